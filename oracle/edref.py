@@ -1,0 +1,402 @@
+"""oracle/edref.py -- ctypes front end and solve orchestration of the oracle.
+
+TEST INFRASTRUCTURE ONLY (the checker): imported by tests/, by
+__graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
+product library.  The arithmetic lives in oracle/edref.c (a C restatement of
+the reference); this file only sequences it the way the reference's
+ed_solve_single does (ED_MAIN.f90:251-289):
+
+  diagonalize_impurity  -> ed_diag_c      ED_DIAG.f90:93-228
+                        -> ed_analysis    ED_DIAG.f90:248-398
+  observables_impurity  (dens/docc)      ED_OBSERVABLES.f90:74-168
+  buildgf_impurity      -> build_gf_normal / lanc_build_gf_normal_c
+                                          build_gf_normal.f90:4-180
+                        -> get_sigma_normal  get_sigma_normal.f90:1-38
+
+The per-sector eigensolver of the reference is SciFortran's sp_eigh (ARPACK,
+ED_DIAG.f90:179-181) for dim > lanc_dim_threshold and LAPACK eigh on the
+dense sector matrix otherwise (:185-193).  Here the dense path is numpy's
+eigh on the matrix that ref_dense_h builds column by column from the oracle
+H*v, and the sparse path is scipy's eigsh -- the same ARPACK family -- driven
+by the oracle H*v.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libedref.so")
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+def build(force=False):
+    """Compile oracle/edref.c into oracle/libedref.so with gcc (OpenMP)."""
+    src = os.path.join(HERE, "edref.c")
+    if not force and os.path.exists(LIB_PATH) and \
+            os.path.getmtime(LIB_PATH) >= os.path.getmtime(src):
+        return LIB_PATH
+    import subprocess
+    subprocess.check_call(["gcc", "-O2", "-fopenmp", "-shared", "-fPIC", "-Wall",
+                           "-o", LIB_PATH, src, "-lm"])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.ref_set_model.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_int, C.c_int]
+        L.ref_binomial.restype = C.c_longlong
+        L.ref_binomial.argtypes = [C.c_int, C.c_int]
+        L.ref_sector_dim.restype = C.c_longlong
+        L.ref_build_sector.restype = C.c_longlong
+        L.ref_build_sector.argtypes = [C.c_int, C.c_int, _i64p]
+        L.ref_c.argtypes = [C.c_int, C.c_int64, _i64p, _dp]
+        L.ref_cdg.argtypes = [C.c_int, C.c_int64, _i64p, _dp]
+        L.ref_binary_search.restype = C.c_longlong
+        L.ref_binary_search.argtypes = [_i64p, C.c_longlong, C.c_int64]
+        L.ref_hxv.argtypes = [_i64p, C.c_longlong, _dp, _dp]
+        L.ref_hxv_omp.argtypes = [_i64p, C.c_longlong, _dp, _dp, C.c_int]
+        L.ref_pythag.restype = C.c_double
+        L.ref_pythag.argtypes = [C.c_double, C.c_double]
+        L.ref_tql2.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.ref_lanc_tridiag.argtypes = [_i64p, C.c_longlong, _dp, C.c_int, _dp, _dp,
+                                       C.c_double, C.c_int, C.POINTER(C.c_int)]
+        L.ref_lanczos_gs.argtypes = [_i64p, C.c_longlong, _dp, C.c_int, C.c_double,
+                                     C.c_int, _dp, C.c_int]
+        L.ref_apply_op.argtypes = [_i64p, C.c_longlong, _dp, _i64p, C.c_longlong,
+                                   C.c_int, C.c_int, _dp]
+        L.ref_add_to_gf.argtypes = [C.c_double, C.c_double, _dp, _dp, C.c_int, C.c_int,
+                                    _dp, C.c_int, _dp, C.c_int, C.c_double, _dp, _dp]
+        L.ref_invg0.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp]
+        L.ref_delta.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp]
+        L.ref_sigma.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
+        L.ref_observables.argtypes = [_i64p, C.c_longlong, _dp, C.c_double, _dp, _dp]
+        L.ref_dense_h.argtypes = [_i64p, C.c_longlong, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_i64p)
+
+
+# ------------------------------------------------------------ wrappers
+def set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, jp=0.0,
+              xmu=0.0, hfmode=True, jhflag=False):
+    bath = np.ascontiguousarray(bath, dtype=np.float64)
+    hloc_f = np.ascontiguousarray(hloc_f, dtype=np.float64)
+    u = np.zeros(3)
+    u[:len(uloc)] = uloc
+    rc = lib().ref_set_model(norb, nbath, nspin, _d(bath), _d(hloc_f), _d(u), ust, jh,
+                             jx, jp, xmu, int(bool(hfmode)), int(bool(jhflag)))
+    if rc != 0:
+        raise ValueError(f"ref_set_model failed ({rc})")
+    return (nbath + 1) * norb
+
+
+def binomial(n1, n2):
+    return lib().ref_binomial(n1, n2)
+
+
+def build_sector(nup, ndw):
+    L = lib()
+    dim = L.ref_build_sector(nup, ndw, None)
+    m = np.empty(dim, dtype=np.int64)
+    L.ref_build_sector(nup, ndw, _i(m))
+    return m
+
+
+def op_c(pos, state, dagger=False):
+    out = C.c_int64(0)
+    s = C.c_double(0)
+    fn = lib().ref_cdg if dagger else lib().ref_c
+    rc = fn(pos, state, C.byref(out), C.byref(s))
+    if rc != 0:
+        raise ValueError("Pauli violation")
+    return out.value, s.value
+
+
+def binary_search(a, value):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return lib().ref_binary_search(_i(a), len(a), value)
+
+
+def hxv(smap, x, nthreads=1):
+    """Complex(8) on-the-fly H*v of the reference (directMatVec_cc)."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.empty_like(x)
+    if nthreads > 1:
+        rc = lib().ref_hxv_omp(_i(smap), len(smap), _d(x.view(np.float64)),
+                               _d(y.view(np.float64)), nthreads)
+    else:
+        rc = lib().ref_hxv(_i(smap), len(smap), _d(x.view(np.float64)), _d(y.view(np.float64)))
+    if rc != 0:
+        raise RuntimeError(f"ref_hxv failed ({rc})")
+    return y
+
+
+def tql2(d, e, z=None):
+    """EISPACK tql2; e has the sub-diagonal in e[1:] (e[0] ignored)."""
+    n = len(d)
+    d = np.array(d, dtype=np.float64)
+    e = np.array(e, dtype=np.float64)
+    z = np.asfortranarray(np.eye(n) if z is None else np.array(z, dtype=np.float64))
+    ierr = lib().ref_tql2(n, _d(d), _d(e), z.ctypes.data_as(_dp))
+    return d, z, ierr
+
+
+def lanc_tridiag(smap, vin, nlanc, threshold=1e-13, nthreads=1):
+    v = np.array(vin, dtype=np.complex128)
+    a = np.zeros(nlanc)
+    b = np.zeros(nlanc)
+    ns = C.c_int(0)
+    rc = lib().ref_lanc_tridiag(_i(smap), len(smap), _d(v.view(np.float64)), nlanc, _d(a),
+                                _d(b), threshold, nthreads, C.byref(ns))
+    if rc != 0:
+        raise RuntimeError(f"ref_lanc_tridiag failed ({rc})")
+    return a, b, ns.value
+
+
+def lanczos_gs(smap, v0, nitermax=512, threshold=1e-13, ncheck=10, nthreads=1):
+    v = np.array(v0, dtype=np.complex128)
+    egs = np.zeros(nitermax)
+    n = lib().ref_lanczos_gs(_i(smap), len(smap), _d(v.view(np.float64)), nitermax,
+                             threshold, ncheck, _d(egs), nthreads)
+    if n <= 0:
+        raise RuntimeError(f"ref_lanczos_gs failed ({n})")
+    return egs[:n], v
+
+
+def apply_op(mapI, psi, mapJ, isite, dagger):
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.zeros(len(mapJ), dtype=np.complex128)
+    rc = lib().ref_apply_op(_i(mapI), len(mapI), _d(psi.view(np.float64)), _i(mapJ),
+                            len(mapJ), isite, int(bool(dagger)), _d(out.view(np.float64)))
+    if rc != 0:
+        raise RuntimeError("ref_apply_op: target state absent")
+    return out
+
+
+def add_to_gf(peso_bz, ei, alanc, blanc, isign, wm, wr, eps, gm, gr):
+    alanc = np.ascontiguousarray(alanc, dtype=np.float64)
+    blanc = np.ascontiguousarray(blanc, dtype=np.float64)
+    wm = np.ascontiguousarray(wm, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    return lib().ref_add_to_gf(peso_bz, ei, _d(alanc), _d(blanc), len(alanc), isign,
+                               _d(wm), len(wm), _d(wr), len(wr), eps,
+                               _d(gm.view(np.float64)), _d(gr.view(np.float64)))
+
+
+def invg0(ispin, iorb, z):
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.empty_like(z)
+    lib().ref_invg0(ispin, iorb, _d(z.view(np.float64)), len(z), _d(out.view(np.float64)))
+    return out
+
+
+def delta(ispin, iorb, z):
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.empty_like(z)
+    lib().ref_delta(ispin, iorb, _d(z.view(np.float64)), len(z), _d(out.view(np.float64)))
+    return out
+
+
+def sigma(ispin, iorb, z, g):
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    g = np.ascontiguousarray(g, dtype=np.complex128)
+    out = np.empty_like(z)
+    lib().ref_sigma(ispin, iorb, _d(z.view(np.float64)), _d(g.view(np.float64)), len(z),
+                    _d(out.view(np.float64)))
+    return out
+
+
+def observables(smap, psi, peso, norb):
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    dens = np.zeros(norb)
+    docc = np.zeros(norb)
+    lib().ref_observables(_i(smap), len(smap), _d(psi.view(np.float64)), peso, _d(dens),
+                          _d(docc))
+    return dens, docc
+
+
+def dense_h(smap):
+    n = len(smap)
+    H = np.zeros((n, n), order="F")
+    rc = lib().ref_dense_h(_i(smap), n, H.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError("ref_dense_h failed")
+    return H
+
+
+# ------------------------------------------------------ solve sequence
+class StateList:
+    """Energy-ordered state list with the reference's insert/pop rules
+    (ED_EIGENSPACE.f90:126-299): a new state goes before existing states of
+    equal energy; pop removes the last (highest) state; emax is only ever
+    raised by inserts (es_pop_state leaves it stale, :284-290)."""
+
+    def __init__(self):
+        self.items = []            # (E, sector, vec)
+        self.emax = -np.inf
+        self.emin = np.inf
+
+    def free(self):
+        self.items = []
+        self.emax = -np.inf
+        self.emin = np.inf
+
+    def insert(self, e, sector, vec):
+        k = 0
+        while k < len(self.items) and not (e <= self.items[k][0]):
+            k += 1
+        self.items.insert(k, (e, sector, vec))
+        self.emax = max(self.emax, e)
+        self.emin = min(self.emin, e)
+
+    def add(self, e, sector, vec, size=None):
+        if size is None or len(self.items) < size:
+            self.insert(e, sector, vec)
+        elif e < self.items[-1][0]:
+            self.items.pop()
+            self.insert(e, sector, vec)
+
+    def pop(self):
+        self.items.pop()
+
+
+def default_params(**kw):
+    p = dict(lanc_nstates_sector=6, lanc_nstates_total=1, lanc_nstates_step=2,
+             lanc_niter=512, lanc_ngfiter=200, lanc_tolerance=1e-12,
+             lanc_dim_threshold=256, lanc_ncv_factor=3, lanc_ncv_add=5,
+             gs_threshold=1e-9, cutoff=1e-9, beta=1000.0, lmats=5000, lreal=5000,
+             wini=-5.0, wfin=5.0, eps=0.01, chain_threshold=1e-13)
+    p.update(kw)
+    return p
+
+
+def _sector_eigs(smap, neigen, p, seed=1234567):
+    dim = len(smap)
+    lanc = not (neigen == dim or dim <= p["lanc_dim_threshold"])
+    if not lanc:
+        H = dense_h(smap)
+        w, U = np.linalg.eigh(H)
+        return w[:neigen], U[:, :neigen]
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    op = LinearOperator((dim, dim), matvec=lambda x: hxv(smap, x.astype(np.complex128)).real,
+                        dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    ncv = min(dim, max(p["lanc_ncv_factor"] * neigen + p["lanc_ncv_add"], 2 * neigen + 1))
+    w, U = eigsh(op, k=neigen, which="SA", tol=p["lanc_tolerance"], ncv=ncv,
+                 v0=rng.uniform(-1, 1, dim), maxiter=10000)
+    order = np.argsort(w)
+    return w[order], U[:, order]
+
+
+def ed_solve(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, xmu=0.0,
+             hfmode=True, params=None, sectors=None):
+    """The reference's ed_solve_single sequence on the oracle (small Ns only).
+    Returns a dict with egs, zeta, states, dens, docc, gmats, greal, smats, sreal
+    (arrays indexed [ispin, iorb, freq])."""
+    p = default_params() if params is None else params
+    ns = set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust, jh, 0.0, 0.0, xmu, hfmode)
+    finite_t = p["lanc_nstates_total"] != 1
+    nstates_sector = p["lanc_nstates_sector"]
+    nstates_total = p["lanc_nstates_total"]
+    if finite_t:
+        nstates_sector += nstates_sector % 2
+        nstates_total += nstates_total % 2
+    sl = StateList()
+    oldzero = 1000.0
+    all_sectors = [(nu, nd) for nu in range(ns + 1) for nd in range(ns + 1)]
+    for (nu, nd) in (all_sectors if sectors is None else sectors):
+        smap = build_sector(nu, nd)
+        dim = len(smap)
+        neigen = min(dim, min(dim, nstates_sector))
+        w, U = _sector_eigs(smap, neigen, p)
+        for i in range(neigen):
+            e = float(w[i])
+            if finite_t:
+                sl.add(e, (nu, nd), U[:, i], size=nstates_total)
+            else:
+                if e < oldzero - 10.0 * p["gs_threshold"]:
+                    oldzero = e
+                    sl.free()
+                    sl.add(e, (nu, nd), U[:, i])
+                elif abs(e - oldzero) <= p["gs_threshold"]:
+                    oldzero = min(oldzero, e)
+                    sl.add(e, (nu, nd), U[:, i])
+    beta = p["beta"]
+    egs = sl.emin
+    if finite_t:
+        zeta = sum(np.exp(-beta * (e - egs)) for e, _, _ in sl.items)
+        ec = sl.emax
+        if not (np.exp(-beta * (ec - egs)) > p["cutoff"]):
+            while np.exp(-beta * (sl.items[-1][0] - egs)) <= p["cutoff"]:
+                sl.pop()
+    else:
+        zeta = float(len(sl.items))
+    # observables
+    dens = np.zeros(norb)
+    docc = np.zeros(norb)
+    for e, (nu, nd), vec in sl.items:
+        peso = (np.exp(-beta * (e - egs)) if finite_t else 1.0) / zeta
+        dd, oo = observables(build_sector(nu, nd), vec, peso, norb)
+        dens += dd
+        docc += oo
+    # Green's functions
+    wm = np.pi / beta * (2 * np.arange(1, p["lmats"] + 1) - 1)
+    wr = np.linspace(p["wini"], p["wfin"], p["lreal"])
+    gm = np.zeros((nspin, norb, p["lmats"]), dtype=np.complex128)
+    gr = np.zeros((nspin, norb, p["lreal"]), dtype=np.complex128)
+    for s in range(nspin):
+        for a in range(norb):
+            isite = a + 1 + s * ns
+            for e, (nu, nd), vec in sl.items:
+                mapI = build_sector(nu, nd)
+                for dagger, isign in ((True, 1), (False, -1)):
+                    ju = nu + (1 if dagger else -1) * (s == 0)
+                    jd = nd + (1 if dagger else -1) * (s == 1)
+                    if ju < 0 or jd < 0 or ju > ns or jd > ns:
+                        continue
+                    mapJ = build_sector(ju, jd)
+                    v = apply_op(mapI, vec.astype(np.complex128), mapJ, isite, dagger)
+                    norm2 = float(np.vdot(v, v).real)
+                    if norm2 == 0.0:
+                        # c^+|psi> (or c|psi>) vanishes: zero spectral weight.  The
+                        # reference divides by sqrt(0) here (build_gf_normal.f90:115).
+                        continue
+                    v = v / np.sqrt(norm2)
+                    nlanc = min(len(mapJ), p["lanc_ngfiter"])
+                    al, bl, _ = lanc_tridiag(mapJ, v, nlanc, p["chain_threshold"])
+                    if finite_t:
+                        if beta * (e - egs) < 200:
+                            peso = norm2 * np.exp(-beta * (e - egs)) / zeta
+                        else:
+                            peso = 0.0
+                    else:
+                        peso = norm2 / zeta
+                    add_to_gf(peso, e, al, bl, isign, wm, wr, p["eps"], gm[s, a], gr[s, a])
+    sm = np.zeros_like(gm)
+    sr = np.zeros_like(gr)
+    for s in range(nspin):
+        for a in range(norb):
+            sm[s, a] = sigma(s, a, 1j * wm, gm[s, a])
+            sr[s, a] = sigma(s, a, wr + 1j * p["eps"], gr[s, a])
+    return dict(egs=egs, zeta=zeta, states=[(e, sec) for e, sec, _ in sl.items],
+                vectors=[v for _, _, v in sl.items], dens=dens, docc=docc, gmats=gm,
+                greal=gr, smats=sm, sreal=sr, wm=wm, wr=wr)

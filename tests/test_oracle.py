@@ -1,0 +1,251 @@
+"""Pin the oracle (oracle/edref.c + oracle/edref.py) before trusting it.
+
+Each check compares the oracle with something it does not share code with:
+the independent Jordan-Wigner fixtures (tests/golden/make_golden.py), library
+routines (math.comb, numpy.searchsorted, scipy eigh_tridiagonal) or closed
+forms (Toeplitz spectrum, U=0 and V=0 limits, Krylov moments).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import edref
+from conftest import GOLDEN
+
+NAMES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+
+
+def set_model_from(g):
+    return edref.set_model(int(g["norb"]), int(g["nbath"]), int(g["nspin"]), g["bath"],
+                           g["hloc_f"], g["uloc"], float(g["ust"]), float(g["jh"]),
+                           float(g["jx"]), float(g["jp"]), float(g["xmu"]),
+                           int(g["hfmode"]), int(g["jhflag"]))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _build():
+    edref.build()
+
+
+def test_binomial_matches_math_comb():
+    for n in range(0, 31):
+        for k in range(-1, n + 1):
+            assert edref.binomial(n, k) == (math.comb(n, k) if k >= 0 else 0)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_sector_maps_and_hxv_match_jordan_wigner(name):
+    g = load(name)
+    ns = set_model_from(g)
+    offs = g["sector_offsets"]
+    k = 0
+    for nu in range(ns + 1):
+        for nd in range(ns + 1):
+            m = edref.build_sector(nu, nd)
+            ref = g["maps"][offs[k]:offs[k + 1]]
+            assert np.array_equal(m, ref), (nu, nd)
+            x = g["x"][offs[k]:offs[k + 1]]
+            y = edref.hxv(m, x)
+            hx = g["hx"][offs[k]:offs[k + 1]]
+            assert np.abs(y.imag).max(initial=0) == 0.0
+            scale = max(1.0, np.abs(hx).max(initial=0))
+            assert np.abs(y.real - hx).max(initial=0) <= 1e-13 * scale, (nu, nd)
+            # complex input: H is real symmetric, so H(x + i x') = Hx + i Hx'
+            if len(m) > 1:
+                yc = edref.hxv(m, x + 1j * x[::-1])
+                assert np.allclose(yc.imag, edref.hxv(m, x[::-1]).real, atol=1e-13 * scale)
+            k += 1
+
+
+def test_openmp_row_blocks_match_serial():
+    g = load("m3_norb2_nb2_jh")
+    set_model_from(g)
+    m = edref.build_sector(3, 3)
+    x = np.random.default_rng(1).standard_normal(len(m)) + 0j
+    assert np.array_equal(edref.hxv(m, x, nthreads=3), edref.hxv(m, x))
+
+
+def test_c_cdg_pauli_and_sign():
+    # sign = (-1)^(occupied positions below pos); Pauli violations raise
+    out, s = edref.op_c(3, 0b0111)
+    assert out == 0b0011 and s == 1.0
+    out, s = edref.op_c(2, 0b0111)
+    assert out == 0b0101 and s == -1.0
+    out, s = edref.op_c(4, 0b0111, dagger=True)
+    assert out == 0b1111 and s == -1.0
+    with pytest.raises(ValueError):
+        edref.op_c(4, 0b0111)
+    with pytest.raises(ValueError):
+        edref.op_c(1, 0b0111, dagger=True)
+
+
+def test_binary_search_matches_searchsorted():
+    rng = np.random.default_rng(7)
+    for n in (0, 1, 2, 3, 10, 257, 1000):
+        a = np.unique(rng.integers(0, 10 * n + 5, size=n))
+        for v in list(a) + [-1, 10 * n + 7]:
+            j = edref.binary_search(a, int(v))
+            pos = np.searchsorted(a, v)
+            expect = pos + 1 if pos < len(a) and a[pos] == v else 0
+            assert j == expect
+
+
+def test_tql2_toeplitz_closed_form_and_vectors():
+    for n in (1, 2, 5, 40, 200):
+        a, b = 0.3, -0.7
+        d = np.full(n, a)
+        e = np.zeros(n)
+        e[1:] = b
+        w, z, ierr = edref.tql2(d, e)
+        assert ierr == 0
+        k = np.arange(1, n + 1)
+        exact = np.sort(a + 2 * b * np.cos(k * np.pi / (n + 1)))
+        assert np.allclose(w, exact, atol=1e-12)
+        T = np.diag(d) + np.diag(e[1:], 1) + np.diag(e[1:], -1)
+        assert np.allclose(T @ z, z * w, atol=1e-11)
+        assert np.allclose(z.T @ z, np.eye(n), atol=1e-11)
+
+
+def test_tql2_matches_scipy_random():
+    from scipy.linalg import eigh_tridiagonal
+    rng = np.random.default_rng(3)
+    for n in (3, 17, 300):
+        d = rng.standard_normal(n)
+        off = rng.uniform(0.01, 1.0, n - 1)
+        e = np.concatenate([[0.0], off])
+        w, z, ierr = edref.tql2(d, e)
+        ws, zs = eigh_tridiagonal(d, off)
+        assert ierr == 0
+        assert np.allclose(w, ws, atol=1e-12)
+        # first components squared are the pole weights used by the GF
+        assert np.allclose(z[0] ** 2, zs[0] ** 2, atol=1e-10)
+
+
+def test_pythag():
+    for a, b in ((3.0, 4.0), (0.0, 0.0), (1e-300, 1e-300), (1e200, 1e200), (-5.0, 12.0)):
+        assert np.isclose(edref.lib().ref_pythag(a, b), math.hypot(a, b), rtol=1e-14)
+
+
+def test_lanczos_chain_reproduces_krylov_moments():
+    g = load("m4_norb3_nb1")
+    set_model_from(g)
+    m = edref.build_sector(3, 3)
+    H = edref.dense_h(m)
+    v = np.random.default_rng(11).standard_normal(len(m))
+    v /= np.linalg.norm(v)
+    n = 12
+    a, b, steps = edref.lanc_tridiag(m, v, n)
+    assert steps == n
+    T = np.diag(a) + np.diag(b[1:], 1) + np.diag(b[1:], -1)
+    for k in range(0, 2 * n - 2):
+        mom_h = v @ np.linalg.matrix_power(H, k) @ v
+        mom_t = np.linalg.matrix_power(T, k)[0, 0]
+        assert np.isclose(mom_h, mom_t, rtol=1e-9, atol=1e-9 * np.abs(H).max() ** k)
+
+
+def test_lanczos_chain_early_exit_on_invariant_subspace():
+    g = load("m1_norb1_nb3")
+    set_model_from(g)
+    m = edref.build_sector(2, 2)
+    H = edref.dense_h(m)
+    w, U = np.linalg.eigh(H)
+    v = U[:, 0] + U[:, 1]            # lives in a 2-dimensional invariant subspace
+    a, b, steps = edref.lanc_tridiag(m, v / np.linalg.norm(v), 10)
+    assert steps == 2
+    assert np.allclose(np.sort(np.linalg.eigvalsh(np.diag(a[:2]) + np.diag(b[1:2], 1)
+                                                   + np.diag(b[1:2], -1))), w[:2])
+    assert np.all(a[2:] == 0) and np.all(b[3:] == 0)
+
+
+@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m3_norb2_nb2_jh", "m4_norb3_nb1"])
+def test_plain_lanczos_ground_state(name):
+    g = load(name)
+    ns = set_model_from(g)
+    offs, eoffs = g["sector_offsets"], g["eig_offsets"]
+    k = 0
+    for nu in range(ns + 1):
+        for nd in range(ns + 1):
+            m = edref.build_sector(nu, nd)
+            e0 = g["eigs"][eoffs[k]]
+            if len(m) > 30:
+                v0 = np.random.default_rng(k).standard_normal(len(m))
+                eg, vec = edref.lanczos_gs(m, v0)
+                assert abs(eg[0] - e0) <= 1e-10 * max(1.0, abs(e0))
+                r = edref.hxv(m, vec) - eg[0] * vec
+                assert np.linalg.norm(r) < 1e-6
+            k += 1
+
+
+def _solve_golden(g, **pkw):
+    wm, wr = g["wm"], g["wr"]
+    p = edref.default_params(beta=float(g["beta"]), lmats=len(wm), lreal=len(wr),
+                             wini=float(wr[0]), wfin=float(wr[-1]), eps=float(g["eps"]),
+                             **pkw)
+    norb = int(g["norb"])
+    return edref.ed_solve(norb, int(g["nbath"]), int(g["nspin"]), g["bath"], g["hloc_f"],
+                          g["uloc"][:norb], float(g["ust"]), float(g["jh"]),
+                          float(g["xmu"]), bool(g["hfmode"]), params=p)
+
+
+@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m2_norb2_nb1", "m4_norb3_nb1", "m6_u0"])
+def test_ed_solve_zero_temperature_matches_lehmann(name):
+    g = load(name)
+    r = _solve_golden(g, lanc_ngfiter=400)
+    assert abs(r["egs"] - g["egs"]) <= 1e-10 * abs(g["egs"])
+    assert len(r["states"]) == int(g["nstates"])
+    assert np.allclose(r["dens"], g["dens"], atol=1e-10)
+    assert np.allclose(r["docc"], g["docc"], atol=1e-10)
+    nsp = int(g["nspin"])
+    assert np.abs(r["gmats"] - g["gmats"][:nsp]).max() < 1e-8
+    # Real axis: the plain (non-reorthogonalised) chain of the reference loses
+    # orthogonality once it runs over a whole small sector, so poles of weight
+    # ~1e-6 are replaced by ghosts; at eps=0.05 that moves G(w+i eps) by ~1e-4
+    # while every Matsubara moment stays exact.
+    assert np.abs(r["greal"] - g["greal"][:nsp]).max() < 5e-4
+
+
+def test_ed_solve_finite_temperature_matches_lehmann():
+    g = load("m5_norb1_nb5_T")
+    r = _solve_golden(g, lanc_nstates_sector=400, lanc_nstates_total=216, lanc_ngfiter=400)
+    assert abs(r["egs"] - g["egs"]) <= 1e-10 * abs(g["egs"])
+    assert np.allclose(r["dens"], g["dens"], atol=1e-8)
+    assert np.allclose(r["docc"], g["docc"], atol=1e-8)
+    assert np.abs(r["gmats"] - g["gmats"]).max() < 1e-7
+
+
+def test_u0_closed_form():
+    """U=0: E0 = 2 * sum of negative single-particle levels and G = 1/(iw+mu-ed-Delta),
+    so Sigma vanishes."""
+    g = load("m6_u0")
+    nb = int(g["nbath"])
+    e, v = g["bath"][:nb], g["bath"][nb:]
+    h1 = np.zeros((nb + 1, nb + 1))
+    h1[0, 0] = -float(g["xmu"])
+    h1[0, 1:] = h1[1:, 0] = v
+    h1[1:, 1:] = np.diag(e)
+    eps = np.linalg.eigvalsh(h1)
+    assert np.isclose(2 * eps[eps < 0].sum(), g["egs"], atol=1e-12)
+    r = _solve_golden(g, lanc_ngfiter=400)
+    z = 1j * g["wm"]
+    g0 = 1.0 / (z + float(g["xmu"]) - (v ** 2 / (z[:, None] - e)).sum(axis=1))
+    assert np.abs(r["gmats"][0, 0] - g0).max() < 1e-9
+    assert np.abs(r["smats"][0, 0]).max() < 1e-7
+
+
+def test_atomic_limit_closed_form():
+    """V=0, hfmode, half filling: G = 1/2 [1/(iw-U/2) + 1/(iw+U/2)],
+    Sigma = U^2/(4 iw) (the bath levels decouple)."""
+    g = load("m7_atomic")
+    U = float(g["uloc"][0])
+    z = 1j * g["wm"]
+    gat = 0.5 * (1 / (z - U / 2) + 1 / (z + U / 2))
+    assert np.abs(g["gmats"][0, 0] - gat).max() < 1e-12
+    r = _solve_golden(g, lanc_nstates_sector=64, lanc_nstates_total=64, lanc_ngfiter=64)
+    assert np.abs(r["gmats"][0, 0] - gat).max() < 1e-9
+    assert np.abs(r["smats"][0, 0] - U * U / (4 * z)).max() < 1e-7
