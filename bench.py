@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark of the impurity-solver hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one FP64 sector H*v over the Ns=16 half-filled sector 8/8
+(dim 12870 x 12870 = 165,636,900; Norb=1, Nbath=15, U=2.5, seeded bath),
+inputs resident in HBM.  Prints ONE JSON line with the metric, the roofline
+of the H*v kernel (algorithmic bytes 16*dim per launch: x read once, y written
+once) measured with HIP events on the library stream, and the CPU baseline:
+the oracle's complex(8) restatement of the reference H*v on a bounded row
+sample of the same sector (rank 0, N=1 only).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dmft-ed_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, sample_rows, threads):
+    """Oracle H*v (combined map + binary search + complex(8), the reference
+    algorithm) on rows [0, sample_rows) of the sector; returns rows/s."""
+    import ctypes as C
+    import numpy as np
+    import edref
+    edref.build()
+    edref.set_model(cfg["norb"], cfg["nbath"], cfg["nspin"], cfg["bath"], cfg["hloc"],
+                    cfg["uloc"], cfg["ust"], cfg["jh"], 0.0, 0.0, cfg["xmu"], True, False)
+    nu, nd = cfg["sector"]
+    t0 = time.time()
+    smap = edref.build_sector(nu, nd)
+    log(f"cpu baseline: map of {len(smap)} codes built in {time.time() - t0:.1f}s")
+    x = np.empty(len(smap), dtype=np.complex128)
+    x.real = np.random.default_rng(1).standard_normal(len(smap))
+    x.imag = 0.0
+    y = np.zeros(sample_rows, dtype=np.complex128)
+    L = edref.lib()
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int64)
+    # y is only written on [0, sample_rows): pass it as the start of hv
+    t0 = time.perf_counter()
+    rc = L.ref_hxv_rows(smap.ctypes.data_as(ip), len(smap), x.view(np.float64).ctypes.data_as(dp),
+                        y.view(np.float64).ctypes.data_as(dp), 0, sample_rows, threads)
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"oracle H*v failed ({rc})")
+    return sample_rows / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--prec", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-rows", type=int, default=2_000_000)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              "(no multi-GPU sharding); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import dmed
+    from dmed import configs
+
+    cfg = configs.config(5)
+    dmed.init(0)
+    dmed.set_model(cfg["norb"], cfg["nbath"], cfg["nspin"], cfg["bath"], cfg["hloc"], cfg["uloc"],
+                   cfg["ust"], cfg["jh"], 0.0, 0.0, cfg["xmu"], True, False)
+    nu, nd = cfg["sector"]
+    sec = dmed.Sector(nu, nd)
+    D = sec.dim
+    prec = dmed.FP64 if a.prec == "fp64" else dmed.FP32
+    esz = 8 if prec == dmed.FP64 else 4
+    log(f"workload {cfg['name']}: dim {D} ({sec.dim_up} x {sec.dim_dw}), {a.prec}")
+    x = dmed.Vec(sec, prec)
+    y = dmed.Vec(sec, prec)
+    x.upload(np.random.default_rng(0).standard_normal(D))
+    L = dmed.lib()
+    for _ in range(a.warmup):
+        dmed._check(L.dmed_hxv_async(sec.h, x.ptr(), y.ptr(), prec))
+    dmed._check(L.dmed_sync())
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        dmed._check(L.dmed_hxv_async(sec.h, x.ptr(), y.ptr(), prec))
+    dmed._check(L.dmed_sync())
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / a.steps
+    import ctypes as C
+    kms = C.c_double()
+    dmed._check(L.dmed_hxv_timed(sec.h, x.ptr(), y.ptr(), prec, a.steps, C.byref(kms)))
+    kernel_ms = kms.value
+    alg_bytes = 2 * esz * D
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    value = a.steps / wall
+    log(f"H*v: {ms_step:.4f} ms/step wall, kernel {kernel_ms:.4f} ms, "
+        f"{achieved:.1f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of HBM peak)")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hxv_ns16_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        key = f"{a.prec}_bytes_per_launch"
+        traffic = t.get(key)
+
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        rows = min(a.cpu_rows, D)
+        rate, dt = cpu_baseline(cfg, rows, threads)
+        cpu = {"value": rate / D, "unit": "Hv/s", "cores": threads, "kind": "port",
+               "sample": f"rows [0,{rows}) of the dim-{D} sector H*v ({dt:.1f}s), oracle "
+                         f"complex(8) map+binary-search restatement, extrapolated to the "
+                         f"full sector"}
+        log(f"cpu baseline: {rate:.3e} rows/s on {threads} threads -> {rate / D:.4f} Hv/s")
+
+    out = {
+        "metric": "H*v/s at dim 1.66e8 (Ns=16 half-filled sector, FP64)",
+        "value": value,
+        "unit": "Hv/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == dmed.FP64 else "f32",
+        "data": "synthetic (seeded SplitMix64 bath, random FP64 vector)",
+        "config": {"workload": cfg["name"], "dim": D, "dim_up": sec.dim_up,
+                   "dim_dw": sec.dim_dw, "norb": 1, "nbath": 15, "U": 2.5,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+        "kernel_ms": kernel_ms,
+        "algorithmic_bytes_per_launch": alg_bytes,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

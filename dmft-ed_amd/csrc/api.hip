@@ -1,0 +1,648 @@
+// api.hip -- the C ABI (include/dmed.h): context, model, sectors, vectors,
+// H*v, eigensolver, Green's-function start vectors, chains and the whole
+// ed_solve sequence.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "dmed_internal.h"
+
+namespace dmed {
+
+Context &ctx()
+{
+    static Context c;
+    return c;
+}
+
+int fail(int code, const std::string &msg)
+{
+    ctx().last_error = msg;
+    return code;
+}
+
+int ensure_partials(size_t n)
+{
+    Context &c = ctx();
+    if (n <= c.partials_cap) return 0;
+    if (c.partials) {
+        hipStreamSynchronize(c.stream);
+        hipFree(c.partials);
+    }
+    c.partials = nullptr;
+    c.partials_cap = 0;
+    DMED_HIP(hipMalloc(&c.partials, n * sizeof(double)));
+    c.partials_cap = n;
+    return 0;
+}
+
+}  // namespace dmed
+
+using namespace dmed;
+
+#define REQUIRE_INIT()                                                     \
+    do {                                                                   \
+        if (!ctx().initialized) return fail(DMED_ESTATE, "dmed_init not called"); \
+    } while (0)
+#define REQUIRE_MODEL()                                                    \
+    do {                                                                   \
+        REQUIRE_INIT();                                                    \
+        if (!ctx().has_model) return fail(DMED_ESTATE, "dmed_set_model not called"); \
+    } while (0)
+
+extern "C" {
+
+int dmed_init(int device)
+{
+    Context &c = ctx();
+    if (c.initialized) {
+        if (device == c.device) return 0;
+        return fail(DMED_ESTATE, "already initialised on another device");
+    }
+    int n = 0;
+    DMED_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(DMED_EINVAL, "no such device");
+    DMED_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    DMED_HIP(hipGetDeviceProperties(&prop, device));
+    c.device = device;
+    c.num_cu = prop.multiProcessorCount;
+    c.lds_max = std::max<size_t>(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+    if (c.lds_max == 0) c.lds_max = 65536;
+    if (c.lds_max > 160 * 1024) c.lds_max = 160 * 1024;
+    DMED_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    DMED_HIP(hipMalloc(&c.dscal, kScalCap * sizeof(double)));
+    DMED_HIP(hipMemset(c.dscal, 0, kScalCap * sizeof(double)));
+    c.initialized = true;
+    if (int rc = ensure_partials(1 << 16)) return rc;
+    return 0;
+}
+
+int dmed_finalize(void)
+{
+    Context &c = ctx();
+    if (!c.initialized) return 0;
+    hipStreamSynchronize(c.stream);
+    hipFree(c.partials);
+    hipFree(c.dscal);
+    hipStreamDestroy(c.stream);
+    c.partials = nullptr;
+    c.partials_cap = 0;
+    c.dscal = nullptr;
+    c.stream = nullptr;
+    c.initialized = false;
+    c.has_model = false;
+    return 0;
+}
+
+const char *dmed_last_error(void) { return ctx().last_error.c_str(); }
+
+int dmed_set_model(const dmed_model *m)
+{
+    REQUIRE_INIT();
+    if (!m || !m->bath || !m->hloc) return fail(DMED_EINVAL, "null model");
+    if (m->norb < 1 || m->norb > 3) return fail(DMED_EINVAL, "Norb must be 1..3 (ED_SETUP.f90:276)");
+    if (m->nspin < 1 || m->nspin > 2) return fail(DMED_EINVAL, "Nspin must be 1..2 (ED_SETUP.f90:275)");
+    if (m->nbath < 0) return fail(DMED_EINVAL, "Nbath < 0");
+    const int ns = (m->nbath + 1) * m->norb;
+    if (ns > kMaxNs) return fail(DMED_EINVAL, "Ns too large for 32-bit spin strings");
+    if (m->jhflag && m->norb > 1)
+        return fail(DMED_EUNSUPP, "spin-exchange/pair-hopping (Jhflag) is not built on the GPU path");
+    Model &md = ctx().model;
+    md.norb = m->norb;
+    md.nbath = m->nbath;
+    md.nspin = m->nspin;
+    md.ns = ns;
+    const int nb = m->nspin * m->norb * m->nbath;
+    md.e.assign(m->bath, m->bath + nb);
+    md.v.assign(m->bath + nb, m->bath + 2 * nb);
+    std::memset(md.hloc, 0, sizeof md.hloc);
+    for (int s = 0; s < m->nspin; ++s)
+        for (int a = 0; a < m->norb; ++a)
+            for (int b = 0; b < m->norb; ++b)
+                md.hloc[s][a][b] = m->hloc[s + m->nspin * (s + m->nspin * (a + m->norb * b))];
+    if (m->nspin == 1)
+        for (int a = 0; a < m->norb; ++a)
+            for (int b = 0; b < m->norb; ++b) md.hloc[1][a][b] = md.hloc[0][a][b];
+    for (int a = 0; a < 3; ++a) md.uloc[a] = m->uloc[a];
+    md.ust = m->ust;
+    md.jh = m->jh;
+    md.jx = m->jx;
+    md.jp = m->jp;
+    md.xmu = m->xmu;
+    md.hfmode = m->hfmode;
+    md.jhflag = m->jhflag;
+    md.generation++;
+    ctx().has_model = true;
+    return 0;
+}
+
+int dmed_get_ns(int32_t *ns)
+{
+    REQUIRE_MODEL();
+    *ns = ctx().model.ns;
+    return 0;
+}
+
+int64_t dmed_binomial(int32_t n1, int32_t n2)
+{
+    // real(8) running product rounded to the nearest integer, ED_SETUP.f90:1258-1275
+    if (n2 < 0) return 0;
+    if (n2 == 0) return 1;
+    double xh = 1.0;
+    for (int i = 1; i <= n2; ++i) xh = xh * (double)(n1 + 1 - i) / (double)i;
+    return (int64_t)(xh + 0.5);
+}
+
+int dmed_sector_create(int32_t nup, int32_t ndw, dmed_sector **out)
+{
+    REQUIRE_MODEL();
+    if (!out) return fail(DMED_EINVAL, "null output");
+    auto s = std::make_unique<dmed_sector>();
+    if (int rc = sector_build(s.get(), nup, ndw)) {
+        sector_release(s.get());
+        return rc;
+    }
+    *out = s.release();
+    return 0;
+}
+
+int dmed_sector_free(dmed_sector *s)
+{
+    if (!s) return 0;
+    if (ctx().initialized) hipStreamSynchronize(ctx().stream);
+    sector_release(s);
+    delete s;
+    return 0;
+}
+
+int dmed_sector_dims(const dmed_sector *s, int64_t *dim, int64_t *dim_up, int64_t *dim_dw)
+{
+    if (!s) return fail(DMED_EINVAL, "null sector");
+    if (dim) *dim = s->dim;
+    if (dim_up) *dim_up = s->up.dim;
+    if (dim_dw) *dim_dw = s->dw.dim;
+    return 0;
+}
+
+int dmed_sector_map(const dmed_sector *s, int64_t *codes)
+{
+    if (!s || !codes) return fail(DMED_EINVAL, "null argument");
+    const int64_t du = s->up.dim, dd = s->dw.dim;
+    for (int64_t id = 0; id < dd; ++id) {
+        const int64_t hi = (int64_t)s->dw.h_str[id] << s->ns;
+        int64_t *o = codes + id * du;
+        for (int64_t iu = 0; iu < du; ++iu) o[iu] = hi | (int64_t)s->up.h_str[iu];
+    }
+    return 0;
+}
+
+int dmed_vec_create(const dmed_sector *s, int32_t prec, dmed_vec **out)
+{
+    REQUIRE_INIT();
+    if (!s || !out) return fail(DMED_EINVAL, "null argument");
+    if (prec != DMED_FP64 && prec != DMED_FP32) return fail(DMED_EINVAL, "bad precision");
+    auto v = std::make_unique<dmed_vec>();
+    v->sec = s;
+    v->prec = prec;
+    v->dim = s->dim;
+    const size_t b = std::max<int64_t>(s->dim, 1) * (prec == DMED_FP32 ? 4 : 8);
+    DMED_HIP(hipMalloc(&v->data, b));
+    DMED_HIP(hipMemsetAsync(v->data, 0, b, ctx().stream));
+    *out = v.release();
+    return 0;
+}
+
+int dmed_vec_free(dmed_vec *v)
+{
+    if (!v) return 0;
+    if (ctx().initialized) hipStreamSynchronize(ctx().stream);
+    hipFree(v->data);
+    delete v;
+    return 0;
+}
+
+void *dmed_vec_data(const dmed_vec *v) { return v ? v->data : nullptr; }
+
+int dmed_vec_upload(dmed_vec *v, const double *host)
+{
+    REQUIRE_INIT();
+    if (!v || !host) return fail(DMED_EINVAL, "null argument");
+    hipStream_t st = ctx().stream;
+    if (v->prec == DMED_FP64) {
+        DMED_HIP(hipMemcpyAsync(v->data, host, v->dim * 8, hipMemcpyHostToDevice, st));
+    } else {
+        double *tmp = nullptr;
+        DMED_HIP(hipMalloc(&tmp, std::max<int64_t>(v->dim, 1) * 8));
+        DMED_HIP(hipMemcpyAsync(tmp, host, v->dim * 8, hipMemcpyHostToDevice, st));
+        int rc = launch_scale_copy(v->data, DMED_FP32, tmp, DMED_FP64, v->dim, 1.0, st);
+        hipStreamSynchronize(st);
+        hipFree(tmp);
+        if (rc) return rc;
+    }
+    DMED_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int dmed_vec_download(const dmed_vec *v, double *host)
+{
+    REQUIRE_INIT();
+    if (!v || !host) return fail(DMED_EINVAL, "null argument");
+    hipStream_t st = ctx().stream;
+    if (v->prec == DMED_FP64) {
+        DMED_HIP(hipMemcpyAsync(host, v->data, v->dim * 8, hipMemcpyDeviceToHost, st));
+    } else {
+        double *tmp = nullptr;
+        DMED_HIP(hipMalloc(&tmp, std::max<int64_t>(v->dim, 1) * 8));
+        int rc = launch_scale_copy(tmp, DMED_FP64, v->data, DMED_FP32, v->dim, 1.0, st);
+        if (!rc) DMED_HIP(hipMemcpyAsync(host, tmp, v->dim * 8, hipMemcpyDeviceToHost, st));
+        hipStreamSynchronize(st);
+        hipFree(tmp);
+        if (rc) return rc;
+    }
+    DMED_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+static int check_sector(const dmed_sector *s)
+{
+    if (!s) return fail(DMED_EINVAL, "null sector");
+    if (s->model_generation != ctx().model.generation)
+        return fail(DMED_ESTATE, "sector was built for an older model; recreate it");
+    return 0;
+}
+
+int dmed_hxv_async(const dmed_sector *s, const void *x, void *y, int32_t prec)
+{
+    REQUIRE_MODEL();
+    if (int rc = check_sector(s)) return rc;
+    if (!x || !y || x == y) return fail(DMED_EINVAL, "x and y must be distinct device buffers");
+    if (prec != DMED_FP64 && prec != DMED_FP32) return fail(DMED_EINVAL, "bad precision");
+    return launch_hxv(s, x, y, prec, kPlain, nullptr, nullptr, nullptr, nullptr, ctx().stream);
+}
+
+int dmed_hxv(const dmed_sector *s, const void *x, void *y, int32_t prec)
+{
+    if (int rc = dmed_hxv_async(s, x, y, prec)) return rc;
+    DMED_HIP(hipStreamSynchronize(ctx().stream));
+    return 0;
+}
+
+int dmed_hxv_timed(const dmed_sector *s, const void *x, void *y, int32_t prec, int32_t nrep,
+                   double *ms_avg)
+{
+    REQUIRE_MODEL();
+    if (int rc = check_sector(s)) return rc;
+    if (!x || !y || x == y || nrep < 1 || !ms_avg) return fail(DMED_EINVAL, "bad argument");
+    hipStream_t st = ctx().stream;
+    hipEvent_t e0, e1;
+    DMED_HIP(hipEventCreate(&e0));
+    DMED_HIP(hipEventCreate(&e1));
+    DMED_HIP(hipEventRecord(e0, st));
+    int rc = 0;
+    for (int r = 0; r < nrep && !rc; ++r)
+        rc = launch_hxv(s, x, y, prec, kPlain, nullptr, nullptr, nullptr, nullptr, st);
+    DMED_HIP(hipEventRecord(e1, st));
+    DMED_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DMED_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms_avg = (double)ms / nrep;
+    return rc;
+}
+
+int dmed_sync(void)
+{
+    REQUIRE_INIT();
+    DMED_HIP(hipStreamSynchronize(ctx().stream));
+    return 0;
+}
+
+int dmed_hxv_host(const dmed_sector *s, const double *x, double *y, int32_t prec)
+{
+    REQUIRE_MODEL();
+    if (int rc = check_sector(s)) return rc;
+    dmed_vec *vx = nullptr, *vy = nullptr;
+    int rc = dmed_vec_create(s, prec, &vx);
+    if (!rc) rc = dmed_vec_create(s, prec, &vy);
+    if (!rc) rc = dmed_vec_upload(vx, x);
+    if (!rc) rc = dmed_hxv(s, vx->data, vy->data, prec);
+    if (!rc) rc = dmed_vec_download(vy, y);
+    dmed_vec_free(vx);
+    dmed_vec_free(vy);
+    return rc;
+}
+
+int dmed_eigh(const dmed_sector *s, int32_t neig, int32_t ncv, int32_t maxit, double tol,
+              int32_t dense_threshold, double *evals, dmed_vec **evecs, double *resid)
+{
+    REQUIRE_MODEL();
+    if (int rc = check_sector(s)) return rc;
+    if (!evals) return fail(DMED_EINVAL, "null evals");
+    if (maxit <= 0) maxit = 20000;
+    if (tol <= 0) tol = 1e-12;
+    return eigh_sector(s, neig, ncv, maxit, tol, dense_threshold, evals, evecs, resid);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- apply c^+ / c
+namespace dmed {
+
+static int32_t host_rank(uint32_t s)
+{
+    int32_t r = 0;
+    int j = 1;
+    while (s) {
+        const int p = __builtin_ctz(s);
+        r += (int32_t)binom(p, j);
+        ++j;
+        s &= s - 1;
+    }
+    return r;
+}
+
+template <typename T>
+__global__ void k_apply_up(const T *__restrict__ psi, int64_t dupI, T *__restrict__ out,
+                           int64_t dupJ, int64_t ddw, const int32_t *__restrict__ src,
+                           const int8_t *__restrict__ sgn, double *__restrict__ partials)
+{
+    __shared__ double red[16];
+    double s2 = 0.0;
+    const int64_t n = dupJ * ddw;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / dupJ, r = e - c * dupJ;
+        const int32_t j = src[r];
+        const double v = j >= 0 ? (double)sgn[r] * (double)psi[j + c * dupI] : 0.0;
+        out[e] = (T)v;
+        s2 += v * v;
+    }
+    double v = s2;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+        partials[blockIdx.x] = t;
+    }
+}
+
+template <typename T>
+__global__ void k_apply_dw(const T *__restrict__ psi, int64_t dup, T *__restrict__ out,
+                           int64_t ddwJ, const int32_t *__restrict__ src,
+                           const int8_t *__restrict__ sgn, double gsign, double *__restrict__ partials)
+{
+    __shared__ double red[16];
+    double s2 = 0.0;
+    const int64_t n = dup * ddwJ;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / dup, r = e - c * dup;
+        const int32_t j = src[c];
+        const double v = j >= 0 ? gsign * (double)sgn[c] * (double)psi[r + (int64_t)j * dup] : 0.0;
+        out[e] = (T)v;
+        s2 += v * v;
+    }
+    double v = s2;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+        partials[blockIdx.x] = t;
+    }
+}
+
+// src/sign of every target string of spin space J from spin space I for
+// c^+ (dagger) or c on bit p; -1 where the operator annihilates the state.
+static void op_tables(const SpinTables &I, const SpinTables &J, int p, int dagger,
+                      std::vector<int32_t> &src, std::vector<int8_t> &sgn)
+{
+    src.assign(J.dim, -1);
+    sgn.assign(J.dim, 0);
+    const uint32_t bit = 1u << p;
+    for (int64_t r = 0; r < J.dim; ++r) {
+        const uint32_t t = J.h_str[r];
+        const bool has = (t & bit) != 0;
+        if (dagger ? !has : has) continue;     // c^+ leaves the bit set, c clears it
+        const uint32_t s = dagger ? (t & ~bit) : (t | bit);
+        src[r] = host_rank(s);
+        sgn[r] = (__builtin_popcount(s & (bit - 1u)) & 1) ? -1 : 1;
+    }
+}
+
+}  // namespace dmed
+
+extern "C" int dmed_apply_op(const dmed_vec *psi, int32_t isite, int32_t dagger,
+                             const dmed_sector *dst, dmed_vec *out, double *norm2)
+{
+    REQUIRE_MODEL();
+    if (!psi || !dst || !out) return fail(DMED_EINVAL, "null argument");
+    if (int rc = check_sector(dst)) return rc;
+    if (int rc = check_sector(psi->sec)) return rc;
+    const dmed_sector *I = psi->sec;
+    const int ns = I->ns;
+    if (isite < 1 || isite > 2 * ns) return fail(DMED_EINVAL, "isite out of range");
+    const bool up = isite <= ns;
+    const int p = up ? isite - 1 : isite - 1 - ns;
+    const int dn = dagger ? 1 : -1;
+    if (dst->nup != I->nup + (up ? dn : 0) || dst->ndw != I->ndw + (up ? 0 : dn))
+        return fail(DMED_EINVAL, "destination sector does not match the operator");
+    if (out->sec != dst || out->prec != psi->prec)
+        return fail(DMED_EINVAL, "output vector must belong to dst with psi's precision");
+    Context &c = ctx();
+    hipStream_t st = c.stream;
+    std::vector<int32_t> src;
+    std::vector<int8_t> sgn;
+    op_tables(up ? I->up : I->dw, up ? dst->up : dst->dw, p, dagger, src, sgn);
+    int32_t *dsrc = nullptr;
+    DMED_HIP(hipMalloc(&dsrc, std::max<size_t>(src.size(), 1) * 5));
+    int8_t *dsgn = reinterpret_cast<int8_t *>(dsrc + std::max<size_t>(src.size(), 1));
+    DMED_HIP(hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, st));
+    DMED_HIP(hipMemcpyAsync(dsgn, sgn.data(), sgn.size(), hipMemcpyHostToDevice, st));
+    const int64_t n = dst->dim;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256));
+    if (int rc = ensure_partials(4096)) { hipFree(dsrc); return rc; }
+    // the down-spin operator passes all up electrons: global sign (-1)^nup
+    const double gsign = (I->nup & 1) ? -1.0 : 1.0;
+    if (psi->prec == DMED_FP64) {
+        if (up)
+            hipLaunchKernelGGL(k_apply_up<double>, dim3(g), dim3(256), 0, st, (const double *)psi->data,
+                               I->up.dim, (double *)out->data, dst->up.dim, dst->dw.dim, dsrc, dsgn,
+                               c.partials);
+        else
+            hipLaunchKernelGGL(k_apply_dw<double>, dim3(g), dim3(256), 0, st, (const double *)psi->data,
+                               I->up.dim, (double *)out->data, dst->dw.dim, dsrc, dsgn, gsign,
+                               c.partials);
+    } else {
+        if (up)
+            hipLaunchKernelGGL(k_apply_up<float>, dim3(g), dim3(256), 0, st, (const float *)psi->data,
+                               I->up.dim, (float *)out->data, dst->up.dim, dst->dw.dim, dsrc, dsgn,
+                               c.partials);
+        else
+            hipLaunchKernelGGL(k_apply_dw<float>, dim3(g), dim3(256), 0, st, (const float *)psi->data,
+                               I->up.dim, (float *)out->data, dst->dw.dim, dsrc, dsgn, gsign,
+                               c.partials);
+    }
+    DMED_HIP(hipGetLastError());
+    double *d = c.dscal + 10;
+    if (int rc = launch_reduce_to(c.partials, (int)g, d, 0, st)) { hipFree(dsrc); return rc; }
+    double h = 0.0;
+    DMED_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipStreamSynchronize(st));
+    hipFree(dsrc);
+    if (norm2) *norm2 = h;
+    return 0;
+}
+
+extern "C" int dmed_chain(const dmed_sector *s, const dmed_vec *v0, int32_t nlanc, int32_t prec,
+                          double threshold, double *alpha, double *beta, int32_t *nsteps)
+{
+    REQUIRE_MODEL();
+    if (int rc = check_sector(s)) return rc;
+    if (!v0 || v0->sec != s) return fail(DMED_EINVAL, "start vector must belong to the sector");
+    if (!alpha || !beta) return fail(DMED_EINVAL, "null alpha/beta");
+    int ns = 0;
+    int rc = chain_run(s, v0->data, v0->prec, nlanc, prec, threshold, alpha, beta, &ns);
+    if (nsteps) *nsteps = ns;
+    return rc;
+}
+
+extern "C" int dmed_gf_add_chain(double peso_bz, double ei, const double *alpha, const double *beta,
+                                 int32_t nlanc, int32_t isign, const double *wm, int32_t lmats,
+                                 const double *wr, int32_t lreal, double eps, double *gmats,
+                                 double *greal)
+{
+    if (nlanc <= 0) return 0;
+    std::vector<double> d(alpha, alpha + nlanc), e(nlanc, 0.0), z(nlanc);
+    for (int i = 1; i < nlanc; ++i) e[i] = beta[i];
+    if (tql2_first_row(nlanc, d.data(), e.data(), z.data()))
+        return fail(DMED_ENOCONV, "tql2 did not converge");
+    for (auto &x : z) x = x * x;
+    gf_add_poles(peso_bz, ei, d.data(), z.data(), nlanc, isign, wm, lmats, wr, lreal, eps, gmats,
+                 greal);
+    return 0;
+}
+
+// --------------------------------------------------------- observables
+namespace dmed {
+
+template <typename T>
+__global__ void k_obs(const T *__restrict__ psi, int64_t dup, int64_t ddw,
+                      const uint8_t *__restrict__ impu, const uint8_t *__restrict__ impd, int norb,
+                      double *__restrict__ partials)
+{
+    __shared__ double red[6][16];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const int64_t n = dup * ddw;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / dup, r = e - c * dup;
+        const double w = (double)psi[e] * (double)psi[e];
+        const unsigned iu = impu[r], id = impd[c];
+        for (int a = 0; a < norb; ++a) {
+            const double nu = (iu >> a) & 1u, nd = (id >> a) & 1u;
+            acc[2 * a] += (nu + nd) * w;
+            acc[2 * a + 1] += nu * nd * w;
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < 2 * norb; ++q) {
+        double v = acc[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[q][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * norb) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[threadIdx.x][i];
+        partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+}  // namespace dmed
+
+extern "C" int dmed_observables(const dmed_vec *psi, double peso, double *dens, double *docc)
+{
+    REQUIRE_MODEL();
+    if (!psi || !dens || !docc) return fail(DMED_EINVAL, "null argument");
+    const dmed_sector *s = psi->sec;
+    if (int rc = check_sector(s)) return rc;
+    Context &c = ctx();
+    hipStream_t st = c.stream;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (s->dim + 255) / 256));
+    if (int rc = ensure_partials(6 * 1024)) return rc;
+    if (psi->prec == DMED_FP64)
+        hipLaunchKernelGGL(k_obs<double>, dim3(g), dim3(256), 0, st, (const double *)psi->data,
+                           s->up.dim, s->dw.dim, s->up.imp, s->dw.imp, s->norb, c.partials);
+    else
+        hipLaunchKernelGGL(k_obs<float>, dim3(g), dim3(256), 0, st, (const float *)psi->data,
+                           s->up.dim, s->dw.dim, s->up.imp, s->dw.imp, s->norb, c.partials);
+    DMED_HIP(hipGetLastError());
+    std::vector<double> h((size_t)2 * s->norb * g);
+    DMED_HIP(hipMemcpyAsync(h.data(), c.partials, h.size() * 8, hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipStreamSynchronize(st));
+    for (int a = 0; a < s->norb; ++a) {
+        double d1 = 0.0, d2 = 0.0;
+        for (unsigned b = 0; b < g; ++b) {
+            d1 += h[(size_t)(2 * a) * g + b];
+            d2 += h[(size_t)(2 * a + 1) * g + b];
+        }
+        dens[a] += peso * d1;
+        docc[a] += peso * d2;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------- bath functions
+extern "C" int dmed_delta(int32_t ispin, int32_t iorb, const double *z, int32_t L, double *delta)
+{
+    REQUIRE_MODEL();
+    const Model &m = ctx().model;
+    if (ispin < 0 || ispin >= m.nspin || iorb < 0 || iorb >= m.norb)
+        return fail(DMED_EINVAL, "spin/orbital out of range");
+    using cd = std::complex<double>;
+    const cd *zz = reinterpret_cast<const cd *>(z);
+    cd *o = reinterpret_cast<cd *>(delta);
+    for (int i = 0; i < L; ++i) {
+        cd d = 0.0;
+        for (int k = 0; k < m.nbath; ++k) {
+            const double v = m.V(ispin, iorb, k);
+            d += v * v / (zz[i] - m.E(ispin, iorb, k));
+        }
+        o[i] = d;
+    }
+    return 0;
+}
+
+extern "C" int dmed_sigma(int32_t ispin, int32_t iorb, const double *z, const double *g, int32_t L,
+                          double *sigma)
+{
+    if (int rc = dmed_delta(ispin, iorb, z, L, sigma)) return rc;
+    const Model &m = ctx().model;
+    using cd = std::complex<double>;
+    const cd *zz = reinterpret_cast<const cd *>(z);
+    const cd *gg = reinterpret_cast<const cd *>(g);
+    cd *o = reinterpret_cast<cd *>(sigma);
+    for (int i = 0; i < L; ++i) {
+        const cd invg0 = zz[i] + m.xmu - m.hloc[ispin][iorb][iorb] - o[i];
+        o[i] = invg0 - 1.0 / gg[i];
+    }
+    return 0;
+}

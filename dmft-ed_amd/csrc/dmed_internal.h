@@ -1,0 +1,167 @@
+// dmed_internal.h -- shared declarations of the gfx950 ED hot-path library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dmed.h"
+
+namespace dmed {
+
+constexpr int kMaxNs = 30;         // spin strings are uint32_t
+constexpr int kMaxTerms = 256;     // one-body hop terms per spin
+
+// -------------------------------------------------------------- errors
+int fail(int code, const std::string &msg);
+#define DMED_HIP(call)                                                      \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess)                                               \
+            return ::dmed::fail(DMED_EHIP, std::string(#call) + ": " +      \
+                                           hipGetErrorString(e_));          \
+    } while (0)
+
+// --------------------------------------------------------------- model
+struct Model {
+    int norb = 0, nbath = 0, nspin = 1, ns = 0;
+    std::vector<double> e, v;          // [spin][orb][k], spin = 0 (up), 1 (down)
+    double hloc[2][3][3] = {};          // spin-diagonal real Hloc
+    double uloc[3] = {}, ust = 0, jh = 0, jx = 0, jp = 0, xmu = 0;
+    int hfmode = 1, jhflag = 0;
+    int generation = 0;                 // bumped by every dmed_set_model
+    double E(int s, int a, int k) const { return e[(s * norb + a) * nbath + k]; }
+    double V(int s, int a, int k) const { return v[(s * norb + a) * nbath + k]; }
+    int bath_bit(int a, int k) const { return norb + a * nbath + k; }   // 0-based
+};
+
+// one-body hop term c^+_i c_j (bit positions, i != j) with coefficient t
+struct HopTerm {
+    int32_t i, j;
+    double t;
+};
+
+// Per-spin tables of a sector: strings of n electrons in Ns orbitals in
+// numeric order, their single-spin diagonal energy, impurity bits and hop
+// lists (target rank, signed coefficient) laid out hop-major [h][r].
+struct SpinTables {
+    int n = 0;
+    int64_t dim = 0;
+    int nhop = 0;
+    uint32_t *str = nullptr;
+    double *diag = nullptr;
+    uint8_t *imp = nullptr;
+    int32_t *hidx = nullptr;
+    double *hval = nullptr;
+    float *hval32 = nullptr;
+    std::vector<uint32_t> h_str;        // host copies (small)
+    std::vector<double> h_diag;
+    std::vector<uint8_t> h_imp;
+    std::vector<int32_t> h_hidx;
+    std::vector<double> h_hval;
+};
+
+struct Context;
+
+}  // namespace dmed
+
+struct dmed_sector {
+    int nup = 0, ndw = 0, ns = 0, norb = 0;
+    int64_t dim = 0;
+    int model_generation = -1;
+    dmed::SpinTables up, dw;
+    int nimp = 1;                       // 2^Norb
+    double *W = nullptr;                // device [nimp*nimp] interaction table W[iu*nimp+id]
+    std::vector<double> h_W;
+};
+
+struct dmed_vec {
+    const dmed_sector *sec = nullptr;
+    int prec = DMED_FP64;
+    int64_t dim = 0;
+    void *data = nullptr;
+};
+
+namespace dmed {
+
+// ------------------------------------------------------------- context
+struct Context {
+    bool initialized = false;
+    int device = 0;
+    int num_cu = 256;
+    size_t lds_max = 160 * 1024;
+    hipStream_t stream = nullptr;
+    Model model;
+    bool has_model = false;
+    // scratch: block partial sums and small device state
+    double *partials = nullptr;
+    size_t partials_cap = 0;            // in doubles
+    double *dscal = nullptr;            // device scalars (chain state), kScalCap doubles
+    std::string last_error;
+};
+Context &ctx();
+constexpr int kScalCap = 4096;
+
+int ensure_partials(size_t n);
+
+// ------------------------------------------------------------- sectors
+int sector_build(dmed_sector *s, int nup, int ndw);
+void sector_release(dmed_sector *s);
+int64_t binom(int n, int k);   // exact, n <= 62
+
+// ---------------------------------------------------------------- H*v
+// Epilogue of the H*v kernel.
+//   kPlain   : y = Hx
+//   kLanczos : y = sx*Hx - b*svp*vp, and block partials of <sx*x, y>
+enum HxvMode { kPlain = 0, kLanczos = 1 };
+
+struct ChainScalars {      // device-resident (inside Context::dscal)
+    double sx;             // scale of the current Lanczos vector
+    double svp;            // scale of the previous one
+    double b;              // previous beta
+    double alpha;          // current alpha
+    double thr;            // early-exit threshold on beta
+    int32_t step;          // 0-based step index
+    int32_t done;          // 1 once |beta| < thr
+    int32_t nlanc;
+    int32_t pad;
+};
+
+int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
+               const void *vp, ChainScalars *cs, double *partials, int *nparts,
+               hipStream_t st);
+
+// ---------------------------------------------------------- vector ops
+int launch_reduce_to(const double *partials, int n, double *out, int op, hipStream_t st);
+int launch_dot(const void *x, const void *y, int64_t n, int prec, double *partials,
+               int *nparts, hipStream_t st);
+int launch_fill_random(void *x, int64_t n, int prec, uint64_t seed, hipStream_t st);
+int launch_scale_copy(void *dst, int dprec, const void *src, int sprec, int64_t n,
+                      double a, hipStream_t st);
+int launch_axpy(void *y, int yprec, const void *x, int xprec, int64_t n, double a,
+                hipStream_t st);
+
+// --------------------------------------------------------- Krylov
+int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
+              double threshold, double *alpha, double *beta, int *nsteps);
+int eigh_sector(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
+                int dense_threshold, double *evals, dmed_vec **evecs, double *resid);
+// lowest_multiplet: the lowest level and its copies within `degen` (at most
+// neig); otherwise the neig lowest with multiplets completed.  *nout pairs.
+int eigh_sector_ex(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
+                   int dense_threshold, double degen, bool lowest_multiplet, int *nout,
+                   double *evals, dmed_vec **evecs, double *resid);
+
+// ------------------------------------------------------------ host LA
+// symmetric eigen-decomposition of a dense n x n matrix (column-major, in
+// place -> eigenvectors), eigenvalues ascending
+int host_syev(int n, double *a, double *w);
+// tql2 of the reference restricted to the first row of Z (the pole weights)
+int tql2_first_row(int n, double *d, double *e, double *z1);
+void gf_add_poles(double peso, double ei, const double *lam, const double *w2, int n,
+                  int isign, const double *wm, int lmats, const double *wr, int lreal,
+                  double eps, double *gm, double *gr);
+
+}  // namespace dmed

@@ -1,0 +1,879 @@
+// krylov.hip -- Lanczos engines on the device: the plain Green's-function
+// chain (sp_lanc_tridiag semantics) and the ground-state eigensolver that
+// replaces sp_eigh.
+//
+// Chain step k (.repo/PLAIN_LANCZOS.f90:87-118, with the vector scale kept in
+// a device scalar instead of a rescaling pass):
+//   K1  w = sx*H v - b*svp*v_prev,   partials of <sx v, w>        (H*v kernel)
+//   F1  alpha_k = sum partials
+//   K2  w -= alpha_k*sx*v,           partials of <w, w>
+//   F2  beta_k = sqrt(sum); v_prev <- v (scale sx); v <- w (scale 1/beta_k)
+// All scalars stay on the device, so a chain is a fixed kernel sequence with
+// no host round trip; reductions are two-level and deterministic.
+#include "dmed_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dmed {
+
+int hxv_nparts(const dmed_sector *s, int prec);
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double bsum(double v, double *sh)
+{
+    v = wsum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    if (w == 0) {
+        v = (lane < nw) ? sh[lane] : 0.0;
+        v = wsum(v);
+    }
+    return v;
+}
+
+constexpr int kVecBS = 256;
+constexpr int kVecGrid = 2048;   // grid-stride vector kernels: fixed partial count
+
+static unsigned vec_grid(int64_t n)
+{
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(kVecGrid, (n + kVecBS - 1) / kVecBS));
+}
+
+template <typename TX, typename TY>
+__global__ void __launch_bounds__(kVecBS) k_dot(const TX *__restrict__ x, const TY *__restrict__ y,
+                                                int64_t n, double *__restrict__ partials)
+{
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        s += (double)x[i] * (double)y[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// fixed-order final sum; op 0: plain sum, 1: sqrt of sum
+__global__ void __launch_bounds__(1024) k_reduce(const double *__restrict__ p, int n,
+                                                 double *__restrict__ out, int op)
+{
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) *out = op == 1 ? sqrt(s) : s;
+}
+
+// chain state init: sx = 1/||v0||, b = 0
+__global__ void __launch_bounds__(1024) k_chain_init(const double *__restrict__ p, int n,
+                                                     ChainScalars *cs, double thr, int nlanc)
+{
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) {
+        cs->sx = 1.0 / sqrt(s);
+        cs->svp = 0.0;
+        cs->b = 0.0;
+        cs->alpha = 0.0;
+        cs->thr = thr;
+        cs->step = 0;
+        cs->done = (s == 0.0) ? 1 : 0;
+        cs->nlanc = nlanc;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_chain_alpha(const double *__restrict__ p, int n,
+                                                      ChainScalars *cs, double *__restrict__ alpha)
+{
+    __shared__ double red[16];
+    if (cs->done) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) {
+        cs->alpha = s;
+        alpha[cs->step] = s;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kVecBS) k_chain_update(T *__restrict__ w, const T *__restrict__ v,
+                                                         int64_t n, const ChainScalars *cs,
+                                                         double *__restrict__ partials)
+{
+    __shared__ double red[16];
+    if (cs->done) return;
+    const double c = cs->alpha * cs->sx;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double wi = (double)w[i] - c * (double)v[i];
+        w[i] = (T)wi;
+        s += wi * wi;
+    }
+    s = bsum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(1024) k_chain_beta(const double *__restrict__ p, int n,
+                                                     ChainScalars *cs, double *__restrict__ beta)
+{
+    __shared__ double red[16];
+    if (cs->done) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) {
+        const double b = sqrt(s);
+        const int k = cs->step;
+        if (k + 1 < cs->nlanc) beta[k + 1] = b;
+        cs->b = b;
+        cs->svp = cs->sx;
+        cs->sx = 1.0 / b;
+        cs->step = k + 1;
+        if (fabs(b) < cs->thr || k + 1 >= cs->nlanc) cs->done = 1;
+    }
+}
+
+// psi += z_k * sx * v  (second pass of the ground-state Lanczos)
+template <typename T>
+__global__ void __launch_bounds__(kVecBS) k_accum(double *__restrict__ psi, const T *__restrict__ v,
+                                                  int64_t n, const ChainScalars *cs,
+                                                  const double *__restrict__ z)
+{
+    const double c = z[cs->step] * cs->sx;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        psi[i] += c * (double)v[i];
+}
+
+template <typename TD, typename TS>
+__global__ void __launch_bounds__(kVecBS) k_scale_copy(TD *__restrict__ d, const TS *__restrict__ s,
+                                                       int64_t n, double a)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = (TD)(a * (double)s[i]);
+}
+
+template <typename TY, typename TX>
+__global__ void __launch_bounds__(kVecBS) k_axpy(TY *__restrict__ y, const TX *__restrict__ x,
+                                                 int64_t n, double a)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (TY)((double)y[i] + a * (double)x[i]);
+}
+
+// counter-based hash PRNG in [-1,1): SplitMix64 of (seed, index)
+__device__ __forceinline__ double hash_uniform(uint64_t seed, uint64_t i)
+{
+    uint64_t z = seed + 0x9e3779b97f4a7c15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kVecBS) k_fill_random(T *__restrict__ x, int64_t n, uint64_t seed)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (T)hash_uniform(seed, (uint64_t)i);
+}
+
+// ----------------------------------------- block-Krylov helpers (FP64)
+// c[j] partials over rows: partials[j*nb + blk] = sum_i V[j*n+i] * w[i], j < m
+__global__ void __launch_bounds__(kVecBS) k_multidot(const double *__restrict__ V, int64_t n, int m,
+                                                     const double *__restrict__ w,
+                                                     double *__restrict__ partials)
+{
+    __shared__ double red[16];
+    const int j = blockIdx.y;
+    const double *vj = V + (int64_t)j * n;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        s += vj[i] * w[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) partials[(int64_t)j * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(1024) k_reduce_rows(const double *__restrict__ p, int nb,
+                                                      double *__restrict__ out)
+{
+    __shared__ double red[16];
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[(int64_t)j * nb + i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) out[j] = s;
+}
+
+// w -= sum_j V_j c_j
+__global__ void __launch_bounds__(kVecBS) k_multiaxpy(const double *__restrict__ V, int64_t n, int m,
+                                                      const double *__restrict__ c,
+                                                      double *__restrict__ w)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < m; ++j) s += V[(int64_t)j * n + i] * c[j];
+        w[i] -= s;
+    }
+}
+
+// in place V[:, 0:k] = V[:, 0:m] S[0:m, 0:k]  (S column-major m x k)
+__global__ void __launch_bounds__(128) k_rotate_basis(double *__restrict__ V, int64_t n, int m,
+                                                      int k, const double *__restrict__ S)
+{
+    extern __shared__ double row[];   // [m][128]
+    const int64_t base = (int64_t)blockIdx.x * 128;
+    const int t = threadIdx.x;
+    const int64_t i = base + t;
+    if (i < n)
+        for (int j = 0; j < m; ++j) row[j * 128 + t] = V[(int64_t)j * n + i];
+    __syncthreads();
+    if (i >= n) return;
+    for (int q = 0; q < k; ++q) {
+        double s = 0.0;
+        for (int j = 0; j < m; ++j) s += row[j * 128 + t] * S[q * m + j];
+        V[(int64_t)q * n + i] = s;
+    }
+}
+
+// ------------------------------------------------------------ launchers
+int launch_reduce_to(const double *partials, int n, double *out, int op, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partials, n, out, op);
+    DMED_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_dot(const void *x, const void *y, int64_t n, int prec, double *partials, int *nparts,
+               hipStream_t st)
+{
+    const unsigned g = vec_grid(n);
+    if (prec == DMED_FP32)
+        hipLaunchKernelGGL((k_dot<float, float>), dim3(g), dim3(kVecBS), 0, st,
+                           (const float *)x, (const float *)y, n, partials);
+    else
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(g), dim3(kVecBS), 0, st,
+                           (const double *)x, (const double *)y, n, partials);
+    DMED_HIP(hipGetLastError());
+    *nparts = (int)g;
+    return 0;
+}
+
+int launch_fill_random(void *x, int64_t n, int prec, uint64_t seed, hipStream_t st)
+{
+    const unsigned g = vec_grid(n);
+    if (prec == DMED_FP32)
+        hipLaunchKernelGGL(k_fill_random<float>, dim3(g), dim3(kVecBS), 0, st, (float *)x, n, seed);
+    else
+        hipLaunchKernelGGL(k_fill_random<double>, dim3(g), dim3(kVecBS), 0, st, (double *)x, n, seed);
+    DMED_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_scale_copy(void *dst, int dprec, const void *src, int sprec, int64_t n, double a,
+                      hipStream_t st)
+{
+    const unsigned g = vec_grid(n);
+    if (dprec == DMED_FP32 && sprec == DMED_FP32)
+        hipLaunchKernelGGL((k_scale_copy<float, float>), dim3(g), dim3(kVecBS), 0, st,
+                           (float *)dst, (const float *)src, n, a);
+    else if (dprec == DMED_FP32)
+        hipLaunchKernelGGL((k_scale_copy<float, double>), dim3(g), dim3(kVecBS), 0, st,
+                           (float *)dst, (const double *)src, n, a);
+    else if (sprec == DMED_FP32)
+        hipLaunchKernelGGL((k_scale_copy<double, float>), dim3(g), dim3(kVecBS), 0, st,
+                           (double *)dst, (const float *)src, n, a);
+    else
+        hipLaunchKernelGGL((k_scale_copy<double, double>), dim3(g), dim3(kVecBS), 0, st,
+                           (double *)dst, (const double *)src, n, a);
+    DMED_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_axpy(void *y, int yprec, const void *x, int xprec, int64_t n, double a, hipStream_t st)
+{
+    const unsigned g = vec_grid(n);
+    if (yprec == DMED_FP64 && xprec == DMED_FP64)
+        hipLaunchKernelGGL((k_axpy<double, double>), dim3(g), dim3(kVecBS), 0, st, (double *)y,
+                           (const double *)x, n, a);
+    else if (yprec == DMED_FP64)
+        hipLaunchKernelGGL((k_axpy<double, float>), dim3(g), dim3(kVecBS), 0, st, (double *)y,
+                           (const float *)x, n, a);
+    else
+        return fail(DMED_EINVAL, "axpy into FP32 not provided");
+    DMED_HIP(hipGetLastError());
+    return 0;
+}
+
+// ----------------------------------------------------------------- chain
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+};
+
+ChainScalars *chain_scalars() { return reinterpret_cast<ChainScalars *>(ctx().dscal); }
+double *chain_alpha() { return ctx().dscal + 16; }
+double *chain_beta() { return ctx().dscal + 16 + (kScalCap - 16) / 3; }
+double *chain_aux() { return ctx().dscal + 16 + 2 * ((kScalCap - 16) / 3); }
+constexpr int kChainMax = (kScalCap - 16) / 3;
+
+// Chain on three rotated work vectors bufs = {v_prev, v, w}; bufs[1] holds
+// the (unnormalised) start vector.  chain_init normalises it through the
+// device scale and zeroes v_prev; chain_steps launches `count` steps.  If
+// psi != nullptr, psi += z[k] * v_k is accumulated before each step (second
+// pass of the ground-state Lanczos), z read from chain_aux().
+int chain_init(const dmed_sector *s, void *bufs[3], int prec, int nlanc_total, double thr)
+{
+    Context &c = ctx();
+    hipStream_t st = c.stream;
+    const int64_t n = s->dim;
+    const size_t esz = prec == DMED_FP32 ? 4 : 8;
+    int np = 0;
+    DMED_HIP(hipMemsetAsync(bufs[0], 0, n * esz, st));
+    if (int rc = launch_dot(bufs[1], bufs[1], n, prec, c.partials, &np, st)) return rc;
+    hipLaunchKernelGGL(k_chain_init, dim3(1), dim3(1024), 0, st, c.partials, np, chain_scalars(),
+                       thr, nlanc_total);
+    DMED_HIP(hipGetLastError());
+    return 0;
+}
+
+int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double *psi)
+{
+    Context &c = ctx();
+    hipStream_t st = c.stream;
+    const int64_t n = s->dim;
+    ChainScalars *cs = chain_scalars();
+    const int nph = hxv_nparts(s, prec);
+    const unsigned g = vec_grid(n);
+    void *vp = bufs[0], *v = bufs[1], *w = bufs[2];
+    for (int k = 0; k < count; ++k) {
+        if (psi) {
+            if (prec == DMED_FP32)
+                hipLaunchKernelGGL(k_accum<float>, dim3(g), dim3(kVecBS), 0, st, psi,
+                                   (const float *)v, n, cs, chain_aux());
+            else
+                hipLaunchKernelGGL(k_accum<double>, dim3(g), dim3(kVecBS), 0, st, psi,
+                                   (const double *)v, n, cs, chain_aux());
+            DMED_HIP(hipGetLastError());
+        }
+        if (int rc = launch_hxv(s, v, w, prec, kLanczos, vp, cs, c.partials, nullptr, st))
+            return rc;
+        hipLaunchKernelGGL(k_chain_alpha, dim3(1), dim3(1024), 0, st, c.partials, nph, cs,
+                           chain_alpha());
+        DMED_HIP(hipGetLastError());
+        if (prec == DMED_FP32)
+            hipLaunchKernelGGL(k_chain_update<float>, dim3(g), dim3(kVecBS), 0, st, (float *)w,
+                               (const float *)v, n, cs, c.partials);
+        else
+            hipLaunchKernelGGL(k_chain_update<double>, dim3(g), dim3(kVecBS), 0, st, (double *)w,
+                               (const double *)v, n, cs, c.partials);
+        DMED_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_chain_beta, dim3(1), dim3(1024), 0, st, c.partials, (int)g, cs,
+                           chain_beta());
+        DMED_HIP(hipGetLastError());
+        void *t = vp;
+        vp = v;
+        v = w;
+        w = t;
+    }
+    bufs[0] = vp;
+    bufs[1] = v;
+    bufs[2] = w;
+    return 0;
+}
+
+}  // namespace
+
+int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
+              double threshold, double *alpha, double *beta, int *nsteps)
+{
+    Context &c = ctx();
+    hipStream_t st = c.stream;
+    if (nlanc <= 0) return fail(DMED_EINVAL, "nlanc must be positive");
+    if (nlanc > kChainMax) return fail(DMED_EINVAL, "nlanc exceeds the chain buffer");
+    const int64_t n = s->dim;
+    const size_t esz = prec == DMED_FP32 ? 4 : 8;
+    if (int rc = ensure_partials(std::max(hxv_nparts(s, prec), kVecGrid) + 16)) return rc;
+    DevBuf b0, b1, b2;
+    DMED_HIP(hipMalloc(&b0.p, std::max<int64_t>(n, 1) * esz));
+    DMED_HIP(hipMalloc(&b1.p, std::max<int64_t>(n, 1) * esz));
+    DMED_HIP(hipMalloc(&b2.p, std::max<int64_t>(n, 1) * esz));
+    if (int rc = launch_scale_copy(b1.p, prec, v0, v0prec, n, 1.0, st)) return rc;
+    DMED_HIP(hipMemsetAsync(chain_alpha(), 0, kChainMax * sizeof(double), st));
+    DMED_HIP(hipMemsetAsync(chain_beta(), 0, kChainMax * sizeof(double), st));
+    void *bufs[3] = {b0.p, b1.p, b2.p};
+    if (int rc = chain_init(s, bufs, prec, nlanc, threshold)) return rc;
+    if (int rc = chain_steps(s, bufs, prec, nlanc, nullptr)) return rc;
+    ChainScalars h;
+    DMED_HIP(hipMemcpyAsync(alpha, chain_alpha(), nlanc * sizeof(double), hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipMemcpyAsync(beta, chain_beta(), nlanc * sizeof(double), hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipMemcpyAsync(&h, chain_scalars(), sizeof h, hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipStreamSynchronize(st));
+    if (nsteps) *nsteps = h.step;
+    return 0;
+}
+
+// ------------------------------------------------------ ground state
+int eigh_sector_ex(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
+                   int dense_threshold, double degen, bool lowest_multiplet, int *nout,
+                   double *evals, dmed_vec **evecs, double *resid);
+constexpr int64_t kTrlMaxDim = 25000000;   // above: plain Lanczos ground state
+
+namespace {
+
+// lowest eigenpair of the symmetric tridiagonal (a[0..n), b[1..n)) via the
+// dense host solver (n is at most a few hundred)
+int tridiag_lowest(int n, const double *a, const double *b, double *theta, double *z)
+{
+    std::vector<double> T((size_t)n * n, 0.0), w(n);
+    for (int i = 0; i < n; ++i) T[(size_t)i * n + i] = a[i];
+    for (int i = 1; i < n; ++i) T[(size_t)i * n + i - 1] = T[(size_t)(i - 1) * n + i] = b[i];
+    if (int rc = host_syev(n, T.data(), w.data())) return rc;
+    *theta = w[0];
+    for (int i = 0; i < n; ++i) z[i] = T[i];
+    return 0;
+}
+
+// dense path: assemble H of a small sector on the host from the sector
+// tables (ED_HAMILTONIAN_MATVEC.f90:185-196 dumps the same matrix) and
+// diagonalise it
+int dense_eigh(const dmed_sector *s, int neig, double *evals, dmed_vec **evecs, double *resid)
+{
+    const int64_t dup = s->up.dim, ddw = s->dw.dim, n = s->dim;
+    std::vector<double> H((size_t)n * n, 0.0), w(n);
+    for (int64_t id = 0; id < ddw; ++id)
+        for (int64_t iu = 0; iu < dup; ++iu) {
+            const int64_t i = iu + id * dup;
+            H[(size_t)i * n + i] += s->up.h_diag[iu] + s->dw.h_diag[id] +
+                                    s->h_W[s->up.h_imp[iu] * s->nimp + s->dw.h_imp[id]];
+            for (int h = 0; h < s->up.nhop; ++h) {
+                const int64_t j = s->up.h_hidx[(size_t)h * dup + iu] + id * dup;
+                H[(size_t)j * n + i] += s->up.h_hval[(size_t)h * dup + iu];
+            }
+            for (int h = 0; h < s->dw.nhop; ++h) {
+                const int64_t j = iu + (int64_t)s->dw.h_hidx[(size_t)h * ddw + id] * dup;
+                H[(size_t)j * n + i] += s->dw.h_hval[(size_t)h * ddw + id];
+            }
+        }
+    if (int rc = host_syev((int)n, H.data(), w.data())) return rc;
+    for (int q = 0; q < neig; ++q) {
+        evals[q] = w[q];
+        if (resid) resid[q] = 0.0;
+        if (evecs) {
+            dmed_vec *v = new dmed_vec;
+            v->sec = s;
+            v->prec = DMED_FP64;
+            v->dim = n;
+            DMED_HIP(hipMalloc(&v->data, std::max<int64_t>(n, 1) * 8));
+            DMED_HIP(hipMemcpy(v->data, H.data() + (size_t)q * n, n * 8, hipMemcpyHostToDevice));
+            evecs[q] = v;
+        }
+    }
+    return 0;
+}
+
+// explicit residual ||H y - theta y|| for a device FP64 vector y
+int residual(const dmed_sector *s, const double *y, double theta, double *work, double *out)
+{
+    Context &c = ctx();
+    hipStream_t st = c.stream;
+    const int64_t n = s->dim;
+    if (int rc = launch_hxv(s, y, work, DMED_FP64, kPlain, nullptr, nullptr, nullptr, nullptr, st))
+        return rc;
+    if (int rc = launch_axpy(work, DMED_FP64, y, DMED_FP64, n, -theta, st)) return rc;
+    int np = 0;
+    if (int rc = launch_dot(work, work, n, DMED_FP64, c.partials, &np, st)) return rc;
+    double *d = ctx().dscal + 8;
+    if (int rc = launch_reduce_to(c.partials, np, d, 1, st)) return rc;
+    DMED_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int dot_host(const double *x, const double *y, int64_t n, double *out)
+{
+    Context &c = ctx();
+    int np = 0;
+    if (int rc = launch_dot(x, y, n, DMED_FP64, c.partials, &np, c.stream)) return rc;
+    double *d = ctx().dscal + 9;
+    if (int rc = launch_reduce_to(c.partials, np, d, 0, c.stream)) return rc;
+    DMED_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    DMED_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+// Plain Lanczos ground state (one eigenpair), memory 4 vectors:
+//   pass 1: recurrence until the lowest Ritz value's residual estimate
+//           |beta_k z_k| <= tol*max(1,|theta|), checked every ncheck steps;
+//   pass 2: identical recurrence, psi = sum_k z_k v_k (lanczos_plain_c,
+//           .repo/PLAIN_LANCZOS.f90:367-384);
+//   refine: explicit residual; if above tol, restart from psi.
+int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec **evec,
+             double *resid)
+{
+    Context &c = ctx();
+    hipStream_t st = c.stream;
+    const int64_t n = s->dim;
+    DevBuf b0, b1, b2;
+    DMED_HIP(hipMalloc(&b0.p, n * 8));
+    DMED_HIP(hipMalloc(&b1.p, n * 8));
+    DMED_HIP(hipMalloc(&b2.p, n * 8));
+    double *psi = nullptr;
+    DMED_HIP(hipMalloc(&psi, n * 8));
+    dmed_vec *out = new dmed_vec{s, DMED_FP64, n, psi};
+    auto cleanup = [&](int rc) {
+        if (rc) { hipFree(out->data); delete out; }
+        return rc;
+    };
+    if (int rc = launch_fill_random(psi, n, DMED_FP64, 1234567ull, st)) return cleanup(rc);
+    const int ncheck = 10;
+    int used = 0;
+    double theta = 0.0, res = INFINITY;
+    std::vector<double> a(kChainMax), b(kChainMax), z(kChainMax);
+    for (int restart = 0; restart < 8 && used < maxit; ++restart) {
+        // pass 1, in chunks of ncheck steps with a host convergence check
+        if (int rc = launch_scale_copy(b1.p, DMED_FP64, psi, DMED_FP64, n, 1.0, st))
+            return cleanup(rc);
+        DMED_HIP(hipMemsetAsync(chain_alpha(), 0, kChainMax * sizeof(double), st));
+        DMED_HIP(hipMemsetAsync(chain_beta(), 0, kChainMax * sizeof(double), st));
+        const int budget = std::min(kChainMax - 1, maxit - used);
+        void *bufs[3] = {b0.p, b1.p, b2.p};
+        if (int rc = chain_init(s, bufs, DMED_FP64, budget, 0.0)) return cleanup(rc);
+        int steps = 0;
+        ChainScalars cs{};
+        for (;;) {
+            const int chunk = std::min(ncheck, budget - steps);
+            if (int rc = chain_steps(s, bufs, DMED_FP64, chunk, nullptr)) return cleanup(rc);
+            steps += chunk;
+            DMED_HIP(hipMemcpyAsync(a.data(), chain_alpha(), steps * 8, hipMemcpyDeviceToHost, st));
+            DMED_HIP(hipMemcpyAsync(b.data(), chain_beta(), (steps + 1) * 8, hipMemcpyDeviceToHost, st));
+            DMED_HIP(hipMemcpyAsync(&cs, chain_scalars(), sizeof cs, hipMemcpyDeviceToHost, st));
+            DMED_HIP(hipStreamSynchronize(st));
+            const int m = std::min(steps, cs.step);
+            if (m < 1) return cleanup(fail(DMED_ENORM, "zero start vector in the ground-state Lanczos"));
+            if (int rc = tridiag_lowest(m, a.data(), b.data(), &theta, z.data())) return cleanup(rc);
+            const double est = fabs(cs.b * z[m - 1]);
+            if (est <= 0.1 * tol * std::max(1.0, fabs(theta)) || cs.done || steps >= budget) {
+                steps = m;
+                break;
+            }
+        }
+        used += steps;
+        // pass 2: rerun the recurrence from the same start accumulating psi
+        DMED_HIP(hipMemcpyAsync(chain_aux(), z.data(), steps * 8, hipMemcpyHostToDevice, st));
+        if (int rc = launch_scale_copy(b1.p, DMED_FP64, psi, DMED_FP64, n, 1.0, st))
+            return cleanup(rc);
+        DMED_HIP(hipMemsetAsync(psi, 0, n * 8, st));
+        void *bufs2[3] = {b0.p, b1.p, b2.p};
+        if (int rc = chain_init(s, bufs2, DMED_FP64, steps, 0.0)) return cleanup(rc);
+        if (int rc = chain_steps(s, bufs2, DMED_FP64, steps, psi)) return cleanup(rc);
+        used += steps;
+        // normalise, Rayleigh quotient and explicit residual
+        double nn = 0.0;
+        if (int rc = dot_host(psi, psi, n, &nn)) return cleanup(rc);
+        if (int rc = launch_scale_copy(psi, DMED_FP64, psi, DMED_FP64, n, 1.0 / sqrt(nn), st))
+            return cleanup(rc);
+        if (int rc = launch_hxv(s, psi, b2.p, DMED_FP64, kPlain, nullptr, nullptr, nullptr, nullptr, st))
+            return cleanup(rc);
+        double rq = 0.0;
+        if (int rc = dot_host(psi, (const double *)b2.p, n, &rq)) return cleanup(rc);
+        theta = rq;
+        if (int rc = residual(s, psi, theta, (double *)b2.p, &res)) return cleanup(rc);
+        used += 2;
+        if (res <= tol * std::max(1.0, fabs(theta))) break;
+    }
+    *eval = theta;
+    if (resid) *resid = res;
+    if (evec) *evec = out;
+    else cleanup(1);
+    return res <= tol * std::max(1.0, fabs(theta)) ? 0 : DMED_ENOCONV;
+}
+
+// Thick-restart Lanczos with full (twice classical Gram-Schmidt)
+// reorthogonalisation in a device basis, for the neig lowest eigenpairs of H
+// restricted to the orthogonal complement of nl locked orthonormal vectors Y
+// (Wu & Simon, SIAM J. Matrix Anal. Appl. 22 (2000) 602).  Ritz vectors are
+// written to X (neig contiguous device vectors).
+int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const double *Y,
+            int nl, uint64_t seed, double *theta_out, double *X, double *resid_out, int *used_out)
+{
+    Context &c = ctx();
+    hipStream_t st = c.stream;
+    const int64_t n = s->dim;
+    const int64_t avail = n - nl;                 // dimension of the complement
+    if (avail < neig) return fail(DMED_EINVAL, "complement smaller than neig");
+    m = std::max(m, neig + 2);
+    m = std::min(m, 64);                          // k_rotate_basis keeps m*128 doubles in LDS
+    m = (int)std::min<int64_t>(m, avail);
+    const bool full = (int64_t)m == avail;
+    double *V = nullptr, *w = nullptr, *dS = nullptr;
+    DMED_HIP(hipMalloc(&V, (size_t)(m + 1) * n * 8));
+    DevBuf gv;
+    gv.p = V;
+    DMED_HIP(hipMalloc(&w, n * 8));
+    DevBuf gw;
+    gw.p = w;
+    DMED_HIP(hipMalloc(&dS, (size_t)m * m * 8 + (size_t)(m + nl + 1) * 8));
+    DevBuf gs;
+    gs.p = dS;
+    double *dc = dS + (size_t)m * m;
+    const unsigned g = vec_grid(n);
+    if (int rc = ensure_partials((size_t)(std::max(m, nl) + 1) * g + 16)) return rc;
+    // project out the columns [0,jmax) of B from w
+    auto project = [&](const double *B, int jmax, double *cacc) -> int {
+        if (jmax <= 0) return 0;
+        hipLaunchKernelGGL(k_multidot, dim3(g, jmax), dim3(kVecBS), 0, st, B, n, jmax, w, c.partials);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(jmax), dim3(1024), 0, st, c.partials, (int)g, dc);
+        DMED_HIP(hipGetLastError());
+        if (cacc) {
+            std::vector<double> cc(jmax);
+            DMED_HIP(hipMemcpyAsync(cc.data(), dc, jmax * 8, hipMemcpyDeviceToHost, st));
+            DMED_HIP(hipStreamSynchronize(st));
+            for (int q = 0; q < jmax; ++q) cacc[q] += cc[q];
+        }
+        hipLaunchKernelGGL(k_multiaxpy, dim3(g), dim3(kVecBS), 0, st, B, n, jmax, dc, w);
+        DMED_HIP(hipGetLastError());
+        return 0;
+    };
+    auto orth = [&](int jmax, double *cacc) -> int {   // CGS2 against Y and V[0..jmax)
+        for (int q = 0; q < jmax; ++q) cacc[q] = 0.0;
+        for (int pass = 0; pass < 2; ++pass) {
+            if (int rc = project(Y, nl, nullptr)) return rc;
+            if (int rc = project(V, jmax, cacc)) return rc;
+        }
+        return 0;
+    };
+    auto normalize_into = [&](double *dst, double *nrm) -> int {
+        double nn = 0.0;
+        if (int rc = dot_host(w, w, n, &nn)) return rc;
+        *nrm = sqrt(nn);
+        return launch_scale_copy(dst, DMED_FP64, w, DMED_FP64, n, nn > 0 ? 1.0 / *nrm : 0.0, st);
+    };
+    std::vector<double> T((size_t)m * m, 0.0), S, theta(m), c1(m + 1);
+    // start vector in the complement of Y
+    if (int rc = launch_fill_random(w, n, DMED_FP64, seed, st)) return rc;
+    if (int rc = orth(0, c1.data())) return rc;
+    double bn = 0.0;
+    if (int rc = normalize_into(V, &bn)) return rc;
+    int k = 0, used = 0;
+    double beta_last = 0.0;
+    bool converged = false;
+    for (;;) {
+        for (int j = k; j < m; ++j) {
+            double *vj = V + (size_t)j * n;
+            if (int rc = launch_hxv(s, vj, w, DMED_FP64, kPlain, nullptr, nullptr, nullptr, nullptr, st))
+                return rc;
+            ++used;
+            if (int rc = orth(j + 1, c1.data())) return rc;
+            for (int q = 0; q <= j; ++q) {
+                const bool arrow = (j == k) && (q < k);
+                if (arrow || q >= j - 1) T[(size_t)q * m + j] = T[(size_t)j * m + q] = c1[q];
+            }
+            if (int rc = normalize_into(V + (size_t)(j + 1) * n, &bn)) return rc;
+            if (j + 1 < m) T[(size_t)(j + 1) * m + j] = T[(size_t)j * m + j + 1] = bn;
+            beta_last = bn;
+        }
+        S = T;
+        if (int rc = host_syev(m, S.data(), theta.data())) return rc;
+        bool ok = true;
+        for (int q = 0; q < neig; ++q) {
+            const double rn = full ? 0.0 : fabs(beta_last * S[(size_t)q * m + (m - 1)]);
+            if (rn > 0.1 * tol * std::max(1.0, fabs(theta[q]))) ok = false;
+        }
+        if (ok || used >= maxit || full) {
+            converged = ok || full;
+            DMED_HIP(hipMemcpyAsync(dS, S.data(), (size_t)m * neig * 8, hipMemcpyHostToDevice, st));
+            const unsigned gb = (unsigned)((n + 127) / 128);
+            hipLaunchKernelGGL(k_rotate_basis, dim3(gb), dim3(128), (size_t)m * 128 * 8, st, V, n, m,
+                               neig, dS);
+            DMED_HIP(hipGetLastError());
+            break;
+        }
+        // thick restart: keep kk Ritz vectors and the residual direction
+        const int kk = std::min(m - 2, neig + std::max(1, (m - neig) / 2));
+        DMED_HIP(hipMemcpyAsync(dS, S.data(), (size_t)m * kk * 8, hipMemcpyHostToDevice, st));
+        const unsigned gb = (unsigned)((n + 127) / 128);
+        hipLaunchKernelGGL(k_rotate_basis, dim3(gb), dim3(128), (size_t)m * 128 * 8, st, V, n, m, kk,
+                           dS);
+        DMED_HIP(hipGetLastError());
+        DMED_HIP(hipMemcpyAsync(V + (size_t)kk * n, V + (size_t)m * n, n * 8,
+                                hipMemcpyDeviceToDevice, st));
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int q = 0; q < kk; ++q) {
+            T[(size_t)q * m + q] = theta[q];
+            const double arrow = beta_last * S[(size_t)q * m + (m - 1)];
+            T[(size_t)q * m + kk] = T[(size_t)kk * m + q] = arrow;
+        }
+        k = kk;
+    }
+    for (int q = 0; q < neig; ++q) {
+        theta_out[q] = theta[q];
+        double r = 0.0;
+        if (int rc = residual(s, V + (size_t)q * n, theta[q], w, &r)) return rc;
+        resid_out[q] = r;
+        if (r > tol * std::max(1.0, fabs(theta[q]))) converged = false;
+        DMED_HIP(hipMemcpyAsync(X + (size_t)q * n, V + (size_t)q * n, n * 8, hipMemcpyDeviceToDevice, st));
+    }
+    DMED_HIP(hipStreamSynchronize(st));
+    if (used_out) *used_out += used;
+    return converged ? 0 : DMED_ENOCONV;
+}
+
+// Lowest neig eigenpairs with complete degenerate multiplets.  A
+// single-vector Krylov space holds one direction per eigenspace, so after a
+// converged run the search is repeated on the orthogonal complement of the
+// kept vectors; a round that brings no new level below the current top ends
+// the search.  lowest_multiplet: return only the lowest level and its copies
+// within degen (absolute), at most neig of them.
+int trl_deflated(const dmed_sector *s, int neig, int m, int maxit, double tol, double degen,
+                 bool lowest_multiplet, int *nout, double *evals, dmed_vec **evecs, double *resid)
+{
+    Context &c = ctx();
+    hipStream_t st = c.stream;
+    const int64_t n = s->dim;
+    double *Y = nullptr, *X = nullptr;
+    DMED_HIP(hipMalloc(&Y, (size_t)neig * n * 8));
+    DevBuf gy;
+    gy.p = Y;
+    const int want = lowest_multiplet ? 1 : neig;
+    DMED_HIP(hipMalloc(&X, (size_t)want * n * 8));
+    DevBuf gx;
+    gx.p = X;
+    struct Kept { double e, r; int slot; };
+    std::vector<Kept> kept;
+    std::vector<double> th(want), rr(want);
+    int used = 0, rcfinal = 0;
+    for (int round = 0; round <= neig; ++round) {
+        const int nl = (int)kept.size();
+        if (n - nl < want) break;
+        int rc = trl_run(s, want, m, std::max(maxit - used, 4 * m), tol, Y, nl,
+                         1234567ull + 7919ull * round, th.data(), X, rr.data(), &used);
+        if (rc == DMED_ENOCONV) rcfinal = rc;
+        else if (rc) return rc;
+        bool added = false;
+        for (int q = 0; q < want; ++q) {
+            if (lowest_multiplet) {
+                if (!kept.empty() && !(th[q] <= kept[0].e + degen)) continue;
+                if ((int)kept.size() >= neig) continue;
+            } else if ((int)kept.size() >= neig) {
+                if (!(th[q] < kept.back().e)) continue;   // nothing new below the top
+                // evict the highest kept level; its slot is reused
+                const int slot = kept.back().slot;
+                kept.pop_back();
+                kept.push_back({th[q], rr[q], slot});
+                DMED_HIP(hipMemcpyAsync(Y + (size_t)slot * n, X + (size_t)q * n, n * 8,
+                                        hipMemcpyDeviceToDevice, st));
+                std::sort(kept.begin(), kept.end(), [](const Kept &x, const Kept &y) { return x.e < y.e; });
+                added = true;
+                continue;
+            }
+            // free slot
+            std::vector<bool> busy(neig, false);
+            for (auto &k : kept) busy[k.slot] = true;
+            int slot = 0;
+            while (busy[slot]) ++slot;
+            DMED_HIP(hipMemcpyAsync(Y + (size_t)slot * n, X + (size_t)q * n, n * 8,
+                                    hipMemcpyDeviceToDevice, st));
+            kept.push_back({th[q], rr[q], slot});
+            std::sort(kept.begin(), kept.end(), [](const Kept &x, const Kept &y) { return x.e < y.e; });
+            added = true;
+        }
+        if (!added || used >= maxit) break;
+        if (lowest_multiplet && (int)kept.size() >= neig) break;
+    }
+    // evicted vectors may have left Y non-compact: the kept slots are what
+    // is returned, in ascending energy
+    *nout = (int)kept.size();
+    for (size_t q = 0; q < kept.size(); ++q) {
+        evals[q] = kept[q].e;
+        if (resid) resid[q] = kept[q].r;
+        if (evecs) {
+            dmed_vec *v = new dmed_vec{s, DMED_FP64, n, nullptr};
+            DMED_HIP(hipMalloc(&v->data, n * 8));
+            DMED_HIP(hipMemcpyAsync(v->data, Y + (size_t)kept[q].slot * n, n * 8,
+                                    hipMemcpyDeviceToDevice, st));
+            evecs[q] = v;
+        }
+    }
+    DMED_HIP(hipStreamSynchronize(st));
+    return rcfinal;
+}
+
+}  // namespace
+
+int eigh_sector(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
+                int dense_threshold, double *evals, dmed_vec **evecs, double *resid)
+{
+    int nout = 0;
+    const int rc = eigh_sector_ex(s, neig, ncv, maxit, tol, dense_threshold, 0.0, false, &nout,
+                                  evals, evecs, resid);
+    return rc;
+}
+
+int eigh_sector_ex(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
+                   int dense_threshold, double degen, bool lowest_multiplet, int *nout,
+                   double *evals, dmed_vec **evecs, double *resid)
+{
+    const int64_t n = s->dim;
+    if (neig < 1 || neig > n) return fail(DMED_EINVAL, "neig out of range");
+    if (n <= dense_threshold || neig == n) {
+        if (!lowest_multiplet) {
+            *nout = neig;
+            return dense_eigh(s, neig, evals, evecs, resid);
+        }
+        // lowest level and its copies among the neig lowest
+        std::vector<double> ev(neig), rs(neig);
+        std::vector<dmed_vec *> vv(neig, nullptr);
+        if (int rc = dense_eigh(s, neig, ev.data(), vv.data(), rs.data())) return rc;
+        int cnt = 0;
+        for (int q = 0; q < neig; ++q) {
+            if (ev[q] <= ev[0] + degen) {
+                evals[cnt] = ev[q];
+                if (resid) resid[cnt] = rs[q];
+                if (evecs) evecs[cnt] = vv[q];
+                else dmed_vec_free(vv[q]);
+                ++cnt;
+            } else {
+                dmed_vec_free(vv[q]);
+            }
+        }
+        *nout = cnt;
+        return 0;
+    }
+    if (int rc = ensure_partials(std::max(hxv_nparts(s, DMED_FP64), kVecGrid) + 16)) return rc;
+    // stored-basis solver with multiplet completion while it fits comfortably;
+    // past that the 4-vector plain Lanczos (ground state only, no in-sector
+    // degeneracy search)
+    int m = ncv > 0 ? ncv : std::max(2 * neig + 10, 20);
+    size_t freeb = 0, totalb = 0;
+    hipMemGetInfo(&freeb, &totalb);
+    const double need = (double)(m + 2 + 2 * neig) * n * 8.0;
+    if ((neig == 1 || lowest_multiplet) && (need > 0.5 * (double)freeb || n > kTrlMaxDim)) {
+        *nout = 1;
+        return plain_gs(s, maxit, tol, evals, evecs ? &evecs[0] : nullptr, resid);
+    }
+    if (need > 0.8 * (double)freeb) return fail(DMED_ENOMEM, "Krylov basis does not fit");
+    return trl_deflated(s, neig, m, maxit, tol, degen, lowest_multiplet, nout, evals, evecs, resid);
+}
+
+}  // namespace dmed

@@ -1,0 +1,316 @@
+"""ctypes binding of libdmed.so (the C ABI declared in include/dmed.h).
+
+Used by the tests and bench.py.  The drop-in interface for DMFT codes is the
+Fortran module DMFT_ED in dmft-ed_amd/fortran/; this module mirrors the same
+C entry points for Python callers.  There is no CPU fallback: if the library
+is missing or no GPU is present, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libdmed.so")
+
+FP64, FP32 = 0, 1
+ERRORS = {-1: "EINVAL", -2: "ESTATE", -3: "EHIP", -4: "ENOMEM", -5: "EUNSUPP",
+          -6: "ENOCONV", -7: "ENORM"}
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+
+
+class DmedError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Model(C.Structure):
+    _fields_ = [("norb", C.c_int32), ("nbath", C.c_int32), ("nspin", C.c_int32),
+                ("bath", _dp), ("hloc", _dp), ("uloc", C.c_double * 3),
+                ("ust", C.c_double), ("jh", C.c_double), ("jx", C.c_double),
+                ("jp", C.c_double), ("xmu", C.c_double), ("hfmode", C.c_int32),
+                ("jhflag", C.c_int32)]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("lanc_nstates_sector", C.c_int32), ("lanc_nstates_total", C.c_int32),
+                ("lanc_nstates_step", C.c_int32), ("lanc_niter", C.c_int32),
+                ("lanc_ngfiter", C.c_int32), ("lanc_ncv_factor", C.c_int32),
+                ("lanc_ncv_add", C.c_int32), ("lanc_dim_threshold", C.c_int32),
+                ("lanc_tolerance", C.c_double), ("gs_threshold", C.c_double),
+                ("cutoff", C.c_double), ("beta", C.c_double), ("eps", C.c_double),
+                ("wini", C.c_double), ("wfin", C.c_double), ("lmats", C.c_int32),
+                ("lreal", C.c_int32), ("prec_gf", C.c_int32), ("batched", C.c_int32),
+                ("verbose", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SolveOut(C.Structure):
+    _fields_ = [("egs", C.c_double), ("zeta", C.c_double), ("nstates", C.c_int32),
+                ("numgs", C.c_int32), ("dens", _dp), ("docc", _dp), ("gmats", _dp),
+                ("greal", _dp), ("smats", _dp), ("sreal", _dp),
+                ("neigen_sector", _i32p), ("lanc_nstates_total", _i32p),
+                ("t_diag", C.c_double), ("t_gf", C.c_double)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every entry point of include/dmed.h
+SIGNATURES = [
+    ("dmed_init", C.c_int, [C.c_int]),
+    ("dmed_finalize", C.c_int, []),
+    ("dmed_last_error", C.c_char_p, []),
+    ("dmed_set_model", C.c_int, [C.POINTER(Model)]),
+    ("dmed_get_ns", C.c_int, [_i32p]),
+    ("dmed_binomial", C.c_int64, [C.c_int32, C.c_int32]),
+    ("dmed_sector_create", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("dmed_sector_free", C.c_int, [C.c_void_p]),
+    ("dmed_sector_dims", C.c_int, [C.c_void_p, _i64p, _i64p, _i64p]),
+    ("dmed_sector_map", C.c_int, [C.c_void_p, _i64p]),
+    ("dmed_vec_create", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("dmed_vec_free", C.c_int, [C.c_void_p]),
+    ("dmed_vec_upload", C.c_int, [C.c_void_p, _dp]),
+    ("dmed_vec_download", C.c_int, [C.c_void_p, _dp]),
+    ("dmed_vec_data", C.c_void_p, [C.c_void_p]),
+    ("dmed_hxv", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+    ("dmed_hxv_host", C.c_int, [C.c_void_p, _dp, _dp, C.c_int32]),
+    ("dmed_hxv_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+    ("dmed_sync", C.c_int, []),
+    ("dmed_hxv_timed", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, _dp]),
+    ("dmed_eigh", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                            C.c_int32, _dp, C.POINTER(C.c_void_p), _dp]),
+    ("dmed_apply_op", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, _dp]),
+    ("dmed_chain", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, _dp,
+                             _dp, _i32p]),
+    ("dmed_gf_add_chain", C.c_int, [C.c_double, C.c_double, _dp, _dp, C.c_int32, C.c_int32,
+                                    _dp, C.c_int32, _dp, C.c_int32, C.c_double, _dp, _dp]),
+    ("dmed_observables", C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
+    ("dmed_solve_params_default", None, [C.POINTER(SolveParams)]),
+    ("dmed_solve", C.c_int, [C.POINTER(SolveParams), C.POINTER(SolveOut)]),
+    ("dmed_sigma", C.c_int, [C.c_int32, C.c_int32, _dp, _dp, C.c_int32, _dp]),
+    ("dmed_delta", C.c_int, [C.c_int32, C.c_int32, _dp, C.c_int32, _dp]),
+]
+
+
+def lib():
+    """Load libdmed.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} not built: run python dmft-ed_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise DmedError(rc, lib().dmed_last_error().decode())
+    return rc
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def init(device=0):
+    _check(lib().dmed_init(device))
+
+
+def finalize():
+    _check(lib().dmed_finalize())
+
+
+def set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, jp=0.0,
+              xmu=0.0, hfmode=True, jhflag=False):
+    """bath: set_dmft_bath layout; hloc_f: real Hloc(Nspin,Nspin,Norb,Norb) in
+    Fortran order (flattened) or a 4-d array."""
+    global _keep
+    bath = np.ascontiguousarray(bath, dtype=np.float64)
+    h = np.asarray(hloc_f, dtype=np.float64)
+    if h.ndim == 4:
+        h = h.flatten(order="F")
+    h = np.ascontiguousarray(h)
+    m = Model()
+    m.norb, m.nbath, m.nspin = norb, nbath, nspin
+    m.bath, m.hloc = _d(bath), _d(h)
+    u = list(uloc) + [0.0] * (3 - len(uloc))
+    for i in range(3):
+        m.uloc[i] = u[i]
+    m.ust, m.jh, m.jx, m.jp, m.xmu = ust, jh, jx, jp, xmu
+    m.hfmode, m.jhflag = int(bool(hfmode)), int(bool(jhflag))
+    _keep = (bath, h)
+    _check(lib().dmed_set_model(C.byref(m)))
+    return (nbath + 1) * norb
+
+
+_keep = None
+
+
+class Sector:
+    def __init__(self, nup, ndw):
+        self.h = C.c_void_p()
+        _check(lib().dmed_sector_create(nup, ndw, C.byref(self.h)))
+        d, du, dd = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().dmed_sector_dims(self.h, C.byref(d), C.byref(du), C.byref(dd)))
+        self.nup, self.ndw = nup, ndw
+        self.dim, self.dim_up, self.dim_dw = d.value, du.value, dd.value
+
+    def free(self):
+        if self.h:
+            lib().dmed_sector_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def map(self):
+        out = np.empty(self.dim, dtype=np.int64)
+        _check(lib().dmed_sector_map(self.h, out.ctypes.data_as(_i64p)))
+        return out
+
+    def hxv(self, x, prec=FP64):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(lib().dmed_hxv_host(self.h, _d(x), _d(y), prec))
+        return y
+
+    def eigh(self, neig=1, ncv=0, maxit=0, tol=1e-12, dense_threshold=256, vectors=True):
+        ev = np.zeros(neig)
+        res = np.zeros(neig)
+        vecs = (C.c_void_p * neig)() if vectors else None
+        rc = lib().dmed_eigh(self.h, neig, ncv, maxit, tol, dense_threshold, _d(ev),
+                             vecs, _d(res))
+        if rc not in (0, -6):
+            _check(rc)
+        out = [Vec.wrap(self, C.c_void_p(v), FP64) for v in vecs] if vectors else None
+        return ev, out, res, rc
+
+    def vec(self, data=None, prec=FP64):
+        v = Vec(self, prec)
+        if data is not None:
+            v.upload(data)
+        return v
+
+
+class Vec:
+    def __init__(self, sector, prec=FP64, handle=None):
+        self.sector = sector
+        self.prec = prec
+        if handle is None:
+            self.h = C.c_void_p()
+            _check(lib().dmed_vec_create(sector.h, prec, C.byref(self.h)))
+        else:
+            self.h = handle
+
+    @classmethod
+    def wrap(cls, sector, handle, prec):
+        return cls(sector, prec, handle)
+
+    def upload(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.shape == (self.sector.dim,)
+        _check(lib().dmed_vec_upload(self.h, _d(x)))
+
+    def download(self):
+        out = np.empty(self.sector.dim)
+        _check(lib().dmed_vec_download(self.h, _d(out)))
+        return out
+
+    def ptr(self):
+        return lib().dmed_vec_data(self.h)
+
+    def free(self):
+        if self.h:
+            lib().dmed_vec_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def apply_op(psi, isite, dagger, dst):
+    out = Vec(dst, psi.prec)
+    n2 = C.c_double()
+    _check(lib().dmed_apply_op(psi.h, isite, int(bool(dagger)), dst.h, out.h, C.byref(n2)))
+    return out, n2.value
+
+
+def chain(sector, v0, nlanc, prec=FP64, threshold=1e-13):
+    a = np.zeros(nlanc)
+    b = np.zeros(nlanc)
+    ns = C.c_int32()
+    _check(lib().dmed_chain(sector.h, v0.h, nlanc, prec, threshold, _d(a), _d(b), C.byref(ns)))
+    return a, b, ns.value
+
+
+def gf_add_chain(peso, ei, alpha, beta, isign, wm, wr, eps, gm, gr):
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    wm = np.ascontiguousarray(wm, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    _check(lib().dmed_gf_add_chain(peso, ei, _d(alpha), _d(beta), len(alpha), isign, _d(wm),
+                                   len(wm), _d(wr), len(wr), eps, _d(gm.view(np.float64)),
+                                   _d(gr.view(np.float64))))
+
+
+def observables(psi, peso, norb):
+    dens = np.zeros(norb)
+    docc = np.zeros(norb)
+    _check(lib().dmed_observables(psi.h, peso, _d(dens), _d(docc)))
+    return dens, docc
+
+
+def sigma(ispin, iorb, z, g):
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    g = np.ascontiguousarray(g, dtype=np.complex128)
+    out = np.empty_like(z)
+    _check(lib().dmed_sigma(ispin, iorb, _d(z.view(np.float64)), _d(g.view(np.float64)),
+                            len(z), _d(out.view(np.float64))))
+    return out
+
+
+def default_params(**kw):
+    p = SolveParams()
+    lib().dmed_solve_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def solve(norb, nspin, ns, params):
+    """ed_solve on the current model.  Returns a dict with arrays shaped like
+    the reference's (Nspin,Nspin,Norb,Norb,L) complex outputs."""
+    p = params
+    g = {}
+    for k, L in (("gmats", p.lmats), ("greal", p.lreal), ("smats", p.lmats), ("sreal", p.lreal)):
+        g[k] = np.zeros((nspin, nspin, norb, norb, L), dtype=np.complex128, order="F")
+    dens = np.zeros(norb)
+    docc = np.zeros(norb)
+    neig = np.zeros((ns + 1) ** 2, dtype=np.int32)
+    ntot = C.c_int32(p.lanc_nstates_total)
+    o = SolveOut()
+    o.dens, o.docc = _d(dens), _d(docc)
+    for k in g:
+        setattr(o, k, g[k].ctypes.data_as(_dp))
+    o.neigen_sector = neig.ctypes.data_as(_i32p)
+    o.lanc_nstates_total = C.pointer(ntot)
+    _check(lib().dmed_solve(C.byref(p), C.byref(o)))
+    return dict(egs=o.egs, zeta=o.zeta, nstates=o.nstates, numgs=o.numgs, dens=dens, docc=docc,
+                neigen_sector=neig, lanc_nstates_total=ntot.value, t_diag=o.t_diag,
+                t_gf=o.t_gf, **g)

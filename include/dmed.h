@@ -1,0 +1,229 @@
+/*
+ * dmed.h -- C ABI of the MI355X (gfx950) DMFT-ED impurity-solver hot path.
+ *
+ * Plain C: pointers, sizes, status codes; no C++ or torch types.  The Fortran
+ * module DMFT_ED (dmft-ed_amd/fortran/) binds these entry points with
+ * ISO_C_BINDING; tests and bench.py bind them with ctypes.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative DMED_E* code on
+ *     failure; dmed_last_error() then describes the failure.  The reference
+ *     aborts with `stop "..."` instead (e.g. ED_HAMILTONIAN_MATVEC.f90:313,317;
+ *     ED_SETUP.f90:1061,1075); the Fortran shim turns a nonzero code into stop.
+ *   - One process, one GPU, one HIP stream.  Calls are host-synchronous
+ *     unless named *_async.  Not thread-safe.
+ *   - State vectors are real (FP64 or FP32).  The reference's vectors are
+ *     complex(8); H is real for a normal bath with real Hloc, so the real
+ *     vector is the reference vector up to a global phase.
+ *   - Vector element order is the reference's sector order
+ *     (ED_SETUP.f90:872-889): i = r_up + r_dw*D_up (0-based), r = rank of the
+ *     spin string in numeric order; i.e. a column-major D_up x D_dw matrix.
+ *   - Arrays that mirror reference arrays use the reference's Fortran layout
+ *     (column-major, first index fastest), e.g. Hloc(Nspin,Nspin,Norb,Norb),
+ *     impGmats(Nspin,Nspin,Norb,Norb,Lmats) as interleaved (re,im) doubles.
+ */
+#ifndef DMED_H
+#define DMED_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DMED_OK 0
+#define DMED_EINVAL (-1)    /* bad argument */
+#define DMED_ESTATE (-2)    /* call out of order (no init / no model / ...) */
+#define DMED_EHIP (-3)      /* HIP runtime error */
+#define DMED_ENOMEM (-4)    /* device or host allocation failed */
+#define DMED_EUNSUPP (-5)   /* model feature not built for the GPU path */
+#define DMED_ENOCONV (-6)   /* eigensolver did not reach the tolerance */
+#define DMED_ENORM (-7)     /* a state that must be normalised is not */
+
+/* precision selector for H*v and the GF chains */
+#define DMED_FP64 0
+#define DMED_FP32 1         /* FP32 vectors, FP64-accumulated reductions */
+
+/* ------------------------------------------------------------ setup */
+/* Select the device and create the library stream.  Replaces the implicit
+ * module initialisation of the reference (init_ed_structure,
+ * ED_SETUP.f90:135-332). */
+int dmed_init(int device);
+int dmed_finalize(void);
+const char *dmed_last_error(void);
+
+/* Impurity model, normal bath, normal ed_mode.
+ *   bath : flat bath vector, set_dmft_bath layout (ED_BATH/dmft_aux.f90:489-512):
+ *          e(ispin,iorb,k) at k + iorb*Nbath + ispin*Nbath*Norb (0-based),
+ *          v(ispin,iorb,k) at the same index + Nspin*Norb*Nbath.
+ *          Length 2*Nspin*Norb*Nbath = get_bath_dimension() (user_aux.f90:17-30).
+ *   hloc : real part of impHloc(Nspin,Nspin,Norb,Norb), Fortran order.
+ * Globals of ED_INPUT_VARS.f90:13-24 (Uloc, Ust, Jh, Jx, Jp, xmu, HFmode,
+ * Jhflag).  Jhflag=1 with Norb>1 returns DMED_EUNSUPP on the GPU path.
+ * Replaces: set_dmft_bath + set_Hloc + the module globals read by
+ * build_hxv.f90 (ED_HAMILTONIAN_MATVEC.f90:289-351). */
+typedef struct dmed_model {
+    int32_t norb, nbath, nspin;
+    const double *bath;
+    const double *hloc;
+    double uloc[3];
+    double ust, jh, jx, jp, xmu;
+    int32_t hfmode, jhflag;
+} dmed_model;
+int dmed_set_model(const dmed_model *m);
+
+/* Ns = (Nbath+1)*Norb (setup_ed_dimensions, ED_SETUP.f90:81-128) */
+int dmed_get_ns(int32_t *ns);
+
+/* binomial(n1,n2) with the reference's rounding (ED_SETUP.f90:1258-1275) */
+int64_t dmed_binomial(int32_t n1, int32_t n2);
+
+/* --------------------------------------------------------- sectors */
+/* A sector (nup, ndw) with its device tables: up/down spin strings, hop
+ * tables (target rank + signed coefficient) per spin, separable diagonal.
+ * Replaces build_sector / setup_Hv_sector (ED_SETUP.f90:861-892,
+ * ED_HAMILTONIAN_MATVEC.f90:99-111); library-owned, freed by
+ * dmed_sector_free. */
+typedef struct dmed_sector dmed_sector;
+int dmed_sector_create(int32_t nup, int32_t ndw, dmed_sector **out);
+int dmed_sector_free(dmed_sector *s);
+int dmed_sector_dims(const dmed_sector *s, int64_t *dim, int64_t *dim_up,
+                     int64_t *dim_dw);
+/* Reference state codes map(i) = iup + idw*2^Ns (ED_SETUP.f90:889), as
+ * int64 (the reference's int32 code overflows at Ns=16).  codes: host array
+ * of length dim. */
+int dmed_sector_map(const dmed_sector *s, int64_t *codes);
+
+/* --------------------------------------------------------- vectors */
+/* Device vectors of a sector, library-owned. */
+typedef struct dmed_vec dmed_vec;
+int dmed_vec_create(const dmed_sector *s, int32_t prec, dmed_vec **out);
+int dmed_vec_free(dmed_vec *v);
+int dmed_vec_upload(dmed_vec *v, const double *host);     /* host FP64, length dim */
+int dmed_vec_download(const dmed_vec *v, double *host);   /* host FP64, length dim */
+void *dmed_vec_data(const dmed_vec *v);                   /* device pointer */
+
+/* ------------------------------------------------------------- H*v */
+/* y = H x on the sector, x and y device pointers of length dim in the given
+ * precision.  Replaces the operator behind the procedure pointer
+ * spHtimesV_cc(Nloc, v, Hv) (ED_VARS_GLOBAL.f90:43-49,100) -- directMatVec_cc
+ * (ED_HAMILTONIAN_MATVEC.f90:289-351) and spMatVec_cc (:221-236). */
+int dmed_hxv(const dmed_sector *s, const void *x_dev, void *y_dev, int32_t prec);
+/* Same product on host FP64 arrays (copies in and out; for tests). */
+int dmed_hxv_host(const dmed_sector *s, const double *x, double *y, int32_t prec);
+/* Same product, launched without a final synchronisation (for timing). */
+int dmed_hxv_async(const dmed_sector *s, const void *x_dev, void *y_dev, int32_t prec);
+int dmed_sync(void);
+/* Measurement entry point (no counterpart in the reference): nrep launches
+ * of the H*v kernel bracketed by HIP events recorded on the library stream;
+ * ms_avg receives the average duration of one launch in milliseconds. */
+int dmed_hxv_timed(const dmed_sector *s, const void *x_dev, void *y_dev, int32_t prec,
+                   int32_t nrep, double *ms_avg);
+
+/* ------------------------------------------------------ eigensolver */
+/* Lowest neig eigenpairs of the sector.  Replaces sp_eigh(MatVec, Dim,
+ * Neigen, Nblock, Nitermax, eval, evec, tol) (call site ED_DIAG.f90:179-181)
+ * and, for dim <= dense_threshold, ed_buildH_c + eigh (ED_DIAG.f90:185-193).
+ *   ncv      : Krylov basis size (Nblock of the reference); 0 = automatic
+ *   maxit    : maximum number of H*v products
+ *   tol      : residual tolerance ||H psi - E psi|| <= tol*max(1,|E|)
+ *   evals    : host array [neig], ascending
+ *   evecs    : array of neig handles created by the call (FP64), or NULL
+ *   resid    : host array [neig] of final residual norms, or NULL
+ * Returns DMED_ENOCONV (results still filled) if tol is not reached. */
+int dmed_eigh(const dmed_sector *s, int32_t neig, int32_t ncv, int32_t maxit,
+              double tol, int32_t dense_threshold, double *evals,
+              dmed_vec **evecs, double *resid);
+
+/* -------------------------------------------------- Green's function */
+/* out = c^+_{isite} psi (dagger=1) or c_{isite} psi (dagger=0), where
+ * isite = iorb + (ispin-1)*Ns is the reference's 1-based spin-orbital index
+ * (impIndex, ED_SETUP.f90:428-431) and out lives in the target sector
+ * (getCDGsector / getCsector, :464-492).  norm2 = <out|out>.  Replaces the
+ * start-vector loop of lanc_build_gf_normal_c (build_gf_normal.f90:105-113,
+ * 144-152). */
+int dmed_apply_op(const dmed_vec *psi, int32_t isite, int32_t dagger,
+                  const dmed_sector *dst, dmed_vec *out, double *norm2);
+
+/* Plain Lanczos tridiagonalisation from v0 (normalised internally, v0 is
+ * not modified): alpha[k], beta[k+1] for k < nlanc-1 with the semantics of
+ * sp_lanc_tridiag (build_gf_normal.f90:124-126; .repo/PLAIN_LANCZOS.f90:154-180),
+ * stopping early when |b| < threshold; unset entries are zero.
+ * prec = DMED_FP32 runs H*v and the vectors in FP32 with FP64 reductions.
+ * nsteps receives the number of alphas produced. */
+int dmed_chain(const dmed_sector *s, const dmed_vec *v0, int32_t nlanc,
+               int32_t prec, double threshold, double *alpha, double *beta,
+               int32_t *nsteps);
+
+/* add_to_lanczos_gf_normal (build_gf_normal.f90:442-488): accumulate
+ *   G(z) += peso_bz * sum_j Z(1,j)^2 / (z - isign*(lambda_j - ei))
+ * with (lambda, Z) the tql2 spectrum of the chain, on z = i*wm[n] and
+ * z = wr[n] + i*eps.  gmats/greal: interleaved complex, lmats / lreal. */
+int dmed_gf_add_chain(double peso_bz, double ei, const double *alpha,
+                      const double *beta, int32_t nlanc, int32_t isign,
+                      const double *wm, int32_t lmats, const double *wr,
+                      int32_t lreal, double eps, double *gmats, double *greal);
+
+/* dens/docc contribution of one state (ED_OBSERVABLES.f90:118-150):
+ * dens[a] += peso*sum|psi|^2 (n_a,up + n_a,dw), docc[a] += peso*sum|psi|^2 n_a,up n_a,dw */
+int dmed_observables(const dmed_vec *psi, double peso, double *dens, double *docc);
+
+/* ------------------------------------------------------ whole solve */
+/* Parameters of ed_solve (names and defaults of ED_INPUT_VARS.f90:111-176). */
+typedef struct dmed_solve_params {
+    int32_t lanc_nstates_sector;   /* 6 */
+    int32_t lanc_nstates_total;    /* 1  (1 => T=0) */
+    int32_t lanc_nstates_step;     /* 2 */
+    int32_t lanc_niter;            /* 512 */
+    int32_t lanc_ngfiter;          /* 200 */
+    int32_t lanc_ncv_factor;       /* 3 */
+    int32_t lanc_ncv_add;          /* 5 */
+    int32_t lanc_dim_threshold;    /* 256 */
+    double lanc_tolerance;         /* 1e-12 */
+    double gs_threshold;           /* 1e-9 */
+    double cutoff;                 /* 1e-9 */
+    double beta;                   /* 1000 */
+    double eps;                    /* 0.01 */
+    double wini, wfin;             /* -5, 5 */
+    int32_t lmats, lreal;          /* 5000, 5000 */
+    int32_t prec_gf;               /* DMED_FP64 / DMED_FP32 chains */
+    int32_t batched;               /* 1: batched sector/chain engine */
+    int32_t verbose;
+    int32_t reserved;
+} dmed_solve_params;
+void dmed_solve_params_default(dmed_solve_params *p);
+
+/* Results of one solve; arrays are caller-allocated in the reference layouts:
+ *   dens, docc        : [Norb]
+ *   gmats, smats      : impGmats/impSmats(Nspin,Nspin,Norb,Norb,Lmats) complex
+ *   greal, sreal      : (Nspin,Nspin,Norb,Norb,Lreal) complex
+ *   neigen_sector     : [(Ns+1)^2] in/out (ED_DIAG.f90:328-343), may be NULL
+ *   lanc_nstates_total: in/out (ED_DIAG.f90:345-396), may be NULL */
+typedef struct dmed_solve_out {
+    double egs, zeta;
+    int32_t nstates, numgs;
+    double *dens, *docc;
+    double *gmats, *greal, *smats, *sreal;
+    int32_t *neigen_sector;
+    int32_t *lanc_nstates_total;
+    double t_diag, t_gf;           /* seconds spent in the two phases */
+} dmed_solve_out;
+
+/* ed_solve_single backend (ED_MAIN.f90:251-289): diagonalize_impurity
+ * (ed_diag_c + ed_analysis), observables (dens/docc), buildgf_impurity
+ * (build_gf_normal + get_sigma_normal), on the current model. */
+int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out);
+
+/* Sigma = G0^-1 - G^-1 of the normal bath on arbitrary points z
+ * (get_sigma_normal.f90:27-38; invg0_bath_mats, ED_BATH_FUNCTIONS.f90:1784-1807).
+ * z, g, sigma: interleaved complex of length L. */
+int dmed_sigma(int32_t ispin, int32_t iorb, const double *z, const double *g,
+               int32_t L, double *sigma);
+/* Delta(z) = sum_k V_k^2/(z-e_k) (delta_bath_mats, ED_BATH_FUNCTIONS.f90:221-258) */
+int dmed_delta(int32_t ispin, int32_t iorb, const double *z, int32_t L, double *delta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMED_H */

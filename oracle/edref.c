@@ -344,6 +344,30 @@ int ref_hxv_omp(const int64_t *map, long long dim, const double *vin,
     return g_abort_flag ? -3 : 0;
 }
 
+/* Rows [first, last) of the same product with nthreads contiguous blocks
+ * (a bounded sample of a large sector for the CPU timing baseline); hv is
+ * written only on those rows. */
+int ref_hxv_rows(const int64_t *map, long long dim, const double *vin, double *hv,
+                 long long first, long long last, int nthreads)
+{
+    g_abort_flag = 0;
+    if (first < 0 || last > dim || first > last) return -1;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+    {
+        int r = omp_get_thread_num(), p = omp_get_num_threads();
+        long long n = last - first, q = n / p, rem = (r == p - 1) ? n % p : 0;
+        hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)hv, first + r * q,
+                 first + (r + 1) * q + rem);
+    }
+#else
+    (void)nthreads;
+    hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)hv, first, last);
+#endif
+    return g_abort_flag ? -3 : 0;
+}
+
 /* ------------------------------------------------------ tql2 / pythag */
 /* pythag: ED_GREENS_FUNCTIONS.f90:513-539 (Moler-Morrison iteration). */
 double ref_pythag(double a, double b)

@@ -67,6 +67,8 @@ def lib():
         L.ref_binary_search.argtypes = [_i64p, C.c_longlong, C.c_int64]
         L.ref_hxv.argtypes = [_i64p, C.c_longlong, _dp, _dp]
         L.ref_hxv_omp.argtypes = [_i64p, C.c_longlong, _dp, _dp, C.c_int]
+        L.ref_hxv_rows.argtypes = [_i64p, C.c_longlong, _dp, _dp, C.c_longlong, C.c_longlong,
+                                   C.c_int]
         L.ref_pythag.restype = C.c_double
         L.ref_pythag.argtypes = [C.c_double, C.c_double]
         L.ref_tql2.argtypes = [C.c_int, _dp, _dp, _dp]

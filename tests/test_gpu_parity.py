@@ -1,0 +1,356 @@
+"""GPU parity: the HIP library (through its C ABI) against the oracle.
+
+Bars (stated per check):
+  maps, signs, operator targets : bit-exact
+  H*v FP64                      : |y - y_ref| <= 1e-13 * max|y_ref|
+  H*v FP32                      : |y - y_ref| <= 2e-6 * max|y_ref|
+  E0                            : 1e-10 relative
+  G(iw), Sigma(iw)              : 1e-8 absolute
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import dmed
+import edref
+from dmed import configs
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+GOLD = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    edref.build()
+    dmed.init(0)
+    yield
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+
+
+def set_both(g):
+    """Same model in the GPU library and in the oracle."""
+    norb = int(g["norb"])
+    args = (norb, int(g["nbath"]), int(g["nspin"]), g["bath"], g["hloc_f"],
+            list(g["uloc"][:3]), float(g["ust"]), float(g["jh"]), float(g["jx"]),
+            float(g["jp"]), float(g["xmu"]), bool(g["hfmode"]), bool(g["jhflag"]))
+    edref.set_model(*args)
+    return dmed.set_model(*args)
+
+
+def set_cfg(c):
+    args = (c["norb"], c["nbath"], c["nspin"], c["bath"], c["hloc"], c["uloc"], c["ust"],
+            c["jh"], 0.0, 0.0, c["xmu"], True, False)
+    edref.set_model(*args)
+    return dmed.set_model(*args)
+
+
+GPU_GOLD = [n for n in GOLD if n != "m3_norb2_nb2_jh"]
+
+
+def test_jhflag_model_is_refused():
+    g = load("m3_norb2_nb2_jh")
+    with pytest.raises(dmed.DmedError) as e:
+        set_both(g)
+    assert e.value.code == -5
+
+
+@pytest.mark.parametrize("name", GPU_GOLD)
+def test_maps_and_hxv_vs_golden(name):
+    g = load(name)
+    ns = set_both(g)
+    offs = g["sector_offsets"]
+    k = 0
+    for nu in range(ns + 1):
+        for nd in range(ns + 1):
+            say(f"[{name}] sector ({nu},{nd})")
+            s = dmed.Sector(nu, nd)
+            ref = g["maps"][offs[k]:offs[k + 1]]
+            assert np.array_equal(s.map(), ref)
+            x = g["x"][offs[k]:offs[k + 1]]
+            hx = g["hx"][offs[k]:offs[k + 1]]
+            scale = max(1.0, np.abs(hx).max(initial=0))
+            y = s.hxv(x)
+            assert np.abs(y - hx).max(initial=0) <= 1e-13 * scale
+            y32 = s.hxv(x, prec=dmed.FP32)
+            assert np.abs(y32 - hx).max(initial=0) <= 2e-6 * scale
+            s.free()
+            k += 1
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 3])
+def test_gs_sector_map_and_hxv_vs_oracle(cfg):
+    c = configs.config(cfg)
+    set_cfg(c)
+    nu, nd = c["sector"]
+    say(f"[config {cfg}] sector {nu},{nd}")
+    s = dmed.Sector(nu, nd)
+    m = edref.build_sector(nu, nd)
+    assert np.array_equal(s.map(), m)
+    rng = np.random.default_rng(cfg)
+    x = rng.standard_normal(s.dim)
+    yref = edref.hxv(m, x, nthreads=8).real
+    scale = np.abs(yref).max()
+    y = s.hxv(x)
+    assert np.abs(y - yref).max() <= 1e-13 * scale
+    y32 = s.hxv(x, prec=dmed.FP32)
+    assert np.abs(y32 - yref).max() <= 2e-6 * scale
+
+
+def test_ns16_hxv_properties():
+    """Ns=16 half filling (dim 165,636,900): size-independent properties --
+    map sorted/unique with the right popcounts, H symmetric (<x,Hy> = <Hx,y>),
+    linear, FP32 consistent with FP64."""
+    c = configs.config(5)
+    set_cfg(c)
+    say("[config 5] sector 8,8")
+    s = dmed.Sector(8, 8)
+    assert s.dim == 12870 * 12870
+    codes = s.map()
+    assert np.all(np.diff(codes) > 0)
+    lo = codes & 0xFFFF
+    hi = codes >> 16
+    for sample in (lo[:: 1 << 12], hi[:: 1 << 12]):
+        assert np.all(np.array([bin(int(v)).count("1") for v in sample]) == 8)
+    del codes, lo, hi
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(s.dim)
+    z = rng.standard_normal(s.dim)
+    hx = s.hxv(x)
+    hz = s.hxv(z)
+    a, b = np.dot(z, hx), np.dot(hz, x)
+    assert abs(a - b) <= 1e-11 * max(abs(a), 1.0)
+    hxz = s.hxv(x + 2.0 * z)
+    assert np.abs(hxz - (hx + 2.0 * hz)).max() <= 1e-12 * np.abs(hx).max()
+    h32 = s.hxv(x, prec=dmed.FP32)
+    assert np.abs(h32 - hx).max() <= 2e-6 * np.abs(hx).max()
+
+
+@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m4_norb3_nb1", "m5_norb1_nb5_T"])
+def test_eigh_vs_golden_spectra(name):
+    g = load(name)
+    ns = set_both(g)
+    eoffs = g["eig_offsets"]
+    k = 0
+    for nu in range(ns + 1):
+        for nd in range(ns + 1):
+            s = dmed.Sector(nu, nd)
+            ref = g["eigs"][eoffs[k]:eoffs[k + 1]]
+            ne = min(s.dim, 4)
+            for thr in (256, 0):     # dense path and Krylov path
+                if thr == 0 and s.dim < 6:
+                    continue
+                say(f"[{name}] eigh ({nu},{nd}) dim={s.dim} thr={thr}")
+                ev, vecs, res, rc = s.eigh(ne, tol=1e-12, dense_threshold=thr)
+                assert rc == 0
+                assert np.allclose(ev, ref[:ne], rtol=0, atol=1e-10 * max(1, abs(ref[0])))
+                for v in vecs:
+                    v.free()
+            s.free()
+            k += 1
+
+
+@pytest.mark.parametrize("cfg", [1, 2])
+def test_ground_state_energy_vs_oracle(cfg):
+    c = configs.config(cfg)
+    set_cfg(c)
+    nu, nd = c["sector"]
+    say(f"[config {cfg}] GS")
+    s = dmed.Sector(nu, nd)
+    m = edref.build_sector(nu, nd)
+    ev, vecs, res, rc = s.eigh(1, tol=1e-12, dense_threshold=0)
+    assert rc == 0 and res[0] <= 1e-12 * max(1, abs(ev[0]))
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    op = LinearOperator((len(m), len(m)), matvec=lambda x: edref.hxv(m, x, nthreads=8).real,
+                        dtype=np.float64)
+    w = eigsh(op, k=1, which="SA", tol=1e-13, v0=np.ones(len(m)))[0]
+    assert abs(ev[0] - w[0]) <= 1e-10 * abs(w[0])
+    psi = vecs[0].download()
+    r = edref.hxv(m, psi, nthreads=8).real - ev[0] * psi
+    assert np.linalg.norm(r) <= 1e-11
+
+
+def test_ns16_u0_ground_state_closed_form():
+    """U=0 at Ns=16 (dim 1.66e8): E0 of the 8/8 sector = 2 * sum of the 8
+    lowest single-particle levels (both spins fill the same levels)."""
+    c = configs.config(5)
+    c = dict(c, uloc=[0.0])
+    set_cfg(c)
+    nb = c["nbath"]
+    e, v = c["bath"][:nb], c["bath"][nb:]
+    h1 = np.zeros((nb + 1, nb + 1))
+    h1[0, 1:] = h1[1:, 0] = v
+    h1[1:, 1:] = np.diag(e)
+    eps = np.linalg.eigvalsh(h1)
+    exact = 2 * eps[:8].sum()
+    say("[config 5, U=0] GS")
+    s = dmed.Sector(8, 8)
+    ev, vecs, res, rc = s.eigh(1, tol=1e-10, dense_threshold=0, maxit=3000)
+    assert abs(ev[0] - exact) <= 1e-10 * abs(exact)
+
+
+@pytest.mark.parametrize("prec", [dmed.FP64, dmed.FP32])
+def test_chain_vs_oracle(prec):
+    """Green's-function chain from c^+|GS> (workload 1, sector 5/4, 200 steps):
+    alpha/beta of the first steps to round-off, G(iw) to 1e-9 (FP64) / 1e-5
+    (FP32 vectors).  Later alpha/beta drift with the plain recurrence's loss of
+    orthogonality -- identically in the reference -- so the comparison of the
+    whole chain goes through G, which depends only on its spectral moments."""
+    c = configs.config(1)
+    set_cfg(c)
+    mI = edref.build_sector(4, 4)
+    w, U = np.linalg.eigh(edref.dense_h(mI))
+    m = edref.build_sector(5, 4)
+    x = edref.apply_op(mI, U[:, 0], m, 1, True).real
+    s = dmed.Sector(5, 4)
+    v = s.vec(x)
+    say(f"[config 1] chain prec={prec}")
+    a, b, n = dmed.chain(s, v, 200, prec=prec)
+    ar, br, nr = edref.lanc_tridiag(m, x, 200)
+    assert n == nr == 200
+    # FP32 vectors: round-off of ~1e-7 per step, amplified as orthogonality is lost
+    nfirst, tol = (20, 1e-11) if prec == dmed.FP64 else (5, 1e-4)
+    assert np.allclose(a[:nfirst], ar[:nfirst], atol=tol)
+    assert np.allclose(b[1:nfirst], br[1:nfirst], atol=tol)
+    wm = np.pi / 100.0 * (2 * np.arange(1, 513) - 1)
+    wr = np.linspace(-5, 5, 11)
+    g1 = np.zeros(512, complex)
+    g2 = np.zeros(512, complex)
+    r1 = np.zeros(11, complex)
+    r2 = np.zeros(11, complex)
+    n2 = float(np.dot(x, x))
+    dmed.gf_add_chain(n2, w[0], a, b, 1, wm, wr, 0.01, g1, r1)
+    edref.add_to_gf(n2, w[0], ar, br, 1, wm, wr, 0.01, g2, r2)
+    assert np.abs(g1 - g2).max() <= (1e-9 if prec == dmed.FP64 else 1e-4)
+
+
+def test_chain_random_start_first_steps():
+    c = configs.config(1)
+    set_cfg(c)
+    s = dmed.Sector(5, 4)
+    m = edref.build_sector(5, 4)
+    x = np.random.default_rng(9).standard_normal(s.dim)
+    a, b, n = dmed.chain(s, s.vec(x), 40)
+    ar, br, nr = edref.lanc_tridiag(m, x, 40)
+    assert n == nr == 40
+    assert np.allclose(a[:25], ar[:25], atol=1e-11)
+    assert np.allclose(b[1:25], br[1:25], atol=1e-11)
+
+
+def test_chain_early_exit():
+    g = load("m1_norb1_nb3")
+    set_both(g)
+    s = dmed.Sector(2, 2)
+    m = edref.build_sector(2, 2)
+    H = edref.dense_h(m)
+    w, U = np.linalg.eigh(H)
+    x = U[:, 0] + U[:, 1]
+    a, b, n = dmed.chain(s, s.vec(x), 10)
+    ar, br, nr = edref.lanc_tridiag(m, x, 10)
+    assert n == nr == 2
+    assert np.allclose(a, ar, atol=1e-12) and np.allclose(b[:3], br[:3], atol=1e-12)
+
+
+def test_gf_add_chain_matches_oracle_tql2():
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal(300)
+    b = np.concatenate([[0.0], rng.uniform(0.05, 1.0, 299)])
+    wm = np.pi / 50 * (2 * np.arange(1, 101) - 1)
+    wr = np.linspace(-4, 4, 41)
+    g1, g2 = np.zeros(100, complex), np.zeros(100, complex)
+    r1, r2 = np.zeros(41, complex), np.zeros(41, complex)
+    dmed.gf_add_chain(0.7, -0.3, a, b, -1, wm, wr, 0.05, g1, r1)
+    edref.add_to_gf(0.7, -0.3, a, b, -1, wm, wr, 0.05, g2, r2)
+    assert np.abs(g1 - g2).max() < 1e-12 and np.abs(r1 - r2).max() < 1e-10
+
+
+@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m4_norb3_nb1", "m5_norb1_nb5_T"])
+def test_apply_op_and_observables_vs_oracle(name):
+    g = load(name)
+    ns = set_both(g)
+    nspin = int(g["nspin"])
+    norb = int(g["norb"])
+    rng = np.random.default_rng(2)
+    for (nu, nd) in [(ns // 2, ns // 2), (1, ns - 1), (ns - 1, 2)]:
+        I = dmed.Sector(nu, nd)
+        mI = edref.build_sector(nu, nd)
+        x = rng.standard_normal(I.dim)
+        psi = I.vec(x)
+        dens, docc = dmed.observables(psi, 0.5, norb)
+        dr, orr = edref.observables(mI, x, 0.5, norb)
+        assert np.allclose(dens, dr, rtol=1e-12) and np.allclose(docc, orr, rtol=1e-12)
+        for s in range(2):
+            for a in range(norb):
+                isite = a + 1 + s * ns
+                for dagger in (1, 0):
+                    ju = nu + (s == 0) * (1 if dagger else -1)
+                    jd = nd + (s == 1) * (1 if dagger else -1)
+                    if not (0 <= ju <= ns and 0 <= jd <= ns):
+                        continue
+                    say(f"[{name}] op site={isite} dag={dagger} ({nu},{nd})->({ju},{jd})")
+                    J = dmed.Sector(ju, jd)
+                    out, n2 = dmed.apply_op(psi, isite, dagger, J)
+                    ref = edref.apply_op(mI, x, edref.build_sector(ju, jd), isite, dagger).real
+                    assert np.array_equal(out.download(), ref)     # copies with signs: exact
+                    assert np.isclose(n2, np.dot(ref, ref), rtol=1e-13)
+
+
+def _solve_both(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, **kw):
+    p = dmed.default_params(**kw)
+    r = dmed.solve(norb, nspin, (nbath + 1) * norb, p)
+    pr = edref.default_params(**{k: v for k, v in kw.items()
+                                 if k not in ("prec_gf", "batched", "verbose")})
+    o = edref.ed_solve(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, True, params=pr)
+    return r, o
+
+
+@pytest.mark.parametrize("name,finite", [("m1_norb1_nb3", False), ("m2_norb2_nb1", False),
+                                          ("m4_norb3_nb1", False), ("m6_u0", False),
+                                          ("m5_norb1_nb5_T", True)])
+def test_solve_vs_oracle(name, finite):
+    g = load(name)
+    set_both(g)
+    norb, nspin = int(g["norb"]), int(g["nspin"])
+    kw = dict(beta=float(g["beta"]), lmats=len(g["wm"]), lreal=len(g["wr"]),
+              wini=float(g["wr"][0]), wfin=float(g["wr"][-1]), eps=float(g["eps"]),
+              lanc_ngfiter=200)
+    if finite:
+        kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
+    say(f"[{name}] solve")
+    r, o = _solve_both(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
+                       float(g["ust"]), float(g["jh"]), float(g["xmu"]), **kw)
+    assert abs(r["egs"] - o["egs"]) <= 1e-10 * abs(o["egs"])
+    assert r["nstates"] == len(o["states"])
+    assert np.allclose(r["dens"], o["dens"], atol=1e-9)
+    assert np.allclose(r["docc"], o["docc"], atol=1e-9)
+    for s in range(nspin):
+        for a in range(norb):
+            assert np.abs(r["gmats"][s, s, a, a] - o["gmats"][s, a]).max() < 1e-8
+            assert np.abs(r["smats"][s, s, a, a] - o["smats"][s, a]).max() < 1e-8
+
+
+def test_solve_config1_vs_oracle():
+    """Workload 1 (Ns=8, the CPU-runnable case): whole ed_solve, T=0,
+    Lmats=4096, chains of 200 steps; E0 1e-10 rel, G and Sigma 1e-8 abs."""
+    c = configs.config(1)
+    set_cfg(c)
+    kw = dict(beta=c["beta"], lmats=c["lmats"], lreal=200, lanc_ngfiter=200)
+    say("[config 1] solve")
+    r, o = _solve_both(1, c["nbath"], 1, c["bath"], c["hloc"], c["uloc"], 0.0, 0.0, 0.0, **kw)
+    assert abs(r["egs"] - o["egs"]) <= 1e-10 * abs(o["egs"])
+    assert [tuple(x[1]) for x in o["states"]] and r["nstates"] == len(o["states"])
+    assert np.abs(r["gmats"][0, 0, 0, 0] - o["gmats"][0, 0]).max() < 1e-8
+    assert np.abs(r["smats"][0, 0, 0, 0] - o["smats"][0, 0]).max() < 1e-8
+    assert np.allclose(r["dens"], o["dens"], atol=1e-10)
+    assert np.allclose(r["docc"], o["docc"], atol=1e-10)
