@@ -92,6 +92,9 @@ SIGNATURES = [
     ("dmed_solve", C.c_int, [C.POINTER(SolveParams), C.POINTER(SolveOut)]),
     ("dmed_sigma", C.c_int, [C.c_int32, C.c_int32, _dp, _dp, C.c_int32, _dp]),
     ("dmed_delta", C.c_int, [C.c_int32, C.c_int32, _dp, C.c_int32, _dp]),
+    ("dmed_chi2_fit", C.c_int, [C.c_int32, C.c_int32, _dp, C.c_int32, C.c_double, C.c_double,
+                                C.c_double, C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp,
+                                _i32p]),
 ]
 
 
@@ -314,3 +317,17 @@ def solve(norb, nspin, ns, params):
     return dict(egs=o.egs, zeta=o.zeta, nstates=o.nstates, numgs=o.numgs, dens=dens, docc=docc,
                 neigen_sector=neig, lanc_nstates_total=ntot.value, t_diag=o.t_diag,
                 t_gf=o.t_gf, **g)
+
+
+def chi2_fit(fg, beta, e, v, scheme="weiss", weight=0, xmu=0.0, hloc=0.0, niter=500,
+             ftol=1e-5):
+    """Fit one orbital's bath (e, v) to fg(iw_n), n < len(fg).  Host-only."""
+    fg = np.ascontiguousarray(fg, dtype=np.complex128)
+    e = np.array(e, dtype=np.float64)
+    v = np.array(v, dtype=np.float64)
+    chi = C.c_double()
+    it = C.c_int32()
+    _check(lib().dmed_chi2_fit(0 if scheme == "weiss" else 1, weight, _d(fg.view(np.float64)),
+                               len(fg), beta, xmu, hloc, len(e), _d(e), _d(v), niter, ftol,
+                               C.byref(chi), C.byref(it)))
+    return e, v, chi.value, it.value

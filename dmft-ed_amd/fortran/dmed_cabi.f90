@@ -1,0 +1,103 @@
+!> ISO_C_BINDING interface to libdmed.so (include/dmed.h).
+!> Derived types mirror the C structs field by field (bind(C) layout).
+module DMED_CABI
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int),parameter,public :: DMED_FP64 = 0, DMED_FP32 = 1
+
+  type,bind(C),public :: dmed_model_t
+     integer(c_int32_t) :: norb, nbath, nspin
+     type(c_ptr)        :: bath, hloc
+     real(c_double)     :: uloc(3)
+     real(c_double)     :: ust, jh, jx, jp, xmu
+     integer(c_int32_t) :: hfmode, jhflag
+  end type dmed_model_t
+
+  type,bind(C),public :: dmed_solve_params_t
+     integer(c_int32_t) :: lanc_nstates_sector, lanc_nstates_total, lanc_nstates_step
+     integer(c_int32_t) :: lanc_niter, lanc_ngfiter, lanc_ncv_factor, lanc_ncv_add
+     integer(c_int32_t) :: lanc_dim_threshold
+     real(c_double)     :: lanc_tolerance, gs_threshold, cutoff, beta, eps, wini, wfin
+     integer(c_int32_t) :: lmats, lreal, prec_gf, batched, verbose, reserved
+  end type dmed_solve_params_t
+
+  type,bind(C),public :: dmed_solve_out_t
+     real(c_double)     :: egs, zeta
+     integer(c_int32_t) :: nstates, numgs
+     type(c_ptr)        :: dens, docc, gmats, greal, smats, sreal
+     type(c_ptr)        :: neigen_sector, lanc_nstates_total
+     real(c_double)     :: t_diag, t_gf
+  end type dmed_solve_out_t
+
+  public :: dmed_init, dmed_finalize, dmed_last_error, dmed_set_model, dmed_solve
+  public :: dmed_solve_params_default, dmed_sigma, dmed_delta, dmed_chi2_fit
+  public :: c_string
+
+  interface
+     integer(c_int) function dmed_init(device) bind(C,name="dmed_init")
+       import :: c_int
+       integer(c_int),value :: device
+     end function dmed_init
+     integer(c_int) function dmed_finalize() bind(C,name="dmed_finalize")
+       import :: c_int
+     end function dmed_finalize
+     type(c_ptr) function dmed_last_error() bind(C,name="dmed_last_error")
+       import :: c_ptr
+     end function dmed_last_error
+     integer(c_int) function dmed_set_model(m) bind(C,name="dmed_set_model")
+       import :: c_int, dmed_model_t
+       type(dmed_model_t),intent(in) :: m
+     end function dmed_set_model
+     subroutine dmed_solve_params_default(p) bind(C,name="dmed_solve_params_default")
+       import :: dmed_solve_params_t
+       type(dmed_solve_params_t),intent(out) :: p
+     end subroutine dmed_solve_params_default
+     integer(c_int) function dmed_solve(p,o) bind(C,name="dmed_solve")
+       import :: c_int, dmed_solve_params_t, dmed_solve_out_t
+       type(dmed_solve_params_t),intent(in) :: p
+       type(dmed_solve_out_t),intent(inout) :: o
+     end function dmed_solve
+     integer(c_int) function dmed_sigma(ispin,iorb,z,g,L,sigma) bind(C,name="dmed_sigma")
+       import :: c_int, c_int32_t, c_double_complex
+       integer(c_int32_t),value :: ispin, iorb, L
+       complex(c_double_complex),intent(in) :: z(*), g(*)
+       complex(c_double_complex),intent(out) :: sigma(*)
+     end function dmed_sigma
+     integer(c_int) function dmed_delta(ispin,iorb,z,L,delta) bind(C,name="dmed_delta")
+       import :: c_int, c_int32_t, c_double_complex
+       integer(c_int32_t),value :: ispin, iorb, L
+       complex(c_double_complex),intent(in) :: z(*)
+       complex(c_double_complex),intent(out) :: delta(*)
+     end function dmed_delta
+     integer(c_int) function dmed_chi2_fit(scheme,weight,fg,L,beta,xmu,hloc_aa,nbath,e,v,&
+          niter,ftol,chi2,iter) bind(C,name="dmed_chi2_fit")
+       import :: c_int, c_int32_t, c_double, c_double_complex
+       integer(c_int32_t),value :: scheme, weight, L, nbath, niter
+       complex(c_double_complex),intent(in) :: fg(*)
+       real(c_double),value :: beta, xmu, hloc_aa, ftol
+       real(c_double),intent(inout) :: e(*), v(*)
+       real(c_double),intent(out) :: chi2
+       integer(c_int32_t),intent(out) :: iter
+     end function dmed_chi2_fit
+  end interface
+
+contains
+
+  !> copy a NUL-terminated C string into a Fortran character variable
+  function c_string(p) result(s)
+    type(c_ptr),intent(in) :: p
+    character(len=256)     :: s
+    character(kind=c_char),pointer :: a(:)
+    integer :: i
+    s = ""
+    if(.not.c_associated(p))return
+    call c_f_pointer(p,a,[256])
+    do i=1,256
+       if(a(i)==c_null_char)exit
+       s(i:i)=a(i)
+    enddo
+  end function c_string
+
+end module DMED_CABI

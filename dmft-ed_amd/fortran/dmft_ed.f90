@@ -1,0 +1,487 @@
+!> DMFT_ED -- Fortran API of the impurity solver, same names, arguments and
+!> input parameters as the reference module (DMFT_ED.f90:1-75), so that an
+!> existing driver relinks unchanged.  The work is done by the gfx950
+!> library libdmed.so through ISO_C_BINDING (module DMED_CABI).
+!>
+!>   ed_read_input            ED_INPUT_VARS.f90:93-202   (same names/defaults)
+!>   ed_init_solver           ED_MAIN.f90:67-107
+!>   ed_solve                 ED_MAIN.f90:251-289
+!>   ed_get_sigma_matsubara   ED_IO/get_sigma_matsubara.f90:2-5
+!>   ed_get_sigma_real, ed_get_gimp_matsubara/real, ed_get_dens, ed_get_docc
+!>   ed_chi2_fitgf            ED_CHI2FIT.f90:86-138 (normal bath)
+!>   get_bath_dimension       ED_BATH/user_aux.f90:11-30
+!>
+!> Scope: ed_mode="normal", bath_type="normal", single impurity site.  Other
+!> modes stop with a message, as the reference stops on unsupported input.
+module DMFT_ED
+  use iso_c_binding
+  use DMED_CABI
+  implicit none
+  private
+
+  !-------------------------------------------------- input variables
+  integer,public              :: Norb=1, Nbath=6, Nspin=1, nloop=100
+  real(8),public              :: Uloc(3)=[2d0,0d0,0d0], Ust=0d0, Jh=0d0, Jx=0d0, Jp=0d0
+  real(8),public              :: beta=1000d0, xmu=0d0, deltasc=0.02d0, dmft_error=1d-5
+  real(8),public              :: sb_field=0.1d0, eps=0.01d0, wini=-5d0, wfin=5d0
+  integer,public              :: Nsuccess=1, Lmats=5000, Lreal=5000, Ltau=1000, Lfit=1000
+  real(8),public              :: nread=0d0, nerr=1d-4, ndelta=0.1d0
+  logical,public              :: ed_twin=.false., ed_sparse_H=.true., chiflag=.false.
+  logical,public              :: Jhflag=.false., HFmode=.true.
+  logical,public              :: ed_print_Sigma=.true., ed_print_G=.true., ed_print_G0=.true.
+  real(8),public              :: cutoff=1d-9, gs_threshold=1d-9, hwband=2d0
+  integer,public              :: lanc_nstates_sector=6, lanc_nstates_total=1, lanc_nstates_step=2
+  integer,public              :: lanc_ncv_factor=3, lanc_ncv_add=5, lanc_niter=512, lanc_ngfiter=200
+  real(8),public              :: lanc_tolerance=1d-12
+  integer,public              :: lanc_dim_threshold=256
+  integer,public              :: cg_niter=500, cg_method=0, cg_stop=0, cg_weight=0
+  real(8),public              :: cg_ftol=1d-5, cg_eps=1d-6
+  character(len=5),public     :: cg_scheme='weiss'
+  character(len=7),public     :: ed_mode='normal', bath_type='normal'
+  logical,public              :: ed_para=.true.
+  real(8),public              :: ed_vsf_ratio=0.1d0, ed_bath_noise_thr=0d0
+  character(len=100),public   :: Hfile='hamiltonian', HLOCfile='inputHLOC.in'
+  integer,public              :: LOGfile=6, ed_verbose=3
+  !> GPU-path extensions (not in the reference): FP32 Green's-function chains
+  !> and the batched sector engine
+  logical,public              :: ed_gpu_fp32_gf=.false., ed_gpu_batched=.false.
+
+  !-------------------------------------------------- solver state
+  complex(8),allocatable,public :: impGmats(:,:,:,:,:), impGreal(:,:,:,:,:)
+  complex(8),allocatable,public :: impSmats(:,:,:,:,:), impSreal(:,:,:,:,:)
+  real(8),allocatable,public    :: ed_dens(:), ed_docc(:)
+  real(8),public                :: ed_egs = 0d0
+  real(8),allocatable,target    :: bath_e(:,:,:), bath_v(:,:,:)
+  real(8),allocatable,target    :: impHloc(:,:,:,:)
+  integer(c_int32_t),allocatable,target :: neigen_sector(:)
+  logical                       :: isetup=.true.
+
+  public :: ed_read_input
+  public :: get_bath_dimension
+  public :: ed_init_solver
+  public :: ed_solve
+  public :: ed_get_sigma_matsubara, ed_get_sigma_real
+  public :: ed_get_gimp_matsubara, ed_get_gimp_real
+  public :: ed_get_dens, ed_get_docc
+  public :: ed_chi2_fitgf
+
+contains
+
+  !---------------------------------------------------------------- input
+  !> Read NAME=value lines (comments after '!').  Names and defaults are
+  !> those of ED_INPUT_VARS.f90:111-176; unknown names are ignored.
+  subroutine ed_read_input(INPUTunit,comm)
+    character(len=*)  :: INPUTunit
+    integer,optional  :: comm
+    integer           :: unit, ios, ieq, iex
+    character(len=512):: line, key, val
+    logical           :: exists
+    inquire(file=trim(INPUTunit),exist=exists)
+    if(.not.exists)then
+       write(LOGfile,"(A)")"ed_read_input: "//trim(INPUTunit)//" not found, using defaults"
+       return
+    endif
+    open(newunit=unit,file=trim(INPUTunit),status='old',action='read')
+    do
+       read(unit,"(A)",iostat=ios)line
+       if(ios/=0)exit
+       iex = index(line,'!')
+       if(iex>0)line=line(:iex-1)
+       ieq = index(line,'=')
+       if(ieq<=1)cycle
+       key = upper(adjustl(line(:ieq-1)))
+       val = adjustl(line(ieq+1:))
+       call set_var(trim(key),trim(val))
+    enddo
+    close(unit)
+    Ltau=max(int(beta),Ltau)
+    if(Lfit>Lmats)Lfit=Lmats
+  end subroutine ed_read_input
+
+  subroutine set_var(key,val)
+    character(len=*) :: key, val
+    integer          :: ios
+    character(len=len(val)+2) :: v
+    v = val
+    select case(key)
+    case("NORB");  read(v,*,iostat=ios)Norb
+    case("NBATH"); read(v,*,iostat=ios)Nbath
+    case("NSPIN"); read(v,*,iostat=ios)Nspin
+    case("ULOC");  read(v,*,iostat=ios)Uloc
+    case("UST");   read(v,*,iostat=ios)Ust
+    case("JH");    read(v,*,iostat=ios)Jh
+    case("JX");    read(v,*,iostat=ios)Jx
+    case("JP");    read(v,*,iostat=ios)Jp
+    case("BETA");  read(v,*,iostat=ios)beta
+    case("XMU");   read(v,*,iostat=ios)xmu
+    case("DELTASC"); read(v,*,iostat=ios)deltasc
+    case("NLOOP"); read(v,*,iostat=ios)nloop
+    case("DMFT_ERROR"); read(v,*,iostat=ios)dmft_error
+    case("SB_FIELD"); read(v,*,iostat=ios)sb_field
+    case("ED_TWIN"); read(v,*,iostat=ios)ed_twin
+    case("ED_SPARSE_H"); read(v,*,iostat=ios)ed_sparse_H
+    case("ED_PRINT_SIGMA"); read(v,*,iostat=ios)ed_print_Sigma
+    case("ED_PRINT_G"); read(v,*,iostat=ios)ed_print_G
+    case("ED_PRINT_G0"); read(v,*,iostat=ios)ed_print_G0
+    case("NSUCCESS"); read(v,*,iostat=ios)Nsuccess
+    case("LMATS"); read(v,*,iostat=ios)Lmats
+    case("LREAL"); read(v,*,iostat=ios)Lreal
+    case("LTAU");  read(v,*,iostat=ios)Ltau
+    case("LFIT");  read(v,*,iostat=ios)Lfit
+    case("NREAD"); read(v,*,iostat=ios)nread
+    case("NERR");  read(v,*,iostat=ios)nerr
+    case("NDELTA"); read(v,*,iostat=ios)ndelta
+    case("WINI");  read(v,*,iostat=ios)wini
+    case("WFIN");  read(v,*,iostat=ios)wfin
+    case("CHIFLAG"); read(v,*,iostat=ios)chiflag
+    case("JHFLAG"); read(v,*,iostat=ios)Jhflag
+    case("HFMODE"); read(v,*,iostat=ios)HFmode
+    case("EPS");   read(v,*,iostat=ios)eps
+    case("CUTOFF"); read(v,*,iostat=ios)cutoff
+    case("GS_THRESHOLD"); read(v,*,iostat=ios)gs_threshold
+    case("HWBAND"); read(v,*,iostat=ios)hwband
+    case("LANC_NSTATES_SECTOR"); read(v,*,iostat=ios)lanc_nstates_sector
+    case("LANC_NSTATES_TOTAL"); read(v,*,iostat=ios)lanc_nstates_total
+    case("LANC_NSTATES_STEP"); read(v,*,iostat=ios)lanc_nstates_step
+    case("LANC_NCV_FACTOR"); read(v,*,iostat=ios)lanc_ncv_factor
+    case("LANC_NCV_ADD"); read(v,*,iostat=ios)lanc_ncv_add
+    case("LANC_NITER"); read(v,*,iostat=ios)lanc_niter
+    case("LANC_NGFITER"); read(v,*,iostat=ios)lanc_ngfiter
+    case("LANC_TOLERANCE"); read(v,*,iostat=ios)lanc_tolerance
+    case("LANC_DIM_THRESHOLD"); read(v,*,iostat=ios)lanc_dim_threshold
+    case("CG_NITER"); read(v,*,iostat=ios)cg_niter
+    case("CG_SCHEME"); cg_scheme=trim(adjustl(v))
+    case("CG_FTOL"); read(v,*,iostat=ios)cg_ftol
+    case("CG_METHOD"); read(v,*,iostat=ios)cg_method
+    case("CG_STOP"); read(v,*,iostat=ios)cg_stop
+    case("CG_EPS"); read(v,*,iostat=ios)cg_eps
+    case("CG_WEIGHT"); read(v,*,iostat=ios)cg_weight
+    case("ED_MODE"); ed_mode=trim(adjustl(v))
+    case("ED_PARA"); read(v,*,iostat=ios)ed_para
+    case("ED_VSF_RATIO"); read(v,*,iostat=ios)ed_vsf_ratio
+    case("ED_BATH_NOISE_THR"); read(v,*,iostat=ios)ed_bath_noise_thr
+    case("BATH_TYPE"); bath_type=trim(adjustl(v))
+    case("HFILE"); Hfile=trim(adjustl(v))
+    case("IMPHFILE"); HLOCfile=trim(adjustl(v))
+    case("LOGFILE"); read(v,*,iostat=ios)LOGfile
+    case("ED_VERBOSE"); read(v,*,iostat=ios)ed_verbose
+    case("ED_GPU_FP32_GF"); read(v,*,iostat=ios)ed_gpu_fp32_gf
+    case("ED_GPU_BATCHED"); read(v,*,iostat=ios)ed_gpu_batched
+    case default
+    end select
+  end subroutine set_var
+
+  function upper(s) result(u)
+    character(len=*) :: s
+    character(len=len(s)) :: u
+    integer :: i, c
+    u = s
+    do i=1,len(s)
+       c = iachar(s(i:i))
+       if(c>=97 .and. c<=122)u(i:i)=achar(c-32)
+    enddo
+  end function upper
+
+  !------------------------------------------------------------- bath
+  !> ( e [Nspin][Norb][Nbath] + v [Nspin][Norb][Nbath] ), user_aux.f90:17-30
+  function get_bath_dimension(Hloc_nn,ispin_) result(bath_size)
+    complex(8),optional,intent(in) :: Hloc_nn(:,:,:,:)
+    integer,optional               :: ispin_
+    integer                        :: bath_size
+    if(bath_type/='normal' .or. ed_mode/='normal')stop "get_bath_dimension: only bath_type=normal, ed_mode=normal"
+    bath_size = Norb*Nbath + Norb*Nbath
+    if(.not.present(ispin_))bath_size=Nspin*bath_size
+  end function get_bath_dimension
+
+  subroutine check_modes()
+    if(ed_mode/='normal')stop "DMFT_ED: only ed_mode=normal is built on the GPU path"
+    if(bath_type/='normal')stop "DMFT_ED: only bath_type=normal is built on the GPU path"
+    if(Nspin>2)stop "ED ERROR: Nspin > 2 is currently not supported"
+    if(Norb>3)stop "ED ERROR: Norb > 3 is currently not supported"
+  end subroutine check_modes
+
+  !> set_dmft_bath (dmft_aux.f90:489-512): flat bath -> e(ispin,iorb,k), v(...)
+  subroutine set_bath(bath)
+    real(8),intent(in) :: bath(:)
+    integer :: s, a, k, io, st
+    st = Nspin*Norb*Nbath
+    do s=1,Nspin
+       do a=1,Norb
+          do k=1,Nbath
+             io = k + (a-1)*Nbath + (s-1)*Nbath*Norb
+             bath_e(s,a,k) = bath(io)
+             bath_v(s,a,k) = bath(st+io)
+          enddo
+       enddo
+    enddo
+  end subroutine set_bath
+
+  subroutine get_bath(bath)
+    real(8),intent(out) :: bath(:)
+    integer :: s, a, k, io, st
+    st = Nspin*Norb*Nbath
+    do s=1,Nspin
+       do a=1,Norb
+          do k=1,Nbath
+             io = k + (a-1)*Nbath + (s-1)*Nbath*Norb
+             bath(io)    = bath_e(s,a,k)
+             bath(st+io) = bath_v(s,a,k)
+          enddo
+       enddo
+    enddo
+  end subroutine get_bath
+
+  !> init_dmft_bath, normal bath (dmft_aux.f90:92-177): levels spread over
+  !> [-hwband, hwband], V = max(0.1, 1/sqrt(Nbath)); then the
+  !> hamiltonian.restart file if present.
+  subroutine init_bath()
+    integer :: i, Nh, unit, s, a, ios
+    real(8) :: de
+    logical :: exists
+    character(len=512) :: line
+    bath_e(:,:,1)     = -hwband
+    bath_e(:,:,Nbath) =  hwband
+    Nh = Nbath/2
+    if(mod(Nbath,2)==0 .and. Nbath>=4)then
+       de = hwband/max(Nh-1,1)
+       bath_e(:,:,Nh)   = -1d-3
+       bath_e(:,:,Nh+1) =  1d-3
+       do i=2,Nh-1
+          bath_e(:,:,i)         = -hwband + (i-1)*de
+          bath_e(:,:,Nbath-i+1) =  hwband - (i-1)*de
+       enddo
+    elseif(mod(Nbath,2)/=0 .and. Nbath>=3)then
+       de = hwband/Nh
+       bath_e(:,:,Nh+1) = 0d0
+       do i=2,Nh
+          bath_e(:,:,i)         = -hwband + (i-1)*de
+          bath_e(:,:,Nbath-i+1) =  hwband - (i-1)*de
+       enddo
+    endif
+    do i=1,Nbath
+       bath_v(:,:,i) = max(0.1d0,1d0/sqrt(dble(Nbath)))
+    enddo
+    inquire(file=trim(Hfile)//".restart",exist=exists)
+    if(exists)then
+       write(LOGfile,"(A)")"Reading bath from file "//trim(Hfile)//".restart"
+       open(newunit=unit,file=trim(Hfile)//".restart",status='old',action='read')
+       read(unit,"(A)",iostat=ios)line
+       do i=1,Nbath
+          read(unit,*,iostat=ios)((bath_e(s,a,i),bath_v(s,a,i),a=1,Norb),s=1,Nspin)
+          if(ios/=0)exit
+       enddo
+       close(unit)
+    endif
+  end subroutine init_bath
+
+  !> save_dmft_bath (dmft_aux.f90:446-465): hamiltonian.used / .restart
+  subroutine save_bath(used)
+    logical,intent(in) :: used
+    integer :: unit, i, s, a
+    character(len=16) :: ext
+    ext = ".restart"; if(used)ext = ".used"
+    open(newunit=unit,file=trim(Hfile)//trim(ext))
+    write(unit,"(90(A21,1X))")(("#Ek_l"//itoa(a)//"_s"//itoa(s),"Vk_l"//itoa(a)//"_s"//itoa(s),a=1,Norb),s=1,Nspin)
+    do i=1,Nbath
+       write(unit,"(90(F21.12,1X))")((bath_e(s,a,i),bath_v(s,a,i),a=1,Norb),s=1,Nspin)
+    enddo
+    close(unit)
+  end subroutine save_bath
+
+  function itoa(i) result(s)
+    integer :: i
+    character(len=:),allocatable :: s
+    character(len=16) :: b
+    write(b,"(I0)")i
+    s = trim(b)
+  end function itoa
+
+  !------------------------------------------------------------ solver
+  subroutine ed_check(rc,where)
+    integer(c_int),intent(in) :: rc
+    character(len=*) :: where
+    if(rc/=0)then
+       write(*,"(A)")"DMFT_ED "//where//" error: "//trim(c_string(dmed_last_error()))
+       error stop "DMFT_ED: GPU library error"
+    endif
+  end subroutine ed_check
+
+  subroutine set_hloc(Hloc)
+    complex(8),intent(in) :: Hloc(Nspin,Nspin,Norb,Norb)
+    if(any(abs(aimag(Hloc))>0d0))stop "DMFT_ED: complex Hloc is not supported on the GPU path (H must be real)"
+    impHloc = dble(Hloc)
+  end subroutine set_hloc
+
+  !> ed_init_solver_single (ED_MAIN.f90:67-107)
+  subroutine ed_init_solver(bath,Hloc)
+    real(8),dimension(:),intent(inout) :: bath
+    complex(8),intent(in)              :: Hloc(Nspin,Nspin,Norb,Norb)
+    integer :: Ns
+    call check_modes()
+    if(size(bath)/=get_bath_dimension())stop "init_ed_solver_single error: wrong bath dimensions"
+    if(isetup)then
+       call ed_check(dmed_init(0_c_int),"dmed_init")
+       if(allocated(impHloc))deallocate(impHloc)
+       allocate(impHloc(Nspin,Nspin,Norb,Norb)); impHloc=0d0
+       allocate(impGmats(Nspin,Nspin,Norb,Norb,Lmats), impSmats(Nspin,Nspin,Norb,Norb,Lmats))
+       allocate(impGreal(Nspin,Nspin,Norb,Norb,Lreal), impSreal(Nspin,Nspin,Norb,Norb,Lreal))
+       allocate(ed_dens(Norb), ed_docc(Norb))
+       allocate(bath_e(Nspin,Norb,Nbath), bath_v(Nspin,Norb,Nbath))
+       Ns = (Nbath+1)*Norb
+       allocate(neigen_sector((Ns+1)**2)); neigen_sector = 0
+       if(lanc_nstates_total/=1)then
+          if(mod(lanc_nstates_sector,2)/=0)lanc_nstates_sector=lanc_nstates_sector+1
+          if(mod(lanc_nstates_total,2)/=0)lanc_nstates_total=lanc_nstates_total+1
+       endif
+    endif
+    call set_hloc(Hloc)
+    bath = 0d0
+    call init_bath()
+    call get_bath(bath)
+    isetup=.false.
+  end subroutine ed_init_solver
+
+  !> ed_solve_single (ED_MAIN.f90:251-289)
+  subroutine ed_solve(bath,Hloc)
+    real(8),dimension(:),intent(in),target :: bath
+    complex(8),optional,intent(in)         :: Hloc(Nspin,Nspin,Norb,Norb)
+    type(dmed_model_t)        :: m
+    type(dmed_solve_params_t) :: p
+    type(dmed_solve_out_t)    :: o
+    real(8),allocatable,target :: bcopy(:), hcopy(:,:,:,:), dens(:), docc(:)
+    complex(8),allocatable,target :: gm(:,:,:,:,:), gr(:,:,:,:,:), sm(:,:,:,:,:), sr(:,:,:,:,:)
+    integer(c_int32_t),target :: ntot
+    if(isetup)stop "ed_solve: call ed_init_solver first"
+    if(present(Hloc))call set_hloc(Hloc)
+    if(size(bath)/=get_bath_dimension())stop "ED_SOLVE_SINGLE Error: wrong bath dimensions"
+    call set_bath(bath)
+    call save_bath(used=.true.)
+    allocate(bcopy(size(bath))); bcopy = bath
+    allocate(hcopy(Nspin,Nspin,Norb,Norb)); hcopy = impHloc
+    m%norb = Norb; m%nbath = Nbath; m%nspin = Nspin
+    m%bath = c_loc(bcopy); m%hloc = c_loc(hcopy)
+    m%uloc = Uloc; m%ust = Ust; m%jh = Jh; m%jx = Jx; m%jp = Jp; m%xmu = xmu
+    m%hfmode = merge(1,0,HFmode); m%jhflag = merge(1,0,Jhflag)
+    call ed_check(dmed_set_model(m),"dmed_set_model")
+    call dmed_solve_params_default(p)
+    p%lanc_nstates_sector = lanc_nstates_sector
+    p%lanc_nstates_total  = lanc_nstates_total
+    p%lanc_nstates_step   = lanc_nstates_step
+    p%lanc_niter   = lanc_niter
+    p%lanc_ngfiter = lanc_ngfiter
+    p%lanc_ncv_factor = lanc_ncv_factor
+    p%lanc_ncv_add = lanc_ncv_add
+    p%lanc_dim_threshold = lanc_dim_threshold
+    p%lanc_tolerance = lanc_tolerance
+    p%gs_threshold = gs_threshold
+    p%cutoff = cutoff
+    p%beta = beta; p%eps = eps; p%wini = wini; p%wfin = wfin
+    p%lmats = Lmats; p%lreal = Lreal
+    p%prec_gf = merge(DMED_FP32,DMED_FP64,ed_gpu_fp32_gf)
+    p%batched = merge(1,0,ed_gpu_batched)
+    p%verbose = ed_verbose
+    allocate(dens(Norb), docc(Norb))
+    allocate(gm(Nspin,Nspin,Norb,Norb,Lmats), sm(Nspin,Nspin,Norb,Norb,Lmats))
+    allocate(gr(Nspin,Nspin,Norb,Norb,Lreal), sr(Nspin,Nspin,Norb,Norb,Lreal))
+    ntot = lanc_nstates_total
+    o%dens = c_loc(dens); o%docc = c_loc(docc)
+    o%gmats = c_loc(gm); o%greal = c_loc(gr); o%smats = c_loc(sm); o%sreal = c_loc(sr)
+    o%neigen_sector = c_loc(neigen_sector)
+    o%lanc_nstates_total = c_loc(ntot)
+    call ed_check(dmed_solve(p,o),"dmed_solve")
+    lanc_nstates_total = ntot
+    impGmats = gm; impGreal = gr; impSmats = sm; impSreal = sr
+    ed_dens = dens; ed_docc = docc; ed_egs = o%egs
+    if(ed_verbose>=1)then
+       write(LOGfile,"(A,F20.12,A,I4,A,F10.3,A,F10.3,A)")"Egs =",o%egs,"  states:",o%nstates,&
+            "  t_diag=",o%t_diag,"s  t_gf=",o%t_gf,"s"
+       write(LOGfile,"(A,10F18.12)")"dens=",ed_dens
+       write(LOGfile,"(A,10F18.12)")"docc=",ed_docc
+    endif
+    if(ed_print_Sigma)call print_impSigma()
+  end subroutine ed_solve
+
+  !> impSigma_l<a><b>_s<s>_iw.ed : columns wm, Im Sigma, Re Sigma (ED_IO/print_impSigma.f90)
+  subroutine print_impSigma()
+    integer :: unit, s, a, i
+    real(8) :: wm
+    do s=1,Nspin
+       do a=1,Norb
+          open(newunit=unit,file="impSigma_l"//itoa(a)//itoa(a)//"_s"//itoa(s)//"_iw.ed")
+          do i=1,Lmats
+             wm = acos(-1d0)/beta*dble(2*i-1)
+             write(unit,"(3ES24.15)")wm,aimag(impSmats(s,s,a,a,i)),dble(impSmats(s,s,a,a,i))
+          enddo
+          close(unit)
+       enddo
+    enddo
+  end subroutine print_impSigma
+
+  !------------------------------------------------------------ getters
+  subroutine ed_get_sigma_matsubara(Smats)
+    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: Smats
+    Smats = impSmats
+  end subroutine ed_get_sigma_matsubara
+
+  subroutine ed_get_sigma_real(Sreal)
+    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lreal),intent(inout) :: Sreal
+    Sreal = impSreal
+  end subroutine ed_get_sigma_real
+
+  subroutine ed_get_gimp_matsubara(Gmats)
+    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: Gmats
+    Gmats = impGmats
+  end subroutine ed_get_gimp_matsubara
+
+  subroutine ed_get_gimp_real(Greal)
+    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lreal),intent(inout) :: Greal
+    Greal = impGreal
+  end subroutine ed_get_gimp_real
+
+  subroutine ed_get_dens(dens)
+    real(8),dimension(Norb) :: dens
+    dens = ed_dens
+  end subroutine ed_get_dens
+
+  subroutine ed_get_docc(docc)
+    real(8),dimension(Norb) :: docc
+    docc = ed_docc
+  end subroutine ed_get_docc
+
+  !------------------------------------------------------------ chi^2 fit
+  !> chi2_fitgf_generic_normal -> chi2_fitgf_normal_normal (ED_CHI2FIT.f90:86-138,
+  !> fitgf_normal_normal.f90:18-156): per orbital, fit (e_k, V_k) of spin ispin
+  !> to fg on the first Lfit Matsubara points, scheme cg_scheme (weiss/delta),
+  !> weights cg_weight; then save the bath (hamiltonian.restart).
+  subroutine ed_chi2_fitgf(fg,bath,ispin)
+    complex(8),dimension(:,:,:,:,:) :: fg
+    real(8),dimension(:)            :: bath
+    integer,optional                :: ispin
+    integer                         :: ispin_, a, L
+    real(8)                         :: chi2
+    integer(c_int32_t)              :: iter, scheme
+    real(8),allocatable             :: e(:), v(:)
+    complex(8),allocatable          :: g(:)
+    ispin_ = 1; if(present(ispin))ispin_ = ispin
+    if(size(fg,1)/=Nspin .or. size(fg,3)/=Norb)stop "chi2_fitgf_generic_normal: wrong fg shape"
+    call set_bath(bath)
+    L = min(Lfit,size(fg,5))
+    allocate(e(Nbath), v(Nbath), g(L))
+    scheme = 0; if(trim(cg_scheme)=='delta')scheme = 1
+    do a=1,Norb
+       g = fg(ispin_,ispin_,a,a,1:L)
+       e = bath_e(ispin_,a,:); v = bath_v(ispin_,a,:)
+       call ed_check(dmed_chi2_fit(scheme,int(cg_weight,c_int32_t),g,int(L,c_int32_t),beta,xmu,&
+            impHloc(ispin_,ispin_,a,a),int(Nbath,c_int32_t),e,v,int(cg_niter,c_int32_t),cg_ftol,&
+            chi2,iter),"dmed_chi2_fit")
+       bath_e(ispin_,a,:) = e; bath_v(ispin_,a,:) = v
+       write(LOGfile,"(A,ES18.9,A,I5,A)")"chi^2|iter= ",chi2," | ",iter,"  <--  Orb"//itoa(a)//" Spin"//itoa(ispin_)
+    enddo
+    if(Nspin==2 .and. .not.present(ispin))then
+       bath_e(2,:,:) = bath_e(1,:,:); bath_v(2,:,:) = bath_v(1,:,:)
+    endif
+    call save_bath(used=.false.)
+    call get_bath(bath)
+  end subroutine ed_chi2_fitgf
+
+end module DMFT_ED

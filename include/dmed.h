@@ -223,6 +223,16 @@ int dmed_sigma(int32_t ispin, int32_t iorb, const double *z, const double *g,
 /* Delta(z) = sum_k V_k^2/(z-e_k) (delta_bath_mats, ED_BATH_FUNCTIONS.f90:221-258) */
 int dmed_delta(int32_t ispin, int32_t iorb, const double *z, int32_t L, double *delta);
 
+/* chi^2 fit of one orbital's normal bath (e_k, V_k), in place, to fg on the
+ * first L Matsubara points w_n = pi/beta*(2n-1): scheme 0 fits
+ * G0 = 1/(iw + xmu - hloc_aa - Delta) ("weiss"), 1 fits Delta ("delta");
+ * weight = cg_weight (0: L, 1: 1, 2: n, 3: w_n).  Host-only.  Replaces
+ * chi2_fitgf_normal_normal (ED_CHI2FIT/fitgf_normal_normal.f90:18-156) and
+ * its fmin_cg call (:86-90). */
+int dmed_chi2_fit(int32_t scheme, int32_t weight, const double *fg, int32_t L, double beta,
+                  double xmu, double hloc_aa, int32_t nbath, double *e, double *v,
+                  int32_t niter, double ftol, double *chi2, int32_t *iter);
+
 #ifdef __cplusplus
 }
 #endif
