@@ -110,6 +110,23 @@ def main():
     log(f"H*v: {ms_step:.4f} ms/step wall, kernel {kernel_ms:.4f} ms, "
         f"{achieved:.1f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of HBM peak)")
 
+    # the FP32 form of the same product (mixed-precision chains use it)
+    extra = {}
+    if prec == dmed.FP64:
+        x32 = dmed.Vec(sec, dmed.FP32)
+        y32 = dmed.Vec(sec, dmed.FP32)
+        dmed._check(L.dmed_hxv_timed(sec.h, x.ptr(), y.ptr(), prec, 1, C.byref(kms)))
+        x32.upload(np.random.default_rng(0).standard_normal(D))
+        k32 = C.c_double()
+        dmed._check(L.dmed_hxv_timed(sec.h, x32.ptr(), y32.ptr(), dmed.FP32, 3, C.byref(k32)))
+        dmed._check(L.dmed_hxv_timed(sec.h, x32.ptr(), y32.ptr(), dmed.FP32, a.steps, C.byref(k32)))
+        ach32 = 8 * D / (k32.value * 1e-3) / 1e9
+        extra["fp32_hxv"] = {"kernel_ms": k32.value, "hv_per_s": 1e3 / k32.value,
+                             "achieved_GBs": ach32, "frac": ach32 / HBM_PEAK_GBS}
+        log(f"FP32 H*v: kernel {k32.value:.4f} ms, {ach32:.1f} GB/s algorithmic")
+        x32.free()
+        y32.free()
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hxv_ns16_traffic.json")
     if os.path.exists(tpath):
@@ -149,6 +166,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "kernel_ms": kernel_ms,
+        "extra": extra,
         "algorithmic_bytes_per_launch": alg_bytes,
     }
     print(json.dumps(out), flush=True)

@@ -56,6 +56,12 @@ struct SpinTables {
     int32_t *hidx = nullptr;
     double *hval = nullptr;
     float *hval32 = nullptr;
+    // packed form for the persistent column kernel: word = rank<<8 | term<<1 | sign,
+    // with |coefficient| = coef[term]; valid when packed == true
+    bool packed = false;
+    int ncoef = 0;
+    uint32_t *hpk = nullptr;            // [nhop][dim]
+    double *coef = nullptr;             // [ncoef]
     std::vector<uint32_t> h_str;        // host copies (small)
     std::vector<double> h_diag;
     std::vector<uint8_t> h_imp;

@@ -126,6 +126,242 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
     }
 }
 
+
+// ------------------------------------------------------------------------
+// Persistent column kernel for large columns (D_up up to 1024*RPT).
+// One 1024-lane workgroup per CU owns a contiguous range of columns of one
+// XCD's share (workgroup b runs on XCD b%8, so concurrent columns of an XCD
+// are neighbours and their down-hop neighbour columns are shared through
+// that XCD's L2).  Per column: the column prefetched into registers during
+// the previous column is stored to LDS; down-hop neighbour columns are
+// streamed from memory four at a time; up hops gather from LDS with packed
+// 4-byte hop words (target rank, coefficient index, sign) whose table stays
+// L2-resident; y is written with non-temporal stores.  Each lane owns rows
+// tid + 1024*r, r < RPT, so its diagonal terms live in registers.
+template <typename T>
+struct ColArgs {
+    const T *__restrict__ x;
+    T *__restrict__ y;
+    int32_t dup, ddw, nhu, nhd, nimp, ncoef;
+    const uint32_t *__restrict__ hu;      // [nhu][dup] packed
+    const double *__restrict__ coef;      // [ncoef]
+    const int32_t *__restrict__ hd_idx;   // [nhd][ddw]
+    const double *__restrict__ hd_val;
+    const double *__restrict__ du;
+    const double *__restrict__ dd;
+    const uint8_t *__restrict__ impu;
+    const uint8_t *__restrict__ impd;
+    const double *__restrict__ W;
+    const T *__restrict__ vp;
+    const ChainScalars *cs;
+    double *__restrict__ partials;
+};
+
+template <typename T, int MODE, int RPT>
+__global__ void __launch_bounds__(1024) k_hxv_col(
+    const T *__restrict__ X, T *__restrict__ Y, int32_t dup, int32_t ddw, int32_t nhu, int32_t nhd,
+    int32_t nimp, int32_t ncoef, const uint32_t *__restrict__ HU, const double *__restrict__ COEF,
+    const int32_t *__restrict__ HDI, const double *__restrict__ HDV, const double *__restrict__ DU,
+    const double *__restrict__ DD, const uint8_t *__restrict__ IMPU, const uint8_t *__restrict__ IMPD,
+    const double *__restrict__ WTAB, const T *__restrict__ VP, const ChainScalars *__restrict__ CS,
+    double *__restrict__ PARTIALS)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *col = reinterpret_cast<T *>(smem);
+    __shared__ T sc[128];
+    __shared__ double wt[8];
+    __shared__ double red[16];
+    constexpr int HDU = 4;
+    double sx = 1.0, coef_vp = 0.0;
+    if (MODE == kLanczos) {
+        if (CS->done) return;
+        sx = CS->sx;
+        coef_vp = CS->b * CS->svp;
+    }
+    for (int t = threadIdx.x; t < ncoef; t += blockDim.x) sc[t] = (T)COEF[t];
+    const int nper = gridDim.x / 8;                  // workgroups per XCD
+    const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+    const int per = (ddw + 7) / 8;
+    const int cbeg = xcd * per, cend = min(ddw, cbeg + per);
+    T dui[RPT];
+    uint32_t imp[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int i = threadIdx.x + r * 1024;
+        dui[r] = i < dup ? (T)DU[i] : (T)0;
+        imp[r] = i < dup ? IMPU[i] : 0u;
+    }
+    int c = cbeg + loc;
+    T nx[RPT];
+    if (c < cend) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = threadIdx.x + r * 1024;
+            nx[r] = i < dup ? X[(int64_t)c * dup + i] : (T)0;
+        }
+    }
+    double dot = 0.0;
+    for (; c < cend; c += nper) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = threadIdx.x + r * 1024;
+            if (i < dup) col[i] = nx[r];
+        }
+        if ((int)threadIdx.x < nimp)
+            wt[threadIdx.x] = DD[c] + WTAB[threadIdx.x * nimp + IMPD[c]];
+        __syncthreads();
+        T acc[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) acc[r] = nx[r] * (dui[r] + (T)wt[imp[r]]);
+        T xo[MODE == kLanczos ? RPT : 1];
+        if (MODE == kLanczos) {
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) xo[r] = nx[r];
+        }
+        const int cn = c + nper;
+        if (cn < cend) {
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int i = threadIdx.x + r * 1024;
+                nx[r] = i < dup ? X[(int64_t)cn * dup + i] : (T)0;
+            }
+        }
+        for (int h0 = 0; h0 < nhd; h0 += HDU) {
+            T v[HDU];
+            const T *xj[HDU];
+#pragma unroll
+            for (int q = 0; q < HDU; ++q) {
+                const int h = min(h0 + q, nhd - 1);
+                v[q] = h0 + q < nhd ? (T)HDV[(int64_t)h * ddw + c] : (T)0;
+                xj[q] = X + (int64_t)HDI[(int64_t)h * ddw + c] * dup;
+            }
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int i = threadIdx.x + r * 1024;
+                if (i < dup) {
+#pragma unroll
+                    for (int q = 0; q < HDU; ++q) acc[r] += v[q] * xj[q][i];
+                }
+            }
+        }
+        for (int h = 0; h < nhu; ++h) {
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int i = threadIdx.x + r * 1024;
+                if (i < dup) {
+                    const uint32_t w = HU[(int64_t)h * dup + i];
+                    T vv = sc[(w >> 1) & 127u];
+                    vv = (w & 1u) ? -vv : vv;
+                    acc[r] += vv * col[w >> 8];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = threadIdx.x + r * 1024;
+            if (i < dup) {
+                const int64_t o = (int64_t)c * dup + i;
+                if (MODE == kPlain) {
+                    __builtin_nontemporal_store(acc[r], &Y[o]);
+                } else {
+                    const double yy = sx * (double)acc[r] - coef_vp * (double)VP[o];
+                    __builtin_nontemporal_store((T)yy, &Y[o]);
+                    dot += sx * (double)xo[r] * yy;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (MODE == kLanczos) {
+        const double s = block_sum(dot, red);
+        if (threadIdx.x == 0) PARTIALS[blockIdx.x] = s;
+    }
+}
+
+// rows per lane the column kernel supports per element type (register budget)
+constexpr int kColMaxRptF64 = 13;
+constexpr int kColMaxRptF32 = 16;
+
+static bool use_col_kernel(const dmed_sector *s, int prec)
+{
+    const Context &c = ctx();
+    const size_t esz = prec == DMED_FP32 ? 4 : 8;
+    const int64_t dup = s->up.dim;
+    const int rpt = (int)((dup + 1023) / 1024);
+    const int maxr = prec == DMED_FP32 ? kColMaxRptF32 : kColMaxRptF64;
+    return s->up.packed && s->up.ncoef <= 128 && s->nimp <= 8 && dup >= 2048 &&
+           rpt <= maxr && (size_t)dup * esz + 2048 <= c.lds_max && s->dw.dim >= 64;
+}
+
+static unsigned col_grid(const dmed_sector *s)
+{
+    // one workgroup per CU, a multiple of the 8 XCDs, never more than columns
+    const int ncu = std::max(8, ctx().num_cu);
+    int g = (ncu / 8) * 8;
+    while (g > 8 && g / 8 > (s->dw.dim + 7) / 8) g -= 8;
+    return (unsigned)g;
+}
+
+template <typename T, int MODE, int RPT>
+static int launch_col_t(const dmed_sector *s, const void *x, void *y, const void *vp,
+                        ChainScalars *cs, double *partials, hipStream_t st)
+{
+    ColArgs<T> a;
+    a.x = static_cast<const T *>(x);
+    a.y = static_cast<T *>(y);
+    a.dup = (int32_t)s->up.dim;
+    a.ddw = (int32_t)s->dw.dim;
+    a.nhu = s->up.nhop;
+    a.nhd = s->dw.nhop;
+    a.nimp = s->nimp;
+    a.ncoef = s->up.ncoef;
+    a.hu = s->up.hpk;
+    a.coef = s->up.coef;
+    a.hd_idx = s->dw.hidx;
+    a.hd_val = s->dw.hval;
+    a.du = s->up.diag;
+    a.dd = s->dw.diag;
+    a.impu = s->up.imp;
+    a.impd = s->dw.imp;
+    a.W = s->W;
+    a.vp = static_cast<const T *>(vp);
+    a.cs = cs;
+    a.partials = partials;
+    const size_t shmem = (size_t)a.dup * sizeof(T);
+    auto kern = k_hxv_col<T, MODE, RPT>;
+    if (shmem > 64 * 1024)
+        DMED_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)shmem));
+    hipLaunchKernelGGL(kern, dim3(col_grid(s)), dim3(1024), shmem, st, a.x, a.y, a.dup, a.ddw,
+                       a.nhu, a.nhd, a.nimp, a.ncoef, a.hu, a.coef, a.hd_idx, a.hd_val, a.du, a.dd,
+                       a.impu, a.impd, a.W, a.vp, a.cs, a.partials);
+    DMED_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T, int MODE>
+static int launch_col_rpt(int rpt, const dmed_sector *s, const void *x, void *y, const void *vp,
+                          ChainScalars *cs, double *partials, hipStream_t st)
+{
+    switch (rpt) {
+#define DMED_RPT(R) \
+    case R: return launch_col_t<T, MODE, R>(s, x, y, vp, cs, partials, st);
+        DMED_RPT(2) DMED_RPT(3) DMED_RPT(4) DMED_RPT(5) DMED_RPT(6) DMED_RPT(7) DMED_RPT(8)
+        DMED_RPT(9) DMED_RPT(10) DMED_RPT(11) DMED_RPT(12) DMED_RPT(13)
+#undef DMED_RPT
+    case 14: case 15: case 16:
+        if constexpr (sizeof(T) == 4) {
+            if (rpt == 14) return launch_col_t<T, MODE, 14>(s, x, y, vp, cs, partials, st);
+            if (rpt == 15) return launch_col_t<T, MODE, 15>(s, x, y, vp, cs, partials, st);
+            return launch_col_t<T, MODE, 16>(s, x, y, vp, cs, partials, st);
+        }
+        break;
+    default:
+        break;
+    }
+    return fail(DMED_EINVAL, "column kernel: unsupported rows per lane");
+}
+
 struct LaunchShape {
     int cb, bs;
     bool lds;
@@ -202,6 +438,16 @@ int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
                hipStream_t st)
 {
     if (s->dim == 0) return 0;
+    if (mode == kLanczos && partials == nullptr) return fail(DMED_EINVAL, "no partials");
+    if (use_col_kernel(s, prec)) {
+        if (nparts) *nparts = (int)col_grid(s);
+        const int rpt = (int)((s->up.dim + 1023) / 1024);
+        if (prec == DMED_FP32)
+            return mode == kPlain ? launch_col_rpt<float, kPlain>(rpt, s, x, y, vp, cs, partials, st)
+                                  : launch_col_rpt<float, kLanczos>(rpt, s, x, y, vp, cs, partials, st);
+        return mode == kPlain ? launch_col_rpt<double, kPlain>(rpt, s, x, y, vp, cs, partials, st)
+                              : launch_col_rpt<double, kLanczos>(rpt, s, x, y, vp, cs, partials, st);
+    }
     const LaunchShape sh = pick_shape(s, prec);
     if (nparts) *nparts = (int)sh.grid;
     if (mode == kLanczos && partials == nullptr) return fail(DMED_EINVAL, "no partials");
@@ -218,6 +464,10 @@ int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
                   : launch_t<double, kLanczos, false>(s, x, y, vp, cs, partials, sh, st);
 }
 
-int hxv_nparts(const dmed_sector *s, int prec) { return (int)pick_shape(s, prec).grid; }
+int hxv_nparts(const dmed_sector *s, int prec)
+{
+    if (use_col_kernel(s, prec)) return (int)col_grid(s);
+    return (int)pick_shape(s, prec).grid;
+}
 
 }  // namespace dmed

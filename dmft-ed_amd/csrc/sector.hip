@@ -16,6 +16,7 @@
 #include "dmed_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace dmed {
@@ -130,6 +131,8 @@ static void release_spin(SpinTables &t)
     hipFree(t.hidx);
     hipFree(t.hval);
     hipFree(t.hval32);
+    hipFree(t.hpk);
+    hipFree(t.coef);
     t = SpinTables();
 }
 
@@ -236,6 +239,32 @@ static int build_spin(const Model &m, int sp, int n, SpinTables &t, hipStream_t 
     DMED_HIP(hipStreamSynchronize(st));
     hipFree(d_h);
     hipFree(d_i);
+    // packed hop words: target rank (24 bits), index of |coefficient| (7 bits), sign
+    std::vector<double> coef;
+    std::vector<uint32_t> pk((size_t)t.nhop * dim);
+    bool ok = dim < (1 << 24);
+    for (size_t q = 0; q < pk.size() && ok; ++q) {
+        const double a = std::fabs(t.h_hval[q]);
+        int id = -1;
+        for (size_t u = 0; u < coef.size(); ++u)
+            if (coef[u] == a) { id = (int)u; break; }
+        if (id < 0) {
+            if (coef.size() >= 128) { ok = false; break; }
+            coef.push_back(a);
+            id = (int)coef.size() - 1;
+        }
+        pk[q] = ((uint32_t)t.h_hidx[q] << 8) | ((uint32_t)id << 1) | (t.h_hval[q] < 0 ? 1u : 0u);
+    }
+    if (ok) {
+        if (coef.empty()) coef.push_back(0.0);
+        t.ncoef = (int)coef.size();
+        DMED_HIP(hipMalloc(&t.hpk, std::max<size_t>(pk.size(), 1) * 4));
+        DMED_HIP(hipMalloc(&t.coef, coef.size() * 8));
+        if (!pk.empty())
+            DMED_HIP(hipMemcpy(t.hpk, pk.data(), pk.size() * 4, hipMemcpyHostToDevice));
+        DMED_HIP(hipMemcpy(t.coef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice));
+        t.packed = true;
+    }
     return 0;
 }
 
