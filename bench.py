@@ -58,6 +58,43 @@ def cpu_baseline(cfg, sample_rows, threads):
     return sample_rows / dt, dt
 
 
+def solve_compare():
+    """Workload 4: Ns=12, all 169 sectors, beta=50, Boltzmann-weighted states.
+    Plain FP64 (serial sectors and chains) against the batched mixed form
+    (concurrent per-stream workers, FP32 chains with FP64 reductions)."""
+    import numpy as np
+    import dmed
+    from dmed import configs
+    c = configs.config(4)
+    dmed.set_model(c["norb"], c["nbath"], c["nspin"], c["bath"], c["hloc"], c["uloc"],
+                   c["ust"], c["jh"], 0.0, 0.0, c["xmu"], True, False)
+    kw = dict(beta=c["beta"], lmats=c["lmats"], lreal=1000, lanc_ngfiter=200,
+              lanc_nstates_sector=4, lanc_nstates_total=16)
+    out = {}
+    res = {}
+    for name, extra in (("fp64_plain", dict(batched=0, prec_gf=dmed.FP64)),
+                        ("mixed_batched", dict(batched=1, prec_gf=dmed.FP32))):
+        t0 = time.perf_counter()
+        r = dmed.solve(1, 1, 12, dmed.default_params(**kw, **extra))
+        dt = time.perf_counter() - t0
+        res[name] = r
+        out[name] = {"seconds": dt, "t_diag": r["t_diag"], "t_gf": r["t_gf"],
+                     "nstates": r["nstates"]}
+        log(f"solve {name}: {dt:.2f}s (diag {r['t_diag']:.2f}s, gf {r['t_gf']:.2f}s), "
+            f"{r['nstates']} states")
+    a64, amx = res["fp64_plain"], res["mixed_batched"]
+    out["speedup"] = out["fp64_plain"]["seconds"] / out["mixed_batched"]["seconds"]
+    out["errors_vs_fp64"] = {
+        "E0_rel": abs(amx["egs"] - a64["egs"]) / abs(a64["egs"]),
+        "dens": float(np.abs(amx["dens"] - a64["dens"]).max()),
+        "docc": float(np.abs(amx["docc"] - a64["docc"]).max()),
+        "G_iw": float(np.abs(amx["gmats"] - a64["gmats"]).max()),
+        "Sigma_iw": float(np.abs(amx["smats"] - a64["smats"]).max()),
+    }
+    log(f"mixed/batched speedup {out['speedup']:.2f}x, errors {out['errors_vs_fp64']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +104,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=2_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--solve", action="store_true",
+                    help="also time the all-sector Ns=12 finite-T solve: plain FP64 vs "
+                         "batched mixed precision")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -126,6 +166,9 @@ def main():
         log(f"FP32 H*v: kernel {k32.value:.4f} ms, {ach32:.1f} GB/s algorithmic")
         x32.free()
         y32.free()
+
+    if a.solve:
+        extra["solve_ns12_all_sectors"] = solve_compare()
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hxv_ns16_traffic.json")

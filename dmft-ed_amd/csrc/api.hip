@@ -23,24 +23,69 @@ Context &ctx()
     return c;
 }
 
+static thread_local Exec *t_exec = nullptr;
+static thread_local std::string t_error;
+
+Exec &ex() { return t_exec ? *t_exec : ctx().exec; }
+void bind_exec(Exec *e) { t_exec = e; }
+
 int fail(int code, const std::string &msg)
 {
-    ctx().last_error = msg;
+    if (t_exec) t_error = msg;          // worker threads keep their own message
+    else ctx().last_error = msg;
     return code;
+}
+
+int exec_create(Exec &e)
+{
+    DMED_HIP(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    DMED_HIP(hipMalloc(&e.dscal, kScalCap * sizeof(double)));
+    DMED_HIP(hipMemsetAsync(e.dscal, 0, kScalCap * sizeof(double), e.stream));
+    DMED_HIP(hipMalloc(&e.partials, (1 << 16) * sizeof(double)));
+    e.partials_cap = 1 << 16;
+    return 0;
+}
+
+void exec_destroy(Exec &e)
+{
+    if (e.stream) hipStreamSynchronize(e.stream);
+    hipFree(e.partials);
+    hipFree(e.dscal);
+    for (int i = 0; i < 12; ++i) hipFree(e.buf[i]);
+    if (e.stream) hipStreamDestroy(e.stream);
+    e = Exec();
+}
+
+int exec_buf(Exec &e, int slot, size_t bytes, void **p)
+{
+    if (slot < 0 || slot >= 12) return fail(DMED_EINVAL, "bad buffer slot");
+    bytes = std::max<size_t>(bytes, 256);
+    if (bytes > e.buf_cap[slot]) {
+        if (e.buf[slot]) {
+            hipStreamSynchronize(e.stream);
+            hipFree(e.buf[slot]);
+        }
+        e.buf[slot] = nullptr;
+        e.buf_cap[slot] = 0;
+        DMED_HIP(hipMalloc(&e.buf[slot], bytes));
+        e.buf_cap[slot] = bytes;
+    }
+    *p = e.buf[slot];
+    return 0;
 }
 
 int ensure_partials(size_t n)
 {
-    Context &c = ctx();
-    if (n <= c.partials_cap) return 0;
-    if (c.partials) {
-        hipStreamSynchronize(c.stream);
-        hipFree(c.partials);
+    Exec &e = ex();
+    if (n <= e.partials_cap) return 0;
+    if (e.partials) {
+        hipStreamSynchronize(e.stream);
+        hipFree(e.partials);
     }
-    c.partials = nullptr;
-    c.partials_cap = 0;
-    DMED_HIP(hipMalloc(&c.partials, n * sizeof(double)));
-    c.partials_cap = n;
+    e.partials = nullptr;
+    e.partials_cap = 0;
+    DMED_HIP(hipMalloc(&e.partials, n * sizeof(double)));
+    e.partials_cap = n;
     return 0;
 }
 
@@ -78,11 +123,9 @@ int dmed_init(int device)
     c.lds_max = std::max<size_t>(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
     if (c.lds_max == 0) c.lds_max = 65536;
     if (c.lds_max > 160 * 1024) c.lds_max = 160 * 1024;
-    DMED_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    DMED_HIP(hipMalloc(&c.dscal, kScalCap * sizeof(double)));
-    DMED_HIP(hipMemset(c.dscal, 0, kScalCap * sizeof(double)));
+    if (int rc = exec_create(c.exec)) return rc;
+    DMED_HIP(hipStreamSynchronize(c.exec.stream));
     c.initialized = true;
-    if (int rc = ensure_partials(1 << 16)) return rc;
     return 0;
 }
 
@@ -90,20 +133,13 @@ int dmed_finalize(void)
 {
     Context &c = ctx();
     if (!c.initialized) return 0;
-    hipStreamSynchronize(c.stream);
-    hipFree(c.partials);
-    hipFree(c.dscal);
-    hipStreamDestroy(c.stream);
-    c.partials = nullptr;
-    c.partials_cap = 0;
-    c.dscal = nullptr;
-    c.stream = nullptr;
+    exec_destroy(c.exec);
     c.initialized = false;
     c.has_model = false;
     return 0;
 }
 
-const char *dmed_last_error(void) { return ctx().last_error.c_str(); }
+const char *dmed_last_error(void) { return t_exec ? t_error.c_str() : ctx().last_error.c_str(); }
 
 int dmed_set_model(const dmed_model *m)
 {
@@ -178,7 +214,7 @@ int dmed_sector_create(int32_t nup, int32_t ndw, dmed_sector **out)
 int dmed_sector_free(dmed_sector *s)
 {
     if (!s) return 0;
-    if (ctx().initialized) hipStreamSynchronize(ctx().stream);
+    if (ctx().initialized) hipStreamSynchronize(ex().stream);
     sector_release(s);
     delete s;
     return 0;
@@ -216,7 +252,7 @@ int dmed_vec_create(const dmed_sector *s, int32_t prec, dmed_vec **out)
     v->dim = s->dim;
     const size_t b = std::max<int64_t>(s->dim, 1) * (prec == DMED_FP32 ? 4 : 8);
     DMED_HIP(hipMalloc(&v->data, b));
-    DMED_HIP(hipMemsetAsync(v->data, 0, b, ctx().stream));
+    DMED_HIP(hipMemsetAsync(v->data, 0, b, ex().stream));
     *out = v.release();
     return 0;
 }
@@ -224,7 +260,7 @@ int dmed_vec_create(const dmed_sector *s, int32_t prec, dmed_vec **out)
 int dmed_vec_free(dmed_vec *v)
 {
     if (!v) return 0;
-    if (ctx().initialized) hipStreamSynchronize(ctx().stream);
+    if (ctx().initialized) hipStreamSynchronize(ex().stream);
     hipFree(v->data);
     delete v;
     return 0;
@@ -236,7 +272,7 @@ int dmed_vec_upload(dmed_vec *v, const double *host)
 {
     REQUIRE_INIT();
     if (!v || !host) return fail(DMED_EINVAL, "null argument");
-    hipStream_t st = ctx().stream;
+    hipStream_t st = ex().stream;
     if (v->prec == DMED_FP64) {
         DMED_HIP(hipMemcpyAsync(v->data, host, v->dim * 8, hipMemcpyHostToDevice, st));
     } else {
@@ -256,7 +292,7 @@ int dmed_vec_download(const dmed_vec *v, double *host)
 {
     REQUIRE_INIT();
     if (!v || !host) return fail(DMED_EINVAL, "null argument");
-    hipStream_t st = ctx().stream;
+    hipStream_t st = ex().stream;
     if (v->prec == DMED_FP64) {
         DMED_HIP(hipMemcpyAsync(host, v->data, v->dim * 8, hipMemcpyDeviceToHost, st));
     } else {
@@ -286,13 +322,13 @@ int dmed_hxv_async(const dmed_sector *s, const void *x, void *y, int32_t prec)
     if (int rc = check_sector(s)) return rc;
     if (!x || !y || x == y) return fail(DMED_EINVAL, "x and y must be distinct device buffers");
     if (prec != DMED_FP64 && prec != DMED_FP32) return fail(DMED_EINVAL, "bad precision");
-    return launch_hxv(s, x, y, prec, kPlain, nullptr, nullptr, nullptr, nullptr, ctx().stream);
+    return launch_hxv(s, x, y, prec, kPlain, nullptr, nullptr, nullptr, nullptr, ex().stream);
 }
 
 int dmed_hxv(const dmed_sector *s, const void *x, void *y, int32_t prec)
 {
     if (int rc = dmed_hxv_async(s, x, y, prec)) return rc;
-    DMED_HIP(hipStreamSynchronize(ctx().stream));
+    DMED_HIP(hipStreamSynchronize(ex().stream));
     return 0;
 }
 
@@ -302,7 +338,7 @@ int dmed_hxv_timed(const dmed_sector *s, const void *x, void *y, int32_t prec, i
     REQUIRE_MODEL();
     if (int rc = check_sector(s)) return rc;
     if (!x || !y || x == y || nrep < 1 || !ms_avg) return fail(DMED_EINVAL, "bad argument");
-    hipStream_t st = ctx().stream;
+    hipStream_t st = ex().stream;
     hipEvent_t e0, e1;
     DMED_HIP(hipEventCreate(&e0));
     DMED_HIP(hipEventCreate(&e1));
@@ -323,7 +359,7 @@ int dmed_hxv_timed(const dmed_sector *s, const void *x, void *y, int32_t prec, i
 int dmed_sync(void)
 {
     REQUIRE_INIT();
-    DMED_HIP(hipStreamSynchronize(ctx().stream));
+    DMED_HIP(hipStreamSynchronize(ex().stream));
     return 0;
 }
 
@@ -467,46 +503,45 @@ extern "C" int dmed_apply_op(const dmed_vec *psi, int32_t isite, int32_t dagger,
     if (out->sec != dst || out->prec != psi->prec)
         return fail(DMED_EINVAL, "output vector must belong to dst with psi's precision");
     Context &c = ctx();
-    hipStream_t st = c.stream;
+    hipStream_t st = ex().stream;
     std::vector<int32_t> src;
     std::vector<int8_t> sgn;
     op_tables(up ? I->up : I->dw, up ? dst->up : dst->dw, p, dagger, src, sgn);
     int32_t *dsrc = nullptr;
-    DMED_HIP(hipMalloc(&dsrc, std::max<size_t>(src.size(), 1) * 5));
+    if (int rc = exec_buf(ex(), 8, std::max<size_t>(src.size(), 1) * 5, (void **)&dsrc)) return rc;
     int8_t *dsgn = reinterpret_cast<int8_t *>(dsrc + std::max<size_t>(src.size(), 1));
     DMED_HIP(hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, st));
     DMED_HIP(hipMemcpyAsync(dsgn, sgn.data(), sgn.size(), hipMemcpyHostToDevice, st));
     const int64_t n = dst->dim;
     const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256));
-    if (int rc = ensure_partials(4096)) { hipFree(dsrc); return rc; }
+    if (int rc = ensure_partials(4096)) return rc;
     // the down-spin operator passes all up electrons: global sign (-1)^nup
     const double gsign = (I->nup & 1) ? -1.0 : 1.0;
     if (psi->prec == DMED_FP64) {
         if (up)
             hipLaunchKernelGGL(k_apply_up<double>, dim3(g), dim3(256), 0, st, (const double *)psi->data,
                                I->up.dim, (double *)out->data, dst->up.dim, dst->dw.dim, dsrc, dsgn,
-                               c.partials);
+                               ex().partials);
         else
             hipLaunchKernelGGL(k_apply_dw<double>, dim3(g), dim3(256), 0, st, (const double *)psi->data,
                                I->up.dim, (double *)out->data, dst->dw.dim, dsrc, dsgn, gsign,
-                               c.partials);
+                               ex().partials);
     } else {
         if (up)
             hipLaunchKernelGGL(k_apply_up<float>, dim3(g), dim3(256), 0, st, (const float *)psi->data,
                                I->up.dim, (float *)out->data, dst->up.dim, dst->dw.dim, dsrc, dsgn,
-                               c.partials);
+                               ex().partials);
         else
             hipLaunchKernelGGL(k_apply_dw<float>, dim3(g), dim3(256), 0, st, (const float *)psi->data,
                                I->up.dim, (float *)out->data, dst->dw.dim, dsrc, dsgn, gsign,
-                               c.partials);
+                               ex().partials);
     }
     DMED_HIP(hipGetLastError());
-    double *d = c.dscal + 10;
-    if (int rc = launch_reduce_to(c.partials, (int)g, d, 0, st)) { hipFree(dsrc); return rc; }
+    double *d = ex().dscal + 10;
+    if (int rc = launch_reduce_to(ex().partials, (int)g, d, 0, st)) return rc;
     double h = 0.0;
     DMED_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, st));
     DMED_HIP(hipStreamSynchronize(st));
-    hipFree(dsrc);
     if (norm2) *norm2 = h;
     return 0;
 }
@@ -585,18 +620,18 @@ extern "C" int dmed_observables(const dmed_vec *psi, double peso, double *dens, 
     const dmed_sector *s = psi->sec;
     if (int rc = check_sector(s)) return rc;
     Context &c = ctx();
-    hipStream_t st = c.stream;
+    hipStream_t st = ex().stream;
     const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (s->dim + 255) / 256));
     if (int rc = ensure_partials(6 * 1024)) return rc;
     if (psi->prec == DMED_FP64)
         hipLaunchKernelGGL(k_obs<double>, dim3(g), dim3(256), 0, st, (const double *)psi->data,
-                           s->up.dim, s->dw.dim, s->up.imp, s->dw.imp, s->norb, c.partials);
+                           s->up.dim, s->dw.dim, s->up.imp, s->dw.imp, s->norb, ex().partials);
     else
         hipLaunchKernelGGL(k_obs<float>, dim3(g), dim3(256), 0, st, (const float *)psi->data,
-                           s->up.dim, s->dw.dim, s->up.imp, s->dw.imp, s->norb, c.partials);
+                           s->up.dim, s->dw.dim, s->up.imp, s->dw.imp, s->norb, ex().partials);
     DMED_HIP(hipGetLastError());
     std::vector<double> h((size_t)2 * s->norb * g);
-    DMED_HIP(hipMemcpyAsync(h.data(), c.partials, h.size() * 8, hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipMemcpyAsync(h.data(), ex().partials, h.size() * 8, hipMemcpyDeviceToHost, st));
     DMED_HIP(hipStreamSynchronize(st));
     for (int a = 0; a < s->norb; ++a) {
         double d1 = 0.0, d2 = 0.0;

@@ -93,24 +93,40 @@ struct dmed_vec {
 namespace dmed {
 
 // ------------------------------------------------------------- context
+constexpr int kScalCap = 4096;
+
+// An execution context: one HIP stream with its own reduction scratch,
+// device scalars (Lanczos chain state) and grow-only work buffers.  The
+// library's calls run on the context bound to the calling host thread
+// (the default one unless a worker bound its own), so independent sectors
+// and chains can run concurrently on separate streams.
+struct Exec {
+    hipStream_t stream = nullptr;
+    double *partials = nullptr;
+    size_t partials_cap = 0;            // in doubles
+    double *dscal = nullptr;            // kScalCap doubles
+    void *buf[12] = {};
+    size_t buf_cap[12] = {};
+};
+int exec_create(Exec &e);
+void exec_destroy(Exec &e);
+Exec &ex();                             // context of the calling thread
+void bind_exec(Exec *e);                // nullptr: back to the default context
+int exec_buf(Exec &e, int slot, size_t bytes, void **p);
+
 struct Context {
     bool initialized = false;
     int device = 0;
     int num_cu = 256;
     size_t lds_max = 160 * 1024;
-    hipStream_t stream = nullptr;
+    Exec exec;                          // default execution context
     Model model;
     bool has_model = false;
-    // scratch: block partial sums and small device state
-    double *partials = nullptr;
-    size_t partials_cap = 0;            // in doubles
-    double *dscal = nullptr;            // device scalars (chain state), kScalCap doubles
     std::string last_error;
 };
 Context &ctx();
-constexpr int kScalCap = 4096;
 
-int ensure_partials(size_t n);
+int ensure_partials(size_t n);          // on ex()
 
 // ------------------------------------------------------------- sectors
 int sector_build(dmed_sector *s, int nup, int ndw);

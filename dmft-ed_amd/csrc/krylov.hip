@@ -331,10 +331,10 @@ struct DevBuf {
     ~DevBuf() { if (p) hipFree(p); }
 };
 
-ChainScalars *chain_scalars() { return reinterpret_cast<ChainScalars *>(ctx().dscal); }
-double *chain_alpha() { return ctx().dscal + 16; }
-double *chain_beta() { return ctx().dscal + 16 + (kScalCap - 16) / 3; }
-double *chain_aux() { return ctx().dscal + 16 + 2 * ((kScalCap - 16) / 3); }
+ChainScalars *chain_scalars() { return reinterpret_cast<ChainScalars *>(ex().dscal); }
+double *chain_alpha() { return ex().dscal + 16; }
+double *chain_beta() { return ex().dscal + 16 + (kScalCap - 16) / 3; }
+double *chain_aux() { return ex().dscal + 16 + 2 * ((kScalCap - 16) / 3); }
 constexpr int kChainMax = (kScalCap - 16) / 3;
 
 // Chain on three rotated work vectors bufs = {v_prev, v, w}; bufs[1] holds
@@ -344,14 +344,13 @@ constexpr int kChainMax = (kScalCap - 16) / 3;
 // pass of the ground-state Lanczos), z read from chain_aux().
 int chain_init(const dmed_sector *s, void *bufs[3], int prec, int nlanc_total, double thr)
 {
-    Context &c = ctx();
-    hipStream_t st = c.stream;
+        hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     const size_t esz = prec == DMED_FP32 ? 4 : 8;
     int np = 0;
     DMED_HIP(hipMemsetAsync(bufs[0], 0, n * esz, st));
-    if (int rc = launch_dot(bufs[1], bufs[1], n, prec, c.partials, &np, st)) return rc;
-    hipLaunchKernelGGL(k_chain_init, dim3(1), dim3(1024), 0, st, c.partials, np, chain_scalars(),
+    if (int rc = launch_dot(bufs[1], bufs[1], n, prec, ex().partials, &np, st)) return rc;
+    hipLaunchKernelGGL(k_chain_init, dim3(1), dim3(1024), 0, st, ex().partials, np, chain_scalars(),
                        thr, nlanc_total);
     DMED_HIP(hipGetLastError());
     return 0;
@@ -359,8 +358,7 @@ int chain_init(const dmed_sector *s, void *bufs[3], int prec, int nlanc_total, d
 
 int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double *psi)
 {
-    Context &c = ctx();
-    hipStream_t st = c.stream;
+        hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     ChainScalars *cs = chain_scalars();
     const int nph = hxv_nparts(s, prec);
@@ -376,19 +374,19 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
                                    (const double *)v, n, cs, chain_aux());
             DMED_HIP(hipGetLastError());
         }
-        if (int rc = launch_hxv(s, v, w, prec, kLanczos, vp, cs, c.partials, nullptr, st))
+        if (int rc = launch_hxv(s, v, w, prec, kLanczos, vp, cs, ex().partials, nullptr, st))
             return rc;
-        hipLaunchKernelGGL(k_chain_alpha, dim3(1), dim3(1024), 0, st, c.partials, nph, cs,
+        hipLaunchKernelGGL(k_chain_alpha, dim3(1), dim3(1024), 0, st, ex().partials, nph, cs,
                            chain_alpha());
         DMED_HIP(hipGetLastError());
         if (prec == DMED_FP32)
             hipLaunchKernelGGL(k_chain_update<float>, dim3(g), dim3(kVecBS), 0, st, (float *)w,
-                               (const float *)v, n, cs, c.partials);
+                               (const float *)v, n, cs, ex().partials);
         else
             hipLaunchKernelGGL(k_chain_update<double>, dim3(g), dim3(kVecBS), 0, st, (double *)w,
-                               (const double *)v, n, cs, c.partials);
+                               (const double *)v, n, cs, ex().partials);
         DMED_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_chain_beta, dim3(1), dim3(1024), 0, st, c.partials, (int)g, cs,
+        hipLaunchKernelGGL(k_chain_beta, dim3(1), dim3(1024), 0, st, ex().partials, (int)g, cs,
                            chain_beta());
         DMED_HIP(hipGetLastError());
         void *t = vp;
@@ -407,17 +405,16 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
 int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
               double threshold, double *alpha, double *beta, int *nsteps)
 {
-    Context &c = ctx();
-    hipStream_t st = c.stream;
+        hipStream_t st = ex().stream;
     if (nlanc <= 0) return fail(DMED_EINVAL, "nlanc must be positive");
     if (nlanc > kChainMax) return fail(DMED_EINVAL, "nlanc exceeds the chain buffer");
     const int64_t n = s->dim;
     const size_t esz = prec == DMED_FP32 ? 4 : 8;
     if (int rc = ensure_partials(std::max(hxv_nparts(s, prec), kVecGrid) + 16)) return rc;
-    DevBuf b0, b1, b2;
-    DMED_HIP(hipMalloc(&b0.p, std::max<int64_t>(n, 1) * esz));
-    DMED_HIP(hipMalloc(&b1.p, std::max<int64_t>(n, 1) * esz));
-    DMED_HIP(hipMalloc(&b2.p, std::max<int64_t>(n, 1) * esz));
+    struct { void *p; } b0, b1, b2;
+    if (int rc = exec_buf(ex(), 0, std::max<int64_t>(n, 1) * esz, &b0.p)) return rc;
+    if (int rc = exec_buf(ex(), 1, std::max<int64_t>(n, 1) * esz, &b1.p)) return rc;
+    if (int rc = exec_buf(ex(), 2, std::max<int64_t>(n, 1) * esz, &b2.p)) return rc;
     if (int rc = launch_scale_copy(b1.p, prec, v0, v0prec, n, 1.0, st)) return rc;
     DMED_HIP(hipMemsetAsync(chain_alpha(), 0, kChainMax * sizeof(double), st));
     DMED_HIP(hipMemsetAsync(chain_beta(), 0, kChainMax * sizeof(double), st));
@@ -495,16 +492,15 @@ int dense_eigh(const dmed_sector *s, int neig, double *evals, dmed_vec **evecs, 
 // explicit residual ||H y - theta y|| for a device FP64 vector y
 int residual(const dmed_sector *s, const double *y, double theta, double *work, double *out)
 {
-    Context &c = ctx();
-    hipStream_t st = c.stream;
+        hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     if (int rc = launch_hxv(s, y, work, DMED_FP64, kPlain, nullptr, nullptr, nullptr, nullptr, st))
         return rc;
     if (int rc = launch_axpy(work, DMED_FP64, y, DMED_FP64, n, -theta, st)) return rc;
     int np = 0;
-    if (int rc = launch_dot(work, work, n, DMED_FP64, c.partials, &np, st)) return rc;
-    double *d = ctx().dscal + 8;
-    if (int rc = launch_reduce_to(c.partials, np, d, 1, st)) return rc;
+    if (int rc = launch_dot(work, work, n, DMED_FP64, ex().partials, &np, st)) return rc;
+    double *d = ex().dscal + 8;
+    if (int rc = launch_reduce_to(ex().partials, np, d, 1, st)) return rc;
     DMED_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, st));
     DMED_HIP(hipStreamSynchronize(st));
     return 0;
@@ -512,13 +508,12 @@ int residual(const dmed_sector *s, const double *y, double theta, double *work, 
 
 int dot_host(const double *x, const double *y, int64_t n, double *out)
 {
-    Context &c = ctx();
-    int np = 0;
-    if (int rc = launch_dot(x, y, n, DMED_FP64, c.partials, &np, c.stream)) return rc;
-    double *d = ctx().dscal + 9;
-    if (int rc = launch_reduce_to(c.partials, np, d, 0, c.stream)) return rc;
-    DMED_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    DMED_HIP(hipStreamSynchronize(c.stream));
+        int np = 0;
+    if (int rc = launch_dot(x, y, n, DMED_FP64, ex().partials, &np, ex().stream)) return rc;
+    double *d = ex().dscal + 9;
+    if (int rc = launch_reduce_to(ex().partials, np, d, 0, ex().stream)) return rc;
+    DMED_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, ex().stream));
+    DMED_HIP(hipStreamSynchronize(ex().stream));
     return 0;
 }
 
@@ -531,13 +526,12 @@ int dot_host(const double *x, const double *y, int64_t n, double *out)
 int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec **evec,
              double *resid)
 {
-    Context &c = ctx();
-    hipStream_t st = c.stream;
+        hipStream_t st = ex().stream;
     const int64_t n = s->dim;
-    DevBuf b0, b1, b2;
-    DMED_HIP(hipMalloc(&b0.p, n * 8));
-    DMED_HIP(hipMalloc(&b1.p, n * 8));
-    DMED_HIP(hipMalloc(&b2.p, n * 8));
+    struct { void *p; } b0, b1, b2;
+    if (int rc = exec_buf(ex(), 0, n * 8, &b0.p)) return rc;
+    if (int rc = exec_buf(ex(), 1, n * 8, &b1.p)) return rc;
+    if (int rc = exec_buf(ex(), 2, n * 8, &b2.p)) return rc;
     double *psi = nullptr;
     DMED_HIP(hipMalloc(&psi, n * 8));
     dmed_vec *out = new dmed_vec{s, DMED_FP64, n, psi};
@@ -617,8 +611,7 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
 int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const double *Y,
             int nl, uint64_t seed, double *theta_out, double *X, double *resid_out, int *used_out)
 {
-    Context &c = ctx();
-    hipStream_t st = c.stream;
+        hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     const int64_t avail = n - nl;                 // dimension of the complement
     if (avail < neig) return fail(DMED_EINVAL, "complement smaller than neig");
@@ -627,23 +620,17 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
     m = (int)std::min<int64_t>(m, avail);
     const bool full = (int64_t)m == avail;
     double *V = nullptr, *w = nullptr, *dS = nullptr;
-    DMED_HIP(hipMalloc(&V, (size_t)(m + 1) * n * 8));
-    DevBuf gv;
-    gv.p = V;
-    DMED_HIP(hipMalloc(&w, n * 8));
-    DevBuf gw;
-    gw.p = w;
-    DMED_HIP(hipMalloc(&dS, (size_t)m * m * 8 + (size_t)(m + nl + 1) * 8));
-    DevBuf gs;
-    gs.p = dS;
+    if (int rc = exec_buf(ex(), 3, (size_t)(m + 1) * n * 8, (void **)&V)) return rc;
+    if (int rc = exec_buf(ex(), 4, n * 8, (void **)&w)) return rc;
+    if (int rc = exec_buf(ex(), 5, (size_t)m * m * 8 + (size_t)(m + nl + 1) * 8, (void **)&dS)) return rc;
     double *dc = dS + (size_t)m * m;
     const unsigned g = vec_grid(n);
     if (int rc = ensure_partials((size_t)(std::max(m, nl) + 1) * g + 16)) return rc;
     // project out the columns [0,jmax) of B from w
     auto project = [&](const double *B, int jmax, double *cacc) -> int {
         if (jmax <= 0) return 0;
-        hipLaunchKernelGGL(k_multidot, dim3(g, jmax), dim3(kVecBS), 0, st, B, n, jmax, w, c.partials);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(jmax), dim3(1024), 0, st, c.partials, (int)g, dc);
+        hipLaunchKernelGGL(k_multidot, dim3(g, jmax), dim3(kVecBS), 0, st, B, n, jmax, w, ex().partials);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(jmax), dim3(1024), 0, st, ex().partials, (int)g, dc);
         DMED_HIP(hipGetLastError());
         if (cacc) {
             std::vector<double> cc(jmax);
@@ -748,17 +735,12 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
 int trl_deflated(const dmed_sector *s, int neig, int m, int maxit, double tol, double degen,
                  bool lowest_multiplet, int *nout, double *evals, dmed_vec **evecs, double *resid)
 {
-    Context &c = ctx();
-    hipStream_t st = c.stream;
+        hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     double *Y = nullptr, *X = nullptr;
-    DMED_HIP(hipMalloc(&Y, (size_t)neig * n * 8));
-    DevBuf gy;
-    gy.p = Y;
+    if (int rc = exec_buf(ex(), 6, (size_t)neig * n * 8, (void **)&Y)) return rc;
     const int want = lowest_multiplet ? 1 : neig;
-    DMED_HIP(hipMalloc(&X, (size_t)want * n * 8));
-    DevBuf gx;
-    gx.p = X;
+    if (int rc = exec_buf(ex(), 7, (size_t)want * n * 8, (void **)&X)) return rc;
     struct Kept { double e, r; int slot; };
     std::vector<Kept> kept;
     std::vector<double> th(want), rr(want);

@@ -313,8 +313,8 @@ int sector_build(dmed_sector *s, int nup, int ndw)
     s->ns = m.ns;
     s->norb = m.norb;
     s->model_generation = m.generation;
-    if (int rc = build_spin(m, 0, nup, s->up, c.stream)) return rc;
-    if (int rc = build_spin(m, 1, ndw, s->dw, c.stream)) return rc;
+    if (int rc = build_spin(m, 0, nup, s->up, ex().stream)) return rc;
+    if (int rc = build_spin(m, 1, ndw, s->dw, ex().stream)) return rc;
     s->dim = s->up.dim * s->dw.dim;
     s->nimp = 1 << m.norb;
     s->h_W.resize(s->nimp * s->nimp);

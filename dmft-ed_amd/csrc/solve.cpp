@@ -6,7 +6,12 @@
 //                          get_sigma_normal (get_sigma_normal.f90:1-38)
 // The state list keeps device vectors (handles), never host copies of them.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cstdlib>
+#include <functional>
+#include <string>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -104,6 +109,45 @@ inline size_t gidx(int s1, int s2, int a, int b, int l, int nspin, int norb)
     return (size_t)s1 + nspin * ((size_t)s2 + nspin * ((size_t)a + norb * ((size_t)b + (size_t)norb * l)));
 }
 
+// Run ntask tasks: serially on the calling thread, or (batched) on a pool of
+// host workers, each bound to its own stream/scratch context, taking tasks
+// from a shared counter.  Task results must not depend on which worker ran
+// them.
+int run_tasks(int batched, int ntask, const std::function<void(int)> &fn)
+{
+    if (!batched || ntask <= 1) {
+        for (int k = 0; k < ntask; ++k) fn(k);
+        return 0;
+    }
+    int nw = 8;
+    if (const char *e = std::getenv("DMED_WORKERS")) nw = std::max(1, std::atoi(e));
+    nw = std::min(nw, ntask);
+    std::vector<Exec> execs(nw);
+    for (auto &e : execs)
+        if (int rc = exec_create(e)) {
+            for (auto &x : execs) exec_destroy(x);
+            return rc;
+        }
+    std::atomic<int> next(0);
+    const int dev = ctx().device;
+    std::vector<std::thread> th;
+    for (int w = 0; w < nw; ++w)
+        th.emplace_back([&, w]() {
+            hipSetDevice(dev);
+            bind_exec(&execs[w]);
+            for (;;) {
+                const int k = next.fetch_add(1);
+                if (k >= ntask) break;
+                fn(k);
+            }
+            hipStreamSynchronize(execs[w].stream);
+            bind_exec(nullptr);
+        });
+    for (auto &t : th) t.join();
+    for (auto &e : execs) exec_destroy(e);
+    return 0;
+}
+
 double now()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -174,49 +218,77 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     double oldzero = 1000.0;
     int rc = 0;
     // ------------------------------------------------------ ed_diag_c
+    // Eigenpairs of every sector (serially, or concurrently on per-thread
+    // streams when batched), then the state-list rules applied in the
+    // reference's sector order, so both forms keep the same states.
+    struct SectorRes {
+        int nout = 0, rc = 0;
+        std::vector<double> ev;
+        std::vector<dmed_vec *> vecs;
+        std::string err;
+    };
+    std::vector<SectorRes> sres(nsectors);
+    std::vector<dmed_sector *> secs(nsectors, nullptr);
     for (int nu = 0; nu <= ns && !rc; ++nu)
-        for (int nd = 0; nd <= ns && !rc; ++nd) {
-            const int is = nu * (ns + 1) + nd;
-            dmed_sector *sec = nullptr;
-            if ((rc = cache.get(nu, nd, ns, &sec))) break;
-            const int64_t dim = sec->dim;
-            const int ne = (int)std::min<int64_t>(dim, neigen[is]);
-            const int nblock = (int)std::min<int64_t>(dim, p->lanc_ncv_factor * ne + p->lanc_ncv_add);
-            std::vector<double> ev(ne), res(ne);
-            std::vector<dmed_vec *> vecs(ne, nullptr);
-            // T=0 keeps only states within gs_threshold of the running minimum
-            // (ED_DIAG.f90:202-213), so only the lowest level of a sector and
-            // its degenerate copies can enter; finite T needs the ne lowest.
-            int nout = 0;
-            rc = eigh_sector_ex(sec, ne, std::max(nblock, 2 * ne + 10), 50 * std::max(p->lanc_niter, 100),
-                                p->lanc_tolerance, p->lanc_dim_threshold, 10.0 * p->gs_threshold,
-                                !finite_t, &nout, ev.data(), vecs.data(), res.data());
-            if (rc == DMED_ENOCONV) rc = 0;      // ARPACK returns its best pairs too
-            if (rc) break;
-            if (p->verbose > 2)
-                std::fprintf(stderr, "sector %d (%d,%d) dim=%lld E0=%.12f\n", is + 1, nu, nd,
-                             (long long)dim, ev[0]);
-            bool used = false;
-            for (int i = 0; i < nout; ++i) {
-                State st{ev[i], is, vecs[i]};
-                bool kept = false;
-                if (finite_t) {
-                    kept = sl.add(st, nst_total);
-                } else if (ev[i] < oldzero - 10.0 * p->gs_threshold) {
-                    oldzero = ev[i];
-                    sl.free_all();
-                    sl.insert(st);
-                    kept = true;
-                } else if (std::fabs(ev[i] - oldzero) <= p->gs_threshold) {
-                    oldzero = std::min(oldzero, ev[i]);
-                    sl.insert(st);
-                    kept = true;
-                }
-                if (!kept) dmed_vec_free(vecs[i]);
-                used = used || kept;
-            }
-            (void)used;
+        for (int nd = 0; nd <= ns && !rc; ++nd)
+            rc = cache.get(nu, nd, ns, &secs[nu * (ns + 1) + nd]);
+    if (rc) return rc;
+    auto solve_sector = [&](int is) {
+        dmed_sector *sec = secs[is];
+        const int64_t dim = sec->dim;
+        const int ne = (int)std::min<int64_t>(dim, neigen[is]);
+        const int nblock = (int)std::min<int64_t>(dim, p->lanc_ncv_factor * ne + p->lanc_ncv_add);
+        SectorRes &r = sres[is];
+        r.ev.assign(ne, 0.0);
+        r.vecs.assign(ne, nullptr);
+        std::vector<double> res(ne);
+        // T=0 keeps only states within gs_threshold of the running minimum
+        // (ED_DIAG.f90:202-213), so only the lowest level of a sector and its
+        // degenerate copies can enter; finite T needs the ne lowest.
+        r.rc = eigh_sector_ex(sec, ne, std::max(nblock, 2 * ne + 10), 50 * std::max(p->lanc_niter, 100),
+                              p->lanc_tolerance, p->lanc_dim_threshold, 10.0 * p->gs_threshold,
+                              !finite_t, &r.nout, r.ev.data(), r.vecs.data(), res.data());
+        if (r.rc == DMED_ENOCONV) r.rc = 0;      // ARPACK returns its best pairs too
+        if (r.rc) r.err = dmed_last_error();
+    };
+    {
+        std::vector<int> order(nsectors);
+        for (int i = 0; i < nsectors; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return secs[x]->dim > secs[y]->dim; });
+        if ((rc = run_tasks(p->batched, nsectors, [&](int k) { solve_sector(order[k]); })))
+            return rc;
+    }
+    for (int is = 0; is < nsectors && !rc; ++is) {
+        SectorRes &r = sres[is];
+        if (r.rc) {
+            rc = fail(r.rc, r.err);
+            break;
         }
+        if (p->verbose > 2 && r.nout > 0)
+            std::fprintf(stderr, "sector %d (%d,%d) dim=%lld E0=%.12f\n", is + 1, is / (ns + 1),
+                         is % (ns + 1), (long long)secs[is]->dim, r.ev[0]);
+        for (int i = 0; i < r.nout; ++i) {
+            State st{r.ev[i], is, r.vecs[i]};
+            bool kept = false;
+            if (finite_t) {
+                kept = sl.add(st, nst_total);
+            } else if (r.ev[i] < oldzero - 10.0 * p->gs_threshold) {
+                oldzero = r.ev[i];
+                sl.free_all();
+                sl.insert(st);
+                kept = true;
+            } else if (std::fabs(r.ev[i] - oldzero) <= p->gs_threshold) {
+                oldzero = std::min(oldzero, r.ev[i]);
+                sl.insert(st);
+                kept = true;
+            }
+            if (!kept) dmed_vec_free(r.vecs[i]);
+            r.vecs[i] = nullptr;
+        }
+    }
+    for (auto &r : sres)
+        for (auto *v : r.vecs) dmed_vec_free(v);
     if (rc) {
         sl.free_all();
         return rc;
@@ -266,49 +338,78 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     if (out->greal) std::fill(out->greal, out->greal + 2 * ng * lr, 0.0);
     std::vector<double> gm(2 * (size_t)lm), gr(2 * (size_t)lr);
     const int ngf = std::max(p->lanc_ngfiter, 1);
-    std::vector<double> al(ngf), bl(ngf);
+    // one task per (spin, orbital, state, particle/hole) chain, in the order
+    // of lanc_build_gf_normal_c (build_gf_normal.f90:83-176)
+    struct Chain {
+        int s, a, state, dagger, isign;
+        dmed_sector *J = nullptr;
+        int nlanc = 0, rc = 0;
+        double norm2 = 0.0;
+        std::vector<double> al, bl;
+        std::string err;
+    };
+    std::vector<Chain> chains;
+    for (int s = 0; s < nspin && !rc; ++s)
+        for (int a = 0; a < norb && !rc; ++a)
+            for (int k = 0; k < (int)sl.items.size() && !rc; ++k) {
+                const int nu = sl.items[k].sector / (ns + 1), nd = sl.items[k].sector % (ns + 1);
+                for (int pass = 0; pass < 2; ++pass) {
+                    const int dagger = pass == 0 ? 1 : 0;
+                    const int ju = nu + (s == 0 ? (dagger ? 1 : -1) : 0);
+                    const int jd = nd + (s == 1 ? (dagger ? 1 : -1) : 0);
+                    if (ju < 0 || jd < 0 || ju > ns || jd > ns) continue;
+                    Chain ch;
+                    ch.s = s;
+                    ch.a = a;
+                    ch.state = k;
+                    ch.dagger = dagger;
+                    ch.isign = dagger ? 1 : -1;
+                    if ((rc = cache.get(ju, jd, ns, &ch.J))) break;
+                    ch.nlanc = (int)std::min<int64_t>(ch.J->dim, ngf);
+                    chains.push_back(std::move(ch));
+                }
+            }
+    auto run_chain = [&](int t) {
+        Chain &ch = chains[t];
+        const State &st = sl.items[ch.state];
+        // start vector in a reusable buffer of the calling context
+        dmed_vec v{ch.J, DMED_FP64, ch.J->dim, nullptr};
+        if ((ch.rc = exec_buf(ex(), 9, std::max<int64_t>(ch.J->dim, 1) * 8, &v.data))) {
+            ch.err = dmed_last_error();
+            return;
+        }
+        const int isite = ch.a + 1 + ch.s * ns;
+        ch.rc = dmed_apply_op(st.vec, isite, ch.dagger, ch.J, &v, &ch.norm2);
+        if (!ch.rc && ch.norm2 > 0.0) {
+            // zero norm: the operator annihilates the state and the chain
+            // carries no weight (the reference divides by 0 here)
+            ch.al.assign(ch.nlanc, 0.0);
+            ch.bl.assign(ch.nlanc, 0.0);
+            int nsteps = 0;
+            ch.rc = chain_run(ch.J, v.data, DMED_FP64, ch.nlanc, p->prec_gf, 1e-13, ch.al.data(),
+                              ch.bl.data(), &nsteps);
+        }
+        if (ch.rc) ch.err = dmed_last_error();
+    };
+    if (!rc) rc = run_tasks(p->batched, (int)chains.size(), run_chain);
+    for (auto &ch : chains)
+        if (!rc && ch.rc) rc = fail(ch.rc, ch.err);
     for (int s = 0; s < nspin && !rc; ++s)
         for (int a = 0; a < norb && !rc; ++a) {
             std::fill(gm.begin(), gm.end(), 0.0);
             std::fill(gr.begin(), gr.end(), 0.0);
-            const int isite = a + 1 + s * ns;
-            for (auto &st : sl.items) {
-                const int nu = st.sector / (ns + 1), nd = st.sector % (ns + 1);
-                for (int pass = 0; pass < 2 && !rc; ++pass) {
-                    const int dagger = pass == 0 ? 1 : 0;
-                    const int isign = pass == 0 ? 1 : -1;
-                    const int ju = nu + (s == 0 ? (dagger ? 1 : -1) : 0);
-                    const int jd = nd + (s == 1 ? (dagger ? 1 : -1) : 0);
-                    if (ju < 0 || jd < 0 || ju > ns || jd > ns) continue;
-                    dmed_sector *J = nullptr;
-                    if ((rc = cache.get(ju, jd, ns, &J))) break;
-                    dmed_vec *v = nullptr;
-                    if ((rc = dmed_vec_create(J, DMED_FP64, &v))) break;
-                    double norm2 = 0.0;
-                    rc = dmed_apply_op(st.vec, isite, dagger, J, v, &norm2);
-                    if (!rc && norm2 > 0.0) {
-                        // zero norm: the operator annihilates the state and the
-                        // chain carries no weight (the reference divides by 0 here)
-                        const int nlanc = (int)std::min<int64_t>(J->dim, ngf);
-                        std::fill(al.begin(), al.end(), 0.0);
-                        std::fill(bl.begin(), bl.end(), 0.0);
-                        int nsteps = 0;
-                        rc = chain_run(J, v->data, DMED_FP64, nlanc, p->prec_gf, 1e-13, al.data(),
-                                       bl.data(), &nsteps);
-                        double peso = 0.0;
-                        if (finite_t) {
-                            if (beta * (st.e - egs) < 200) peso = norm2 * std::exp(-beta * (st.e - egs)) / zeta;
-                        } else {
-                            peso = norm2 / zeta;
-                        }
-                        if (!rc)
-                            rc = dmed_gf_add_chain(peso, st.e, al.data(), bl.data(), nlanc, isign,
-                                                   wm.data(), lm, wr.data(), lr, p->eps, gm.data(),
-                                                   gr.data());
-                    }
-                    dmed_vec_free(v);
+            for (auto &ch : chains) {
+                if (ch.s != s || ch.a != a || !(ch.norm2 > 0.0)) continue;
+                const State &st = sl.items[ch.state];
+                double peso = 0.0;
+                if (finite_t) {
+                    if (beta * (st.e - egs) < 200) peso = ch.norm2 * std::exp(-beta * (st.e - egs)) / zeta;
+                } else {
+                    peso = ch.norm2 / zeta;
                 }
-                if (rc) break;
+                if ((rc = dmed_gf_add_chain(peso, st.e, ch.al.data(), ch.bl.data(), ch.nlanc, ch.isign,
+                                            wm.data(), lm, wr.data(), lr, p->eps, gm.data(), gr.data())))
+                    break;
             }
             if (rc) break;
             for (int l = 0; l < lm && out->gmats; ++l) {
