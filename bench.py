@@ -61,7 +61,8 @@ def cpu_baseline(cfg, sample_rows, threads):
 def solve_compare():
     """Workload 4: Ns=12, all 169 sectors, beta=50, Boltzmann-weighted states.
     Plain FP64 (serial sectors and chains) against the batched mixed form
-    (concurrent per-stream workers, FP32 chains with FP64 reductions)."""
+    (concurrent per-stream workers; chains of 16 FP64 steps, then FP32 with
+    FP64 reductions)."""
     import numpy as np
     import dmed
     from dmed import configs
@@ -73,7 +74,7 @@ def solve_compare():
     out = {}
     res = {}
     for name, extra in (("fp64_plain", dict(batched=0, prec_gf=dmed.FP64)),
-                        ("mixed_batched", dict(batched=1, prec_gf=dmed.FP32))):
+                        ("mixed_batched", dict(batched=1, prec_gf=dmed.MIXED))):
         t0 = time.perf_counter()
         r = dmed.solve(1, 1, 12, dmed.default_params(**kw, **extra))
         dt = time.perf_counter() - t0

@@ -51,14 +51,14 @@ void exec_destroy(Exec &e)
     if (e.stream) hipStreamSynchronize(e.stream);
     hipFree(e.partials);
     hipFree(e.dscal);
-    for (int i = 0; i < 12; ++i) hipFree(e.buf[i]);
+    for (int i = 0; i < kExecSlots; ++i) hipFree(e.buf[i]);
     if (e.stream) hipStreamDestroy(e.stream);
     e = Exec();
 }
 
 int exec_buf(Exec &e, int slot, size_t bytes, void **p)
 {
-    if (slot < 0 || slot >= 12) return fail(DMED_EINVAL, "bad buffer slot");
+    if (slot < 0 || slot >= kExecSlots) return fail(DMED_EINVAL, "bad buffer slot");
     bytes = std::max<size_t>(bytes, 256);
     if (bytes > e.buf_cap[slot]) {
         if (e.buf[slot]) {
@@ -508,7 +508,7 @@ extern "C" int dmed_apply_op(const dmed_vec *psi, int32_t isite, int32_t dagger,
     std::vector<int8_t> sgn;
     op_tables(up ? I->up : I->dw, up ? dst->up : dst->dw, p, dagger, src, sgn);
     int32_t *dsrc = nullptr;
-    if (int rc = exec_buf(ex(), 8, std::max<size_t>(src.size(), 1) * 5, (void **)&dsrc)) return rc;
+    if (int rc = exec_buf(ex(), kSlotOpTable, std::max<size_t>(src.size(), 1) * 5, (void **)&dsrc)) return rc;
     int8_t *dsgn = reinterpret_cast<int8_t *>(dsrc + std::max<size_t>(src.size(), 1));
     DMED_HIP(hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, st));
     DMED_HIP(hipMemcpyAsync(dsgn, sgn.data(), sgn.size(), hipMemcpyHostToDevice, st));

@@ -100,13 +100,26 @@ constexpr int kScalCap = 4096;
 // library's calls run on the context bound to the calling host thread
 // (the default one unless a worker bound its own), so independent sectors
 // and chains can run concurrently on separate streams.
+// Work-buffer slots of an execution context.
+constexpr int kSlotChain = 0;           // 0..2: Lanczos chain vectors (v_prev, v, w)
+constexpr int kSlotTrlBasis = 3;        // thick-restart basis
+constexpr int kSlotTrlW = 4;            // thick-restart work vector
+constexpr int kSlotTrlSmall = 5;        // projected matrix and coefficients
+constexpr int kSlotDeflY = 6;           // locked eigenvectors of a deflation round
+constexpr int kSlotDeflX = 7;           // eigenvectors collected over rounds
+constexpr int kSlotOpTable = 8;         // c / c^+ source-target tables
+constexpr int kSlotStart = 9;           // GF chain start vector
+constexpr int kSlotChainLo = 12;        // 12..14: FP32 tail of a mixed chain
+constexpr int kExecSlots = 16;
+constexpr int kMixedHead = 16;          // FP64 steps at the head of a mixed chain
+
 struct Exec {
     hipStream_t stream = nullptr;
     double *partials = nullptr;
     size_t partials_cap = 0;            // in doubles
     double *dscal = nullptr;            // kScalCap doubles
-    void *buf[12] = {};
-    size_t buf_cap[12] = {};
+    void *buf[kExecSlots] = {};
+    size_t buf_cap[kExecSlots] = {};
 };
 int exec_create(Exec &e);
 void exec_destroy(Exec &e);
@@ -166,8 +179,10 @@ int launch_axpy(void *y, int yprec, const void *x, int xprec, int64_t n, double 
                 hipStream_t st);
 
 // --------------------------------------------------------- Krylov
+// prec: DMED_FP64, DMED_FP32, or DMED_MIXED (`head` FP64 steps, then FP32;
+// head < 0 selects kMixedHead)
 int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
-              double threshold, double *alpha, double *beta, int *nsteps);
+              double threshold, double *alpha, double *beta, int *nsteps, int head = -1);
 int eigh_sector(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
                 int dense_threshold, double *evals, dmed_vec **evecs, double *resid);
 // lowest_multiplet: the lowest level and its copies within `degen` (at most

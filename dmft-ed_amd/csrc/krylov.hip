@@ -403,24 +403,44 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
 }  // namespace
 
 int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
-              double threshold, double *alpha, double *beta, int *nsteps)
+              double threshold, double *alpha, double *beta, int *nsteps, int head)
 {
-        hipStream_t st = ex().stream;
+    hipStream_t st = ex().stream;
     if (nlanc <= 0) return fail(DMED_EINVAL, "nlanc must be positive");
     if (nlanc > kChainMax) return fail(DMED_EINVAL, "nlanc exceeds the chain buffer");
+    if (prec != DMED_FP64 && prec != DMED_FP32 && prec != DMED_MIXED)
+        return fail(DMED_EINVAL, "bad chain precision");
+    if (head < 0) head = kMixedHead;
+    // DMED_MIXED: the first `head` steps in FP64, the rest in FP32.  The low
+    // moments of the spectral function, which fix G(iw) at all but the
+    // highest resolution, come from the first steps; FP32 rounding there is
+    // what dominates the error of an all-FP32 chain.
+    const int nhead = prec == DMED_MIXED ? std::min(head, nlanc) : (prec == DMED_FP64 ? nlanc : 0);
+    const int p0 = nhead > 0 ? DMED_FP64 : DMED_FP32;
     const int64_t n = s->dim;
-    const size_t esz = prec == DMED_FP32 ? 4 : 8;
-    if (int rc = ensure_partials(std::max(hxv_nparts(s, prec), kVecGrid) + 16)) return rc;
-    struct { void *p; } b0, b1, b2;
-    if (int rc = exec_buf(ex(), 0, std::max<int64_t>(n, 1) * esz, &b0.p)) return rc;
-    if (int rc = exec_buf(ex(), 1, std::max<int64_t>(n, 1) * esz, &b1.p)) return rc;
-    if (int rc = exec_buf(ex(), 2, std::max<int64_t>(n, 1) * esz, &b2.p)) return rc;
-    if (int rc = launch_scale_copy(b1.p, prec, v0, v0prec, n, 1.0, st)) return rc;
+    const size_t n1 = (size_t)std::max<int64_t>(n, 1);
+    const size_t esz = p0 == DMED_FP32 ? 4 : 8;
+    int nparts = std::max(hxv_nparts(s, DMED_FP64), hxv_nparts(s, DMED_FP32));
+    if (int rc = ensure_partials(std::max(nparts, kVecGrid) + 16)) return rc;
+    void *bufs[3];
+    for (int k = 0; k < 3; ++k)
+        if (int rc = exec_buf(ex(), kSlotChain + k, n1 * esz, &bufs[k])) return rc;
+    if (int rc = launch_scale_copy(bufs[1], p0, v0, v0prec, n, 1.0, st)) return rc;
     DMED_HIP(hipMemsetAsync(chain_alpha(), 0, kChainMax * sizeof(double), st));
     DMED_HIP(hipMemsetAsync(chain_beta(), 0, kChainMax * sizeof(double), st));
-    void *bufs[3] = {b0.p, b1.p, b2.p};
-    if (int rc = chain_init(s, bufs, prec, nlanc, threshold)) return rc;
-    if (int rc = chain_steps(s, bufs, prec, nlanc, nullptr)) return rc;
+    if (int rc = chain_init(s, bufs, p0, nlanc, threshold)) return rc;
+    if (nhead > 0 && nhead < nlanc) {
+        if (int rc = chain_steps(s, bufs, DMED_FP64, nhead, nullptr)) return rc;
+        // the recurrence needs v and v_prev; w is overwritten by the next step
+        void *lo[3];
+        for (int k = 0; k < 3; ++k)
+            if (int rc = exec_buf(ex(), kSlotChainLo + k, n1 * 4, &lo[k])) return rc;
+        if (int rc = launch_scale_copy(lo[0], DMED_FP32, bufs[0], DMED_FP64, n, 1.0, st)) return rc;
+        if (int rc = launch_scale_copy(lo[1], DMED_FP32, bufs[1], DMED_FP64, n, 1.0, st)) return rc;
+        if (int rc = chain_steps(s, lo, DMED_FP32, nlanc - nhead, nullptr)) return rc;
+    } else {
+        if (int rc = chain_steps(s, bufs, p0, nlanc, nullptr)) return rc;
+    }
     ChainScalars h;
     DMED_HIP(hipMemcpyAsync(alpha, chain_alpha(), nlanc * sizeof(double), hipMemcpyDeviceToHost, st));
     DMED_HIP(hipMemcpyAsync(beta, chain_beta(), nlanc * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -529,9 +549,9 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
         hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     struct { void *p; } b0, b1, b2;
-    if (int rc = exec_buf(ex(), 0, n * 8, &b0.p)) return rc;
-    if (int rc = exec_buf(ex(), 1, n * 8, &b1.p)) return rc;
-    if (int rc = exec_buf(ex(), 2, n * 8, &b2.p)) return rc;
+    if (int rc = exec_buf(ex(), kSlotChain + 0, n * 8, &b0.p)) return rc;
+    if (int rc = exec_buf(ex(), kSlotChain + 1, n * 8, &b1.p)) return rc;
+    if (int rc = exec_buf(ex(), kSlotChain + 2, n * 8, &b2.p)) return rc;
     double *psi = nullptr;
     DMED_HIP(hipMalloc(&psi, n * 8));
     dmed_vec *out = new dmed_vec{s, DMED_FP64, n, psi};
@@ -620,9 +640,9 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
     m = (int)std::min<int64_t>(m, avail);
     const bool full = (int64_t)m == avail;
     double *V = nullptr, *w = nullptr, *dS = nullptr;
-    if (int rc = exec_buf(ex(), 3, (size_t)(m + 1) * n * 8, (void **)&V)) return rc;
-    if (int rc = exec_buf(ex(), 4, n * 8, (void **)&w)) return rc;
-    if (int rc = exec_buf(ex(), 5, (size_t)m * m * 8 + (size_t)(m + nl + 1) * 8, (void **)&dS)) return rc;
+    if (int rc = exec_buf(ex(), kSlotTrlBasis, (size_t)(m + 1) * n * 8, (void **)&V)) return rc;
+    if (int rc = exec_buf(ex(), kSlotTrlW, n * 8, (void **)&w)) return rc;
+    if (int rc = exec_buf(ex(), kSlotTrlSmall, (size_t)m * m * 8 + (size_t)(m + nl + 1) * 8, (void **)&dS)) return rc;
     double *dc = dS + (size_t)m * m;
     const unsigned g = vec_grid(n);
     if (int rc = ensure_partials((size_t)(std::max(m, nl) + 1) * g + 16)) return rc;
@@ -738,9 +758,9 @@ int trl_deflated(const dmed_sector *s, int neig, int m, int maxit, double tol, d
         hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     double *Y = nullptr, *X = nullptr;
-    if (int rc = exec_buf(ex(), 6, (size_t)neig * n * 8, (void **)&Y)) return rc;
+    if (int rc = exec_buf(ex(), kSlotDeflY, (size_t)neig * n * 8, (void **)&Y)) return rc;
     const int want = lowest_multiplet ? 1 : neig;
-    if (int rc = exec_buf(ex(), 7, (size_t)want * n * 8, (void **)&X)) return rc;
+    if (int rc = exec_buf(ex(), kSlotDeflX, (size_t)want * n * 8, (void **)&X)) return rc;
     struct Kept { double e, r; int slot; };
     std::vector<Kept> kept;
     std::vector<double> th(want), rr(want);

@@ -181,7 +181,7 @@ extern "C" void dmed_solve_params_default(dmed_solve_params *p)
     p->prec_gf = DMED_FP64;
     p->batched = 0;
     p->verbose = 0;
-    p->reserved = 0;
+    p->gf_fp64_head = kMixedHead;
 }
 
 extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
@@ -374,7 +374,7 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
         const State &st = sl.items[ch.state];
         // start vector in a reusable buffer of the calling context
         dmed_vec v{ch.J, DMED_FP64, ch.J->dim, nullptr};
-        if ((ch.rc = exec_buf(ex(), 9, std::max<int64_t>(ch.J->dim, 1) * 8, &v.data))) {
+        if ((ch.rc = exec_buf(ex(), kSlotStart, std::max<int64_t>(ch.J->dim, 1) * 8, &v.data))) {
             ch.err = dmed_last_error();
             return;
         }
@@ -387,7 +387,7 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
             ch.bl.assign(ch.nlanc, 0.0);
             int nsteps = 0;
             ch.rc = chain_run(ch.J, v.data, DMED_FP64, ch.nlanc, p->prec_gf, 1e-13, ch.al.data(),
-                              ch.bl.data(), &nsteps);
+                              ch.bl.data(), &nsteps, p->gf_fp64_head);
         }
         if (ch.rc) ch.err = dmed_last_error();
     };

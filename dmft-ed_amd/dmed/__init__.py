@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "libdmed.so")
 
-FP64, FP32 = 0, 1
+FP64, FP32, MIXED = 0, 1, 2   # MIXED: chains only (FP64 head steps, then FP32)
 ERRORS = {-1: "EINVAL", -2: "ESTATE", -3: "EHIP", -4: "ENOMEM", -5: "EUNSUPP",
           -6: "ENOCONV", -7: "ENORM"}
 
@@ -45,7 +45,7 @@ class SolveParams(C.Structure):
                 ("cutoff", C.c_double), ("beta", C.c_double), ("eps", C.c_double),
                 ("wini", C.c_double), ("wfin", C.c_double), ("lmats", C.c_int32),
                 ("lreal", C.c_int32), ("prec_gf", C.c_int32), ("batched", C.c_int32),
-                ("verbose", C.c_int32), ("reserved", C.c_int32)]
+                ("verbose", C.c_int32), ("gf_fp64_head", C.c_int32)]
 
 
 class SolveOut(C.Structure):

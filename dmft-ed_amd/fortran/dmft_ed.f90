@@ -377,7 +377,7 @@ contains
     p%cutoff = cutoff
     p%beta = beta; p%eps = eps; p%wini = wini; p%wfin = wfin
     p%lmats = Lmats; p%lreal = Lreal
-    p%prec_gf = merge(DMED_FP32,DMED_FP64,ed_gpu_fp32_gf)
+    p%prec_gf = merge(DMED_MIXED,DMED_FP64,ed_gpu_fp32_gf)
     p%batched = merge(1,0,ed_gpu_batched)
     p%verbose = ed_verbose
     allocate(dens(Norb), docc(Norb))

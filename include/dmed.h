@@ -44,6 +44,7 @@ extern "C" {
 /* precision selector for H*v and the GF chains */
 #define DMED_FP64 0
 #define DMED_FP32 1         /* FP32 vectors, FP64-accumulated reductions */
+#define DMED_MIXED 2        /* Lanczos chains only: FP64 head steps, then FP32 */
 
 /* ------------------------------------------------------------ setup */
 /* Select the device and create the library stream.  Replaces the implicit
@@ -150,7 +151,9 @@ int dmed_apply_op(const dmed_vec *psi, int32_t isite, int32_t dagger,
  * not modified): alpha[k], beta[k+1] for k < nlanc-1 with the semantics of
  * sp_lanc_tridiag (build_gf_normal.f90:124-126; .repo/PLAIN_LANCZOS.f90:154-180),
  * stopping early when |b| < threshold; unset entries are zero.
- * prec = DMED_FP32 runs H*v and the vectors in FP32 with FP64 reductions.
+ * prec = DMED_FP32 runs H*v and the vectors in FP32 with FP64 reductions;
+ * prec = DMED_MIXED runs the first 16 steps in FP64 and the rest in FP32
+ * (the rounding of the first steps dominates the error of an FP32 chain).
  * nsteps receives the number of alphas produced. */
 int dmed_chain(const dmed_sector *s, const dmed_vec *v0, int32_t nlanc,
                int32_t prec, double threshold, double *alpha, double *beta,
@@ -187,10 +190,10 @@ typedef struct dmed_solve_params {
     double eps;                    /* 0.01 */
     double wini, wfin;             /* -5, 5 */
     int32_t lmats, lreal;          /* 5000, 5000 */
-    int32_t prec_gf;               /* DMED_FP64 / DMED_FP32 chains */
+    int32_t prec_gf;               /* DMED_FP64 / DMED_FP32 / DMED_MIXED chains */
     int32_t batched;               /* 1: batched sector/chain engine */
     int32_t verbose;
-    int32_t reserved;
+    int32_t gf_fp64_head;          /* 16: FP64 steps of a DMED_MIXED chain */
 } dmed_solve_params;
 void dmed_solve_params_default(dmed_solve_params *p);
 
