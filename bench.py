@@ -103,7 +103,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--prec", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-rows", type=int, default=2_000_000)
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="rows of the oracle H*v sample (default: the whole sector)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--solve", action="store_true",
                     help="also time the all-sector Ns=12 finite-T solve: plain FP64 vs "
@@ -151,8 +152,22 @@ def main():
     log(f"H*v: {ms_step:.4f} ms/step wall, kernel {kernel_ms:.4f} ms, "
         f"{achieved:.1f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of HBM peak)")
 
-    # the FP32 form of the same product (mixed-precision chains use it)
+    # a plain streaming copy of one vector: the practical HBM ceiling on this
+    # box, and the calibration kernel for the HBM traffic counters
     extra = {}
+    xc = dmed.Vec(sec, dmed.FP64)
+    for _ in range(2):
+        xc.copy_from(x)
+    t0 = time.perf_counter()
+    ncopy = 5
+    for _ in range(ncopy):
+        xc.copy_from(x)
+    tc = (time.perf_counter() - t0) / ncopy
+    extra["copy_fp64"] = {"ms_wall": 1e3 * tc, "GBs": 16 * D / tc / 1e9}
+    log(f"copy of one FP64 vector: {1e3 * tc:.4f} ms wall, {16 * D / tc / 1e9:.1f} GB/s")
+    xc.free()
+
+    # the FP32 form of the same product (mixed-precision chains use it)
     if prec == dmed.FP64:
         x32 = dmed.Vec(sec, dmed.FP32)
         y32 = dmed.Vec(sec, dmed.FP32)
@@ -182,12 +197,14 @@ def main():
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        rows = min(a.cpu_rows, D)
+        rows = min(a.cpu_rows, D) if a.cpu_rows > 0 else D
         rate, dt = cpu_baseline(cfg, rows, threads)
+        what = ("one full H*v of the dim-%d sector" % D if rows == D else
+                f"rows [0,{rows}) of the dim-{D} sector H*v, extrapolated to the full sector")
         cpu = {"value": rate / D, "unit": "Hv/s", "cores": threads, "kind": "port",
-               "sample": f"rows [0,{rows}) of the dim-{D} sector H*v ({dt:.1f}s), oracle "
-                         f"complex(8) map+binary-search restatement, extrapolated to the "
-                         f"full sector"}
+               "sample": f"{what} ({dt:.1f}s): the oracle's complex(8) restatement of "
+                         f"directMatVec_cc (state map + binary search per hop), "
+                         f"{threads} OpenMP threads over row blocks"}
         log(f"cpu baseline: {rate:.3e} rows/s on {threads} threads -> {rate / D:.4f} Hv/s")
 
     out = {

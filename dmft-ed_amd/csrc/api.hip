@@ -268,6 +268,20 @@ int dmed_vec_free(dmed_vec *v)
 
 void *dmed_vec_data(const dmed_vec *v) { return v ? v->data : nullptr; }
 
+int dmed_vec_copy(dmed_vec *dst, const dmed_vec *src)
+{
+    REQUIRE_INIT();
+    if (!dst || !src) return fail(DMED_EINVAL, "null argument");
+    if (dst->sec != src->sec || dst->dim != src->dim)
+        return fail(DMED_EINVAL, "vectors belong to different sectors");
+    if (dst == src) return 0;
+    hipStream_t st = ex().stream;
+    if (int rc = launch_scale_copy(dst->data, dst->prec, src->data, src->prec, src->dim, 1.0, st))
+        return rc;
+    DMED_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
 int dmed_vec_upload(dmed_vec *v, const double *host)
 {
     REQUIRE_INIT();

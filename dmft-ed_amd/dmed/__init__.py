@@ -75,6 +75,7 @@ SIGNATURES = [
     ("dmed_vec_upload", C.c_int, [C.c_void_p, _dp]),
     ("dmed_vec_download", C.c_int, [C.c_void_p, _dp]),
     ("dmed_vec_data", C.c_void_p, [C.c_void_p]),
+    ("dmed_vec_copy", C.c_int, [C.c_void_p, C.c_void_p]),
     ("dmed_hxv", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
     ("dmed_hxv_host", C.c_int, [C.c_void_p, _dp, _dp, C.c_int32]),
     ("dmed_hxv_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
@@ -229,6 +230,10 @@ class Vec:
         out = np.empty(self.sector.dim)
         _check(lib().dmed_vec_download(self.h, _d(out)))
         return out
+
+    def copy_from(self, src):
+        """self = src on the device (precision converted)."""
+        _check(lib().dmed_vec_copy(self.h, src.h))
 
     def ptr(self):
         return lib().dmed_vec_data(self.h)

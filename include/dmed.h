@@ -104,6 +104,10 @@ int dmed_vec_free(dmed_vec *v);
 int dmed_vec_upload(dmed_vec *v, const double *host);     /* host FP64, length dim */
 int dmed_vec_download(const dmed_vec *v, double *host);   /* host FP64, length dim */
 void *dmed_vec_data(const dmed_vec *v);                   /* device pointer */
+/* dst = src on the device, converting FP64 <-> FP32; both vectors belong to
+ * the same sector.  A streaming copy (read + write of dim elements); the
+ * reference copies vectors with Fortran assignment (ED_EIGENSPACE.f90:177-178). */
+int dmed_vec_copy(dmed_vec *dst, const dmed_vec *src);
 
 /* ------------------------------------------------------------- H*v */
 /* y = H x on the sector, x and y device pointers of length dim in the given

@@ -397,3 +397,18 @@ def test_mixed_precision_chains_error_bounds():
         ds = np.abs(r64["smats"] - r["smats"]).max()
         say(f"chains prec={prec}: max|dG|={dg:.3e} max|dSigma|={ds:.3e}")
         assert dg < gbar and ds < sbar
+
+
+def test_vec_copy_converts_precision():
+    """dmed_vec_copy: FP64 copy is exact, FP64 -> FP32 rounds to nearest."""
+    c = configs.config(1)
+    set_cfg(c)
+    s = dmed.Sector(4, 4)
+    x = np.random.default_rng(5).standard_normal(s.dim)
+    a = s.vec(x)
+    b = dmed.Vec(s, dmed.FP64)
+    b.copy_from(a)
+    assert np.array_equal(b.download(), x)
+    f = dmed.Vec(s, dmed.FP32)
+    f.copy_from(a)
+    assert np.array_equal(f.download(), x.astype(np.float32).astype(np.float64))
