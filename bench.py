@@ -96,6 +96,76 @@ def solve_compare():
     return out
 
 
+def lanczos_step_rate(sec, D, prec, nsteps=40):
+    """Wall time per plain Lanczos step (fused H*v epilogue + update + two
+    one-block reductions) from a random start on the benchmark sector."""
+    import numpy as np
+    import dmed
+    v0 = dmed.Vec(sec, dmed.FP64)
+    v0.upload(np.random.default_rng(2).standard_normal(D))
+    dmed.chain(sec, v0, 4, prec=prec)
+    t0 = time.perf_counter()
+    a, b, n = dmed.chain(sec, v0, nsteps, prec=prec)
+    dt = (time.perf_counter() - t0) / max(n, 1)
+    v0.free()
+    esz = 8 if prec == dmed.FP64 else 4
+    return {"ms_per_step": 1e3 * dt, "algorithmic_GBs": 6 * esz * D / dt / 1e9,
+            "frac": 6 * esz * D / dt / 1e9 / HBM_PEAK_GBS, "steps": n}
+
+
+def solve5_compare():
+    """Workload 5 sector pipeline (Ns=16, sector 8/8, dim 1.66e8): ground
+    state to residual 1e-12, particle and hole chains of 500 steps from it,
+    G(iw) and Sigma(iw) on 4096 Matsubara points at beta=100.  Plain FP64
+    against the mixed form (ground state started on FP32 vectors and finished
+    in FP64; chains of 16 FP64 steps, then FP32)."""
+    import numpy as np
+    import dmed
+    from dmed import configs
+    c = configs.config(5)
+    dmed.set_model(c["norb"], c["nbath"], c["nspin"], c["bath"], c["hloc"], c["uloc"],
+                   c["ust"], c["jh"], 0.0, 0.0, c["xmu"], True, False)
+    s, sp, sh = dmed.Sector(8, 8), dmed.Sector(9, 8), dmed.Sector(7, 8)
+    wm = np.pi / c["beta"] * (2 * np.arange(c["lmats"]) + 1)
+    z = 1j * wm
+    out, res = {}, {}
+    for name, pgs, pgf in (("fp64_plain", dmed.FP64, dmed.FP64),
+                           ("mixed", dmed.MIXED, dmed.MIXED)):
+        t0 = time.perf_counter()
+        e0, psi, r0, rc = s.ground_state(prec=pgs, tol=1e-12, maxit=6000)
+        t_gs = time.perf_counter() - t0
+        dens, docc = dmed.observables(psi, 1.0, 1)
+        g = np.zeros(c["lmats"], complex)
+        gr = np.zeros(1, complex)
+        t1 = time.perf_counter()
+        for dst, dag, sgn in ((sp, 1, 1), (sh, 0, -1)):
+            v, n2 = dmed.apply_op(psi, 1, dag, dst)
+            a, b, n = dmed.chain(dst, v, c["nlanc"], prec=pgf)
+            dmed.gf_add_chain(n2, e0, a, b, sgn, wm, np.zeros(1), 0.01, g, gr)
+            v.free()
+        t_gf = time.perf_counter() - t1
+        psi.free()
+        sig = dmed.sigma(0, 0, z, g)
+        res[name] = dict(e0=e0, dens=dens, docc=docc, g=g, s=sig)
+        out[name] = {"seconds": time.perf_counter() - t0, "t_gs": t_gs, "t_gf": t_gf,
+                     "E0": e0, "gs_residual": r0, "gs_rc": rc}
+        log(f"workload 5 {name}: {out[name]['seconds']:.2f}s (GS {t_gs:.2f}s, chains "
+            f"{t_gf:.2f}s), E0 {e0:.12f}, residual {r0:.2e}")
+    a64, amx = res["fp64_plain"], res["mixed"]
+    out["speedup"] = out["fp64_plain"]["seconds"] / out["mixed"]["seconds"]
+    out["errors_vs_fp64"] = {
+        "E0_rel": abs(amx["e0"] - a64["e0"]) / abs(a64["e0"]),
+        "dens": float(np.abs(amx["dens"] - a64["dens"]).max()),
+        "docc": float(np.abs(amx["docc"] - a64["docc"]).max()),
+        "G_iw": float(np.abs(amx["g"] - a64["g"]).max()),
+        "Sigma_iw": float(np.abs(amx["s"] - a64["s"]).max()),
+    }
+    log(f"workload 5 mixed speedup {out['speedup']:.2f}x, errors {out['errors_vs_fp64']}")
+    for v in (s, sp, sh):
+        v.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +179,9 @@ def main():
     ap.add_argument("--solve", action="store_true",
                     help="also time the all-sector Ns=12 finite-T solve: plain FP64 vs "
                          "batched mixed precision")
+    ap.add_argument("--solve5", action="store_true",
+                    help="also time the Ns=16 sector pipeline (ground state + 500-step chains): "
+                         "plain FP64 vs mixed precision")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -183,8 +256,18 @@ def main():
         x32.free()
         y32.free()
 
+    for p_, nm in ((dmed.FP64, "fp64"), (dmed.FP32, "fp32")):
+        extra[f"lanczos_step_{nm}"] = st = lanczos_step_rate(sec, D, p_)
+        log(f"Lanczos step {nm}: {st['ms_per_step']:.3f} ms, {st['algorithmic_GBs']:.1f} GB/s "
+            f"algorithmic (48 B/element FP64, 24 FP32)")
+
     if a.solve:
         extra["solve_ns12_all_sectors"] = solve_compare()
+    if a.solve5:
+        sec.free()
+        x.free()
+        y.free()
+        extra["solve_ns16_sector_pipeline"] = solve5_compare()
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hxv_ns16_traffic.json")

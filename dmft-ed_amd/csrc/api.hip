@@ -336,7 +336,7 @@ int dmed_hxv_async(const dmed_sector *s, const void *x, void *y, int32_t prec)
     if (int rc = check_sector(s)) return rc;
     if (!x || !y || x == y) return fail(DMED_EINVAL, "x and y must be distinct device buffers");
     if (prec != DMED_FP64 && prec != DMED_FP32) return fail(DMED_EINVAL, "bad precision");
-    return launch_hxv(s, x, y, prec, kPlain, nullptr, nullptr, nullptr, nullptr, ex().stream);
+    return launch_hxv(s, x, y, prec, kPlain, nullptr, nullptr, nullptr, ex().stream);
 }
 
 int dmed_hxv(const dmed_sector *s, const void *x, void *y, int32_t prec)
@@ -359,7 +359,7 @@ int dmed_hxv_timed(const dmed_sector *s, const void *x, void *y, int32_t prec, i
     DMED_HIP(hipEventRecord(e0, st));
     int rc = 0;
     for (int r = 0; r < nrep && !rc; ++r)
-        rc = launch_hxv(s, x, y, prec, kPlain, nullptr, nullptr, nullptr, nullptr, st);
+        rc = launch_hxv(s, x, y, prec, kPlain, nullptr, nullptr, nullptr, st);
     DMED_HIP(hipEventRecord(e1, st));
     DMED_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -401,6 +401,17 @@ int dmed_eigh(const dmed_sector *s, int32_t neig, int32_t ncv, int32_t maxit, do
     if (maxit <= 0) maxit = 20000;
     if (tol <= 0) tol = 1e-12;
     return eigh_sector(s, neig, ncv, maxit, tol, dense_threshold, evals, evecs, resid);
+}
+
+int dmed_ground_state(const dmed_sector *s, int32_t prec, int32_t maxit, double tol,
+                      double *eval, dmed_vec **evec, double *resid)
+{
+    REQUIRE_MODEL();
+    if (int rc = check_sector(s)) return rc;
+    if (!eval) return fail(DMED_EINVAL, "null eval");
+    if (maxit <= 0) maxit = 20000;
+    if (tol <= 0) tol = 1e-12;
+    return ground_state(s, prec, maxit, tol, eval, evec, resid);
 }
 
 }  // extern "C"

@@ -158,15 +158,16 @@ struct ChainScalars {      // device-resident (inside Context::dscal)
     double b;              // previous beta
     double alpha;          // current alpha
     double thr;            // early-exit threshold on beta
+    double q;              // <x_new, x_old> of the stored vectors after the last update
     int32_t step;          // 0-based step index
     int32_t done;          // 1 once |beta| < thr
     int32_t nlanc;
     int32_t pad;
 };
 
+// mode kLanczos: y = sx*H*x with the block partials of <x, y> (cs->sx)
 int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
-               const void *vp, ChainScalars *cs, double *partials, int *nparts,
-               hipStream_t st);
+               ChainScalars *cs, double *partials, int *nparts, hipStream_t st);
 
 // ---------------------------------------------------------- vector ops
 int launch_reduce_to(const double *partials, int n, double *out, int op, hipStream_t st);
@@ -187,9 +188,15 @@ int eigh_sector(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
                 int dense_threshold, double *evals, dmed_vec **evecs, double *resid);
 // lowest_multiplet: the lowest level and its copies within `degen` (at most
 // neig); otherwise the neig lowest with multiplets completed.  *nout pairs.
+// gs_prec (DMED_FP64 / DMED_MIXED) applies where the ground state comes
+// from the plain Lanczos solver.
 int eigh_sector_ex(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
                    int dense_threshold, double degen, bool lowest_multiplet, int *nout,
-                   double *evals, dmed_vec **evecs, double *resid);
+                   double *evals, dmed_vec **evecs, double *resid, int gs_prec = DMED_FP64);
+// plain two-pass Lanczos ground state with residual restarts; DMED_MIXED runs
+// the first restarts on FP32 vectors and finishes in FP64
+int ground_state(const dmed_sector *s, int prec, int maxit, double tol, double *eval,
+                 dmed_vec **evec, double *resid);
 
 // ------------------------------------------------------------ host LA
 // symmetric eigen-decomposition of a dense n x n matrix (column-major, in

@@ -14,8 +14,10 @@
 // workgroup owns `cb` whole columns: it stages them in LDS with coalesced
 // loads, gathers the up hops from LDS, and reads the down-hop neighbour
 // columns straight from HBM with lanes running along the contiguous up
-// index (coalesced sweeps).  The epilogue can fuse the first half of a
-// Lanczos step (w = sx*Hv - b*svp*v_prev and the block partials of <v,w>).
+// index (coalesced sweeps).  In Lanczos mode the stored vector x carries the
+// scale sx (v = sx*x): the kernel writes y = Hv = sx*H*x and the block
+// partials of <x, y>; the -b*v_prev and -alpha*v terms of the step are left to
+// the streaming update kernel, so no third vector is read here.
 #include "dmed_internal.h"
 
 #include <algorithm>
@@ -37,7 +39,6 @@ struct HxvArgs {
     const uint8_t *__restrict__ impu;
     const uint8_t *__restrict__ impd;
     const double *__restrict__ W;         // [nimp][nimp]
-    const T *__restrict__ vp;             // Lanczos epilogue
     const ChainScalars *cs;
     double *__restrict__ partials;
 };
@@ -78,11 +79,10 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
     double *wt = reinterpret_cast<double *>(smem + colbytes);
     const T *__restrict__ xb = a.x + c0 * dup;
 
-    double sx = 1.0, coef_vp = 0.0;
+    double sx = 1.0;
     if (MODE == kLanczos) {
         if (a.cs->done) return;
         sx = a.cs->sx;
-        coef_vp = a.cs->b * a.cs->svp;
     }
     if (LDS)
         for (int e = threadIdx.x; e < nel; e += blockDim.x) col[e] = xb[e];
@@ -115,9 +115,9 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
         if (MODE == kPlain) {
             a.y[o] = acc;
         } else {
-            const double yy = sx * (double)acc - coef_vp * (double)a.vp[o];
-            a.y[o] = (T)yy;
-            dot += sx * (double)xo * yy;
+            const T yy = (T)(sx * (double)acc);
+            a.y[o] = yy;
+            dot += (double)xo * (double)yy;
         }
     }
     if (MODE == kLanczos) {
@@ -136,8 +136,8 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
 // the previous column is stored to LDS; down-hop neighbour columns are
 // streamed from memory four at a time; up hops gather from LDS with packed
 // 4-byte hop words (target rank, coefficient index, sign) whose table stays
-// L2-resident; y is written with non-temporal stores.  Each lane owns rows
-// tid + 1024*r, r < RPT, so its diagonal terms live in registers.
+// L2-resident; in FP64 the diagonal is added last from the LDS copy of x; y
+// is written with non-temporal stores.  Each lane owns rows tid + 1024*r, r < RPT.
 template <typename T>
 struct ColArgs {
     const T *__restrict__ x;
@@ -152,7 +152,6 @@ struct ColArgs {
     const uint8_t *__restrict__ impu;
     const uint8_t *__restrict__ impd;
     const double *__restrict__ W;
-    const T *__restrict__ vp;
     const ChainScalars *cs;
     double *__restrict__ partials;
 };
@@ -163,7 +162,7 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
     int32_t nimp, int32_t ncoef, const uint32_t *__restrict__ HU, const double *__restrict__ COEF,
     const int32_t *__restrict__ HDI, const double *__restrict__ HDV, const double *__restrict__ DU,
     const double *__restrict__ DD, const uint8_t *__restrict__ IMPU, const uint8_t *__restrict__ IMPD,
-    const double *__restrict__ WTAB, const T *__restrict__ VP, const ChainScalars *__restrict__ CS,
+    const double *__restrict__ WTAB, const ChainScalars *__restrict__ CS,
     double *__restrict__ PARTIALS)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -172,24 +171,37 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
     __shared__ double wt[8];
     __shared__ double red[16];
     constexpr int HDU = 4;
-    double sx = 1.0, coef_vp = 0.0;
+    double sx = 1.0;
     if (MODE == kLanczos) {
         if (CS->done) return;
         sx = CS->sx;
-        coef_vp = CS->b * CS->svp;
     }
     for (int t = threadIdx.x; t < ncoef; t += blockDim.x) sc[t] = (T)COEF[t];
     const int nper = gridDim.x / 8;                  // workgroups per XCD
     const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
     const int per = (ddw + 7) / 8;
     const int cbeg = xcd * per, cend = min(ddw, cbeg + per);
-    T dui[RPT];
-    uint32_t imp[RPT];
+    // FP64: the up diagonal is re-read from L2 and added last (registers hold
+    // the prefetch and the sums; keeping it resident spills at 13 rows per
+    // lane).  FP32: it stays in registers and starts the sum.
+    constexpr bool kDiagFirst = sizeof(T) == 4;
+    T dui[kDiagFirst ? RPT : 1];
+    if (kDiagFirst) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = threadIdx.x + r * 1024;
+            dui[r] = i < dup ? (T)DU[i] : (T)0;
+        }
+    }
+    // impurity bits of the lane's rows, four per register
+    constexpr int NP = (RPT + 3) / 4;
+    uint32_t imp4[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) imp4[q] = 0u;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         const int i = threadIdx.x + r * 1024;
-        dui[r] = i < dup ? (T)DU[i] : (T)0;
-        imp[r] = i < dup ? IMPU[i] : 0u;
+        imp4[r / 4] |= (i < dup ? (uint32_t)IMPU[i] : 0u) << (8 * (r % 4));
     }
     int c = cbeg + loc;
     T nx[RPT];
@@ -212,11 +224,13 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
         __syncthreads();
         T acc[RPT];
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) acc[r] = nx[r] * (dui[r] + (T)wt[imp[r]]);
-        T xo[MODE == kLanczos ? RPT : 1];
-        if (MODE == kLanczos) {
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) xo[r] = nx[r];
+        for (int r = 0; r < RPT; ++r) {
+            if (kDiagFirst) {
+                const uint32_t ib = (imp4[r / 4] >> (8 * (r % 4))) & 0xffu;
+                acc[r] = nx[r] * (dui[r] + (T)wt[ib]);
+            } else {
+                acc[r] = (T)0;
+            }
         }
         const int cn = c + nper;
         if (cn < cend) {
@@ -256,17 +270,24 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
                 }
             }
         }
+        // (FP64) diagonal last, with the lane's own x from LDS
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const int i = threadIdx.x + r * 1024;
             if (i < dup) {
                 const int64_t o = (int64_t)c * dup + i;
+                const T xi = col[i];
+                T hv = acc[r];
+                if (!kDiagFirst) {
+                    const uint32_t ib = (imp4[r / 4] >> (8 * (r % 4))) & 0xffu;
+                    hv += (T)(DU[i] + wt[ib]) * xi;
+                }
                 if (MODE == kPlain) {
-                    __builtin_nontemporal_store(acc[r], &Y[o]);
+                    __builtin_nontemporal_store(hv, &Y[o]);
                 } else {
-                    const double yy = sx * (double)acc[r] - coef_vp * (double)VP[o];
-                    __builtin_nontemporal_store((T)yy, &Y[o]);
-                    dot += sx * (double)xo[r] * yy;
+                    const T yy = (T)(sx * (double)hv);
+                    __builtin_nontemporal_store(yy, &Y[o]);
+                    dot += (double)xi * (double)yy;
                 }
             }
         }
@@ -303,8 +324,8 @@ static unsigned col_grid(const dmed_sector *s)
 }
 
 template <typename T, int MODE, int RPT>
-static int launch_col_t(const dmed_sector *s, const void *x, void *y, const void *vp,
-                        ChainScalars *cs, double *partials, hipStream_t st)
+static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScalars *cs,
+                        double *partials, hipStream_t st)
 {
     ColArgs<T> a;
     a.x = static_cast<const T *>(x);
@@ -324,7 +345,6 @@ static int launch_col_t(const dmed_sector *s, const void *x, void *y, const void
     a.impu = s->up.imp;
     a.impd = s->dw.imp;
     a.W = s->W;
-    a.vp = static_cast<const T *>(vp);
     a.cs = cs;
     a.partials = partials;
     const size_t shmem = (size_t)a.dup * sizeof(T);
@@ -334,26 +354,26 @@ static int launch_col_t(const dmed_sector *s, const void *x, void *y, const void
                                      (int)shmem));
     hipLaunchKernelGGL(kern, dim3(col_grid(s)), dim3(1024), shmem, st, a.x, a.y, a.dup, a.ddw,
                        a.nhu, a.nhd, a.nimp, a.ncoef, a.hu, a.coef, a.hd_idx, a.hd_val, a.du, a.dd,
-                       a.impu, a.impd, a.W, a.vp, a.cs, a.partials);
+                       a.impu, a.impd, a.W, a.cs, a.partials);
     DMED_HIP(hipGetLastError());
     return 0;
 }
 
 template <typename T, int MODE>
-static int launch_col_rpt(int rpt, const dmed_sector *s, const void *x, void *y, const void *vp,
+static int launch_col_rpt(int rpt, const dmed_sector *s, const void *x, void *y,
                           ChainScalars *cs, double *partials, hipStream_t st)
 {
     switch (rpt) {
 #define DMED_RPT(R) \
-    case R: return launch_col_t<T, MODE, R>(s, x, y, vp, cs, partials, st);
+    case R: return launch_col_t<T, MODE, R>(s, x, y, cs, partials, st);
         DMED_RPT(2) DMED_RPT(3) DMED_RPT(4) DMED_RPT(5) DMED_RPT(6) DMED_RPT(7) DMED_RPT(8)
         DMED_RPT(9) DMED_RPT(10) DMED_RPT(11) DMED_RPT(12) DMED_RPT(13)
 #undef DMED_RPT
     case 14: case 15: case 16:
         if constexpr (sizeof(T) == 4) {
-            if (rpt == 14) return launch_col_t<T, MODE, 14>(s, x, y, vp, cs, partials, st);
-            if (rpt == 15) return launch_col_t<T, MODE, 15>(s, x, y, vp, cs, partials, st);
-            return launch_col_t<T, MODE, 16>(s, x, y, vp, cs, partials, st);
+            if (rpt == 14) return launch_col_t<T, MODE, 14>(s, x, y, cs, partials, st);
+            if (rpt == 15) return launch_col_t<T, MODE, 15>(s, x, y, cs, partials, st);
+            return launch_col_t<T, MODE, 16>(s, x, y, cs, partials, st);
         }
         break;
     default:
@@ -393,8 +413,8 @@ static LaunchShape pick_shape(const dmed_sector *s, int prec)
 }
 
 template <typename T, int MODE, bool LDS>
-static int launch_t(const dmed_sector *s, const void *x, void *y, const void *vp,
-                    ChainScalars *cs, double *partials, const LaunchShape &sh, hipStream_t st)
+static int launch_t(const dmed_sector *s, const void *x, void *y, ChainScalars *cs,
+                    double *partials, const LaunchShape &sh, hipStream_t st)
 {
     HxvArgs<T> a;
     a.x = static_cast<const T *>(x);
@@ -420,7 +440,6 @@ static int launch_t(const dmed_sector *s, const void *x, void *y, const void *vp
     a.impu = s->up.imp;
     a.impd = s->dw.imp;
     a.W = s->W;
-    a.vp = static_cast<const T *>(vp);
     a.cs = cs;
     a.partials = partials;
     auto kern = k_hxv<T, MODE, LDS>;
@@ -434,8 +453,7 @@ static int launch_t(const dmed_sector *s, const void *x, void *y, const void *vp
 }
 
 int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
-               const void *vp, ChainScalars *cs, double *partials, int *nparts,
-               hipStream_t st)
+               ChainScalars *cs, double *partials, int *nparts, hipStream_t st)
 {
     if (s->dim == 0) return 0;
     if (mode == kLanczos && partials == nullptr) return fail(DMED_EINVAL, "no partials");
@@ -443,25 +461,25 @@ int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
         if (nparts) *nparts = (int)col_grid(s);
         const int rpt = (int)((s->up.dim + 1023) / 1024);
         if (prec == DMED_FP32)
-            return mode == kPlain ? launch_col_rpt<float, kPlain>(rpt, s, x, y, vp, cs, partials, st)
-                                  : launch_col_rpt<float, kLanczos>(rpt, s, x, y, vp, cs, partials, st);
-        return mode == kPlain ? launch_col_rpt<double, kPlain>(rpt, s, x, y, vp, cs, partials, st)
-                              : launch_col_rpt<double, kLanczos>(rpt, s, x, y, vp, cs, partials, st);
+            return mode == kPlain ? launch_col_rpt<float, kPlain>(rpt, s, x, y, cs, partials, st)
+                                  : launch_col_rpt<float, kLanczos>(rpt, s, x, y, cs, partials, st);
+        return mode == kPlain ? launch_col_rpt<double, kPlain>(rpt, s, x, y, cs, partials, st)
+                              : launch_col_rpt<double, kLanczos>(rpt, s, x, y, cs, partials, st);
     }
     const LaunchShape sh = pick_shape(s, prec);
     if (nparts) *nparts = (int)sh.grid;
     if (mode == kLanczos && partials == nullptr) return fail(DMED_EINVAL, "no partials");
     const bool f32 = prec == DMED_FP32;
     if (mode == kPlain) {
-        if (f32) return sh.lds ? launch_t<float, kPlain, true>(s, x, y, vp, cs, partials, sh, st)
-                               : launch_t<float, kPlain, false>(s, x, y, vp, cs, partials, sh, st);
-        return sh.lds ? launch_t<double, kPlain, true>(s, x, y, vp, cs, partials, sh, st)
-                      : launch_t<double, kPlain, false>(s, x, y, vp, cs, partials, sh, st);
+        if (f32) return sh.lds ? launch_t<float, kPlain, true>(s, x, y, cs, partials, sh, st)
+                               : launch_t<float, kPlain, false>(s, x, y, cs, partials, sh, st);
+        return sh.lds ? launch_t<double, kPlain, true>(s, x, y, cs, partials, sh, st)
+                      : launch_t<double, kPlain, false>(s, x, y, cs, partials, sh, st);
     }
-    if (f32) return sh.lds ? launch_t<float, kLanczos, true>(s, x, y, vp, cs, partials, sh, st)
-                           : launch_t<float, kLanczos, false>(s, x, y, vp, cs, partials, sh, st);
-    return sh.lds ? launch_t<double, kLanczos, true>(s, x, y, vp, cs, partials, sh, st)
-                  : launch_t<double, kLanczos, false>(s, x, y, vp, cs, partials, sh, st);
+    if (f32) return sh.lds ? launch_t<float, kLanczos, true>(s, x, y, cs, partials, sh, st)
+                           : launch_t<float, kLanczos, false>(s, x, y, cs, partials, sh, st);
+    return sh.lds ? launch_t<double, kLanczos, true>(s, x, y, cs, partials, sh, st)
+                  : launch_t<double, kLanczos, false>(s, x, y, cs, partials, sh, st);
 }
 
 int hxv_nparts(const dmed_sector *s, int prec)

@@ -86,6 +86,7 @@ __global__ void __launch_bounds__(1024) k_chain_init(const double *__restrict__ 
         cs->svp = 0.0;
         cs->b = 0.0;
         cs->alpha = 0.0;
+        cs->q = 0.0;
         cs->thr = thr;
         cs->step = 0;
         cs->done = (s == 0.0) ? 1 : 0;
@@ -102,28 +103,41 @@ __global__ void __launch_bounds__(1024) k_chain_alpha(const double *__restrict__
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
     s = bsum(s, red);
     if (threadIdx.x == 0) {
-        cs->alpha = s;
-        alpha[cs->step] = s;
+        // alpha = <v, Hv - b v_prev> = sx*(<x, y> - b*svp*<x, x_prev>)
+        const double a = cs->sx * (s - cs->b * cs->svp * cs->q);
+        cs->alpha = a;
+        alpha[cs->step] = a;
     }
 }
 
+// w = Hv - b*v_prev - alpha*v in place of y = Hv (stored scales sx, svp), with
+// the block partials of ||w||^2 (first gridDim.x) and <w, x> (next gridDim.x)
 template <typename T>
 __global__ void __launch_bounds__(kVecBS) k_chain_update(T *__restrict__ w, const T *__restrict__ v,
-                                                         int64_t n, const ChainScalars *cs,
+                                                         const T *__restrict__ vp, int64_t n,
+                                                         const ChainScalars *cs,
                                                          double *__restrict__ partials)
 {
     __shared__ double red[16];
     if (cs->done) return;
-    const double c = cs->alpha * cs->sx;
-    double s = 0.0;
+    const double ca = cs->alpha * cs->sx, cb = cs->b * cs->svp;
+    double s1 = 0.0, s2 = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const double wi = (double)w[i] - c * (double)v[i];
-        w[i] = (T)wi;
-        s += wi * wi;
+        const double vi = (double)v[i];
+        const T wt = (T)((double)w[i] - cb * (double)vp[i] - ca * vi);
+        w[i] = wt;
+        const double wi = (double)wt;
+        s1 += wi * wi;
+        s2 += wi * vi;
     }
-    s = bsum(s, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    s1 = bsum(s1, red);
+    __syncthreads();
+    s2 = bsum(s2, red);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s1;
+        partials[gridDim.x + blockIdx.x] = s2;
+    }
 }
 
 __global__ void __launch_bounds__(1024) k_chain_beta(const double *__restrict__ p, int n,
@@ -131,14 +145,20 @@ __global__ void __launch_bounds__(1024) k_chain_beta(const double *__restrict__ 
 {
     __shared__ double red[16];
     if (cs->done) return;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    double s = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        s += p[i];
+        q += p[n + i];
+    }
     s = bsum(s, red);
+    __syncthreads();
+    q = bsum(q, red);
     if (threadIdx.x == 0) {
         const double b = sqrt(s);
         const int k = cs->step;
         if (k + 1 < cs->nlanc) beta[k + 1] = b;
         cs->b = b;
+        cs->q = q;
         cs->svp = cs->sx;
         cs->sx = 1.0 / b;
         cs->step = k + 1;
@@ -374,17 +394,17 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
                                    (const double *)v, n, cs, chain_aux());
             DMED_HIP(hipGetLastError());
         }
-        if (int rc = launch_hxv(s, v, w, prec, kLanczos, vp, cs, ex().partials, nullptr, st))
+        if (int rc = launch_hxv(s, v, w, prec, kLanczos, cs, ex().partials, nullptr, st))
             return rc;
         hipLaunchKernelGGL(k_chain_alpha, dim3(1), dim3(1024), 0, st, ex().partials, nph, cs,
                            chain_alpha());
         DMED_HIP(hipGetLastError());
         if (prec == DMED_FP32)
             hipLaunchKernelGGL(k_chain_update<float>, dim3(g), dim3(kVecBS), 0, st, (float *)w,
-                               (const float *)v, n, cs, ex().partials);
+                               (const float *)v, (const float *)vp, n, cs, ex().partials);
         else
             hipLaunchKernelGGL(k_chain_update<double>, dim3(g), dim3(kVecBS), 0, st, (double *)w,
-                               (const double *)v, n, cs, ex().partials);
+                               (const double *)v, (const double *)vp, n, cs, ex().partials);
         DMED_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_chain_beta, dim3(1), dim3(1024), 0, st, ex().partials, (int)g, cs,
                            chain_beta());
@@ -421,7 +441,7 @@ int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int p
     const size_t n1 = (size_t)std::max<int64_t>(n, 1);
     const size_t esz = p0 == DMED_FP32 ? 4 : 8;
     int nparts = std::max(hxv_nparts(s, DMED_FP64), hxv_nparts(s, DMED_FP32));
-    if (int rc = ensure_partials(std::max(nparts, kVecGrid) + 16)) return rc;
+    if (int rc = ensure_partials(std::max(nparts, 2 * kVecGrid) + 16)) return rc;
     void *bufs[3];
     for (int k = 0; k < 3; ++k)
         if (int rc = exec_buf(ex(), kSlotChain + k, n1 * esz, &bufs[k])) return rc;
@@ -453,7 +473,7 @@ int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int p
 // ------------------------------------------------------ ground state
 int eigh_sector_ex(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
                    int dense_threshold, double degen, bool lowest_multiplet, int *nout,
-                   double *evals, dmed_vec **evecs, double *resid);
+                   double *evals, dmed_vec **evecs, double *resid, int gs_prec);
 constexpr int64_t kTrlMaxDim = 25000000;   // above: plain Lanczos ground state
 
 namespace {
@@ -514,7 +534,7 @@ int residual(const dmed_sector *s, const double *y, double theta, double *work, 
 {
         hipStream_t st = ex().stream;
     const int64_t n = s->dim;
-    if (int rc = launch_hxv(s, y, work, DMED_FP64, kPlain, nullptr, nullptr, nullptr, nullptr, st))
+    if (int rc = launch_hxv(s, y, work, DMED_FP64, kPlain, nullptr, nullptr, nullptr, st))
         return rc;
     if (int rc = launch_axpy(work, DMED_FP64, y, DMED_FP64, n, -theta, st)) return rc;
     int np = 0;
@@ -544,16 +564,23 @@ int dot_host(const double *x, const double *y, int64_t n, double *out)
 //           .repo/PLAIN_LANCZOS.f90:367-384);
 //   refine: explicit residual; if above tol, restart from psi.
 int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec **evec,
-             double *resid)
+             double *resid, int prec)
 {
-        hipStream_t st = ex().stream;
+    hipStream_t st = ex().stream;
+    if (prec != DMED_FP64 && prec != DMED_MIXED)
+        return fail(DMED_EINVAL, "ground-state precision must be DMED_FP64 or DMED_MIXED");
     const int64_t n = s->dim;
-    struct { void *p; } b0, b1, b2;
-    if (int rc = exec_buf(ex(), kSlotChain + 0, n * 8, &b0.p)) return rc;
-    if (int rc = exec_buf(ex(), kSlotChain + 1, n * 8, &b1.p)) return rc;
-    if (int rc = exec_buf(ex(), kSlotChain + 2, n * 8, &b2.p)) return rc;
+    const size_t n1 = (size_t)std::max<int64_t>(n, 1);
+    void *b64[3], *b32[3] = {nullptr, nullptr, nullptr};
+    for (int k = 0; k < 3; ++k)
+        if (int rc = exec_buf(ex(), kSlotChain + k, n1 * 8, &b64[k])) return rc;
+    if (prec == DMED_MIXED)
+        for (int k = 0; k < 3; ++k)
+            if (int rc = exec_buf(ex(), kSlotChainLo + k, n1 * 4, &b32[k])) return rc;
+    const int nparts = std::max(hxv_nparts(s, DMED_FP64), hxv_nparts(s, DMED_FP32));
+    if (int rc = ensure_partials(std::max(nparts, 2 * kVecGrid) + 16)) return rc;
     double *psi = nullptr;
-    DMED_HIP(hipMalloc(&psi, n * 8));
+    DMED_HIP(hipMalloc(&psi, n1 * 8));
     dmed_vec *out = new dmed_vec{s, DMED_FP64, n, psi};
     auto cleanup = [&](int rc) {
         if (rc) { hipFree(out->data); delete out; }
@@ -564,20 +591,28 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
     int used = 0;
     double theta = 0.0, res = INFINITY;
     std::vector<double> a(kChainMax), b(kChainMax), z(kChainMax);
-    for (int restart = 0; restart < 8 && used < maxit; ++restart) {
+    // DMED_MIXED: restarts run the recurrence on FP32 vectors (FP64 scalars
+    // and reductions) until the FP64 residual of the accumulated vector is
+    // below kMixedSwitch; the later restarts, which take the residual down to
+    // tol, run in FP64.  The Rayleigh quotient and the residual are FP64.
+    constexpr double kMixedSwitch = 1e-5;
+    int stage = prec == DMED_MIXED ? DMED_FP32 : DMED_FP64;
+    for (int restart = 0; restart < 12 && used < maxit; ++restart) {
+        void **bb = stage == DMED_FP32 ? b32 : b64;
+        const double stol = stage == DMED_FP32 ? std::max(tol, 1e-7) : tol;
         // pass 1, in chunks of ncheck steps with a host convergence check
-        if (int rc = launch_scale_copy(b1.p, DMED_FP64, psi, DMED_FP64, n, 1.0, st))
+        if (int rc = launch_scale_copy(bb[1], stage, psi, DMED_FP64, n, 1.0, st))
             return cleanup(rc);
         DMED_HIP(hipMemsetAsync(chain_alpha(), 0, kChainMax * sizeof(double), st));
         DMED_HIP(hipMemsetAsync(chain_beta(), 0, kChainMax * sizeof(double), st));
         const int budget = std::min(kChainMax - 1, maxit - used);
-        void *bufs[3] = {b0.p, b1.p, b2.p};
-        if (int rc = chain_init(s, bufs, DMED_FP64, budget, 0.0)) return cleanup(rc);
+        void *bufs[3] = {bb[0], bb[1], bb[2]};
+        if (int rc = chain_init(s, bufs, stage, budget, 0.0)) return cleanup(rc);
         int steps = 0;
         ChainScalars cs{};
         for (;;) {
             const int chunk = std::min(ncheck, budget - steps);
-            if (int rc = chain_steps(s, bufs, DMED_FP64, chunk, nullptr)) return cleanup(rc);
+            if (int rc = chain_steps(s, bufs, stage, chunk, nullptr)) return cleanup(rc);
             steps += chunk;
             DMED_HIP(hipMemcpyAsync(a.data(), chain_alpha(), steps * 8, hipMemcpyDeviceToHost, st));
             DMED_HIP(hipMemcpyAsync(b.data(), chain_beta(), (steps + 1) * 8, hipMemcpyDeviceToHost, st));
@@ -587,34 +622,36 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
             if (m < 1) return cleanup(fail(DMED_ENORM, "zero start vector in the ground-state Lanczos"));
             if (int rc = tridiag_lowest(m, a.data(), b.data(), &theta, z.data())) return cleanup(rc);
             const double est = fabs(cs.b * z[m - 1]);
-            if (est <= 0.1 * tol * std::max(1.0, fabs(theta)) || cs.done || steps >= budget) {
+            if (est <= 0.1 * stol * std::max(1.0, fabs(theta)) || cs.done || steps >= budget) {
                 steps = m;
                 break;
             }
         }
         used += steps;
-        // pass 2: rerun the recurrence from the same start accumulating psi
+        // pass 2: rerun the recurrence from the same start accumulating psi (FP64)
         DMED_HIP(hipMemcpyAsync(chain_aux(), z.data(), steps * 8, hipMemcpyHostToDevice, st));
-        if (int rc = launch_scale_copy(b1.p, DMED_FP64, psi, DMED_FP64, n, 1.0, st))
+        if (int rc = launch_scale_copy(bb[1], stage, psi, DMED_FP64, n, 1.0, st))
             return cleanup(rc);
         DMED_HIP(hipMemsetAsync(psi, 0, n * 8, st));
-        void *bufs2[3] = {b0.p, b1.p, b2.p};
-        if (int rc = chain_init(s, bufs2, DMED_FP64, steps, 0.0)) return cleanup(rc);
-        if (int rc = chain_steps(s, bufs2, DMED_FP64, steps, psi)) return cleanup(rc);
+        void *bufs2[3] = {bb[0], bb[1], bb[2]};
+        if (int rc = chain_init(s, bufs2, stage, steps, 0.0)) return cleanup(rc);
+        if (int rc = chain_steps(s, bufs2, stage, steps, psi)) return cleanup(rc);
         used += steps;
-        // normalise, Rayleigh quotient and explicit residual
+        // normalise, FP64 Rayleigh quotient and explicit FP64 residual
         double nn = 0.0;
         if (int rc = dot_host(psi, psi, n, &nn)) return cleanup(rc);
         if (int rc = launch_scale_copy(psi, DMED_FP64, psi, DMED_FP64, n, 1.0 / sqrt(nn), st))
             return cleanup(rc);
-        if (int rc = launch_hxv(s, psi, b2.p, DMED_FP64, kPlain, nullptr, nullptr, nullptr, nullptr, st))
+        if (int rc = launch_hxv(s, psi, b64[2], DMED_FP64, kPlain, nullptr, nullptr, nullptr, st))
             return cleanup(rc);
         double rq = 0.0;
-        if (int rc = dot_host(psi, (const double *)b2.p, n, &rq)) return cleanup(rc);
+        if (int rc = dot_host(psi, (const double *)b64[2], n, &rq)) return cleanup(rc);
         theta = rq;
-        if (int rc = residual(s, psi, theta, (double *)b2.p, &res)) return cleanup(rc);
+        if (int rc = residual(s, psi, theta, (double *)b64[2], &res)) return cleanup(rc);
         used += 2;
         if (res <= tol * std::max(1.0, fabs(theta))) break;
+        if (stage == DMED_FP32 && (res <= kMixedSwitch * std::max(1.0, fabs(theta)) || restart >= 3))
+            stage = DMED_FP64;
     }
     *eval = theta;
     if (resid) *resid = res;
@@ -688,7 +725,7 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
     for (;;) {
         for (int j = k; j < m; ++j) {
             double *vj = V + (size_t)j * n;
-            if (int rc = launch_hxv(s, vj, w, DMED_FP64, kPlain, nullptr, nullptr, nullptr, nullptr, st))
+            if (int rc = launch_hxv(s, vj, w, DMED_FP64, kPlain, nullptr, nullptr, nullptr, st))
                 return rc;
             ++used;
             if (int rc = orth(j + 1, c1.data())) return rc;
@@ -828,13 +865,20 @@ int eigh_sector(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
 {
     int nout = 0;
     const int rc = eigh_sector_ex(s, neig, ncv, maxit, tol, dense_threshold, 0.0, false, &nout,
-                                  evals, evecs, resid);
+                                  evals, evecs, resid, DMED_FP64);
     return rc;
+}
+
+int ground_state(const dmed_sector *s, int prec, int maxit, double tol, double *eval,
+                 dmed_vec **evec, double *resid)
+{
+    if (s->dim < 1) return fail(DMED_EINVAL, "empty sector");
+    return plain_gs(s, maxit, tol, eval, evec, resid, prec);
 }
 
 int eigh_sector_ex(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
                    int dense_threshold, double degen, bool lowest_multiplet, int *nout,
-                   double *evals, dmed_vec **evecs, double *resid)
+                   double *evals, dmed_vec **evecs, double *resid, int gs_prec)
 {
     const int64_t n = s->dim;
     if (neig < 1 || neig > n) return fail(DMED_EINVAL, "neig out of range");
@@ -872,7 +916,7 @@ int eigh_sector_ex(const dmed_sector *s, int neig, int ncv, int maxit, double to
     const double need = (double)(m + 2 + 2 * neig) * n * 8.0;
     if ((neig == 1 || lowest_multiplet) && (need > 0.5 * (double)freeb || n > kTrlMaxDim)) {
         *nout = 1;
-        return plain_gs(s, maxit, tol, evals, evecs ? &evecs[0] : nullptr, resid);
+        return plain_gs(s, maxit, tol, evals, evecs ? &evecs[0] : nullptr, resid, gs_prec);
     }
     if (need > 0.8 * (double)freeb) return fail(DMED_ENOMEM, "Krylov basis does not fit");
     return trl_deflated(s, neig, m, maxit, tol, degen, lowest_multiplet, nout, evals, evecs, resid);

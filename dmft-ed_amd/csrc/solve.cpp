@@ -182,6 +182,8 @@ extern "C" void dmed_solve_params_default(dmed_solve_params *p)
     p->batched = 0;
     p->verbose = 0;
     p->gf_fp64_head = kMixedHead;
+    p->prec_gs = DMED_FP64;
+    p->pad_ = 0;
 }
 
 extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
@@ -190,6 +192,10 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     if (!c.initialized) return fail(DMED_ESTATE, "dmed_init not called");
     if (!c.has_model) return fail(DMED_ESTATE, "dmed_set_model not called");
     if (!p || !out) return fail(DMED_EINVAL, "null argument");
+    if (p->prec_gf != DMED_FP64 && p->prec_gf != DMED_FP32 && p->prec_gf != DMED_MIXED)
+        return fail(DMED_EINVAL, "prec_gf must be DMED_FP64, DMED_FP32 or DMED_MIXED");
+    if (p->prec_gs != DMED_FP64 && p->prec_gs != DMED_MIXED)
+        return fail(DMED_EINVAL, "prec_gs must be DMED_FP64 or DMED_MIXED");
     const Model &m = c.model;
     const int ns = m.ns, norb = m.norb, nspin = m.nspin;
     const int nsectors = (ns + 1) * (ns + 1);
@@ -247,7 +253,8 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
         // degenerate copies can enter; finite T needs the ne lowest.
         r.rc = eigh_sector_ex(sec, ne, std::max(nblock, 2 * ne + 10), 50 * std::max(p->lanc_niter, 100),
                               p->lanc_tolerance, p->lanc_dim_threshold, 10.0 * p->gs_threshold,
-                              !finite_t, &r.nout, r.ev.data(), r.vecs.data(), res.data());
+                              !finite_t, &r.nout, r.ev.data(), r.vecs.data(), res.data(),
+                              p->prec_gs);
         if (r.rc == DMED_ENOCONV) r.rc = 0;      // ARPACK returns its best pairs too
         if (r.rc) r.err = dmed_last_error();
     };

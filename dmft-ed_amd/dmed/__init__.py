@@ -45,7 +45,8 @@ class SolveParams(C.Structure):
                 ("cutoff", C.c_double), ("beta", C.c_double), ("eps", C.c_double),
                 ("wini", C.c_double), ("wfin", C.c_double), ("lmats", C.c_int32),
                 ("lreal", C.c_int32), ("prec_gf", C.c_int32), ("batched", C.c_int32),
-                ("verbose", C.c_int32), ("gf_fp64_head", C.c_int32)]
+                ("verbose", C.c_int32), ("gf_fp64_head", C.c_int32),
+                ("prec_gs", C.c_int32), ("pad_", C.c_int32)]
 
 
 class SolveOut(C.Structure):
@@ -83,6 +84,8 @@ SIGNATURES = [
     ("dmed_hxv_timed", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, _dp]),
     ("dmed_eigh", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                             C.c_int32, _dp, C.POINTER(C.c_void_p), _dp]),
+    ("dmed_ground_state", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, _dp,
+                                    C.POINTER(C.c_void_p), _dp]),
     ("dmed_apply_op", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, _dp]),
     ("dmed_chain", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, _dp,
                              _dp, _i32p]),
@@ -199,6 +202,16 @@ class Sector:
             _check(rc)
         out = [Vec.wrap(self, C.c_void_p(v), FP64) for v in vecs] if vectors else None
         return ev, out, res, rc
+
+    def ground_state(self, prec=FP64, maxit=0, tol=1e-12):
+        """Plain-Lanczos ground state (FP64 or MIXED): (E0, Vec, residual, rc)."""
+        e = C.c_double()
+        r = C.c_double()
+        h = C.c_void_p()
+        rc = lib().dmed_ground_state(self.h, prec, maxit, tol, C.byref(e), C.byref(h), C.byref(r))
+        if rc not in (0, -6):
+            _check(rc)
+        return e.value, Vec.wrap(self, h, FP64), r.value, rc
 
     def vec(self, data=None, prec=FP64):
         v = Vec(self, prec)

@@ -20,7 +20,7 @@ module DMED_CABI
      integer(c_int32_t) :: lanc_niter, lanc_ngfiter, lanc_ncv_factor, lanc_ncv_add
      integer(c_int32_t) :: lanc_dim_threshold
      real(c_double)     :: lanc_tolerance, gs_threshold, cutoff, beta, eps, wini, wfin
-     integer(c_int32_t) :: lmats, lreal, prec_gf, batched, verbose, gf_fp64_head
+     integer(c_int32_t) :: lmats, lreal, prec_gf, batched, verbose, gf_fp64_head, prec_gs, pad_
   end type dmed_solve_params_t
 
   type,bind(C),public :: dmed_solve_out_t

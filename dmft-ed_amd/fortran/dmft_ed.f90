@@ -45,6 +45,7 @@ module DMFT_ED
   !> GPU-path extensions (not in the reference): FP32 Green's-function chains
   !> and the batched sector engine
   logical,public              :: ed_gpu_fp32_gf=.false., ed_gpu_batched=.false.
+  logical,public              :: ed_gpu_mixed_gs=.false.
 
   !-------------------------------------------------- solver state
   complex(8),allocatable,public :: impGmats(:,:,:,:,:), impGreal(:,:,:,:,:)
@@ -167,6 +168,7 @@ contains
     case("ED_VERBOSE"); read(v,*,iostat=ios)ed_verbose
     case("ED_GPU_FP32_GF"); read(v,*,iostat=ios)ed_gpu_fp32_gf
     case("ED_GPU_BATCHED"); read(v,*,iostat=ios)ed_gpu_batched
+    case("ED_GPU_MIXED_GS"); read(v,*,iostat=ios)ed_gpu_mixed_gs
     case default
     end select
   end subroutine set_var
@@ -379,6 +381,7 @@ contains
     p%lmats = Lmats; p%lreal = Lreal
     p%prec_gf = merge(DMED_MIXED,DMED_FP64,ed_gpu_fp32_gf)
     p%batched = merge(1,0,ed_gpu_batched)
+    p%prec_gs = merge(DMED_MIXED,DMED_FP64,ed_gpu_mixed_gs)
     p%verbose = ed_verbose
     allocate(dens(Norb), docc(Norb))
     allocate(gm(Nspin,Nspin,Norb,Norb,Lmats), sm(Nspin,Nspin,Norb,Norb,Lmats))

@@ -141,6 +141,17 @@ int dmed_eigh(const dmed_sector *s, int32_t neig, int32_t ncv, int32_t maxit,
               double tol, int32_t dense_threshold, double *evals,
               dmed_vec **evecs, double *resid);
 
+/* Ground state by two-pass plain Lanczos with explicit-residual restarts
+ * (lanczos_plain_c, .repo/PLAIN_LANCZOS.f90:286-385; the solver the
+ * reference names for sectors where a stored Krylov basis does not fit).
+ * prec = DMED_FP64, or DMED_MIXED: the first restarts run the recurrence on
+ * FP32 vectors (FP64 scalars and reductions) until the FP64 residual of the
+ * accumulated vector is below 1e-5*max(1,|E|); the remaining restarts run in
+ * FP64.  The Rayleigh quotient and residual are always FP64.
+ *   evec : handle created by the call (FP64, normalised), or NULL */
+int dmed_ground_state(const dmed_sector *s, int32_t prec, int32_t maxit, double tol,
+                      double *eval, dmed_vec **evec, double *resid);
+
 /* -------------------------------------------------- Green's function */
 /* out = c^+_{isite} psi (dagger=1) or c_{isite} psi (dagger=0), where
  * isite = iorb + (ispin-1)*Ns is the reference's 1-based spin-orbital index
@@ -198,6 +209,8 @@ typedef struct dmed_solve_params {
     int32_t batched;               /* 1: batched sector/chain engine */
     int32_t verbose;
     int32_t gf_fp64_head;          /* 16: FP64 steps of a DMED_MIXED chain */
+    int32_t prec_gs;               /* DMED_FP64 / DMED_MIXED plain-Lanczos ground states */
+    int32_t pad_;
 } dmed_solve_params;
 void dmed_solve_params_default(dmed_solve_params *p);
 
@@ -224,10 +237,11 @@ int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out);
 
 /* Sigma = G0^-1 - G^-1 of the normal bath on arbitrary points z
  * (get_sigma_normal.f90:27-38; invg0_bath_mats, ED_BATH_FUNCTIONS.f90:1784-1807).
- * z, g, sigma: interleaved complex of length L. */
+ * ispin, iorb: 0-based.  z, g, sigma: interleaved complex of length L. */
 int dmed_sigma(int32_t ispin, int32_t iorb, const double *z, const double *g,
                int32_t L, double *sigma);
-/* Delta(z) = sum_k V_k^2/(z-e_k) (delta_bath_mats, ED_BATH_FUNCTIONS.f90:221-258) */
+/* Delta(z) = sum_k V_k^2/(z-e_k) (delta_bath_mats, ED_BATH_FUNCTIONS.f90:221-258);
+ * ispin, iorb 0-based */
 int dmed_delta(int32_t ispin, int32_t iorb, const double *z, int32_t L, double *delta);
 
 /* chi^2 fit of one orbital's normal bath (e_k, V_k), in place, to fg on the

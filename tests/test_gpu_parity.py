@@ -179,6 +179,15 @@ def test_ground_state_energy_vs_oracle(cfg):
     psi = vecs[0].download()
     r = edref.hxv(m, psi, nthreads=8).real - ev[0] * psi
     assert np.linalg.norm(r) <= 1e-11
+    # the plain-Lanczos solver, FP64 and mixed (FP32 restarts, FP64 finish)
+    for prec in (dmed.FP64, dmed.MIXED):
+        say(f"[config {cfg}] plain-Lanczos GS prec={prec}")
+        e0, v, res0, rc0 = s.ground_state(prec=prec, tol=1e-12)
+        assert rc0 == 0 and res0 <= 1e-12 * max(1, abs(e0))
+        assert abs(e0 - w[0]) <= 1e-10 * abs(w[0])
+        psi = v.download()
+        r = edref.hxv(m, psi, nthreads=8).real - e0 * psi
+        assert np.linalg.norm(r) <= 1e-11
 
 
 def test_ns16_u0_ground_state_closed_form():
@@ -198,6 +207,9 @@ def test_ns16_u0_ground_state_closed_form():
     s = dmed.Sector(8, 8)
     ev, vecs, res, rc = s.eigh(1, tol=1e-10, dense_threshold=0, maxit=3000)
     assert abs(ev[0] - exact) <= 1e-10 * abs(exact)
+    say("[config 5, U=0] GS, mixed precision")
+    e0, v, res0, rc0 = s.ground_state(prec=dmed.MIXED, tol=1e-10, maxit=3000)
+    assert rc0 == 0 and abs(e0 - exact) <= 1e-10 * abs(exact)
 
 
 @pytest.mark.parametrize("prec", [dmed.FP64, dmed.FP32, dmed.MIXED])
