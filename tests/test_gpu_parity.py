@@ -248,6 +248,75 @@ def test_chain_vs_oracle(prec):
     assert np.abs(g1 - g2).max() <= gtol
 
 
+def test_chain_and_gf_vs_oracle_config2():
+    """Workload 2 (Ns=12, GS sector 6/6, dim 853,776): particle and hole
+    chains of 300 steps from the GPU ground state, against the oracle's
+    complex(8) sp_lanc_tridiag restatement from the same start vectors:
+    first 20 alpha/beta to 1e-11, then G(iw) on 4096 Matsubara points at
+    beta=200 and Sigma(iw) to the 1e-8 parity bar."""
+    c = configs.config(2)
+    set_cfg(c)
+    nu, nd = c["sector"]
+    s = dmed.Sector(nu, nd)
+    ev, vecs, res, rc = s.eigh(1, tol=1e-12, dense_threshold=0)
+    assert rc == 0
+    e0 = ev[0]
+    psi = vecs[0].download()
+    mI = edref.build_sector(nu, nd)
+    wm = np.pi / c["beta"] * (2 * np.arange(c["lmats"]) + 1)
+    g_gpu = np.zeros(c["lmats"], complex)
+    g_ref = np.zeros(c["lmats"], complex)
+    gr = np.zeros(1, complex)
+    for (tu, td), dag, sgn in (((nu + 1, nd), True, 1), ((nu - 1, nd), False, -1)):
+        say(f"[config 2] chain into sector {tu}/{td}, {c['nlanc']} steps")
+        mJ = edref.build_sector(tu, td)
+        x = edref.apply_op(mI, psi, mJ, 1, dag).real
+        t = dmed.Sector(tu, td)
+        v, n2 = dmed.apply_op(vecs[0], 1, dag, t)
+        assert np.array_equal(v.download(), x)            # start vector bit-exact
+        a, b, n = dmed.chain(t, v, c["nlanc"])
+        ar, br, nr = edref.lanc_tridiag(mJ, x, c["nlanc"], nthreads=16)
+        assert n == nr
+        assert np.allclose(a[:20], ar[:20], atol=1e-11)
+        assert np.allclose(b[1:20], br[1:20], atol=1e-11)
+        n2r = float(x @ x)
+        assert abs(n2 - n2r) <= 1e-12 * n2r            # reduction order
+        dmed.gf_add_chain(n2, e0, a, b, sgn, wm, np.zeros(1), 0.01, g_gpu, gr)
+        edref.add_to_gf(n2r, e0, ar, br, sgn, wm, np.zeros(1), 0.01, g_ref, gr)
+    dg = np.abs(g_gpu - g_ref).max()
+    z = 1j * wm
+    ds = np.abs(dmed.sigma(0, 0, z, g_gpu) - edref.sigma(0, 0, z, g_ref)).max()
+    say(f"[config 2] max|dG| {dg:.2e}, max|dSigma| {ds:.2e}")
+    assert dg <= 1e-8 and ds <= 1e-8
+
+
+def test_config3_ground_state_and_chain_first_steps():
+    """Workload 3 (Ns=14, two-orbital Kanamori, GS sector 7/7, dim 1.18e7):
+    ground-state residual against the oracle's H*v, and the first 6
+    alpha/beta of the orbital-1 particle chain (sector 8/7, dim 1.03e7)
+    against the oracle from the same start vector (bounded oracle work)."""
+    c = configs.config(3)
+    set_cfg(c)
+    nu, nd = c["sector"]
+    s = dmed.Sector(nu, nd)
+    say("[config 3] GS")
+    ev, vecs, res, rc = s.eigh(1, tol=1e-12, dense_threshold=0)
+    assert rc == 0
+    psi = vecs[0].download()
+    mI = edref.build_sector(nu, nd)
+    r = edref.hxv(mI, psi, nthreads=16).real - ev[0] * psi
+    assert np.linalg.norm(r) <= 1e-11 * max(1.0, abs(ev[0]))
+    say("[config 3] chain first steps")
+    mJ = edref.build_sector(nu + 1, nd)
+    x = edref.apply_op(mI, psi, mJ, 1, True).real
+    t = dmed.Sector(nu + 1, nd)
+    v, n2 = dmed.apply_op(vecs[0], 1, True, t)
+    assert np.array_equal(v.download(), x)
+    a, b, n = dmed.chain(t, v, 6)
+    ar, br, nr = edref.lanc_tridiag(mJ, x, 6, nthreads=16)
+    assert np.allclose(a, ar, atol=1e-11) and np.allclose(b, br, atol=1e-11)
+
+
 def test_chain_random_start_first_steps():
     c = configs.config(1)
     set_cfg(c)
