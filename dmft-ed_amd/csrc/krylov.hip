@@ -242,6 +242,44 @@ __global__ void __launch_bounds__(1024) k_reduce_rows(const double *__restrict__
     if (threadIdx.x == 0) out[j] = s;
 }
 
+// out[j] = sum of row j of the partials; acc[j] = (first ? 0 : acc[j]) + out[j]
+__global__ void __launch_bounds__(1024) k_reduce_rows_acc(const double *__restrict__ p, int nb,
+                                                          double *__restrict__ out,
+                                                          double *__restrict__ acc, int first)
+{
+    __shared__ double red[16];
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[(int64_t)j * nb + i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) {
+        out[j] = s;
+        acc[j] = first ? s : acc[j] + s;
+    }
+}
+
+// *nrm = sqrt(sum p)
+__global__ void __launch_bounds__(1024) k_reduce_norm(const double *__restrict__ p, int n,
+                                                      double *__restrict__ nrm)
+{
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) *nrm = sqrt(s);
+}
+
+// dst = src / *nrm (0 when the norm is 0)
+__global__ void __launch_bounds__(kVecBS) k_scale_inv(double *__restrict__ dst,
+                                                      const double *__restrict__ src, int64_t n,
+                                                      const double *__restrict__ nrm)
+{
+    const double a = *nrm > 0.0 ? 1.0 / *nrm : 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = a * src[i];
+}
+
 // w -= sum_j V_j c_j
 __global__ void __launch_bounds__(kVecBS) k_multiaxpy(const double *__restrict__ V, int64_t n, int m,
                                                       const double *__restrict__ c,
@@ -679,61 +717,77 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
     double *V = nullptr, *w = nullptr, *dS = nullptr;
     if (int rc = exec_buf(ex(), kSlotTrlBasis, (size_t)(m + 1) * n * 8, (void **)&V)) return rc;
     if (int rc = exec_buf(ex(), kSlotTrlW, n * 8, (void **)&w)) return rc;
-    if (int rc = exec_buf(ex(), kSlotTrlSmall, (size_t)m * m * 8 + (size_t)(m + nl + 1) * 8, (void **)&dS)) return rc;
+    // dS: Ritz rotation (m*m), projection coefficients of one pass (m+nl+1),
+    // and per expansion step j the accumulated CGS2 coefficients (j+1) and
+    // the new norm, in rows of m+2 (kept on the device until the restart)
+    const size_t ldc = (size_t)m + 2;
+    if (int rc = exec_buf(ex(), kSlotTrlSmall,
+                          ((size_t)m * m + (size_t)(m + nl + 1) + (size_t)m * ldc) * 8, (void **)&dS))
+        return rc;
     double *dc = dS + (size_t)m * m;
+    double *dcoef = dc + (m + nl + 1);
     const unsigned g = vec_grid(n);
     if (int rc = ensure_partials((size_t)(std::max(m, nl) + 1) * g + 16)) return rc;
-    // project out the columns [0,jmax) of B from w
-    auto project = [&](const double *B, int jmax, double *cacc) -> int {
+    // project out the columns [0,jmax) of B from w; with cacc (device), the
+    // coefficients are accumulated there (first: overwrite).  No host sync.
+    auto project = [&](const double *B, int jmax, double *cacc, bool first) -> int {
         if (jmax <= 0) return 0;
         hipLaunchKernelGGL(k_multidot, dim3(g, jmax), dim3(kVecBS), 0, st, B, n, jmax, w, ex().partials);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(jmax), dim3(1024), 0, st, ex().partials, (int)g, dc);
-        DMED_HIP(hipGetLastError());
-        if (cacc) {
-            std::vector<double> cc(jmax);
-            DMED_HIP(hipMemcpyAsync(cc.data(), dc, jmax * 8, hipMemcpyDeviceToHost, st));
-            DMED_HIP(hipStreamSynchronize(st));
-            for (int q = 0; q < jmax; ++q) cacc[q] += cc[q];
-        }
+        if (cacc)
+            hipLaunchKernelGGL(k_reduce_rows_acc, dim3(jmax), dim3(1024), 0, st, ex().partials,
+                               (int)g, dc, cacc, first ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_reduce_rows, dim3(jmax), dim3(1024), 0, st, ex().partials, (int)g, dc);
         hipLaunchKernelGGL(k_multiaxpy, dim3(g), dim3(kVecBS), 0, st, B, n, jmax, dc, w);
         DMED_HIP(hipGetLastError());
         return 0;
     };
     auto orth = [&](int jmax, double *cacc) -> int {   // CGS2 against Y and V[0..jmax)
-        for (int q = 0; q < jmax; ++q) cacc[q] = 0.0;
         for (int pass = 0; pass < 2; ++pass) {
-            if (int rc = project(Y, nl, nullptr)) return rc;
-            if (int rc = project(V, jmax, cacc)) return rc;
+            if (int rc = project(Y, nl, nullptr, false)) return rc;
+            if (int rc = project(V, jmax, cacc, pass == 0)) return rc;
         }
         return 0;
     };
+    // dst = w/||w||, the norm left at nrm (device)
     auto normalize_into = [&](double *dst, double *nrm) -> int {
-        double nn = 0.0;
-        if (int rc = dot_host(w, w, n, &nn)) return rc;
-        *nrm = sqrt(nn);
-        return launch_scale_copy(dst, DMED_FP64, w, DMED_FP64, n, nn > 0 ? 1.0 / *nrm : 0.0, st);
+        int np = 0;
+        if (int rc = launch_dot(w, w, n, DMED_FP64, ex().partials, &np, st)) return rc;
+        hipLaunchKernelGGL(k_reduce_norm, dim3(1), dim3(1024), 0, st, ex().partials, np, nrm);
+        hipLaunchKernelGGL(k_scale_inv, dim3(g), dim3(kVecBS), 0, st, dst, w, n, nrm);
+        DMED_HIP(hipGetLastError());
+        return 0;
     };
-    std::vector<double> T((size_t)m * m, 0.0), S, theta(m), c1(m + 1);
+    std::vector<double> T((size_t)m * m, 0.0), S, theta(m), hc((size_t)m * ldc);
     // start vector in the complement of Y
     if (int rc = launch_fill_random(w, n, DMED_FP64, seed, st)) return rc;
-    if (int rc = orth(0, c1.data())) return rc;
-    double bn = 0.0;
-    if (int rc = normalize_into(V, &bn)) return rc;
+    if (int rc = orth(0, nullptr)) return rc;
+    if (int rc = normalize_into(V, dcoef + m + 1)) return rc;
     int k = 0, used = 0;
     double beta_last = 0.0;
     bool converged = false;
     for (;;) {
+        // expansion k..m-1 entirely on the stream; one copy of the
+        // coefficients per restart
         for (int j = k; j < m; ++j) {
             double *vj = V + (size_t)j * n;
             if (int rc = launch_hxv(s, vj, w, DMED_FP64, kPlain, nullptr, nullptr, nullptr, st))
                 return rc;
             ++used;
-            if (int rc = orth(j + 1, c1.data())) return rc;
+            double *cj = dcoef + (size_t)j * ldc;
+            if (int rc = orth(j + 1, cj)) return rc;
+            if (int rc = normalize_into(V + (size_t)(j + 1) * n, cj + m + 1)) return rc;
+        }
+        DMED_HIP(hipMemcpyAsync(hc.data() + (size_t)k * ldc, dcoef + (size_t)k * ldc,
+                                (size_t)(m - k) * ldc * 8, hipMemcpyDeviceToHost, st));
+        DMED_HIP(hipStreamSynchronize(st));
+        for (int j = k; j < m; ++j) {
+            const double *c1 = hc.data() + (size_t)j * ldc;
             for (int q = 0; q <= j; ++q) {
                 const bool arrow = (j == k) && (q < k);
                 if (arrow || q >= j - 1) T[(size_t)q * m + j] = T[(size_t)j * m + q] = c1[q];
             }
-            if (int rc = normalize_into(V + (size_t)(j + 1) * n, &bn)) return rc;
+            const double bn = c1[m + 1];
             if (j + 1 < m) T[(size_t)(j + 1) * m + j] = T[(size_t)j * m + j + 1] = bn;
             beta_last = bn;
         }
