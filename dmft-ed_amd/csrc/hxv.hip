@@ -399,13 +399,18 @@ static LaunchShape pick_shape(const dmed_sector *s, int prec)
     const size_t wtb = 8 * 64 * (size_t)s->nimp;
     const size_t lds_budget = c.lds_max - 1024;
     sh.lds = colb + wtb + 16 <= lds_budget;
-    // aim for ~4-8K elements per workgroup, bounded by the LDS budget
-    int cb = (int)std::max<int64_t>(1, std::min<int64_t>(ddw, 8192 / std::max<int64_t>(dup, 1)));
+    // ~2K elements per workgroup, but never fewer than two workgroups per CU
+    // while the sector has columns for them; at least ~256 elements each
+    const int64_t dupc = std::max<int64_t>(dup, 1);
+    int64_t cb = std::max<int64_t>(1, (2048 + dupc - 1) / dupc);
+    const int64_t fill = std::max<int64_t>(1, ddw / (2 * std::max(c.num_cu, 1)));
+    if (cb > fill) cb = std::max<int64_t>(fill, (256 + dupc - 1) / dupc);
+    cb = std::min<int64_t>(std::min<int64_t>(cb, ddw), 64);
     if (sh.lds)
         while (cb > 1 && ((size_t)cb * colb + 16 + 8 * (size_t)cb * s->nimp) > 64 * 1024) --cb;
-    sh.cb = std::min<int>(cb, 64);
+    sh.cb = (int)std::max<int64_t>(cb, 1);
     const int64_t nel = (int64_t)sh.cb * dup;
-    sh.bs = nel >= 8192 ? 1024 : (nel >= 2048 ? 512 : 256);
+    sh.bs = nel >= 4096 ? 1024 : (nel >= 1024 ? 512 : 256);
     sh.shmem = (sh.lds ? (((size_t)sh.cb * colb + 15) & ~(size_t)15) : 0) +
                8 * (size_t)sh.cb * s->nimp;
     sh.grid = (unsigned)((ddw + sh.cb - 1) / sh.cb);

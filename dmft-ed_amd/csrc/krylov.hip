@@ -218,9 +218,11 @@ __global__ void __launch_bounds__(kVecBS) k_fill_random(T *__restrict__ x, int64
 // c[j] partials over rows: partials[j*nb + blk] = sum_i V[j*n+i] * w[i], j < m
 __global__ void __launch_bounds__(kVecBS) k_multidot(const double *__restrict__ V, int64_t n, int m,
                                                      const double *__restrict__ w,
-                                                     double *__restrict__ partials)
+                                                     double *__restrict__ partials,
+                                                     const int *__restrict__ gate)
 {
     __shared__ double red[16];
+    if (gate && !*gate) return;
     const int j = blockIdx.y;
     const double *vj = V + (int64_t)j * n;
     double s = 0.0;
@@ -231,23 +233,14 @@ __global__ void __launch_bounds__(kVecBS) k_multidot(const double *__restrict__ 
     if (threadIdx.x == 0) partials[(int64_t)j * gridDim.x + blockIdx.x] = s;
 }
 
-__global__ void __launch_bounds__(1024) k_reduce_rows(const double *__restrict__ p, int nb,
-                                                      double *__restrict__ out)
-{
-    __shared__ double red[16];
-    const int j = blockIdx.x;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[(int64_t)j * nb + i];
-    s = bsum(s, red);
-    if (threadIdx.x == 0) out[j] = s;
-}
-
 // out[j] = sum of row j of the partials; acc[j] = (first ? 0 : acc[j]) + out[j]
 __global__ void __launch_bounds__(1024) k_reduce_rows_acc(const double *__restrict__ p, int nb,
                                                           double *__restrict__ out,
-                                                          double *__restrict__ acc, int first)
+                                                          double *__restrict__ acc, int first,
+                                                          const int *__restrict__ gate)
 {
     __shared__ double red[16];
+    if (gate && !*gate) return;
     const int j = blockIdx.x;
     double s = 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[(int64_t)j * nb + i];
@@ -256,6 +249,19 @@ __global__ void __launch_bounds__(1024) k_reduce_rows_acc(const double *__restri
         out[j] = s;
         acc[j] = first ? s : acc[j] + s;
     }
+}
+
+// DGKS test after one Gram-Schmidt pass: *gate = 1 when ||w||^2 (sum of p)
+// fell below half of the squared norm before it (*n0sq), i.e. cancellation
+// large enough that a second pass is needed
+__global__ void __launch_bounds__(1024) k_dgks(const double *__restrict__ p, int n,
+                                               const double *__restrict__ n0sq, int *__restrict__ gate)
+{
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) *gate = (s < 0.5 * *n0sq) ? 1 : 0;
 }
 
 // *nrm = sqrt(sum p)
@@ -283,8 +289,10 @@ __global__ void __launch_bounds__(kVecBS) k_scale_inv(double *__restrict__ dst,
 // w -= sum_j V_j c_j
 __global__ void __launch_bounds__(kVecBS) k_multiaxpy(const double *__restrict__ V, int64_t n, int m,
                                                       const double *__restrict__ c,
-                                                      double *__restrict__ w)
+                                                      double *__restrict__ w,
+                                                      const int *__restrict__ gate)
 {
+    if (gate && !*gate) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         double s = 0.0;
@@ -722,31 +730,47 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
     // the new norm, in rows of m+2 (kept on the device until the restart)
     const size_t ldc = (size_t)m + 2;
     if (int rc = exec_buf(ex(), kSlotTrlSmall,
-                          ((size_t)m * m + (size_t)(m + nl + 1) + (size_t)m * ldc) * 8, (void **)&dS))
+                          ((size_t)m * m + (size_t)(m + nl + 1) + (size_t)m * ldc + 4) * 8,
+                          (void **)&dS))
         return rc;
     double *dc = dS + (size_t)m * m;
     double *dcoef = dc + (m + nl + 1);
     const unsigned g = vec_grid(n);
     if (int rc = ensure_partials((size_t)(std::max(m, nl) + 1) * g + 16)) return rc;
     // project out the columns [0,jmax) of B from w; with cacc (device), the
-    // coefficients are accumulated there (first: overwrite).  No host sync.
-    auto project = [&](const double *B, int jmax, double *cacc, bool first) -> int {
+    // coefficients are accumulated there (first: overwrite).  `gate`
+    // (device flag) skips the projection when 0.  No host sync.
+    int *dgate = reinterpret_cast<int *>(dcoef + (size_t)m * ldc);
+    double *dn0 = dcoef + (size_t)m * ldc + 1;
+    auto project = [&](const double *B, int jmax, double *cacc, bool first, const int *gate) -> int {
         if (jmax <= 0) return 0;
-        hipLaunchKernelGGL(k_multidot, dim3(g, jmax), dim3(kVecBS), 0, st, B, n, jmax, w, ex().partials);
+        hipLaunchKernelGGL(k_multidot, dim3(g, jmax), dim3(kVecBS), 0, st, B, n, jmax, w,
+                           ex().partials, gate);
         if (cacc)
             hipLaunchKernelGGL(k_reduce_rows_acc, dim3(jmax), dim3(1024), 0, st, ex().partials,
-                               (int)g, dc, cacc, first ? 1 : 0);
+                               (int)g, dc, cacc, first ? 1 : 0, gate);
         else
-            hipLaunchKernelGGL(k_reduce_rows, dim3(jmax), dim3(1024), 0, st, ex().partials, (int)g, dc);
-        hipLaunchKernelGGL(k_multiaxpy, dim3(g), dim3(kVecBS), 0, st, B, n, jmax, dc, w);
+            hipLaunchKernelGGL(k_reduce_rows_acc, dim3(jmax), dim3(1024), 0, st, ex().partials,
+                               (int)g, dc, dc, 1, gate);
+        hipLaunchKernelGGL(k_multiaxpy, dim3(g), dim3(kVecBS), 0, st, B, n, jmax, dc, w, gate);
         DMED_HIP(hipGetLastError());
         return 0;
     };
-    auto orth = [&](int jmax, double *cacc) -> int {   // CGS2 against Y and V[0..jmax)
-        for (int pass = 0; pass < 2; ++pass) {
-            if (int rc = project(Y, nl, nullptr, false)) return rc;
-            if (int rc = project(V, jmax, cacc, pass == 0)) return rc;
-        }
+    // classical Gram-Schmidt against Y and V[0..jmax), repeated once when the
+    // first pass cancelled more than half of ||w||^2 (Daniel-Gragg-Kaufman-
+    // Stewart criterion), decided on the device
+    auto orth = [&](int jmax, double *cacc) -> int {
+        if (jmax + nl <= 0) return 0;
+        int np = 0;
+        if (int rc = launch_dot(w, w, n, DMED_FP64, ex().partials, &np, st)) return rc;
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, ex().partials, np, dn0, 0);
+        if (int rc = project(Y, nl, nullptr, false, nullptr)) return rc;
+        if (int rc = project(V, jmax, cacc, true, nullptr)) return rc;
+        if (int rc = launch_dot(w, w, n, DMED_FP64, ex().partials, &np, st)) return rc;
+        hipLaunchKernelGGL(k_dgks, dim3(1), dim3(1024), 0, st, ex().partials, np, dn0, dgate);
+        if (int rc = project(Y, nl, nullptr, false, dgate)) return rc;
+        if (int rc = project(V, jmax, cacc, false, dgate)) return rc;
+        DMED_HIP(hipGetLastError());
         return 0;
     };
     // dst = w/||w||, the norm left at nrm (device)
