@@ -880,11 +880,28 @@ int trl_deflated(const dmed_sector *s, int neig, int m, int maxit, double tol, d
     std::vector<Kept> kept;
     std::vector<double> th(want), rr(want);
     int used = 0, rcfinal = 0;
+    constexpr double confirm_tol = 1e-6;
     for (int round = 0; round <= neig; ++round) {
         const int nl = (int)kept.size();
         if (n - nl < want) break;
-        int rc = trl_run(s, want, m, std::max(maxit - used, 4 * m), tol, Y, nl,
+        // rounds after the first only have to show that the complement holds
+        // nothing below the kept levels: a loose solve settles that unless a
+        // Ritz value, less its residual bound, comes within reach
+        const double tol_r = (nl > 0 && confirm_tol > tol) ? confirm_tol : tol;
+        int rc = trl_run(s, want, m, std::max(maxit - used, 4 * m), tol_r, Y, nl,
                          1234567ull + 7919ull * round, th.data(), X, rr.data(), &used);
+        if (tol_r > tol && (rc == 0 || rc == DMED_ENOCONV)) {
+            bool reach = false;
+            for (int q = 0; q < want; ++q) {
+                const double lo = th[q] - rr[q];
+                if (lowest_multiplet ? (lo <= kept[0].e + degen)
+                                     : ((int)kept.size() < neig || lo < kept.back().e))
+                    reach = true;
+            }
+            if (!reach) break;
+            rc = trl_run(s, want, m, std::max(maxit - used, 4 * m), tol, Y, nl,
+                         1234567ull + 7919ull * round, th.data(), X, rr.data(), &used);
+        }
         if (rc == DMED_ENOCONV) rcfinal = rc;
         else if (rc) return rc;
         bool added = false;

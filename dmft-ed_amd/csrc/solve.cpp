@@ -235,11 +235,16 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     };
     std::vector<SectorRes> sres(nsectors);
     std::vector<dmed_sector *> secs(nsectors, nullptr);
+    const auto tsec0 = std::chrono::steady_clock::now();
     for (int nu = 0; nu <= ns && !rc; ++nu)
         for (int nd = 0; nd <= ns && !rc; ++nd)
             rc = cache.get(nu, nd, ns, &secs[nu * (ns + 1) + nd]);
     if (rc) return rc;
+    if (p->verbose > 1)
+        std::fprintf(stderr, "sector tables: %.3f s\n",
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - tsec0).count());
     auto solve_sector = [&](int is) {
+        const auto t0s = std::chrono::steady_clock::now();
         dmed_sector *sec = secs[is];
         const int64_t dim = sec->dim;
         const int ne = (int)std::min<int64_t>(dim, neigen[is]);
@@ -257,6 +262,10 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
                               p->prec_gs);
         if (r.rc == DMED_ENOCONV) r.rc = 0;      // ARPACK returns its best pairs too
         if (r.rc) r.err = dmed_last_error();
+        if (p->verbose > 1)
+            std::fprintf(stderr, "sector %d dim=%lld neig=%d nout=%d %.3f ms\n", is, (long long)dim,
+                         ne, r.nout,
+                         1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0s).count());
     };
     {
         std::vector<int> order(nsectors);

@@ -420,6 +420,16 @@ def test_solve_vs_oracle(name, finite):
         for a in range(norb):
             assert np.abs(r["gmats"][s, s, a, a] - o["gmats"][s, a]).max() < 1e-8
             assert np.abs(r["smats"][s, s, a, a] - o["smats"][s, a]).max() < 1e-8
+            # real axis, z = w + i*eps (a reported check; the parity bar is on
+            # the Matsubara axis): near a pole |dG| ~ w*|d pole|/eps^2, so
+            # round-off in the chain's poles (ghost copies included) and the
+            # free basis of degenerate multiplets show as up to ~1e-4 pointwise
+            # at eps=0.01; compared in L2 over the grid, relative to the
+            # function's norm (or to an RMS of 1 where it vanishes, Sigma at U=0)
+            def rel(x, y):
+                return np.linalg.norm(x - y) / max(np.linalg.norm(y), np.sqrt(len(y)))
+            assert rel(r["greal"][s, s, a, a], o["greal"][s, a]) < 1e-4
+            assert rel(r["sreal"][s, s, a, a], o["sreal"][s, a]) < 1e-3
 
 
 def test_solve_config1_vs_oracle():
