@@ -111,7 +111,7 @@ constexpr int kSlotOpTable = 8;         // c / c^+ source-target tables
 constexpr int kSlotStart = 9;           // GF chain start vector
 constexpr int kSlotChainLo = 12;        // 12..14: FP32 tail of a mixed chain
 constexpr int kExecSlots = 16;
-constexpr int kMixedHead = 16;          // FP64 steps at the head of a mixed chain
+constexpr int kMixedHead = 32;          // FP64 steps at the head of a mixed chain
 
 struct Exec {
     hipStream_t stream = nullptr;

@@ -167,7 +167,7 @@ int dmed_apply_op(const dmed_vec *psi, int32_t isite, int32_t dagger,
  * sp_lanc_tridiag (build_gf_normal.f90:124-126; .repo/PLAIN_LANCZOS.f90:154-180),
  * stopping early when |b| < threshold; unset entries are zero.
  * prec = DMED_FP32 runs H*v and the vectors in FP32 with FP64 reductions;
- * prec = DMED_MIXED runs the first 16 steps in FP64 and the rest in FP32
+ * prec = DMED_MIXED runs the first 32 steps in FP64 and the rest in FP32
  * (the rounding of the first steps dominates the error of an FP32 chain).
  * nsteps receives the number of alphas produced. */
 int dmed_chain(const dmed_sector *s, const dmed_vec *v0, int32_t nlanc,
@@ -208,7 +208,7 @@ typedef struct dmed_solve_params {
     int32_t prec_gf;               /* DMED_FP64 / DMED_FP32 / DMED_MIXED chains */
     int32_t batched;               /* 1: batched sector/chain engine */
     int32_t verbose;
-    int32_t gf_fp64_head;          /* 16: FP64 steps of a DMED_MIXED chain */
+    int32_t gf_fp64_head;          /* 32: FP64 steps of a DMED_MIXED chain */
     int32_t prec_gs;               /* DMED_FP64 / DMED_MIXED plain-Lanczos ground states */
     int32_t pad_;
 } dmed_solve_params;
