@@ -1,0 +1,70 @@
+"""The Fortran DMFT_ED layer end to end on the GPU: the Bethe-lattice driver
+(ed_read_input -> ed_init_solver with a hamiltonian.restart bath -> ed_solve
+-> ed_get_sigma_matsubara -> local G -> ed_chi2_fitgf) for one DMFT
+iteration on workload 1, against the oracle's solve of the same bath.
+Bars: E0 1e-10 relative; dens, docc 1e-9; G_loc(iw) 1e-8 absolute."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import edref
+from dmed import configs
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "dmft-ed_amd", "fortran", "build", "ed_hm_bethe")
+
+
+def _read_restart(path, nbath, norb=1, nspin=1):
+    rows = [list(map(float, l.split())) for l in open(path).read().splitlines()[1:] if l.strip()]
+    st = nspin * norb * nbath
+    b = np.zeros(2 * st)
+    for k, r in enumerate(rows[:nbath]):
+        q = 0
+        for s in range(nspin):
+            for a in range(norb):
+                io = k + a * nbath + s * nbath * norb
+                b[io], b[st + io] = r[q], r[q + 1]
+                q += 2
+    return b
+
+
+def test_bethe_driver_one_iteration(tmp_path):
+    assert os.path.exists(EXE), "Fortran driver not built (__graft_entry__.build())"
+    c = configs.config(1)
+    lmats = 1024
+    (tmp_path / "inputED.in").write_text("\n".join([
+        "NORB=1", "NBATH=7", "NSPIN=1", "ULOC=2.0", "XMU=0.0", f"BETA={c['beta']}",
+        f"LMATS={lmats}", "LREAL=100", "NLOOP=1", "LANC_NGFITER=200",
+        "LANC_NSTATES_TOTAL=1", "HFMODE=T", "CG_SCHEME=weiss", ""]))
+    configs.write_restart(str(tmp_path / "hamiltonian.restart"), c)
+    bath = _read_restart(str(tmp_path / "hamiltonian.restart"), c["nbath"])
+    print("[fortran driver] one DMFT iteration, workload 1", flush=True)
+    r = subprocess.run([EXE, "inputED.in"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    used = _read_restart(str(tmp_path / "hamiltonian.used"), c["nbath"])
+    assert np.array_equal(used, bath)          # the solve ran on the restart bath
+    summ = {l.split()[0]: [float(x) for x in l.split()[1:]]
+            for l in open(tmp_path / "dmft_summary.ed") if l.strip()}
+    edref.set_model(1, c["nbath"], 1, bath, c["hloc"], [2.0], 0.0, 0.0, 0.0, 0.0, 0.0, True, False)
+    p = edref.default_params(beta=c["beta"], lmats=lmats, lreal=100, lanc_ngfiter=200)
+    o = edref.ed_solve(1, c["nbath"], 1, bath, c["hloc"], [2.0], 0.0, 0.0, 0.0, True, params=p)
+    assert abs(summ["egs"][0] - o["egs"]) <= 1e-10 * abs(o["egs"])
+    assert abs(summ["dens"][0] - o["dens"][0]) <= 1e-9
+    assert abs(summ["docc"][0] - o["docc"][0]) <= 1e-9
+    # G_loc of the semicircular DOS (half-bandwidth 1, 500 points) from Sigma
+    g = np.loadtxt(tmp_path / "Gloc_iw.ed")
+    wm = np.pi / c["beta"] * (2 * np.arange(lmats) + 1)
+    assert np.allclose(g[:, 0], wm, rtol=1e-14)
+    e = np.linspace(-1.0, 1.0, 500)
+    wte = 2.0 / np.pi * np.sqrt(np.maximum(0.0, 1.0 - e ** 2)) * (e[1] - e[0])
+    zeta = 1j * wm - o["smats"][0, 0]
+    gl = (wte[None, :] / (zeta[:, None] - e[None, :])).sum(1)
+    assert np.abs(g[:, 2] + 1j * g[:, 1] - gl).max() <= 1e-8
+    # the fit wrote a new bath
+    new = _read_restart(str(tmp_path / "hamiltonian.restart"), c["nbath"])
+    assert np.all(np.isfinite(new)) and not np.array_equal(new, bath)
