@@ -23,6 +23,7 @@
  *   - Delta / G0^{-1} / Sigma (normal bath)                 ED_BATH_FUNCTIONS.f90:221-258,1784-1807;
  *                                                           get_sigma_normal.f90:1-38
  *   - dens / docc of a state                                ED_OBSERVABLES.f90:118-150
+ *   - local observables and energies of a state             ED_OBSERVABLES.f90:74-150, 450-669
  *
  * Deviation (stated): state codes are int64, so Ns=16 does not overflow as the
  * reference's integer(4) code does (ED_SETUP.f90:889).
@@ -708,6 +709,129 @@ void ref_observables(const int64_t *map, long long dim, const double *psi,
             docc[a] += nu * nd * w;
         }
     }
+}
+
+/* Local observables and energies of one state, accumulated into out[42]
+ * (observables_impurity, ED_OBSERVABLES.f90:74-150, normal mode, and
+ * local_energy_impurity, :450-669).  Layout (Norb <= 3, [a] = orbital,
+ * [a*3+b] = orbital pair):
+ *   0 dens[3]  3 dens_up[3]  6 dens_dw[3]  9 docc[3]  12 magz[3]
+ *   15 sz2[9]  24 n2[9]  33 s2tot
+ *   34 eknot  35 epot  36 ehartree  37 dust  38 dund  39 dse  40 dph  41 eint
+ * The diagonal terms carry gs_weight = peso*|psi_i|^2.  As in the reference,
+ * the off-diagonal Hloc terms of eknot (:536-543) and the spin-exchange and
+ * pair-hopping terms of epot/dse/dph (:597-647) are summed over states
+ * without peso, and epot includes ehartree (:669); eint is never assigned
+ * by the reference and stays 0. */
+int ref_local_obs(const int64_t *map, long long dim, const double *psi, double peso, double *out)
+{
+    const zdouble *p = (const zdouble *)psi;
+    const int no = g_norb, ns = g_ns, sd = g_nspin - 1;
+    for (long long i = 0; i < dim; ++i) {
+        const int64_t m = map[i];
+        const double w = peso * (creal(p[i]) * creal(p[i]) + cimag(p[i]) * cimag(p[i]));
+        double nup[3], ndw[3], sz[3], nt[3], szs = 0.0;
+        for (int a = 0; a < no; ++a) {
+            nup[a] = (double)((m >> a) & 1);
+            ndw[a] = (double)((m >> (a + ns)) & 1);
+            sz[a] = 0.5 * (nup[a] - ndw[a]);
+            nt[a] = nup[a] + ndw[a];
+            szs += sz[a];
+        }
+        for (int a = 0; a < no; ++a) {
+            out[0 + a] += nt[a] * w;
+            out[3 + a] += nup[a] * w;
+            out[6 + a] += ndw[a] * w;
+            out[9 + a] += nup[a] * ndw[a] * w;
+            out[12 + a] += (nup[a] - ndw[a]) * w;
+            out[15 + a * 3 + a] += sz[a] * sz[a] * w;
+            out[24 + a * 3 + a] += nt[a] * nt[a] * w;
+            for (int b = a + 1; b < no; ++b) {
+                out[15 + a * 3 + b] += sz[a] * sz[b] * w;
+                out[15 + b * 3 + a] += sz[b] * sz[a] * w;
+                out[24 + a * 3 + b] += nt[a] * nt[b] * w;
+                out[24 + b * 3 + a] += nt[b] * nt[a] * w;
+            }
+        }
+        out[33] += szs * szs * w;
+        /* eknot: eloc(ispin,iorb) = impHloc(ispin,ispin,iorb,iorb) */
+        for (int a = 0; a < no; ++a)
+            out[34] += g_hloc[0][0][a][a] * nup[a] * w + g_hloc[sd][sd][a][a] * ndw[a] * w;
+        for (int a = 0; a < no; ++a)
+            for (int b = 0; b < no; ++b) {
+                int64_t k1, k2;
+                double s1, s2;
+                if (nup[a] == 0.0 && nup[b] == 1.0) {
+                    ref_c(b + 1, m, &k1, &s1);
+                    ref_cdg(a + 1, k1, &k2, &s2);
+                    long long j = ref_binary_search(map, dim, k2);
+                    if (j > 0) out[34] += creal(g_hloc[0][0][a][b] * s1 * s2 * p[i] * conj(p[j - 1]));
+                }
+                if (ndw[a] == 0.0 && ndw[b] == 1.0) {
+                    ref_c(b + 1 + ns, m, &k1, &s1);
+                    ref_cdg(a + 1 + ns, k1, &k2, &s2);
+                    long long j = ref_binary_search(map, dim, k2);
+                    if (j > 0) out[34] += creal(g_hloc[sd][sd][a][b] * s1 * s2 * p[i] * conj(p[j - 1]));
+                }
+            }
+        /* epot: density-density terms */
+        for (int a = 0; a < no; ++a) out[35] += g_uloc[a] * nup[a] * ndw[a] * w;
+        for (int a = 0; a < no; ++a)
+            for (int b = a + 1; b < no; ++b) {
+                out[35] += g_ust * (nup[a] * ndw[b] + nup[b] * ndw[a]) * w;
+                out[37] += (nup[a] * ndw[b] + nup[b] * ndw[a]) * w;
+                out[35] += (g_ust - g_jh) * (nup[a] * nup[b] + ndw[a] * ndw[b]) * w;
+                out[38] += (nup[a] * nup[b] + ndw[a] * ndw[b]) * w;
+            }
+        if (no > 1 && g_jhflag) {
+            for (int a = 0; a < no; ++a)
+                for (int b = 0; b < no; ++b) {
+                    if (a == b) continue;
+                    int64_t k1, k2, k3, k4;
+                    double s1, s2, s3, s4;
+                    /* spin exchange: c_b,up c_a,dw cdg_b,dw cdg_a,up */
+                    if (nup[b] == 1.0 && ndw[a] == 1.0 && ndw[b] == 0.0 && nup[a] == 0.0) {
+                        ref_c(b + 1, m, &k1, &s1);
+                        ref_c(a + 1 + ns, k1, &k2, &s2);
+                        ref_cdg(b + 1 + ns, k2, &k3, &s3);
+                        ref_cdg(a + 1, k3, &k4, &s4);
+                        long long j = ref_binary_search(map, dim, k4);
+                        if (j > 0) {
+                            const double v = creal(s1 * s2 * s3 * s4 * p[i] * conj(p[j - 1]));
+                            out[35] += g_jx * v;
+                            out[39] += v;
+                        }
+                    }
+                    /* pair hopping: c_b,up c_b,dw cdg_a,dw cdg_a,up */
+                    if (nup[b] == 1.0 && ndw[b] == 1.0 && ndw[a] == 0.0 && nup[a] == 0.0) {
+                        ref_c(b + 1, m, &k1, &s1);
+                        ref_c(b + 1 + ns, k1, &k2, &s2);
+                        ref_cdg(a + 1 + ns, k2, &k3, &s3);
+                        ref_cdg(a + 1, k3, &k4, &s4);
+                        long long j = ref_binary_search(map, dim, k4);
+                        if (j > 0) {
+                            const double v = creal(s1 * s2 * s3 * s4 * p[i] * conj(p[j - 1]));
+                            out[35] += g_jp * v;
+                            out[40] += v;
+                        }
+                    }
+                }
+        }
+        if (g_hfmode) {
+            double eh = 0.0;
+            for (int a = 0; a < no; ++a)
+                eh += -0.5 * g_uloc[a] * (nup[a] + ndw[a]) * w + 0.25 * g_uloc[a] * w;
+            for (int a = 0; a < no; ++a)
+                for (int b = a + 1; b < no; ++b) {
+                    const double nn = nup[a] + ndw[a] + nup[b] + ndw[b];
+                    eh += -0.5 * g_ust * nn * w + 0.25 * g_ust * w;
+                    eh += -0.5 * (g_ust - g_jh) * nn * w + 0.25 * (g_ust - g_jh) * w;
+                }
+            out[36] += eh;
+            out[35] += eh;
+        }
+    }
+    return 0;
 }
 
 /* Dense sector Hamiltonian (real part) by applying H to unit vectors: the

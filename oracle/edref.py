@@ -9,6 +9,7 @@ ed_solve_single does (ED_MAIN.f90:251-289):
   diagonalize_impurity  -> ed_diag_c      ED_DIAG.f90:93-228
                         -> ed_analysis    ED_DIAG.f90:248-398
   observables_impurity  (dens/docc)      ED_OBSERVABLES.f90:74-168
+  local observables and energies         ED_OBSERVABLES.f90:74-150, 450-669
   buildgf_impurity      -> build_gf_normal / lanc_build_gf_normal_c
                                           build_gf_normal.f90:4-180
                         -> get_sigma_normal  get_sigma_normal.f90:1-38
@@ -85,6 +86,7 @@ def lib():
         L.ref_sigma.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp]
         L.ref_observables.argtypes = [_i64p, C.c_longlong, _dp, C.c_double, _dp, _dp]
         L.ref_dense_h.argtypes = [_i64p, C.c_longlong, _dp]
+        L.ref_local_obs.argtypes = [_i64p, C.c_longlong, _dp, C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -236,6 +238,35 @@ def observables(smap, psi, peso, norb):
     return dens, docc
 
 
+OBS_FIELDS = (("dens", 0, 3), ("dens_up", 3, 3), ("dens_dw", 6, 3), ("docc", 9, 3),
+              ("magz", 12, 3), ("sz2", 15, 9), ("n2", 24, 9), ("s2tot", 33, 1),
+              ("eknot", 34, 1), ("epot", 35, 1), ("ehartree", 36, 1), ("dust", 37, 1),
+              ("dund", 38, 1), ("dse", 39, 1), ("dph", 40, 1), ("eint", 41, 1))
+NOBS = 42
+
+
+def local_obs(smap, psi, peso, acc=None):
+    """Accumulate observables_impurity / local_energy_impurity of one state
+    into the flat 42-vector (layout OBS_FIELDS)."""
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.zeros(NOBS) if acc is None else acc
+    lib().ref_local_obs(_i(smap), len(smap), _d(psi.view(np.float64)), peso, _d(out))
+    return out
+
+
+def obs_dict(v, norb):
+    """Flat 42-vector -> dict; per-orbital arrays trimmed to norb, pairs to norb x norb."""
+    d = {}
+    for k, o, n in OBS_FIELDS:
+        if n == 1:
+            d[k] = float(v[o])
+        elif n == 3:
+            d[k] = np.array(v[o:o + norb])
+        else:
+            d[k] = np.array(v[o:o + 9]).reshape(3, 3)[:norb, :norb]
+    return d
+
+
 def dense_h(smap):
     n = len(smap)
     H = np.zeros((n, n), order="F")
@@ -355,11 +386,14 @@ def ed_solve(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, xmu=0.0,
     # observables
     dens = np.zeros(norb)
     docc = np.zeros(norb)
+    acc = np.zeros(NOBS)
     for e, (nu, nd), vec in sl.items:
         peso = (np.exp(-beta * (e - egs)) if finite_t else 1.0) / zeta
-        dd, oo = observables(build_sector(nu, nd), vec, peso, norb)
+        m = build_sector(nu, nd)
+        dd, oo = observables(m, vec, peso, norb)
         dens += dd
         docc += oo
+        local_obs(m, vec, peso, acc)
     # Green's functions
     wm = np.pi / beta * (2 * np.arange(1, p["lmats"] + 1) - 1)
     wr = np.linspace(p["wini"], p["wfin"], p["lreal"])
@@ -400,5 +434,6 @@ def ed_solve(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, xmu=0.0,
             sm[s, a] = sigma(s, a, 1j * wm, gm[s, a])
             sr[s, a] = sigma(s, a, wr + 1j * p["eps"], gr[s, a])
     return dict(egs=egs, zeta=zeta, states=[(e, sec) for e, sec, _ in sl.items],
-                vectors=[v for _, _, v in sl.items], dens=dens, docc=docc, gmats=gm,
+                vectors=[v for _, _, v in sl.items], dens=dens, docc=docc,
+                obs=obs_dict(acc, norb), gmats=gm,
                 greal=gr, smats=sm, sreal=sr, wm=wm, wr=wr)

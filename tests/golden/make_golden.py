@@ -18,6 +18,10 @@ bit-twiddling formula with the oracle (oracle/edref.c) or with the HIP library:
 * Spectra come from dense numpy.linalg.eigh, Green's functions from the exact
   Lehmann sum over all eigenstates of the N+1 / N-1 sectors, with the
   reference's T=0 / finite-T weights (ED_DIAG.f90:202-213, build_gf_normal.f90:457-466).
+* Local observables and energies of every kept state are expectation values
+  of operator matrices (n, n n, S S, the Hloc, interaction, Hartree-shift,
+  spin-exchange and pair-hopping pieces of H), stored per state with its
+  weight; the pieces are checked to add up to H.
 
 Everything is seeded (numpy default_rng); the output files are plain .npz
 (no pickles).
@@ -112,6 +116,94 @@ def model_hamiltonian(mdl):
     return H.tocsr(), c, ns
 
 
+OBS_COLS = (["dens_up%d" % a for a in range(3)] + ["dens_dw%d" % a for a in range(3)] +
+            ["docc%d" % a for a in range(3)] +
+            ["sz2_%d%d" % (a, b) for a in range(3) for b in range(3)] +
+            ["n2_%d%d" % (a, b) for a in range(3) for b in range(3)] +
+            ["s2tot", "eknot_diag", "eknot_off", "epot_dd", "ehartree", "dust", "dund",
+             "dse", "dph"])
+
+
+def local_operators(mdl, c, ns):
+    """The pieces of H as separate Fock-space operators, and the local
+    observables as operators, all from Jordan-Wigner products."""
+    norb, nbath, nspin = mdl["norb"], mdl["nbath"], mdl["nspin"]
+    cd = [op.T.tocsr() for op in c]
+    n = [cd[p] @ c[p] for p in range(2 * ns)]
+    dim = c[0].shape[0]
+    zero = sp.csr_matrix((dim, dim))
+    ident = sp.identity(dim, format="csr")
+    e, v, hloc = mdl["e"], mdl["v"], mdl["hloc"]
+    U, Ust, Jh = mdl["uloc"], mdl["ust"], mdl["jh"]
+    sd = nspin - 1
+    up = lambda a: a
+    dw = lambda a: a + ns
+    bath = lambda a, k: norb + a * nbath + k
+    ops = {}
+    ops["hloc_diag"] = sum((hloc[0][a][a] * n[up(a)] + hloc[sd][a][a] * n[dw(a)]
+                            for a in range(norb)), zero)
+    ops["hloc_off"] = sum((hloc[0][a][b] * (cd[up(a)] @ c[up(b)]) +
+                           hloc[sd][a][b] * (cd[dw(a)] @ c[dw(b)])
+                           for a in range(norb) for b in range(norb) if a != b), zero)
+    ops["mu"] = -mdl["xmu"] * sum((n[up(a)] + n[dw(a)] for a in range(norb)), zero)
+    dd = sum((U[a] * (n[up(a)] @ n[dw(a)]) for a in range(norb)), zero)
+    dust = zero
+    dund = zero
+    hf = zero
+    for a in range(norb):
+        for b in range(a + 1, norb):
+            dust = dust + n[up(a)] @ n[dw(b)] + n[up(b)] @ n[dw(a)]
+            dund = dund + n[up(a)] @ n[up(b)] + n[dw(a)] @ n[dw(b)]
+    dd = dd + Ust * dust + (Ust - Jh) * dund
+    if mdl["hfmode"]:
+        for a in range(norb):
+            hf = hf - 0.5 * U[a] * (n[up(a)] + n[dw(a)]) + 0.25 * U[a] * ident
+            for b in range(a + 1, norb):
+                nt = n[up(a)] + n[dw(a)] + n[up(b)] + n[dw(b)]
+                hf = hf - 0.5 * Ust * nt + 0.25 * Ust * ident
+                hf = hf - 0.5 * (Ust - Jh) * nt + 0.25 * (Ust - Jh) * ident
+    ops["epot_dd"], ops["ehartree"], ops["dust"], ops["dund"] = dd, hf, dust, dund
+    se = zero
+    ph = zero
+    if norb > 1 and mdl["jhflag"]:
+        for a in range(norb):
+            for b in range(norb):
+                if a != b:
+                    se = se + cd[up(a)] @ cd[dw(b)] @ c[dw(a)] @ c[up(b)]
+                    ph = ph + cd[up(a)] @ cd[dw(a)] @ c[dw(b)] @ c[up(b)]
+    ops["dse"], ops["dph"] = se, ph
+    ops["bath"] = sum((e[0][a][k] * n[up(bath(a, k))] + e[sd][a][k] * n[dw(bath(a, k))]
+                       for a in range(norb) for k in range(nbath)), zero)
+    ops["hyb"] = sum((v[0][a][k] * (cd[up(a)] @ c[up(bath(a, k))] + cd[up(bath(a, k))] @ c[up(a)]) +
+                      v[sd][a][k] * (cd[dw(a)] @ c[dw(bath(a, k))] + cd[dw(bath(a, k))] @ c[dw(a)])
+                      for a in range(norb) for k in range(nbath)), zero)
+    ops["n_up"] = [n[up(a)] for a in range(norb)]
+    ops["n_dw"] = [n[dw(a)] for a in range(norb)]
+    return ops
+
+
+def state_observables(ops, norb, full):
+    """<full| O |full> for the columns of OBS_COLS (orbital slots >= norb are 0)."""
+    ev = lambda O: float(full @ (O @ full))
+    out = dict.fromkeys(OBS_COLS, 0.0)
+    nu, nd = ops["n_up"], ops["n_dw"]
+    szo = [0.5 * (nu[a] - nd[a]) for a in range(norb)]
+    nto = [nu[a] + nd[a] for a in range(norb)]
+    for a in range(norb):
+        out["dens_up%d" % a] = ev(nu[a])
+        out["dens_dw%d" % a] = ev(nd[a])
+        out["docc%d" % a] = ev(nu[a] @ nd[a])
+        for b in range(norb):
+            out["sz2_%d%d" % (a, b)] = ev(szo[a] @ szo[b])
+            out["n2_%d%d" % (a, b)] = ev(nto[a] @ nto[b])
+    stot = sum(szo[1:], szo[0])
+    out["s2tot"] = ev(stot @ stot)
+    for k in ("eknot_diag", "eknot_off", "epot_dd", "ehartree", "dust", "dund", "dse", "dph"):
+        key = {"eknot_diag": "hloc_diag", "eknot_off": "hloc_off"}.get(k, k)
+        out[k] = ev(ops[key])
+    return np.array([out[k] for k in OBS_COLS])
+
+
 def sector_codes(ns, nup, ndw):
     """Reference ordering: idw ascending (outer), iup ascending (inner)."""
     ups = [s for s in range(2 ** ns) if bin(s).count("1") == nup]
@@ -190,6 +282,12 @@ def lehmann_gf(H, c, ns, mdl, beta, wm, wr, eps, finite_t, gs_threshold=1e-9,
     gr = np.zeros((nspin, norb, len(wr)), dtype=complex)
     dens = np.zeros(norb)
     docc = np.zeros(norb)
+    ops = local_operators(mdl, c, ns)
+    # the pieces add up to H (checks the decomposition used below)
+    parts = (ops["hloc_diag"] + ops["hloc_off"] + ops["mu"] + ops["epot_dd"] + ops["ehartree"] +
+             mdl["jx"] * ops["dse"] + mdl["jp"] * ops["dph"] + ops["bath"] + ops["hyb"])
+    assert abs(parts - H).max() < 1e-12
+    obs_peso, obs_vals = [], []
     for E, (nu, nd), vec in states:
         peso = (np.exp(-beta * (E - egs)) if finite_t else 1.0) / zeta
         idx = spectra[(nu, nd)][0]
@@ -200,6 +298,8 @@ def lehmann_gf(H, c, ns, mdl, beta, wm, wr, eps, finite_t, gs_threshold=1e-9,
             docc[a] += peso * np.sum(np.abs(vec) ** 2 * nua * nda)
         full = np.zeros(H.shape[0])
         full[idx] = vec
+        obs_peso.append(peso)
+        obs_vals.append(state_observables(ops, norb, full))
         for s in range(nspin):
             for a in range(norb):
                 p = a + s * ns
@@ -220,7 +320,8 @@ def lehmann_gf(H, c, ns, mdl, beta, wm, wr, eps, finite_t, gs_threshold=1e-9,
                         wt = peso * amp[k] ** 2
                         gm[s, a] += wt / (1j * wm + (wj[k] - E))
                         gr[s, a] += wt / (wr + 1j * eps + (wj[k] - E))
-    return egs, zeta, len(states), gm, gr, dens, docc, spectra
+    return (egs, zeta, len(states), gm, gr, dens, docc, spectra, np.array(obs_peso),
+            np.array(obs_vals))
 
 
 def write_model(name, mdl, rng, beta=50.0, lmats=64, lreal=48, eps=0.05,
@@ -228,7 +329,7 @@ def write_model(name, mdl, rng, beta=50.0, lmats=64, lreal=48, eps=0.05,
     H, c, ns = model_hamiltonian(mdl)
     wm = np.pi / beta * (2 * np.arange(1, lmats + 1) - 1)
     wr = np.linspace(-4.0, 4.0, lreal)
-    egs, zeta, nstates, gm, gr, dens, docc, spectra = lehmann_gf(
+    egs, zeta, nstates, gm, gr, dens, docc, spectra, opeso, ovals = lehmann_gf(
         H, c, ns, mdl, beta, wm, wr, eps, finite_t, cutoff_states=cutoff_states)
     out = dict(norb=mdl["norb"], nbath=mdl["nbath"], nspin=mdl["nspin"],
                bath=bath_vector(mdl), hloc_f=hloc_fortran(mdl),
@@ -236,7 +337,8 @@ def write_model(name, mdl, rng, beta=50.0, lmats=64, lreal=48, eps=0.05,
                jx=mdl["jx"], jp=mdl["jp"], xmu=mdl["xmu"], hfmode=mdl["hfmode"],
                jhflag=mdl["jhflag"], beta=beta, eps=eps, wm=wm, wr=wr,
                egs=egs, zeta=zeta, nstates=nstates, gmats=gm, greal=gr,
-               dens=dens, docc=docc, finite_t=int(finite_t))
+               dens=dens, docc=docc, finite_t=int(finite_t),
+               obs_peso=opeso, obs_vals=ovals)
     if cutoff_states is not None:
         out["cutoff_states"] = cutoff_states
     # per-sector map, seeded H*x, lowest eigenvalues

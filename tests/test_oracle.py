@@ -249,3 +249,78 @@ def test_atomic_limit_closed_form():
     r = _solve_golden(g, lanc_nstates_sector=64, lanc_nstates_total=64, lanc_ngfiter=64)
     assert np.abs(r["gmats"][0, 0] - gat).max() < 1e-9
     assert np.abs(r["smats"][0, 0] - U * U / (4 * z)).max() < 1e-7
+
+
+# columns of the per-state observables in the fixtures (make_golden.OBS_COLS)
+_COLS = (["dens_up%d" % a for a in range(3)] + ["dens_dw%d" % a for a in range(3)] +
+         ["docc%d" % a for a in range(3)] +
+         ["sz2_%d%d" % (a, b) for a in range(3) for b in range(3)] +
+         ["n2_%d%d" % (a, b) for a in range(3) for b in range(3)] +
+         ["s2tot", "eknot_diag", "eknot_off", "epot_dd", "ehartree", "dust", "dund",
+          "dse", "dph"])
+
+
+def _golden_local_obs(g):
+    """The reference's accumulation of the fixture's per-state expectation
+    values: gs_weight terms weighted by peso; off-diagonal Hloc and the
+    spin-exchange / pair-hopping terms summed without it (as the reference's
+    local_energy_impurity does, ED_OBSERVABLES.f90:536-543, 597-647)."""
+    w = g["obs_peso"]
+    V = g["obs_vals"]
+    col = {k: V[:, i] for i, k in enumerate(_COLS)}
+    ws = lambda k: float(w @ col[k])
+    us = lambda k: float(col[k].sum())
+    norb = int(g["norb"])
+    d = {}
+    d["dens_up"] = np.array([ws("dens_up%d" % a) for a in range(norb)])
+    d["dens_dw"] = np.array([ws("dens_dw%d" % a) for a in range(norb)])
+    d["dens"] = d["dens_up"] + d["dens_dw"]
+    d["magz"] = d["dens_up"] - d["dens_dw"]
+    d["docc"] = np.array([ws("docc%d" % a) for a in range(norb)])
+    d["sz2"] = np.array([[ws("sz2_%d%d" % (a, b)) for b in range(norb)] for a in range(norb)])
+    d["n2"] = np.array([[ws("n2_%d%d" % (a, b)) for b in range(norb)] for a in range(norb)])
+    d["s2tot"] = ws("s2tot")
+    d["eknot"] = ws("eknot_diag") + us("eknot_off")
+    d["ehartree"] = ws("ehartree")
+    d["dust"] = ws("dust")
+    d["dund"] = ws("dund")
+    d["dse"] = us("dse")
+    d["dph"] = us("dph")
+    d["epot"] = (ws("epot_dd") + d["ehartree"] + float(g["jx"]) * d["dse"] +
+                 float(g["jp"]) * d["dph"])
+    d["eint"] = 0.0
+    return d
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_local_observables_match_jordan_wigner(name):
+    """ref_local_obs over the fixture's kept states (recomputed from the
+    oracle's dense sector matrices) against the operator expectation values
+    of the Jordan-Wigner construction: n, n n, S S, the Hloc, interaction,
+    Hartree-shift, spin-exchange and pair-hopping parts of H."""
+    g = load(name)
+    ns = set_model_from(g)
+    norb = int(g["norb"])
+    finite = bool(int(g["finite_t"]))
+    beta = float(g["beta"])
+    egs = float(g["egs"])
+    states = []
+    for nu in range(ns + 1):
+        for nd in range(ns + 1):
+            m = edref.build_sector(nu, nd)
+            w, U = np.linalg.eigh(edref.dense_h(m))
+            for k in range(len(w)):
+                keep = (w[k] - egs < float(g["cutoff_states"])) if finite else \
+                    abs(w[k] - egs) <= 1e-9
+                if keep:
+                    states.append((w[k], m, U[:, k]))
+    assert len(states) == len(g["obs_peso"])
+    zeta = sum(np.exp(-beta * (e - egs)) for e, _, _ in states) if finite else len(states)
+    acc = np.zeros(edref.NOBS)
+    for e, m, vec in states:
+        peso = (np.exp(-beta * (e - egs)) if finite else 1.0) / zeta
+        edref.local_obs(m, vec, peso, acc)
+    r = edref.obs_dict(acc, norb)
+    ref = _golden_local_obs(g)
+    for k, v in ref.items():
+        assert np.allclose(r[k], v, atol=1e-10, rtol=0), (k, r[k], v)
