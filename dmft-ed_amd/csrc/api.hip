@@ -636,6 +636,48 @@ __global__ void k_obs(const T *__restrict__ psi, int64_t dup, int64_t ddw,
     }
 }
 
+// Joint weight of the impurity occupations: P[bu*nimp + bd] = sum |psi|^2 over
+// the elements whose up / down strings carry impurity bits bu / bd.  A block
+// owns whole columns (fixed bd); per column the lanes sum by bu in registers
+// and the block reduces in a fixed order, so the result is deterministic.
+// partials[bin * gridDim.x + block], nimp <= 8.
+template <typename T>
+__global__ void __launch_bounds__(256) k_imp_hist(const T *__restrict__ psi, int64_t dup, int64_t ddw,
+                                                  const uint8_t *__restrict__ impu,
+                                                  const uint8_t *__restrict__ impd, int nimp,
+                                                  double *__restrict__ partials)
+{
+    __shared__ double red[8][4];
+    __shared__ double hist[64];
+    for (int t = threadIdx.x; t < 64; t += blockDim.x) hist[t] = 0.0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int64_t c = blockIdx.x; c < ddw; c += gridDim.x) {
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const T *col = psi + c * dup;
+        for (int64_t r = threadIdx.x; r < dup; r += blockDim.x) {
+            const double x = (double)col[r];
+            const unsigned iu = impu[r];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] += (iu == (unsigned)q) ? x * x : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double v = acc[q];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) red[q][wv] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < nimp) {
+            double t = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[threadIdx.x][w];
+            hist[threadIdx.x * nimp + impd[c]] += t;
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < nimp * nimp; t += blockDim.x)
+        partials[(int64_t)t * gridDim.x + blockIdx.x] = hist[t];
+}
+
 }  // namespace dmed
 
 extern "C" int dmed_observables(const dmed_vec *psi, double peso, double *dens, double *docc)
@@ -667,6 +709,122 @@ extern "C" int dmed_observables(const dmed_vec *psi, double peso, double *dens, 
         dens[a] += peso * d1;
         docc[a] += peso * d2;
     }
+    return 0;
+}
+
+// Local observables and energies of one normalised state (observables_impurity,
+// ED_OBSERVABLES.f90:74-150; local_energy_impurity, :450-669).  Every diagonal
+// quantity is a function of the impurity bits only, so it follows from the
+// joint weight P(bu, bd); the off-diagonal Hloc part of eknot is
+// <psi|c+_a c_b|psi> = <c_a psi, c_b psi>, from the annihilated vectors.
+extern "C" int dmed_local_observables(const dmed_vec *psi, double peso, dmed_local_obs *acc)
+{
+    REQUIRE_MODEL();
+    if (!psi || !acc) return fail(DMED_EINVAL, "null argument");
+    const dmed_sector *s = psi->sec;
+    if (int rc = check_sector(s)) return rc;
+    const Model &md = ctx().model;
+    const int no = md.norb, nimp = s->nimp, sd = md.nspin - 1;
+    if (nimp > 8) return fail(DMED_EUNSUPP, "local observables: Norb <= 3");
+    hipStream_t st = ex().stream;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, s->dw.dim));
+    if (int rc = ensure_partials((size_t)64 * g + 16)) return rc;
+    if (psi->prec == DMED_FP64)
+        hipLaunchKernelGGL(dmed::k_imp_hist<double>, dim3(g), dim3(256), 0, st,
+                           (const double *)psi->data, s->up.dim, s->dw.dim, s->up.imp, s->dw.imp,
+                           nimp, ex().partials);
+    else
+        hipLaunchKernelGGL(dmed::k_imp_hist<float>, dim3(g), dim3(256), 0, st,
+                           (const float *)psi->data, s->up.dim, s->dw.dim, s->up.imp, s->dw.imp,
+                           nimp, ex().partials);
+    DMED_HIP(hipGetLastError());
+    std::vector<double> h((size_t)nimp * nimp * g);
+    DMED_HIP(hipMemcpyAsync(h.data(), ex().partials, h.size() * 8, hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipStreamSynchronize(st));
+    dmed_local_obs o{};
+    for (int bu = 0; bu < nimp; ++bu)
+        for (int bd = 0; bd < nimp; ++bd) {
+            double P = 0.0;
+            for (unsigned b = 0; b < g; ++b) P += h[(size_t)(bu * nimp + bd) * g + b];
+            const double w = peso * P;
+            if (w == 0.0) continue;
+            double nup[3] = {}, ndw[3] = {}, sz[3] = {}, nt[3] = {}, szs = 0.0;
+            for (int a = 0; a < no; ++a) {
+                nup[a] = (bu >> a) & 1;
+                ndw[a] = (bd >> a) & 1;
+                sz[a] = 0.5 * (nup[a] - ndw[a]);
+                nt[a] = nup[a] + ndw[a];
+                szs += sz[a];
+            }
+            for (int a = 0; a < no; ++a) {
+                o.dens[a] += nt[a] * w;
+                o.dens_up[a] += nup[a] * w;
+                o.dens_dw[a] += ndw[a] * w;
+                o.docc[a] += nup[a] * ndw[a] * w;
+                o.magz[a] += (nup[a] - ndw[a]) * w;
+                for (int b = 0; b < no; ++b) {
+                    o.sz2[a * 3 + b] += sz[a] * sz[b] * w;
+                    o.n2[a * 3 + b] += nt[a] * nt[b] * w;
+                }
+                o.eknot += (md.hloc[0][a][a] * nup[a] + md.hloc[sd][a][a] * ndw[a]) * w;
+                o.epot += md.uloc[a] * nup[a] * ndw[a] * w;
+            }
+            o.s2tot += szs * szs * w;
+            for (int a = 0; a < no; ++a)
+                for (int b = a + 1; b < no; ++b) {
+                    const double du = nup[a] * ndw[b] + nup[b] * ndw[a];
+                    const double dn = nup[a] * nup[b] + ndw[a] * ndw[b];
+                    o.epot += md.ust * du * w + (md.ust - md.jh) * dn * w;
+                    o.dust += du * w;
+                    o.dund += dn * w;
+                }
+            if (md.hfmode) {
+                double eh = 0.0;
+                for (int a = 0; a < no; ++a) {
+                    eh += -0.5 * md.uloc[a] * nt[a] * w + 0.25 * md.uloc[a] * w;
+                    for (int b = a + 1; b < no; ++b) {
+                        const double nn = nt[a] + nt[b];
+                        eh += -0.5 * md.ust * nn * w + 0.25 * md.ust * w;
+                        eh += -0.5 * (md.ust - md.jh) * nn * w + 0.25 * (md.ust - md.jh) * w;
+                    }
+                }
+                o.ehartree += eh;
+                o.epot += eh;
+            }
+        }
+    // off-diagonal Hloc: sum_{a != b} Hloc_ab <c_a psi, c_b psi> per spin (weight 1)
+    for (int sp = 0; sp < 2; ++sp) {
+        const double(*hl)[3] = md.hloc[sp == 0 ? 0 : sd];
+        bool any = false;
+        for (int a = 0; a < no; ++a)
+            for (int b = 0; b < no; ++b) any = any || (a != b && hl[a][b] != 0.0);
+        const int nocc = sp == 0 ? s->nup : s->ndw;
+        if (!any || nocc == 0) continue;
+        if (psi->prec != DMED_FP64)
+            return fail(DMED_EINVAL, "off-diagonal Hloc observables need an FP64 state");
+        dmed_sector *t = nullptr;
+        if (int rc = dmed_sector_create(s->nup - (sp == 0), s->ndw - (sp == 1), &t)) return rc;
+        std::vector<dmed_vec *> cv(no, nullptr);
+        int rc = 0;
+        for (int a = 0; a < no && !rc; ++a) {
+            rc = dmed_vec_create(t, DMED_FP64, &cv[a]);
+            double n2 = 0.0;
+            if (!rc) rc = dmed_apply_op(psi, a + 1 + sp * s->ns, 0, t, cv[a], &n2);
+        }
+        for (int a = 0; a < no && !rc; ++a)
+            for (int b = 0; b < no && !rc; ++b) {
+                if (a == b || hl[a][b] == 0.0) continue;
+                double d = 0.0;
+                rc = dot_host((const double *)cv[a]->data, (const double *)cv[b]->data, t->dim, &d);
+                o.eknot += hl[a][b] * d;
+            }
+        for (auto *v : cv) dmed_vec_free(v);
+        dmed_sector_free(t);
+        if (rc) return rc;
+    }
+    double *dst = reinterpret_cast<double *>(acc);
+    const double *src = reinterpret_cast<const double *>(&o);
+    for (size_t q = 0; q < sizeof(dmed_local_obs) / sizeof(double); ++q) dst[q] += src[q];
     return 0;
 }
 

@@ -179,6 +179,8 @@ int launch_scale_copy(void *dst, int dprec, const void *src, int sprec, int64_t 
 int launch_axpy(void *y, int yprec, const void *x, int xprec, int64_t n, double a,
                 hipStream_t st);
 
+int dot_host(const double *x, const double *y, int64_t n, double *out);
+
 // --------------------------------------------------------- Krylov
 // prec: DMED_FP64, DMED_FP32, or DMED_MIXED (`head` FP64 steps, then FP32;
 // head < 0 selects kMixedHead)

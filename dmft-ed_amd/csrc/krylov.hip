@@ -522,6 +522,18 @@ int eigh_sector_ex(const dmed_sector *s, int neig, int ncv, int maxit, double to
                    double *evals, dmed_vec **evecs, double *resid, int gs_prec);
 constexpr int64_t kTrlMaxDim = 25000000;   // above: plain Lanczos ground state
 
+// <x, y> of two device FP64 vectors, returned to the host
+int dot_host(const double *x, const double *y, int64_t n, double *out)
+{
+        int np = 0;
+    if (int rc = launch_dot(x, y, n, DMED_FP64, ex().partials, &np, ex().stream)) return rc;
+    double *d = ex().dscal + 9;
+    if (int rc = launch_reduce_to(ex().partials, np, d, 0, ex().stream)) return rc;
+    DMED_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, ex().stream));
+    DMED_HIP(hipStreamSynchronize(ex().stream));
+    return 0;
+}
+
 namespace {
 
 // lowest eigenpair of the symmetric tridiagonal (a[0..n), b[1..n)) via the
@@ -592,16 +604,6 @@ int residual(const dmed_sector *s, const double *y, double theta, double *work, 
     return 0;
 }
 
-int dot_host(const double *x, const double *y, int64_t n, double *out)
-{
-        int np = 0;
-    if (int rc = launch_dot(x, y, n, DMED_FP64, ex().partials, &np, ex().stream)) return rc;
-    double *d = ex().dscal + 9;
-    if (int rc = launch_reduce_to(ex().partials, np, d, 0, ex().stream)) return rc;
-    DMED_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, ex().stream));
-    DMED_HIP(hipStreamSynchronize(ex().stream));
-    return 0;
-}
 
 // Plain Lanczos ground state (one eigenpair), memory 4 vectors:
 //   pass 1: recurrence until the lowest Ritz value's residual estimate

@@ -338,11 +338,13 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     }
     const double t1 = now();
     // ---------------------------------------------------- observables
-    std::vector<double> dens(norb, 0.0), docc(norb, 0.0);
+    // observables_impurity / local_energy_impurity over the kept states
+    dmed_local_obs obs{};
     for (auto &s : sl.items) {
         const double peso = (finite_t ? std::exp(-beta * (s.e - egs)) : 1.0) / zeta;
-        if ((rc = dmed_observables(s.vec, peso, dens.data(), docc.data()))) break;
+        if ((rc = dmed_local_observables(s.vec, peso, &obs))) break;
     }
+    std::vector<double> dens(obs.dens, obs.dens + norb), docc(obs.docc, obs.docc + norb);
     // ------------------------------------------------ Green's functions
     const int lm = p->lmats, lr = p->lreal;
     std::vector<double> wm(lm), wr(lr);
@@ -466,6 +468,7 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     out->numgs = numgs;
     if (out->dens) std::copy(dens.begin(), dens.end(), out->dens);
     if (out->docc) std::copy(docc.begin(), docc.end(), out->docc);
+    if (out->obs) *out->obs = obs;
     if (out->neigen_sector) std::copy(neigen.begin(), neigen.end(), out->neigen_sector);
     if (out->lanc_nstates_total) *out->lanc_nstates_total = nst_total;
     out->t_diag = t1 - t0;

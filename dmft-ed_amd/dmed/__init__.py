@@ -49,12 +49,34 @@ class SolveParams(C.Structure):
                 ("prec_gs", C.c_int32), ("pad_", C.c_int32)]
 
 
+class LocalObs(C.Structure):
+    _fields_ = [("dens", C.c_double * 3), ("dens_up", C.c_double * 3),
+                ("dens_dw", C.c_double * 3), ("docc", C.c_double * 3), ("magz", C.c_double * 3),
+                ("sz2", C.c_double * 9), ("n2", C.c_double * 9), ("s2tot", C.c_double),
+                ("eknot", C.c_double), ("epot", C.c_double), ("ehartree", C.c_double),
+                ("dust", C.c_double), ("dund", C.c_double), ("dse", C.c_double),
+                ("dph", C.c_double), ("eint", C.c_double)]
+
+    def as_dict(self, norb):
+        d = {}
+        for k, t in self._fields_:
+            v = getattr(self, k)
+            if t is C.c_double:
+                d[k] = float(v)
+            elif len(v) == 3:
+                d[k] = np.array(v[:norb])
+            else:
+                d[k] = np.array(v[:]).reshape(3, 3)[:norb, :norb]
+        return d
+
+
 class SolveOut(C.Structure):
     _fields_ = [("egs", C.c_double), ("zeta", C.c_double), ("nstates", C.c_int32),
                 ("numgs", C.c_int32), ("dens", _dp), ("docc", _dp), ("gmats", _dp),
                 ("greal", _dp), ("smats", _dp), ("sreal", _dp),
                 ("neigen_sector", _i32p), ("lanc_nstates_total", _i32p),
-                ("t_diag", C.c_double), ("t_gf", C.c_double)]
+                ("t_diag", C.c_double), ("t_gf", C.c_double),
+                ("obs", C.POINTER(LocalObs))]
 
 
 _lib = None
@@ -77,6 +99,7 @@ SIGNATURES = [
     ("dmed_vec_download", C.c_int, [C.c_void_p, _dp]),
     ("dmed_vec_data", C.c_void_p, [C.c_void_p]),
     ("dmed_vec_copy", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("dmed_local_observables", C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
     ("dmed_hxv", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
     ("dmed_hxv_host", C.c_int, [C.c_void_p, _dp, _dp, C.c_int32]),
     ("dmed_hxv_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
@@ -331,10 +354,19 @@ def solve(norb, nspin, ns, params):
         setattr(o, k, g[k].ctypes.data_as(_dp))
     o.neigen_sector = neig.ctypes.data_as(_i32p)
     o.lanc_nstates_total = C.pointer(ntot)
+    obs = LocalObs()
+    o.obs = C.pointer(obs)
     _check(lib().dmed_solve(C.byref(p), C.byref(o)))
     return dict(egs=o.egs, zeta=o.zeta, nstates=o.nstates, numgs=o.numgs, dens=dens, docc=docc,
                 neigen_sector=neig, lanc_nstates_total=ntot.value, t_diag=o.t_diag,
-                t_gf=o.t_gf, **g)
+                t_gf=o.t_gf, obs=obs.as_dict(norb), **g)
+
+
+def local_observables(psi, peso, acc=None):
+    """Accumulate one normalised state's local observables into acc (LocalObs)."""
+    acc = LocalObs() if acc is None else acc
+    _check(lib().dmed_local_observables(psi.h, peso, C.byref(acc)))
+    return acc
 
 
 def chi2_fit(fg, beta, e, v, scheme="weiss", weight=0, xmu=0.0, hloc=0.0, niter=500,

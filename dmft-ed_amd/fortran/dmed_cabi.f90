@@ -23,12 +23,21 @@ module DMED_CABI
      integer(c_int32_t) :: lmats, lreal, prec_gf, batched, verbose, gf_fp64_head, prec_gs, pad_
   end type dmed_solve_params_t
 
+  !> dmed_local_obs: orbital arrays (3), orbital pairs (9, [a*3+b] in C)
+  type,bind(C),public :: dmed_local_obs_t
+     real(c_double) :: dens(3), dens_up(3), dens_dw(3), docc(3), magz(3)
+     real(c_double) :: sz2(9), n2(9)
+     real(c_double) :: s2tot
+     real(c_double) :: eknot, epot, ehartree, dust, dund, dse, dph, eint
+  end type dmed_local_obs_t
+
   type,bind(C),public :: dmed_solve_out_t
      real(c_double)     :: egs, zeta
      integer(c_int32_t) :: nstates, numgs
      type(c_ptr)        :: dens, docc, gmats, greal, smats, sreal
      type(c_ptr)        :: neigen_sector, lanc_nstates_total
      real(c_double)     :: t_diag, t_gf
+     type(c_ptr)        :: obs
   end type dmed_solve_out_t
 
   public :: dmed_init, dmed_finalize, dmed_last_error, dmed_set_model, dmed_solve

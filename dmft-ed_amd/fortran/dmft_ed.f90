@@ -51,7 +51,13 @@ module DMFT_ED
   complex(8),allocatable,public :: impGmats(:,:,:,:,:), impGreal(:,:,:,:,:)
   complex(8),allocatable,public :: impSmats(:,:,:,:,:), impSreal(:,:,:,:,:)
   real(8),allocatable,public    :: ed_dens(:), ed_docc(:)
+  real(8),allocatable,public    :: ed_dens_up(:), ed_dens_dw(:)
   real(8),public                :: ed_egs = 0d0
+  !> local energies and double occupancies of the last solve
+  !> (local_energy_impurity, ED_OBSERVABLES.f90:450-669)
+  real(8),public                :: ed_Epot=0d0, ed_Eint=0d0, ed_Ehartree=0d0, ed_Eknot=0d0
+  real(8),public                :: ed_Dust=0d0, ed_Dund=0d0, ed_Dse=0d0, ed_Dph=0d0
+  type(dmed_local_obs_t),target :: ed_obs
   real(8),allocatable,target    :: bath_e(:,:,:), bath_v(:,:,:)
   real(8),allocatable,target    :: impHloc(:,:,:,:)
   integer(c_int32_t),allocatable,target :: neigen_sector(:)
@@ -64,6 +70,9 @@ module DMFT_ED
   public :: ed_get_sigma_matsubara, ed_get_sigma_real
   public :: ed_get_gimp_matsubara, ed_get_gimp_real
   public :: ed_get_dens, ed_get_docc
+  public :: ed_get_dens_up, ed_get_dens_dw, ed_get_mag
+  public :: ed_get_eimp, ed_get_epot, ed_get_eint, ed_get_ehartree, ed_get_eknot
+  public :: ed_get_doubles, ed_get_dust, ed_get_dund, ed_get_dse, ed_get_dph
   public :: ed_chi2_fitgf
 
 contains
@@ -328,6 +337,9 @@ contains
        allocate(impGmats(Nspin,Nspin,Norb,Norb,Lmats), impSmats(Nspin,Nspin,Norb,Norb,Lmats))
        allocate(impGreal(Nspin,Nspin,Norb,Norb,Lreal), impSreal(Nspin,Nspin,Norb,Norb,Lreal))
        allocate(ed_dens(Norb), ed_docc(Norb))
+       if(allocated(ed_dens_up))deallocate(ed_dens_up)
+       if(allocated(ed_dens_dw))deallocate(ed_dens_dw)
+       allocate(ed_dens_up(Norb), ed_dens_dw(Norb)); ed_dens_up = 0d0; ed_dens_dw = 0d0
        allocate(bath_e(Nspin,Norb,Nbath), bath_v(Nspin,Norb,Nbath))
        Ns = (Nbath+1)*Norb
        allocate(neigen_sector((Ns+1)**2)); neigen_sector = 0
@@ -391,10 +403,15 @@ contains
     o%gmats = c_loc(gm); o%greal = c_loc(gr); o%smats = c_loc(sm); o%sreal = c_loc(sr)
     o%neigen_sector = c_loc(neigen_sector)
     o%lanc_nstates_total = c_loc(ntot)
+    o%obs = c_loc(ed_obs)
     call ed_check(dmed_solve(p,o),"dmed_solve")
     lanc_nstates_total = ntot
     impGmats = gm; impGreal = gr; impSmats = sm; impSreal = sr
     ed_dens = dens; ed_docc = docc; ed_egs = o%egs
+    ed_dens_up = ed_obs%dens_up(1:Norb); ed_dens_dw = ed_obs%dens_dw(1:Norb)
+    ed_Epot = ed_obs%epot; ed_Eint = ed_obs%eint; ed_Ehartree = ed_obs%ehartree
+    ed_Eknot = ed_obs%eknot
+    ed_Dust = ed_obs%dust; ed_Dund = ed_obs%dund; ed_Dse = ed_obs%dse; ed_Dph = ed_obs%dph
     if(ed_verbose>=1)then
        write(LOGfile,"(A,F20.12,A,I4,A,F10.3,A,F10.3,A)")"Egs =",o%egs,"  states:",o%nstates,&
             "  t_diag=",o%t_diag,"s  t_gf=",o%t_gf,"s"
@@ -450,6 +467,74 @@ contains
     real(8),dimension(Norb) :: docc
     docc = ed_docc
   end subroutine ed_get_docc
+
+  !> ED_IO/get_dens.f90, get_mag.f90 (ed_get_mag_1: dens_up - dens_dw)
+  subroutine ed_get_dens_up(dens)
+    real(8),dimension(Norb) :: dens
+    dens = ed_dens_up
+  end subroutine ed_get_dens_up
+
+  subroutine ed_get_dens_dw(dens)
+    real(8),dimension(Norb) :: dens
+    dens = ed_dens_dw
+  end subroutine ed_get_dens_dw
+
+  subroutine ed_get_mag(mag)
+    real(8),dimension(Norb) :: mag
+    mag = ed_dens_up - ed_dens_dw
+  end subroutine ed_get_mag
+
+  !> ED_IO/get_eimp.f90: [Epot, Eint, Ehartree, Eknot]
+  subroutine ed_get_eimp(eimp)
+    real(8),dimension(4) :: eimp
+    eimp = [ed_Epot,ed_Eint,ed_Ehartree,ed_Eknot]
+  end subroutine ed_get_eimp
+
+  subroutine ed_get_epot(e)
+    real(8) :: e
+    e = ed_Epot
+  end subroutine ed_get_epot
+
+  subroutine ed_get_eint(e)
+    real(8) :: e
+    e = ed_Eint
+  end subroutine ed_get_eint
+
+  subroutine ed_get_ehartree(e)
+    real(8) :: e
+    e = ed_Ehartree
+  end subroutine ed_get_ehartree
+
+  subroutine ed_get_eknot(e)
+    real(8) :: e
+    e = ed_Eknot
+  end subroutine ed_get_eknot
+
+  !> ED_IO/get_doubles.f90: [Dust, Dund, Dse, Dph]
+  subroutine ed_get_doubles(d)
+    real(8),dimension(4) :: d
+    d = [ed_Dust,ed_Dund,ed_Dse,ed_Dph]
+  end subroutine ed_get_doubles
+
+  subroutine ed_get_dust(d)
+    real(8) :: d
+    d = ed_Dust
+  end subroutine ed_get_dust
+
+  subroutine ed_get_dund(d)
+    real(8) :: d
+    d = ed_Dund
+  end subroutine ed_get_dund
+
+  subroutine ed_get_dse(d)
+    real(8) :: d
+    d = ed_Dse
+  end subroutine ed_get_dse
+
+  subroutine ed_get_dph(d)
+    real(8) :: d
+    d = ed_Dph
+  end subroutine ed_get_dph
 
   !------------------------------------------------------------ chi^2 fit
   !> chi2_fitgf_generic_normal -> chi2_fitgf_normal_normal (ED_CHI2FIT.f90:86-138,

@@ -13,7 +13,8 @@ program ed_hm_bethe
   implicit none
   integer                  :: iloop, Nb, Le, i, ie, nsucc, unit
   logical                  :: converged
-  real(8)                  :: wband, wmixing, de, err, pi
+  real(8)                  :: wband, wmixing, de, err, pi, eimp(4), dbl(4)
+  real(8),allocatable      :: mag(:)
   real(8),allocatable      :: bath(:), bath_(:), He(:), Wte(:), wm(:)
   complex(8),allocatable   :: Hloc(:,:,:,:), Weiss(:,:,:,:,:), Weiss_old(:,:,:,:,:)
   complex(8),allocatable   :: Smats(:,:,:,:,:), Gloc(:,:,:,:,:)
@@ -26,7 +27,7 @@ program ed_hm_bethe
   wband = 1d0; Le = 500; wmixing = 0.5d0
   call read_extra(trim(finput))
 
-  allocate(He(Le), Wte(Le), wm(Lmats))
+  allocate(He(Le), Wte(Le), wm(Lmats), mag(Norb))
   de = 2d0*wband/dble(Le-1)
   do ie=1,Le
      He(ie) = -wband + (ie-1)*de
@@ -74,6 +75,12 @@ program ed_hm_bethe
   write(unit,"(A,ES24.15)")"egs ",ed_egs
   write(unit,"(A,10ES24.15)")"dens ",ed_dens
   write(unit,"(A,10ES24.15)")"docc ",ed_docc
+  call ed_get_eimp(eimp)
+  call ed_get_doubles(dbl)
+  call ed_get_mag(mag)
+  write(unit,"(A,4ES24.15)")"eimp ",eimp
+  write(unit,"(A,4ES24.15)")"doubles ",dbl
+  write(unit,"(A,10ES24.15)")"mag ",mag
   close(unit)
 
 contains

@@ -187,6 +187,24 @@ int dmed_gf_add_chain(double peso_bz, double ei, const double *alpha,
  * dens[a] += peso*sum|psi|^2 (n_a,up + n_a,dw), docc[a] += peso*sum|psi|^2 n_a,up n_a,dw */
 int dmed_observables(const dmed_vec *psi, double peso, double *dens, double *docc);
 
+/* Local observables and energies (observables_impurity, ED_OBSERVABLES.f90:74-150,
+ * and local_energy_impurity, :450-669, normal mode).  Orbital arrays [a],
+ * orbital pairs [a*3+b], Norb <= 3.  epot includes ehartree; eint is never
+ * assigned by the reference and stays 0; dse and dph need Jhflag, which the
+ * library does not support, and stay 0. */
+typedef struct dmed_local_obs {
+    double dens[3], dens_up[3], dens_dw[3], docc[3], magz[3];
+    double sz2[9], n2[9];
+    double s2tot;
+    double eknot, epot, ehartree, dust, dund, dse, dph, eint;
+} dmed_local_obs;
+
+/* Accumulate one state into acc: the terms the reference weights with
+ * gs_weight = peso*|psi_i|^2 get peso; the off-diagonal Hloc terms of eknot
+ * get weight 1, as the reference sums them without peso
+ * (ED_OBSERVABLES.f90:536-543).  psi must be normalised. */
+int dmed_local_observables(const dmed_vec *psi, double peso, dmed_local_obs *acc);
+
 /* ------------------------------------------------------ whole solve */
 /* Parameters of ed_solve (names and defaults of ED_INPUT_VARS.f90:111-176). */
 typedef struct dmed_solve_params {
@@ -228,6 +246,7 @@ typedef struct dmed_solve_out {
     int32_t *neigen_sector;
     int32_t *lanc_nstates_total;
     double t_diag, t_gf;           /* seconds spent in the two phases */
+    dmed_local_obs *obs;           /* local observables and energies, or NULL */
 } dmed_solve_out;
 
 /* ed_solve_single backend (ED_MAIN.f90:251-289): diagonalize_impurity

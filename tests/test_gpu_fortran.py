@@ -56,6 +56,13 @@ def test_bethe_driver_one_iteration(tmp_path):
     assert abs(summ["egs"][0] - o["egs"]) <= 1e-10 * abs(o["egs"])
     assert abs(summ["dens"][0] - o["dens"][0]) <= 1e-9
     assert abs(summ["docc"][0] - o["docc"][0]) <= 1e-9
+    # ed_get_eimp = [Epot, Eint, Ehartree, Eknot], ed_get_doubles, ed_get_mag
+    ob = o["obs"]
+    assert np.allclose(summ["eimp"], [ob["epot"], ob["eint"], ob["ehartree"], ob["eknot"]],
+                       atol=1e-9, rtol=0)
+    assert np.allclose(summ["doubles"], [ob["dust"], ob["dund"], ob["dse"], ob["dph"]],
+                       atol=1e-9, rtol=0)
+    assert np.allclose(summ["mag"], ob["magz"], atol=1e-9, rtol=0)
     # G_loc of the semicircular DOS (half-bandwidth 1, 500 points) from Sigma
     g = np.loadtxt(tmp_path / "Gloc_iw.ed")
     wm = np.pi / c["beta"] * (2 * np.arange(lmats) + 1)

@@ -416,6 +416,9 @@ def test_solve_vs_oracle(name, finite):
     assert r["nstates"] == len(o["states"])
     assert np.allclose(r["dens"], o["dens"], atol=1e-9)
     assert np.allclose(r["docc"], o["docc"], atol=1e-9)
+    # observables_impurity / local_energy_impurity
+    for k, v in o["obs"].items():
+        assert np.allclose(r["obs"][k], v, atol=1e-9, rtol=0), (k, r["obs"][k], v)
     for s in range(nspin):
         for a in range(norb):
             assert np.abs(r["gmats"][s, s, a, a] - o["gmats"][s, a]).max() < 1e-8
@@ -503,3 +506,25 @@ def test_vec_copy_converts_precision():
     f = dmed.Vec(s, dmed.FP32)
     f.copy_from(a)
     assert np.array_equal(f.download(), x.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("name", ["m2_norb2_nb1", "m4_norb3_nb1", "m5_norb1_nb5_T"])
+def test_local_observables_vs_oracle(name):
+    """dmed_local_observables on GPU eigenvectors against the oracle's
+    restatement on the same vectors: every field to 1e-12 (m2 has
+    off-diagonal Hloc, m4 three orbitals, m5 two spins)."""
+    g = load(name)
+    set_both(g)
+    ns = (int(g["nbath"]) + 1) * int(g["norb"])
+    norb = int(g["norb"])
+    for nu, nd in ((ns // 2, ns // 2), (ns // 2 + 1, ns // 2 - 1), (1, ns - 1)):
+        s = dmed.Sector(nu, nd)
+        ev, vecs, res, rc = s.eigh(min(3, s.dim), dense_threshold=0 if s.dim > 8 else 256)
+        m = edref.build_sector(nu, nd)
+        for q, v in enumerate(vecs):
+            peso = 0.3 + 0.2 * q
+            acc = dmed.local_observables(v, peso)
+            r = acc.as_dict(norb)
+            o = edref.obs_dict(edref.local_obs(m, v.download(), peso), norb)
+            for k in o:
+                assert np.allclose(r[k], o[k], atol=1e-12, rtol=0), (name, nu, nd, k, r[k], o[k])
