@@ -96,6 +96,64 @@ def solve_compare():
     return out
 
 
+def configs_table(threads):
+    """Workloads 1-3 beside the oracle on the host cores: GPU FP64 H*v of the
+    ground-state sector (HIP events) against one full oracle H*v (the
+    reference's map + binary-search algorithm, `threads` OpenMP threads), and
+    for workload 1 the whole ed_solve on both (T=0, Lmats=4096, 200-step chains)."""
+    import ctypes as C
+    import numpy as np
+    import dmed
+    import edref
+    from dmed import configs
+    edref.build()
+    out = {}
+    for n in (1, 2, 3):
+        c = configs.config(n)
+        args = (c["norb"], c["nbath"], c["nspin"], c["bath"], c["hloc"], c["uloc"], c["ust"],
+                c["jh"], 0.0, 0.0, c["xmu"], True, False)
+        dmed.set_model(*args)
+        edref.set_model(*args)
+        nu, nd = c["sector"]
+        s = dmed.Sector(nu, nd)
+        x = dmed.Vec(s, dmed.FP64)
+        y = dmed.Vec(s, dmed.FP64)
+        xh = np.random.default_rng(n).standard_normal(s.dim)
+        x.upload(xh)
+        L = dmed.lib()
+        kms = C.c_double()
+        dmed._check(L.dmed_hxv_timed(s.h, x.ptr(), y.ptr(), dmed.FP64, 3, C.byref(kms)))
+        dmed._check(L.dmed_hxv_timed(s.h, x.ptr(), y.ptr(), dmed.FP64, 20, C.byref(kms)))
+        m = edref.build_sector(nu, nd)
+        t0 = time.perf_counter()
+        edref.hxv(m, xh, nthreads=threads)
+        tc = time.perf_counter() - t0
+        row = {"workload": c["name"], "dim": s.dim, "gpu_hxv_ms": kms.value,
+               "gpu_hxv_GBs": 16 * s.dim / kms.value / 1e6, "cpu_hxv_ms": 1e3 * tc,
+               "cpu_threads": threads}
+        x.free()
+        y.free()
+        s.free()
+        if n == 1:
+            kw = dict(beta=c["beta"], lmats=c["lmats"], lreal=200, lanc_ngfiter=200)
+            t0 = time.perf_counter()
+            r = dmed.solve(1, 1, 8, dmed.default_params(**kw))
+            tg = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            o = edref.ed_solve(1, c["nbath"], 1, c["bath"], c["hloc"], c["uloc"], 0.0, 0.0, 0.0,
+                               True, params=edref.default_params(**kw))
+            to = time.perf_counter() - t0
+            row.update(gpu_solve_s=tg, cpu_solve_s=to, cpu_solve_kind="oracle ed_solve "
+                       "(dense/eigsh sector eigensolves, C chains, 1 thread)",
+                       solve_E0_rel=abs(r["egs"] - o["egs"]) / abs(o["egs"]),
+                       solve_Sigma_iw=float(np.abs(r["smats"][0, 0, 0, 0] - o["smats"][0, 0]).max()))
+        log(f"{c['name']}: GPU H*v {row['gpu_hxv_ms']:.4f} ms, oracle H*v {row['cpu_hxv_ms']:.1f} ms"
+            + (f"; solve GPU {row['gpu_solve_s']:.2f}s vs oracle {row['cpu_solve_s']:.2f}s"
+               if n == 1 else ""))
+        out[f"workload{n}"] = row
+    return out
+
+
 def lanczos_step_rate(sec, D, prec, nsteps=40):
     """Wall time per plain Lanczos step (fused H*v epilogue + update + two
     one-block reductions) from a random start on the benchmark sector."""
@@ -186,6 +244,8 @@ def main():
     ap.add_argument("--solve", action="store_true",
                     help="also time the all-sector Ns=12 finite-T solve: plain FP64 vs "
                          "batched mixed precision")
+    ap.add_argument("--configs", action="store_true",
+                    help="also time workloads 1-3 beside the oracle (H*v; workload-1 solve)")
     ap.add_argument("--solve5", action="store_true",
                     help="also time the Ns=16 sector pipeline (ground state + 500-step chains): "
                          "plain FP64 vs mixed precision")
@@ -268,6 +328,8 @@ def main():
         log(f"Lanczos step {nm}: {st['ms_per_step']:.3f} ms, {st['algorithmic_GBs']:.1f} GB/s "
             f"algorithmic (48 B/element FP64, 24 FP32)")
 
+    if a.configs:
+        extra["workloads_1_3_vs_oracle"] = configs_table(a.cpu_threads or min(16, os.cpu_count() or 1))
     if a.solve:
         extra["solve_ns12_all_sectors"] = solve_compare()
     if a.solve5:
