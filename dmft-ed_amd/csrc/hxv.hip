@@ -310,15 +310,39 @@ static bool use_col_kernel(const dmed_sector *s, int prec)
     const int64_t dup = s->up.dim;
     const int rpt = (int)((dup + 1023) / 1024);
     const int maxr = prec == DMED_FP32 ? kColMaxRptF32 : kColMaxRptF64;
-    return s->up.packed && s->up.ncoef <= 128 && s->nimp <= 8 && dup >= 2048 &&
+    return s->up.packed && s->up.ncoef <= 128 && s->nimp <= 8 && dup >= 512 &&
            rpt <= maxr && (size_t)dup * esz + 2048 <= c.lds_max && s->dw.dim >= 64;
 }
 
-static unsigned col_grid(const dmed_sector *s)
+template <typename T, int RPT>
+static int col_fit_per_cu(size_t shmem)
 {
-    // one workgroup per CU, a multiple of the 8 XCDs, never more than columns
+    int nb = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_hxv_col<T, kLanczos, RPT>,
+                                                     1024, shmem) != hipSuccess)
+        nb = 1;
+    return nb;
+}
+
+// Workgroups of the persistent column kernel: as many per CU as its
+// registers and LDS allow, at most two (a third lowered the rate at
+// D_up = 3432), in multiples of the 8 XCDs, never more than the columns.
+static unsigned col_grid(const dmed_sector *s, int prec)
+{
     const int ncu = std::max(8, ctx().num_cu);
-    int g = (ncu / 8) * 8;
+    const int64_t dup = s->up.dim;
+    const int rpt = (int)((dup + 1023) / 1024);
+    const bool f32 = prec == DMED_FP32;
+    const size_t shmem = (size_t)dup * (f32 ? 4 : 8);
+    int fit = 1;
+    switch (rpt) {
+#define DMED_FIT(R) \
+    case R: fit = f32 ? col_fit_per_cu<float, R>(shmem) : col_fit_per_cu<double, R>(shmem); break;
+        DMED_FIT(1) DMED_FIT(2) DMED_FIT(3) DMED_FIT(4) DMED_FIT(5) DMED_FIT(6) DMED_FIT(7)
+#undef DMED_FIT
+    default: fit = 1; break;
+    }
+    int g = (ncu / 8) * 8 * std::max(1, std::min(fit, 2));
     while (g > 8 && g / 8 > (s->dw.dim + 7) / 8) g -= 8;
     return (unsigned)g;
 }
@@ -352,7 +376,7 @@ static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScala
     if (shmem > 64 * 1024)
         DMED_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)shmem));
-    hipLaunchKernelGGL(kern, dim3(col_grid(s)), dim3(1024), shmem, st, a.x, a.y, a.dup, a.ddw,
+    hipLaunchKernelGGL(kern, dim3(col_grid(s, sizeof(T) == 4 ? DMED_FP32 : DMED_FP64)), dim3(1024), shmem, st, a.x, a.y, a.dup, a.ddw,
                        a.nhu, a.nhd, a.nimp, a.ncoef, a.hu, a.coef, a.hd_idx, a.hd_val, a.du, a.dd,
                        a.impu, a.impd, a.W, a.cs, a.partials);
     DMED_HIP(hipGetLastError());
@@ -366,7 +390,7 @@ static int launch_col_rpt(int rpt, const dmed_sector *s, const void *x, void *y,
     switch (rpt) {
 #define DMED_RPT(R) \
     case R: return launch_col_t<T, MODE, R>(s, x, y, cs, partials, st);
-        DMED_RPT(2) DMED_RPT(3) DMED_RPT(4) DMED_RPT(5) DMED_RPT(6) DMED_RPT(7) DMED_RPT(8)
+        DMED_RPT(1) DMED_RPT(2) DMED_RPT(3) DMED_RPT(4) DMED_RPT(5) DMED_RPT(6) DMED_RPT(7) DMED_RPT(8)
         DMED_RPT(9) DMED_RPT(10) DMED_RPT(11) DMED_RPT(12) DMED_RPT(13)
 #undef DMED_RPT
     case 14: case 15: case 16:
@@ -463,7 +487,7 @@ int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
     if (s->dim == 0) return 0;
     if (mode == kLanczos && partials == nullptr) return fail(DMED_EINVAL, "no partials");
     if (use_col_kernel(s, prec)) {
-        if (nparts) *nparts = (int)col_grid(s);
+        if (nparts) *nparts = (int)col_grid(s, prec);
         const int rpt = (int)((s->up.dim + 1023) / 1024);
         if (prec == DMED_FP32)
             return mode == kPlain ? launch_col_rpt<float, kPlain>(rpt, s, x, y, cs, partials, st)
@@ -489,7 +513,7 @@ int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
 
 int hxv_nparts(const dmed_sector *s, int prec)
 {
-    if (use_col_kernel(s, prec)) return (int)col_grid(s);
+    if (use_col_kernel(s, prec)) return (int)col_grid(s, prec);
     return (int)pick_shape(s, prec).grid;
 }
 
