@@ -383,6 +383,9 @@ def main():
     write_model("m6_u0", make_model(rng, 1, 4, 1, [0.0], xmu=0.0), rng)
     write_model("m7_atomic", make_model(rng, 1, 2, 1, [2.0], vzero=True), rng, beta=10.0,
                 finite_t=True, cutoff_states=50.0)
+    # two orbitals, Nbath=2 (Ns=6), no Hartree shift (hfmode=F), no spin-flip terms
+    write_model("m8_norb2_nb2_nohf", make_model(rng, 2, 2, 1, [2.2, 1.8], ust=1.4, jh=0.2,
+                                                 xmu=0.4, hfmode=0), rng)
 
 
 if __name__ == "__main__":

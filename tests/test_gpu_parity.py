@@ -388,17 +388,18 @@ def test_apply_op_and_observables_vs_oracle(name):
                     assert np.isclose(n2, np.dot(ref, ref), rtol=1e-13)
 
 
-def _solve_both(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, **kw):
+def _solve_both(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode=True, **kw):
     p = dmed.default_params(**kw)
     r = dmed.solve(norb, nspin, (nbath + 1) * norb, p)
     pr = edref.default_params(**{k: v for k, v in kw.items()
                                  if k not in ("prec_gf", "batched", "verbose")})
-    o = edref.ed_solve(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, True, params=pr)
+    o = edref.ed_solve(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode, params=pr)
     return r, o
 
 
 @pytest.mark.parametrize("name,finite", [("m1_norb1_nb3", False), ("m2_norb2_nb1", False),
                                           ("m4_norb3_nb1", False), ("m6_u0", False),
+                                          ("m8_norb2_nb2_nohf", False),
                                           ("m5_norb1_nb5_T", True)])
 def test_solve_vs_oracle(name, finite):
     g = load(name)
@@ -411,7 +412,8 @@ def test_solve_vs_oracle(name, finite):
         kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
     say(f"[{name}] solve")
     r, o = _solve_both(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
-                       float(g["ust"]), float(g["jh"]), float(g["xmu"]), **kw)
+                       float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]), **kw)
+    assert abs(o["egs"] - float(g["egs"])) <= 1e-10 * abs(float(g["egs"]))
     assert abs(r["egs"] - o["egs"]) <= 1e-10 * abs(o["egs"])
     assert r["nstates"] == len(o["states"])
     assert np.allclose(r["dens"], o["dens"], atol=1e-9)
@@ -528,3 +530,41 @@ def test_local_observables_vs_oracle(name):
             o = edref.obs_dict(edref.local_obs(m, v.download(), peso), norb)
             for k in o:
                 assert np.allclose(r[k], o[k], atol=1e-12, rtol=0), (name, nu, nd, k, r[k], o[k])
+
+
+def test_error_paths_return_codes():
+    """Invalid calls return the documented codes and leave the library usable."""
+    g = load("m1_norb1_nb3")
+    set_both(g)
+    ns = (int(g["nbath"]) + 1) * int(g["norb"])
+
+    def code(fn, *a):
+        with pytest.raises(dmed.DmedError) as e:
+            fn(*a)
+        return e.value.code
+
+    EINVAL, ESTATE = -1, -2
+    assert code(dmed.Sector, ns + 1, 0) == EINVAL
+    assert code(dmed.Sector, -1, 0) == EINVAL
+    s = dmed.Sector(2, 2)
+    v = s.vec(np.random.default_rng(0).standard_normal(s.dim))
+    assert code(dmed.chain, s, v, 0) == EINVAL
+    assert code(dmed.chain, s, v, 100000) == EINVAL
+    assert code(dmed.chain, s, v, 10, 7) == EINVAL
+    assert code(s.ground_state, dmed.FP32) == EINVAL
+    wrong = dmed.Sector(2, 3)                    # c^+_up needs (3, 2)
+    assert code(dmed.apply_op, v, 1, True, wrong) == EINVAL
+    w = dmed.Vec(wrong, dmed.FP64)
+    assert code(w.copy_from, v) == EINVAL
+    L = dmed.lib()
+    assert L.dmed_hxv(s.h, v.ptr(), v.ptr(), dmed.FP64) == EINVAL
+    assert L.dmed_hxv(s.h, v.ptr(), w.ptr(), 5) == EINVAL
+    assert code(dmed.solve, 1, 1, ns, dmed.default_params(prec_gs=dmed.FP32)) == EINVAL
+    assert code(dmed.solve, 1, 1, ns, dmed.default_params(prec_gf=9)) == EINVAL
+    # a sector built for an older model is refused
+    set_both(g)
+    assert code(s.hxv, np.zeros(s.dim)) == ESTATE
+    # and the library keeps working
+    s2 = dmed.Sector(2, 2)
+    y = s2.hxv(np.ones(s2.dim))
+    assert np.all(np.isfinite(y))
