@@ -12,6 +12,10 @@
 // no host round trip; reductions are two-level and deterministic.
 #include "dmed_internal.h"
 
+#include <cstdlib>
+#include <cstdio>
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -422,15 +426,29 @@ int chain_init(const dmed_sector *s, void *bufs[3], int prec, int nlanc_total, d
     return 0;
 }
 
+// Lanczos steps are captured into a HIP graph when a run is long enough:
+// three steps bring the (v_prev, v, w) rotation back to the start, so one
+// captured three-step graph is replayed.  DMED_GRAPHS=0 turns this off.
+constexpr int kGraphMinSteps = 30;
+
+static bool graphs_enabled()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("DMED_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double *psi)
 {
-        hipStream_t st = ex().stream;
+    hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     ChainScalars *cs = chain_scalars();
     const int nph = hxv_nparts(s, prec);
     const unsigned g = vec_grid(n);
-    void *vp = bufs[0], *v = bufs[1], *w = bufs[2];
-    for (int k = 0; k < count; ++k) {
+    // the kernels of one step; no memsets, no host synchronisation
+    auto one = [&](void *vp, void *v, void *w) -> int {
         if (psi) {
             if (prec == DMED_FP32)
                 hipLaunchKernelGGL(k_accum<float>, dim3(g), dim3(kVecBS), 0, st, psi,
@@ -438,27 +456,74 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
             else
                 hipLaunchKernelGGL(k_accum<double>, dim3(g), dim3(kVecBS), 0, st, psi,
                                    (const double *)v, n, cs, chain_aux());
-            DMED_HIP(hipGetLastError());
         }
         if (int rc = launch_hxv(s, v, w, prec, kLanczos, cs, ex().partials, nullptr, st))
             return rc;
         hipLaunchKernelGGL(k_chain_alpha, dim3(1), dim3(1024), 0, st, ex().partials, nph, cs,
                            chain_alpha());
-        DMED_HIP(hipGetLastError());
         if (prec == DMED_FP32)
             hipLaunchKernelGGL(k_chain_update<float>, dim3(g), dim3(kVecBS), 0, st, (float *)w,
                                (const float *)v, (const float *)vp, n, cs, ex().partials);
         else
             hipLaunchKernelGGL(k_chain_update<double>, dim3(g), dim3(kVecBS), 0, st, (double *)w,
                                (const double *)v, (const double *)vp, n, cs, ex().partials);
-        DMED_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_chain_beta, dim3(1), dim3(1024), 0, st, ex().partials, (int)g, cs,
                            chain_beta());
-        DMED_HIP(hipGetLastError());
+        return hipGetLastError() == hipSuccess ? 0 : fail(DMED_EHIP, "chain step launch failed");
+    };
+    void *vp = bufs[0], *v = bufs[1], *w = bufs[2];
+    auto rotate = [&]() {
         void *t = vp;
         vp = v;
         v = w;
         w = t;
+    };
+    int k = 0;
+    if (count >= kGraphMinSteps && graphs_enabled()) {
+        // one eager step first (kernel attributes are set outside the capture)
+        if (int rc = one(vp, v, w)) return rc;
+        rotate();
+        ++k;
+        hipGraph_t gr = nullptr;
+        hipGraphExec_t ge = nullptr;
+        bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            int rc = 0;
+            for (int q = 0; q < 3 && !rc; ++q) {
+                rc = one(vp, v, w);
+                rotate();
+            }
+            // after three rotations the buffers are back where they started
+            ok = hipStreamEndCapture(st, &gr) == hipSuccess && rc == 0 && gr != nullptr;
+            ok = ok && hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) == hipSuccess;
+        }
+        bool launched_fail = false;
+        if (ok) {
+            const int reps = (count - k) / 3;
+            for (int r = 0; r < reps; ++r)
+                if (hipGraphLaunch(ge, st) != hipSuccess) {
+                    launched_fail = true;
+                    break;
+                }
+            if (!launched_fail) k += 3 * reps;
+            DMED_HIP(hipStreamSynchronize(st));
+        }
+        if (ge) hipGraphExecDestroy(ge);
+        if (gr) hipGraphDestroy(gr);
+        // a capture that could not be built leaves the chain where the eager
+        // step left it and the loop below carries on; a failed replay cannot
+        (void)hipGetLastError();
+        if (!ok) {
+            static std::atomic<bool> warned{false};
+            if (!warned.exchange(true))
+                std::fprintf(stderr, "dmed: Lanczos-step graph capture unavailable, "
+                                     "launching the steps one by one\n");
+        }
+        if (launched_fail) return fail(DMED_EHIP, "chain graph launch failed");
+    }
+    for (; k < count; ++k) {
+        if (int rc = one(vp, v, w)) return rc;
+        rotate();
     }
     bufs[0] = vp;
     bufs[1] = v;

@@ -568,3 +568,40 @@ def test_error_paths_return_codes():
     s2 = dmed.Sector(2, 2)
     y = s2.hxv(np.ones(s2.dim))
     assert np.all(np.isfinite(y))
+
+
+def test_chain_graph_replays_match_eager():
+    """Graph-captured Lanczos steps (chains of >= 30 steps) give alpha/beta
+    bit-identical to eager launches, on repeated runs with eager work and
+    synchronisation in between, in FP64 and FP32, for the tile and the
+    column H*v kernels (workload 1 sector 5/4, workload 2 sector 7/6)."""
+    import subprocess
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, 'dmft-ed_amd'); import dmed; "
+        "from dmed import configs; dmed.init(0); out = []\n"
+        "for n, sec in ((1, (5, 4)), (2, (7, 6))):\n"
+        "    c = configs.config(n)\n"
+        "    dmed.set_model(c['norb'], c['nbath'], c['nspin'], c['bath'], c['hloc'], c['uloc'], "
+        "c['ust'], c['jh'], 0.0, 0.0, c['xmu'], True, False)\n"
+        "    s = dmed.Sector(*sec); v = s.vec(np.random.default_rng(3).standard_normal(s.dim))\n"
+        "    for prec in (dmed.FP64, dmed.FP32, dmed.MIXED):\n"
+        "        for rep in range(3):\n"
+        "            a, b, k = dmed.chain(s, v, 64, prec=prec); out.append(np.concatenate([a, b]))\n"
+        "            s.hxv(np.ones(s.dim))\n"
+        "np.save(sys.argv[1], np.array(out))\n")
+    import os, tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with tempfile.TemporaryDirectory() as td:
+        res = {}
+        for flag in ("1", "0"):
+            f = os.path.join(td, f"g{flag}.npy")
+            env = dict(os.environ, DMED_GRAPHS=flag)
+            r = subprocess.run([sys.executable, "-c", code, f], cwd=root, env=env,
+                               capture_output=True, text=True, timeout=300)
+            assert r.returncode == 0, r.stderr[-2000:]
+            res[flag] = np.load(f)
+    say(f"graph vs eager: {res['1'].shape[0]} chains")
+    assert np.array_equal(res["1"], res["0"])
+    # the three repetitions of each chain agree with each other too
+    g = res["1"].reshape(-1, 3, res["1"].shape[1])
+    assert all(np.array_equal(x[0], x[1]) and np.array_equal(x[0], x[2]) for x in g)
