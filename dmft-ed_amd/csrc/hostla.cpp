@@ -173,15 +173,26 @@ void gf_add_poles(double peso, double ei, const double *lam, const double *w2, i
                   const double *wm, int lmats, const double *wr, int lreal, double eps, double *gm,
                   double *gr)
 {
-    using cd = std::complex<double>;
-    cd *g1 = reinterpret_cast<cd *>(gm);
-    cd *g2 = reinterpret_cast<cd *>(gr);
+    // pw / (z - isign*de) with the reciprocal written out:
+    //   Matsubara z = i*wm:      (-s*de - i*wm) / (de^2 + wm^2)
+    //   real axis z = w + i*eps: ((w - s*de) - i*eps) / ((w - s*de)^2 + eps^2)
     for (int j = 0; j < n; ++j) {
-        const double de = lam[j] - ei;
+        const double sde = (double)isign * (lam[j] - ei);
         const double pw = peso * w2[j];
         if (pw == 0.0) continue;
-        for (int i = 0; i < lmats; ++i) g1[i] += pw / (cd(0.0, wm[i]) - (double)isign * de);
-        for (int i = 0; i < lreal; ++i) g2[i] += pw / (cd(wr[i], eps) - (double)isign * de);
+        const double sde2 = sde * sde;
+        for (int i = 0; i < lmats; ++i) {
+            const double q = pw / (sde2 + wm[i] * wm[i]);
+            gm[2 * i] -= sde * q;
+            gm[2 * i + 1] -= wm[i] * q;
+        }
+        const double eps2 = eps * eps;
+        for (int i = 0; i < lreal; ++i) {
+            const double x = wr[i] - sde;
+            const double q = pw / (x * x + eps2);
+            gr[2 * i] += x * q;
+            gr[2 * i + 1] -= eps * q;
+        }
     }
 }
 

@@ -148,6 +148,25 @@ int run_tasks(int batched, int ntask, const std::function<void(int)> &fn)
     return 0;
 }
 
+// Host-only parallel loop (no device context): tasks write disjoint outputs,
+// so the result does not depend on the thread count.
+void host_parallel(int ntask, const std::function<void(int)> &fn)
+{
+    int nw = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    nw = std::min(nw, ntask);
+    if (nw <= 1) {
+        for (int k = 0; k < ntask; ++k) fn(k);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int w = 0; w < nw; ++w)
+        th.emplace_back([&]() {
+            for (int k; (k = next.fetch_add(1)) < ntask;) fn(k);
+        });
+    for (auto &t : th) t.join();
+}
+
 double now()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -364,6 +383,7 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
         int nlanc = 0, rc = 0;
         double norm2 = 0.0;
         std::vector<double> al, bl;
+        std::vector<double> lam, w2;    // poles: tql2 eigenvalues and Z(1,j)^2
         std::string err;
     };
     std::vector<Chain> chains;
@@ -406,6 +426,16 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
             int nsteps = 0;
             ch.rc = chain_run(ch.J, v.data, DMED_FP64, ch.nlanc, p->prec_gf, 1e-13, ch.al.data(),
                               ch.bl.data(), &nsteps, p->gf_fp64_head);
+            // poles of the chain (add_to_lanczos_gf_normal: tql2 with Z = I)
+            if (!ch.rc) {
+                ch.lam = ch.al;
+                std::vector<double> e(ch.nlanc, 0.0);
+                for (int i = 1; i < ch.nlanc; ++i) e[i] = ch.bl[i];
+                ch.w2.assign(ch.nlanc, 0.0);
+                if (tql2_first_row(ch.nlanc, ch.lam.data(), e.data(), ch.w2.data()))
+                    ch.rc = fail(DMED_ENOCONV, "tql2 did not converge");
+                for (auto &z : ch.w2) z *= z;
+            }
         }
         if (ch.rc) ch.err = dmed_last_error();
     };
@@ -416,6 +446,10 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
         for (int a = 0; a < norb && !rc; ++a) {
             std::fill(gm.begin(), gm.end(), 0.0);
             std::fill(gr.begin(), gr.end(), 0.0);
+            // pole sums of the chains of (s, a), in chain order for every
+            // frequency, split over frequency blocks (deterministic either way)
+            struct Term { const Chain *ch; double peso, e; };
+            std::vector<Term> terms;
             for (auto &ch : chains) {
                 if (ch.s != s || ch.a != a || !(ch.norm2 > 0.0)) continue;
                 const State &st = sl.items[ch.state];
@@ -425,11 +459,18 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
                 } else {
                     peso = ch.norm2 / zeta;
                 }
-                if ((rc = dmed_gf_add_chain(peso, st.e, ch.al.data(), ch.bl.data(), ch.nlanc, ch.isign,
-                                            wm.data(), lm, wr.data(), lr, p->eps, gm.data(), gr.data())))
-                    break;
+                terms.push_back({&ch, peso, st.e});
             }
-            if (rc) break;
+            const int nfreq = lm + lr, nblk = std::max(1, std::min(64, nfreq / 64));
+            host_parallel(nblk, [&](int b) {
+                const int f0 = (int)((int64_t)nfreq * b / nblk), f1 = (int)((int64_t)nfreq * (b + 1) / nblk);
+                const int m0 = std::min(f0, lm), m1 = std::min(f1, lm);
+                const int r0 = std::max(f0, lm) - lm, r1 = std::max(f1, lm) - lm;
+                for (const Term &t : terms)
+                    gf_add_poles(t.peso, t.e, t.ch->lam.data(), t.ch->w2.data(), t.ch->nlanc,
+                                 t.ch->isign, wm.data() + m0, m1 - m0, wr.data() + r0, r1 - r0,
+                                 p->eps, gm.data() + 2 * m0, gr.data() + 2 * r0);
+            });
             for (int l = 0; l < lm && out->gmats; ++l) {
                 const size_t o = gidx(s, s, a, a, l, nspin, norb);
                 out->gmats[2 * o] = gm[2 * l];
