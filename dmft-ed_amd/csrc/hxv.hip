@@ -129,40 +129,32 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
 
 // ------------------------------------------------------------------------
 // Persistent column kernel for large columns (D_up up to 1024*RPT).
-// One 1024-lane workgroup per CU owns a contiguous range of columns of one
-// XCD's share (workgroup b runs on XCD b%8, so concurrent columns of an XCD
-// are neighbours and their down-hop neighbour columns are shared through
-// that XCD's L2).  Per column: the column prefetched into registers during
-// the previous column is stored to LDS; down-hop neighbour columns are
-// streamed from memory four at a time; up hops gather from LDS with packed
-// 4-byte hop words (target rank, coefficient index, sign) whose table stays
-// L2-resident; in FP64 the diagonal is added last from the LDS copy of x; y
-// is written with non-temporal stores.  Each lane owns rows tid + 1024*r, r < RPT.
-template <typename T>
-struct ColArgs {
-    const T *__restrict__ x;
-    T *__restrict__ y;
-    int32_t dup, ddw, nhu, nhd, nimp, ncoef;
-    const uint32_t *__restrict__ hu;      // [nhu][dup] packed
-    const double *__restrict__ coef;      // [ncoef]
-    const int32_t *__restrict__ hd_idx;   // [nhd][ddw]
-    const double *__restrict__ hd_val;
-    const double *__restrict__ du;
-    const double *__restrict__ dd;
-    const uint8_t *__restrict__ impu;
-    const uint8_t *__restrict__ impd;
-    const double *__restrict__ W;
-    const ChainScalars *cs;
-    double *__restrict__ partials;
-};
-
+// One 1024-lane workgroup per CU (two where registers and LDS allow) walks
+// columns c = xcd*nper + loc + k*grid: chunks of nper consecutive columns are
+// dealt round-robin to the 8 XCDs (workgroup b runs on XCD b%8), so each
+// XCD's concurrent columns are neighbours and the whole chip works on one
+// window of consecutive columns.  Per column:
+//   1. x(:,c) is copied into LDS by LDS DMA (global_load_lds), which needs no
+//      registers and overlaps the down hops;
+//   2. down hops: for each neighbour column (HDU at a time) every lane loads
+//      its RPT rows, then accumulates;
+//   3. wait for the DMA, barrier; up hops gather from LDS through the packed
+//      4-byte hop words (target rank, coefficient index, sign), whose table
+//      stays L2-resident;
+//   4. the diagonal d_up + d_dw + W is added last from the LDS copy of x, and
+//      y is written with non-temporal stores.
+// Each lane owns rows tid + 1024*r, r < RPT.  Rows past D_up load row D_up-1
+// (in bounds) and are never stored: the inner loops carry no per-row branch,
+// so all loads of a hop step are in flight before the first is used.
+// Summation order per row: down hops in table order, up hops in table order,
+// then the diagonal.
 template <typename T, int MODE, int RPT>
 __global__ void __launch_bounds__(1024) k_hxv_col(
     const T *__restrict__ X, T *__restrict__ Y, int32_t dup, int32_t ddw, int32_t nhu, int32_t nhd,
     int32_t nimp, int32_t ncoef, const uint32_t *__restrict__ HU, const double *__restrict__ COEF,
     const int32_t *__restrict__ HDI, const double *__restrict__ HDV, const double *__restrict__ DU,
     const double *__restrict__ DD, const uint8_t *__restrict__ IMPU, const uint8_t *__restrict__ IMPD,
-    const double *__restrict__ WTAB, const ChainScalars *__restrict__ CS,
+    const double *__restrict__ WTAB, int32_t dma16, const ChainScalars *__restrict__ CS,
     double *__restrict__ PARTIALS)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -170,7 +162,9 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
     __shared__ T sc[128];
     __shared__ double wt[8];
     __shared__ double red[16];
-    constexpr int HDU = 4;
+    // neighbour columns in flight: two where the registers allow (FP32 up to
+    // 13 rows per lane), else one; FP64 spills at two
+    constexpr int HDU = (sizeof(T) == 4 && RPT <= 13) ? 2 : 1;
     double sx = 1.0;
     if (MODE == kLanczos) {
         if (CS->done) return;
@@ -179,20 +173,9 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
     for (int t = threadIdx.x; t < ncoef; t += blockDim.x) sc[t] = (T)COEF[t];
     const int nper = gridDim.x / 8;                  // workgroups per XCD
     const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
-    const int per = (ddw + 7) / 8;
-    const int cbeg = xcd * per, cend = min(ddw, cbeg + per);
-    // FP64: the up diagonal is re-read from L2 and added last (registers hold
-    // the prefetch and the sums; keeping it resident spills at 13 rows per
-    // lane).  FP32: it stays in registers and starts the sum.
-    constexpr bool kDiagFirst = sizeof(T) == 4;
-    T dui[kDiagFirst ? RPT : 1];
-    if (kDiagFirst) {
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int i = threadIdx.x + r * 1024;
-            dui[r] = i < dup ? (T)DU[i] : (T)0;
-        }
-    }
+    const int last = dup - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int colbytes = dup * (int)sizeof(T);
     // impurity bits of the lane's rows, four per register
     constexpr int NP = (RPT + 3) / 4;
     uint32_t imp4[NP];
@@ -200,46 +183,37 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
     for (int q = 0; q < NP; ++q) imp4[q] = 0u;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        const int i = threadIdx.x + r * 1024;
-        imp4[r / 4] |= (i < dup ? (uint32_t)IMPU[i] : 0u) << (8 * (r % 4));
-    }
-    int c = cbeg + loc;
-    T nx[RPT];
-    if (c < cend) {
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int i = threadIdx.x + r * 1024;
-            nx[r] = i < dup ? X[(int64_t)c * dup + i] : (T)0;
-        }
+        const int ic = min((int)threadIdx.x + r * 1024, last);
+        imp4[r / 4] |= (uint32_t)IMPU[ic] << (8 * (r % 4));
     }
     double dot = 0.0;
-    for (; c < cend; c += nper) {
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int i = threadIdx.x + r * 1024;
-            if (i < dup) col[i] = nx[r];
+    for (int c = xcd * nper + loc; c < ddw; c += gridDim.x) {
+        const char *xc = reinterpret_cast<const char *>(X + (int64_t)c * dup);
+        // 1. x(:,c) -> LDS: 16-byte granules (1 KB per wave instruction)
+        //    when every column start is 16-byte aligned, else 4-byte ones
+        if (dma16) {
+            for (int k = wave; k * 1024 < colbytes; k += 16) {
+                const int off = k * 1024 + lane * 16;
+                if (off < colbytes)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(xc + off),
+                        (__attribute__((address_space(3))) void *)(smem + k * 1024), 16, 0, 0);
+            }
+        } else {
+            for (int k = wave; k * 256 < colbytes; k += 16) {
+                const int off = k * 256 + lane * 4;
+                if (off < colbytes)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(xc + off),
+                        (__attribute__((address_space(3))) void *)(smem + k * 256), 4, 0, 0);
+            }
         }
         if ((int)threadIdx.x < nimp)
             wt[threadIdx.x] = DD[c] + WTAB[threadIdx.x * nimp + IMPD[c]];
-        __syncthreads();
         T acc[RPT];
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            if (kDiagFirst) {
-                const uint32_t ib = (imp4[r / 4] >> (8 * (r % 4))) & 0xffu;
-                acc[r] = nx[r] * (dui[r] + (T)wt[ib]);
-            } else {
-                acc[r] = (T)0;
-            }
-        }
-        const int cn = c + nper;
-        if (cn < cend) {
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int i = threadIdx.x + r * 1024;
-                nx[r] = i < dup ? X[(int64_t)cn * dup + i] : (T)0;
-            }
-        }
+        for (int r = 0; r < RPT; ++r) acc[r] = (T)0;
+        // 2. down hops
         for (int h0 = 0; h0 < nhd; h0 += HDU) {
             T v[HDU];
             const T *xj[HDU];
@@ -249,39 +223,39 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
                 v[q] = h0 + q < nhd ? (T)HDV[(int64_t)h * ddw + c] : (T)0;
                 xj[q] = X + (int64_t)HDI[(int64_t)h * ddw + c] * dup;
             }
+            T g[HDU][RPT];
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int i = threadIdx.x + r * 1024;
-                if (i < dup) {
+            for (int q = 0; q < HDU; ++q)
 #pragma unroll
-                    for (int q = 0; q < HDU; ++q) acc[r] += v[q] * xj[q][i];
-                }
-            }
+                for (int r = 0; r < RPT; ++r) g[q][r] = xj[q][min((int)threadIdx.x + r * 1024, last)];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r)
+#pragma unroll
+                for (int q = 0; q < HDU; ++q) acc[r] += v[q] * g[q][r];
         }
+        // 3. the DMA has landed for every wave, then up hops from LDS
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
         for (int h = 0; h < nhu; ++h) {
+            uint32_t w[RPT];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) w[r] = HU[(int64_t)h * dup + min((int)threadIdx.x + r * 1024, last)];
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
-                const int i = threadIdx.x + r * 1024;
-                if (i < dup) {
-                    const uint32_t w = HU[(int64_t)h * dup + i];
-                    T vv = sc[(w >> 1) & 127u];
-                    vv = (w & 1u) ? -vv : vv;
-                    acc[r] += vv * col[w >> 8];
-                }
+                T vv = sc[(w[r] >> 1) & 127u];
+                vv = (w[r] & 1u) ? -vv : vv;
+                acc[r] += vv * col[w[r] >> 8];
             }
         }
-        // (FP64) diagonal last, with the lane's own x from LDS
+        // 4. diagonal last, with the lane's own x from LDS
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const int i = threadIdx.x + r * 1024;
             if (i < dup) {
                 const int64_t o = (int64_t)c * dup + i;
                 const T xi = col[i];
-                T hv = acc[r];
-                if (!kDiagFirst) {
-                    const uint32_t ib = (imp4[r / 4] >> (8 * (r % 4))) & 0xffu;
-                    hv += (T)(DU[i] + wt[ib]) * xi;
-                }
+                const uint32_t ib = (imp4[r / 4] >> (8 * (r % 4))) & 0xffu;
+                const T hv = acc[r] + (T)(DU[i] + wt[ib]) * xi;
                 if (MODE == kPlain) {
                     __builtin_nontemporal_store(hv, &Y[o]);
                 } else {
@@ -303,6 +277,13 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
 constexpr int kColMaxRptF64 = 13;
 constexpr int kColMaxRptF32 = 16;
 
+// dynamic LDS of the column kernel: the column rounded up to whole 1 KB DMA
+// chunks (the last 16-byte-granule chunk may be partial)
+static size_t col_lds_bytes(int64_t dup, size_t esz)
+{
+    return ((size_t)dup * esz + 1023) & ~(size_t)1023;
+}
+
 static bool use_col_kernel(const dmed_sector *s, int prec)
 {
     const Context &c = ctx();
@@ -311,7 +292,7 @@ static bool use_col_kernel(const dmed_sector *s, int prec)
     const int rpt = (int)((dup + 1023) / 1024);
     const int maxr = prec == DMED_FP32 ? kColMaxRptF32 : kColMaxRptF64;
     return s->up.packed && s->up.ncoef <= 128 && s->nimp <= 8 && dup >= 512 &&
-           rpt <= maxr && (size_t)dup * esz + 2048 <= c.lds_max && s->dw.dim >= 64;
+           rpt <= maxr && col_lds_bytes(dup, esz) + 2048 <= c.lds_max && s->dw.dim >= 64;
 }
 
 template <typename T, int RPT>
@@ -333,7 +314,7 @@ static unsigned col_grid(const dmed_sector *s, int prec)
     const int64_t dup = s->up.dim;
     const int rpt = (int)((dup + 1023) / 1024);
     const bool f32 = prec == DMED_FP32;
-    const size_t shmem = (size_t)dup * (f32 ? 4 : 8);
+    const size_t shmem = col_lds_bytes(dup, f32 ? 4 : 8);
     int fit = 1;
     switch (rpt) {
 #define DMED_FIT(R) \
@@ -351,34 +332,20 @@ template <typename T, int MODE, int RPT>
 static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScalars *cs,
                         double *partials, hipStream_t st)
 {
-    ColArgs<T> a;
-    a.x = static_cast<const T *>(x);
-    a.y = static_cast<T *>(y);
-    a.dup = (int32_t)s->up.dim;
-    a.ddw = (int32_t)s->dw.dim;
-    a.nhu = s->up.nhop;
-    a.nhd = s->dw.nhop;
-    a.nimp = s->nimp;
-    a.ncoef = s->up.ncoef;
-    a.hu = s->up.hpk;
-    a.coef = s->up.coef;
-    a.hd_idx = s->dw.hidx;
-    a.hd_val = s->dw.hval;
-    a.du = s->up.diag;
-    a.dd = s->dw.diag;
-    a.impu = s->up.imp;
-    a.impd = s->dw.imp;
-    a.W = s->W;
-    a.cs = cs;
-    a.partials = partials;
-    const size_t shmem = (size_t)a.dup * sizeof(T);
+    const int64_t dup = s->up.dim;
+    const size_t shmem = col_lds_bytes(dup, sizeof(T));
+    // 16-byte DMA granules need every column start 16-byte aligned
+    const int dma16 = ((uintptr_t)x % 16 == 0) && ((size_t)dup * sizeof(T)) % 16 == 0;
     auto kern = k_hxv_col<T, MODE, RPT>;
     if (shmem > 64 * 1024)
         DMED_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)shmem));
-    hipLaunchKernelGGL(kern, dim3(col_grid(s, sizeof(T) == 4 ? DMED_FP32 : DMED_FP64)), dim3(1024), shmem, st, a.x, a.y, a.dup, a.ddw,
-                       a.nhu, a.nhd, a.nimp, a.ncoef, a.hu, a.coef, a.hd_idx, a.hd_val, a.du, a.dd,
-                       a.impu, a.impd, a.W, a.cs, a.partials);
+    hipLaunchKernelGGL(kern, dim3(col_grid(s, sizeof(T) == 4 ? DMED_FP32 : DMED_FP64)), dim3(1024),
+                       shmem, st, static_cast<const T *>(x), static_cast<T *>(y), (int32_t)dup,
+                       (int32_t)s->dw.dim, (int32_t)s->up.nhop, (int32_t)s->dw.nhop, (int32_t)s->nimp,
+                       (int32_t)s->up.ncoef, s->up.hpk, s->up.coef, s->dw.hidx, s->dw.hval,
+                       s->up.diag, s->dw.diag, s->up.imp, s->dw.imp, s->W, (int32_t)dma16,
+                       (const ChainScalars *)cs, partials);
     DMED_HIP(hipGetLastError());
     return 0;
 }

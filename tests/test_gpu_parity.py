@@ -108,6 +108,36 @@ def test_gs_sector_map_and_hxv_vs_oracle(cfg):
     assert np.abs(y32 - yref).max() <= 2e-6 * scale
 
 
+def test_column_kernel_odd_column_length_vs_oracle():
+    """Ns=13 sector (5,5): D_up = 1287 is odd, so neither FP64 nor FP32
+    columns start on 16-byte boundaries and the column kernel copies x into
+    LDS in 4-byte granules.  Bars as for the workloads; the chain mode
+    (Lanczos epilogue) is checked through the first chain coefficients."""
+    from dmed.configs import SplitMix64, bath_vector, hloc_fortran
+    rng = SplitMix64(77)
+    nb = 12
+    e = [rng.uniform(-1.0, 1.0) for _ in range(nb)]
+    v = [0.3 + 0.2 * rng.uniform() for _ in range(nb)]
+    args = (1, nb, 1, bath_vector([[e]], [[v]]), hloc_fortran(1, 1), [2.5, 0.0, 0.0], 0.0, 0.0,
+            0.0, 0.0, 0.0, True, False)
+    edref.set_model(*args)
+    dmed.set_model(*args)
+    say("[Ns=13] sector 5,5")
+    s = dmed.Sector(5, 5)
+    assert s.dim == 1287 * 1287
+    m = edref.build_sector(5, 5)
+    x = np.random.default_rng(13).standard_normal(s.dim)
+    yref = edref.hxv(m, x, nthreads=8).real
+    scale = np.abs(yref).max()
+    assert np.abs(s.hxv(x) - yref).max() <= 1e-13 * scale
+    assert np.abs(s.hxv(x, prec=dmed.FP32) - yref).max() <= 2e-6 * scale
+    a, b, n = dmed.chain(s, s.vec(x), 6)
+    ar, br, nr = edref.lanc_tridiag(m, x, 6, nthreads=8)
+    assert n == nr == 6
+    assert np.allclose(a, ar, atol=1e-11)
+    assert np.allclose(b[1:], br[1:], atol=1e-11)
+
+
 def test_ns16_hxv_properties():
     """Ns=16 half filling (dim 165,636,900): size-independent properties --
     map sorted/unique with the right popcounts, H symmetric (<x,Hy> = <Hx,y>),
