@@ -219,90 +219,148 @@ __global__ void __launch_bounds__(kVecBS) k_fill_random(T *__restrict__ x, int64
 }
 
 // ----------------------------------------- block-Krylov helpers (FP64)
-// c[j] partials over rows: partials[j*nb + blk] = sum_i V[j*n+i] * w[i], j < m
-__global__ void __launch_bounds__(kVecBS) k_multidot(const double *__restrict__ V, int64_t n, int m,
-                                                     const double *__restrict__ w,
-                                                     double *__restrict__ partials,
-                                                     const int *__restrict__ gate)
+// One classical Gram-Schmidt pass and its norms in two launches, with no
+// cross-block synchronisation: a producer writes per-block partials, and
+// every block of the consumer re-sums the few it needs in index order (the
+// same values in the same order in every block, hence the same result).
+// Cross-block tickets would need agent-scope fences, which on a multi-XCD
+// chip write back and invalidate the per-XCD L2 caches.  The grid of these
+// kernels is capped so the re-summed partials stay small.
+constexpr int kGsGrid = 512;
+constexpr int kGsMaxRows = 160;     // Y and V rows (nl + m <= 64 + 64) plus slack
+
+// sum of n partials in index order, valid in every thread
+__device__ __forceinline__ double psum_all(const double *__restrict__ p, int n, double *red)
 {
-    __shared__ double red[16];
-    if (gate && !*gate) return;
-    const int j = blockIdx.y;
-    const double *vj = V + (int64_t)j * n;
     double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        s += vj[i] * w[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
     s = bsum(s, red);
-    if (threadIdx.x == 0) partials[(int64_t)j * gridDim.x + blockIdx.x] = s;
+    __shared__ double out;
+    if (threadIdx.x == 0) out = s;
+    __syncthreads();
+    s = out;
+    __syncthreads();
+    return s;
 }
 
-// out[j] = sum of row j of the partials; acc[j] = (first ? 0 : acc[j]) + out[j]
-__global__ void __launch_bounds__(1024) k_reduce_rows_acc(const double *__restrict__ p, int nb,
-                                                          double *__restrict__ out,
-                                                          double *__restrict__ acc, int first,
-                                                          const int *__restrict__ gate)
+// c[r] = sum of the g partials of row r of P (row stride g), r < nr, into
+// shared c; sixteen rows at a time with all their loads in flight, reduced
+// by wave shuffles and one pass over the waves (a fixed order for a given
+// g and block size).  red: shared, >= 16 * waves per block.
+__device__ __forceinline__ void rowsums(const double *__restrict__ P, int nr, int g,
+                                        double *c, double *red)
 {
-    __shared__ double red[16];
-    if (gate && !*gate) return;
-    const int j = blockIdx.x;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[(int64_t)j * nb + i];
-    s = bsum(s, red);
-    if (threadIdx.x == 0) {
-        out[j] = s;
-        acc[j] = first ? s : acc[j] + s;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int r0 = 0; r0 < nr; r0 += 16) {
+        double s[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            s[q] = 0.0;
+            if (r0 + q < nr)
+                for (int i = threadIdx.x; i < g; i += blockDim.x) s[q] += P[(int64_t)(r0 + q) * g + i];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s[q] = wsum(s[q]);
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) red[q * nw + w] = s[q];
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < 16 && r0 + (int)threadIdx.x < nr) {
+            double t = 0.0;
+            for (int k = 0; k < nw; ++k) t += red[threadIdx.x * nw + k];
+            c[r0 + threadIdx.x] = t;
+        }
+        __syncthreads();
     }
 }
 
-// DGKS test after one Gram-Schmidt pass: *gate = 1 when ||w||^2 (sum of p)
-// fell below half of the squared norm before it (*n0sq), i.e. cancellation
-// large enough that a second pass is needed
-__global__ void __launch_bounds__(1024) k_dgks(const double *__restrict__ p, int n,
-                                               const double *__restrict__ n0sq, int *__restrict__ gate)
+// Partials of the projections of w on the rows Y_0..Y_{ny-1}, V_0..V_{nv-1}
+// and, with one extra row, on w itself: P[r*gridDim.x + block].
+// Second pass (dgks != 0): runs only when the first pass cancelled more than
+// half of ||w||^2 (Daniel-Gragg-Kaufman-Stewart), judged from the first
+// pass's partials: n0 (||w||^2 before, g0 values) and n1 (after, g0 values).
+__global__ void __launch_bounds__(kVecBS) k_proj_dot(
+    const double *__restrict__ Y, int ny, const double *__restrict__ V, int nv,
+    const double *__restrict__ w, int64_t n, double *__restrict__ P, int dgks,
+    const double *__restrict__ n0, const double *__restrict__ n1)
 {
     __shared__ double red[16];
+    if (dgks) {
+        const double a0 = psum_all(n0, gridDim.x, red);
+        const double a1 = psum_all(n1, gridDim.x, red);
+        if (!(a1 < 0.5 * a0)) return;
+    }
+    const int r = blockIdx.y;
+    const double *b = r < ny ? Y + (int64_t)r * n : (r < ny + nv ? V + (int64_t)(r - ny) * n : w);
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
-    s = bsum(s, red);
-    if (threadIdx.x == 0) *gate = (s < 0.5 * *n0sq) ? 1 : 0;
-}
-
-// *nrm = sqrt(sum p)
-__global__ void __launch_bounds__(1024) k_reduce_norm(const double *__restrict__ p, int n,
-                                                      double *__restrict__ nrm)
-{
-    __shared__ double red[16];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
-    s = bsum(s, red);
-    if (threadIdx.x == 0) *nrm = sqrt(s);
-}
-
-// dst = src / *nrm (0 when the norm is 0)
-__global__ void __launch_bounds__(kVecBS) k_scale_inv(double *__restrict__ dst,
-                                                      const double *__restrict__ src, int64_t n,
-                                                      const double *__restrict__ nrm)
-{
-    const double a = *nrm > 0.0 ? 1.0 / *nrm : 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = a * src[i];
+        s += b[i] * w[i];
+    s = bsum(s, red);
+    if (threadIdx.x == 0) P[(int64_t)r * gridDim.x + blockIdx.x] = s;
 }
 
-// w -= sum_j V_j c_j
-__global__ void __launch_bounds__(kVecBS) k_multiaxpy(const double *__restrict__ V, int64_t n, int m,
-                                                      const double *__restrict__ c,
-                                                      double *__restrict__ w,
-                                                      const int *__restrict__ gate)
+// w -= sum_r Y_r cy_r + sum_r V_r cv_r, the coefficients summed from the
+// row partials P of k_proj_dot; partials of the new ||w||^2 into Pn.
+// Block 0 stores the V coefficients at acc (first: overwrite, else add).
+// dgks: second pass, gated as in k_proj_dot.
+__global__ void __launch_bounds__(kVecBS) k_proj_axpy(
+    const double *__restrict__ Y, int ny, const double *__restrict__ V, int nv,
+    double *__restrict__ w, int64_t n, const double *__restrict__ P, double *__restrict__ Pn,
+    double *__restrict__ acc, int first, int dgks, const double *__restrict__ n0,
+    const double *__restrict__ n1)
 {
-    if (gate && !*gate) return;
+    __shared__ double red[16];
+    __shared__ double red16[16 * (kVecBS / 64)];
+    __shared__ double c[kGsMaxRows];
+    if (dgks) {
+        const double a0 = psum_all(n0, gridDim.x, red);
+        const double a1 = psum_all(n1, gridDim.x, red);
+        if (!(a1 < 0.5 * a0)) return;
+    }
+    const int nr = ny + nv;
+    rowsums(P, nr, gridDim.x, c, red16);
+    if (blockIdx.x == 0 && acc)
+        for (int r = ny + (int)threadIdx.x; r < nr; r += blockDim.x)
+            acc[r - ny] = first ? c[r] : acc[r - ny] + c[r];
+    double s2 = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         double s = 0.0;
-        for (int j = 0; j < m; ++j) s += V[(int64_t)j * n + i] * c[j];
-        w[i] -= s;
+        for (int j = 0; j < ny; ++j) s += Y[(int64_t)j * n + i] * c[j];
+        for (int j = 0; j < nv; ++j) s += V[(int64_t)j * n + i] * c[ny + j];
+        const double x = w[i] - s;
+        w[i] = x;
+        s2 += x * x;
     }
+    s2 = bsum(s2, red);
+    if (threadIdx.x == 0) Pn[blockIdx.x] = s2;
+}
+
+// dst = w / ||w||, with ||w||^2 the sum of the partials p1 (after the first
+// pass) or, when the second pass ran (same test as above on n0, p1), p2;
+// 0 when the norm is 0.  Block 0 stores the norm at *nrm.  Without a
+// projection (noproj) p1 holds the partials of ||w||^2 directly.
+__global__ void __launch_bounds__(kVecBS) k_normalize(double *__restrict__ dst,
+                                                      const double *__restrict__ w, int64_t n,
+                                                      const double *__restrict__ n0,
+                                                      const double *__restrict__ p1,
+                                                      const double *__restrict__ p2, int noproj,
+                                                      double *__restrict__ nrm)
+{
+    __shared__ double red[16];
+    double q = psum_all(p1, gridDim.x, red);
+    if (!noproj) {
+        const double a0 = psum_all(n0, gridDim.x, red);
+        if (q < 0.5 * a0) q = psum_all(p2, gridDim.x, red);
+    }
+    const double r = sqrt(q);
+    const double a = r > 0.0 ? 1.0 / r : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nrm = r;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = a * w[i];
 }
 
 // in place V[:, 0:k] = V[:, 0:m] S[0:m, 0:k]  (S column-major m x k)
@@ -792,60 +850,56 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
     double *V = nullptr, *w = nullptr, *dS = nullptr;
     if (int rc = exec_buf(ex(), kSlotTrlBasis, (size_t)(m + 1) * n * 8, (void **)&V)) return rc;
     if (int rc = exec_buf(ex(), kSlotTrlW, n * 8, (void **)&w)) return rc;
-    // dS: Ritz rotation (m*m), projection coefficients of one pass (m+nl+1),
-    // and per expansion step j the accumulated CGS2 coefficients (j+1) and
-    // the new norm, in rows of m+2 (kept on the device until the restart)
+    // dS: Ritz rotation (m*m), and per expansion step j the accumulated
+    // CGS2 coefficients (j+1) and the new norm, in rows of m+2 (kept on the
+    // device until the restart)
     const size_t ldc = (size_t)m + 2;
-    if (int rc = exec_buf(ex(), kSlotTrlSmall,
-                          ((size_t)m * m + (size_t)(m + nl + 1) + (size_t)m * ldc + 4) * 8,
+    if (nl + m > kGsMaxRows) return fail(DMED_EINVAL, "too many projection rows");
+    if (int rc = exec_buf(ex(), kSlotTrlSmall, ((size_t)m * m + (size_t)m * ldc + 4) * 8,
                           (void **)&dS))
         return rc;
-    double *dc = dS + (size_t)m * m;
-    double *dcoef = dc + (m + nl + 1);
-    const unsigned g = vec_grid(n);
-    if (int rc = ensure_partials((size_t)(std::max(m, nl) + 1) * g + 16)) return rc;
-    // project out the columns [0,jmax) of B from w; with cacc (device), the
-    // coefficients are accumulated there (first: overwrite).  `gate`
-    // (device flag) skips the projection when 0.  No host sync.
-    int *dgate = reinterpret_cast<int *>(dcoef + (size_t)m * ldc);
-    double *dn0 = dcoef + (size_t)m * ldc + 1;
-    auto project = [&](const double *B, int jmax, double *cacc, bool first, const int *gate) -> int {
-        if (jmax <= 0) return 0;
-        hipLaunchKernelGGL(k_multidot, dim3(g, jmax), dim3(kVecBS), 0, st, B, n, jmax, w,
-                           ex().partials, gate);
-        if (cacc)
-            hipLaunchKernelGGL(k_reduce_rows_acc, dim3(jmax), dim3(1024), 0, st, ex().partials,
-                               (int)g, dc, cacc, first ? 1 : 0, gate);
-        else
-            hipLaunchKernelGGL(k_reduce_rows_acc, dim3(jmax), dim3(1024), 0, st, ex().partials,
-                               (int)g, dc, dc, 1, gate);
-        hipLaunchKernelGGL(k_multiaxpy, dim3(g), dim3(kVecBS), 0, st, B, n, jmax, dc, w, gate);
-        DMED_HIP(hipGetLastError());
-        return 0;
-    };
-    // classical Gram-Schmidt against Y and V[0..jmax), repeated once when the
-    // first pass cancelled more than half of ||w||^2 (Daniel-Gragg-Kaufman-
-    // Stewart criterion), decided on the device
+    double *dcoef = dS + (size_t)m * m;
+    // partial sums of one orthogonalisation: rows of the first pass
+    // (nl + m + 1, the last one ||w||^2 before it), ||w||^2 after it, rows
+    // of the second pass, ||w||^2 after that
+    const unsigned g = (unsigned)std::min<unsigned>(vec_grid(n), kGsGrid);
+    const int nr = nl + m;
+    if (int rc = ensure_partials((size_t)(2 * nr + 3) * g + 16)) return rc;
+    double *P1 = ex().partials, *Pn1 = P1 + (size_t)(nr + 1) * g, *P2 = Pn1 + g, *Pn2 = P2 + (size_t)nr * g;
+    // classical Gram-Schmidt of w against Y[0..nl) and V[0..jmax), repeated
+    // once when the first pass cancelled more than half of ||w||^2 (Daniel-
+    // Gragg-Kaufman-Stewart criterion, decided on the device); the V
+    // coefficients of both passes are accumulated at cacc when given.  Four
+    // launches, no host sync; normalize_into then needs no reduction launch.
+    bool noproj = false;
+    const double *n0_last = nullptr;
     auto orth = [&](int jmax, double *cacc) -> int {
-        if (jmax + nl <= 0) return 0;
-        int np = 0;
-        if (int rc = launch_dot(w, w, n, DMED_FP64, ex().partials, &np, st)) return rc;
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, ex().partials, np, dn0, 0);
-        if (int rc = project(Y, nl, nullptr, false, nullptr)) return rc;
-        if (int rc = project(V, jmax, cacc, true, nullptr)) return rc;
-        if (int rc = launch_dot(w, w, n, DMED_FP64, ex().partials, &np, st)) return rc;
-        hipLaunchKernelGGL(k_dgks, dim3(1), dim3(1024), 0, st, ex().partials, np, dn0, dgate);
-        if (int rc = project(Y, nl, nullptr, false, dgate)) return rc;
-        if (int rc = project(V, jmax, cacc, false, dgate)) return rc;
+        const int rows = nl + jmax;
+        noproj = rows <= 0;
+        if (noproj) {
+            // only ||w||^2: the w row of k_proj_dot, into Pn1
+            hipLaunchKernelGGL(k_proj_dot, dim3(g, 1), dim3(kVecBS), 0, st, Y, 0, V, 0, w, n, Pn1,
+                               0, (const double *)nullptr, (const double *)nullptr);
+            DMED_HIP(hipGetLastError());
+            return 0;
+        }
+        const double *n0 = P1 + (size_t)rows * g;   // the w row
+        hipLaunchKernelGGL(k_proj_dot, dim3(g, rows + 1), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n,
+                           P1, 0, (const double *)nullptr, (const double *)nullptr);
+        hipLaunchKernelGGL(k_proj_axpy, dim3(g), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n, P1, Pn1,
+                           cacc, 1, 0, (const double *)nullptr, (const double *)nullptr);
+        hipLaunchKernelGGL(k_proj_dot, dim3(g, rows), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n, P2,
+                           1, n0, Pn1);
+        hipLaunchKernelGGL(k_proj_axpy, dim3(g), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n, P2, Pn2,
+                           cacc, 0, 1, n0, Pn1);
         DMED_HIP(hipGetLastError());
+        n0_last = n0;
         return 0;
     };
-    // dst = w/||w||, the norm left at nrm (device)
+    // dst = w/||w|| with the norm from the partials orth left; stored at nrm
     auto normalize_into = [&](double *dst, double *nrm) -> int {
-        int np = 0;
-        if (int rc = launch_dot(w, w, n, DMED_FP64, ex().partials, &np, st)) return rc;
-        hipLaunchKernelGGL(k_reduce_norm, dim3(1), dim3(1024), 0, st, ex().partials, np, nrm);
-        hipLaunchKernelGGL(k_scale_inv, dim3(g), dim3(kVecBS), 0, st, dst, w, n, nrm);
+        hipLaunchKernelGGL(k_normalize, dim3(g), dim3(kVecBS), 0, st, dst, w, n,
+                           noproj ? Pn1 : n0_last, Pn1, Pn2, noproj ? 1 : 0, nrm);
         DMED_HIP(hipGetLastError());
         return 0;
     };
