@@ -104,12 +104,43 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
         const T *xc = LDS ? (col + cc * dup) : (xb + (int64_t)cc * dup);
         const T xo = xc[i];
         T acc = (T)(a.du[i] + wt[cc * a.nimp + a.impu[i]]) * xo;
-        for (int h = 0; h < a.nhu; ++h)
+        // hops four at a time: every index and value load of a group is issued
+        // before its gathers, and the gathers before the adds, which keep the
+        // table order
+        int h = 0;
+        for (; h + 4 <= a.nhu; h += 4) {
+            int32_t id[4];
+            T v[4], g[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                id[q] = a.hu_idx[(int64_t)(h + q) * dup + i];
+                v[q] = a.hu_val[(int64_t)(h + q) * dup + i];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g[q] = xc[id[q]];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc += v[q] * g[q];
+        }
+        for (; h < a.nhu; ++h)
             acc += a.hu_val[(int64_t)h * dup + i] * xc[a.hu_idx[(int64_t)h * dup + i]];
         const int64_t c = c0 + cc;
-        for (int h = 0; h < a.nhd; ++h) {
-            const int64_t j = a.hd_idx[(int64_t)h * a.ddw + c];
-            acc += a.hd_val[(int64_t)h * a.ddw + c] * a.x[j * dup + i];
+        h = 0;
+        for (; h + 4 <= a.nhd; h += 4) {
+            int64_t j[4];
+            T v[4], g[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                j[q] = a.hd_idx[(int64_t)(h + q) * a.ddw + c];
+                v[q] = a.hd_val[(int64_t)(h + q) * a.ddw + c];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g[q] = a.x[j[q] * dup + i];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc += v[q] * g[q];
+        }
+        for (; h < a.nhd; ++h) {
+            const int64_t jj = a.hd_idx[(int64_t)h * a.ddw + c];
+            acc += a.hd_val[(int64_t)h * a.ddw + c] * a.x[jj * dup + i];
         }
         const int64_t o = c0 * dup + e;
         if (MODE == kPlain) {
