@@ -21,6 +21,12 @@
 #include "dmed_internal.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 namespace dmed {
 
@@ -336,6 +342,59 @@ static int col_fit_per_cu(size_t shmem)
     return nb;
 }
 
+// ------------------------------------------------------------ autotuner
+// Opt-in (DMED_TUNE=1).  The first H*v of a sector shape times the launch
+// shapes that give the same y bit for bit: the tile kernel's columns per
+// workgroup and workgroup size, and the column kernel's workgroups per CU.
+// It keeps the fastest shape for that sector shape and precision.
+// DMED_TUNE_FILE=path loads earlier choices on first use and appends new
+// ones, so later runs reuse them.  Lanczos-mode partial sums follow the
+// grid, so a tuned run's alpha/beta can differ from the fixed rules' in the
+// last bits.  By default the fixed rules below apply, and runs are
+// reproducible bit for bit.
+struct TuneChoice {
+    int cb = 0, bs = 0;      // tile kernel (0: fixed rule)
+    int wgs = 0;             // column kernel workgroups per CU (0: fixed rule)
+};
+using TuneKey = std::tuple<int64_t, int64_t, int, int, int, int>;
+
+static TuneKey tune_key(const dmed_sector *s, int prec)
+{
+    return TuneKey{s->up.dim, s->dw.dim, s->up.nhop, s->dw.nhop, s->nimp,
+                   prec == DMED_FP32 ? DMED_FP32 : DMED_FP64};
+}
+
+struct TuneTable {
+    std::mutex mu;
+    std::map<TuneKey, TuneChoice> tab;
+    bool loaded = false;
+};
+
+static TuneTable &tune_table()
+{
+    static TuneTable t;
+    return t;
+}
+
+static bool tune_enabled()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("DMED_TUNE");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
+// choice recorded for this shape (no tuning here); zeros when none
+static TuneChoice tune_get(const dmed_sector *s, int prec)
+{
+    if (!tune_enabled()) return TuneChoice{};
+    TuneTable &t = tune_table();
+    std::lock_guard<std::mutex> lk(t.mu);
+    auto it = t.tab.find(tune_key(s, prec));
+    return it == t.tab.end() ? TuneChoice{} : it->second;
+}
+
 // Workgroups of the persistent column kernel: as many per CU as its
 // registers and LDS allow, at most two (a third lowered the rate at
 // D_up = 3432), in multiples of the 8 XCDs, never more than the columns.
@@ -354,14 +413,17 @@ static unsigned col_grid(const dmed_sector *s, int prec)
 #undef DMED_FIT
     default: fit = 1; break;
     }
-    int g = (ncu / 8) * 8 * std::max(1, std::min(fit, 2));
+    int per_cu = std::max(1, std::min(fit, 2));
+    const TuneChoice tc = tune_get(s, prec);
+    if (tc.wgs > 0) per_cu = std::min(tc.wgs, std::max(1, fit));
+    int g = (ncu / 8) * 8 * per_cu;
     while (g > 8 && g / 8 > (s->dw.dim + 7) / 8) g -= 8;
     return (unsigned)g;
 }
 
 template <typename T, int MODE, int RPT>
 static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScalars *cs,
-                        double *partials, hipStream_t st)
+                        double *partials, unsigned grid, hipStream_t st)
 {
     const int64_t dup = s->up.dim;
     const size_t shmem = col_lds_bytes(dup, sizeof(T));
@@ -371,7 +433,7 @@ static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScala
     if (shmem > 64 * 1024)
         DMED_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)shmem));
-    hipLaunchKernelGGL(kern, dim3(col_grid(s, sizeof(T) == 4 ? DMED_FP32 : DMED_FP64)), dim3(1024),
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(1024),
                        shmem, st, static_cast<const T *>(x), static_cast<T *>(y), (int32_t)dup,
                        (int32_t)s->dw.dim, (int32_t)s->up.nhop, (int32_t)s->dw.nhop, (int32_t)s->nimp,
                        (int32_t)s->up.ncoef, s->up.hpk, s->up.coef, s->dw.hidx, s->dw.hval,
@@ -383,19 +445,19 @@ static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScala
 
 template <typename T, int MODE>
 static int launch_col_rpt(int rpt, const dmed_sector *s, const void *x, void *y,
-                          ChainScalars *cs, double *partials, hipStream_t st)
+                          ChainScalars *cs, double *partials, unsigned grid, hipStream_t st)
 {
     switch (rpt) {
 #define DMED_RPT(R) \
-    case R: return launch_col_t<T, MODE, R>(s, x, y, cs, partials, st);
+    case R: return launch_col_t<T, MODE, R>(s, x, y, cs, partials, grid, st);
         DMED_RPT(1) DMED_RPT(2) DMED_RPT(3) DMED_RPT(4) DMED_RPT(5) DMED_RPT(6) DMED_RPT(7) DMED_RPT(8)
         DMED_RPT(9) DMED_RPT(10) DMED_RPT(11) DMED_RPT(12) DMED_RPT(13)
 #undef DMED_RPT
     case 14: case 15: case 16:
         if constexpr (sizeof(T) == 4) {
-            if (rpt == 14) return launch_col_t<T, MODE, 14>(s, x, y, cs, partials, st);
-            if (rpt == 15) return launch_col_t<T, MODE, 15>(s, x, y, cs, partials, st);
-            return launch_col_t<T, MODE, 16>(s, x, y, cs, partials, st);
+            if (rpt == 14) return launch_col_t<T, MODE, 14>(s, x, y, cs, partials, grid, st);
+            if (rpt == 15) return launch_col_t<T, MODE, 15>(s, x, y, cs, partials, grid, st);
+            return launch_col_t<T, MODE, 16>(s, x, y, cs, partials, grid, st);
         }
         break;
     default:
@@ -433,6 +495,11 @@ static LaunchShape pick_shape(const dmed_sector *s, int prec)
     sh.cb = (int)std::max<int64_t>(cb, 1);
     const int64_t nel = (int64_t)sh.cb * dup;
     sh.bs = nel >= 4096 ? 1024 : (nel >= 1024 ? 512 : 256);
+    const TuneChoice tc = tune_get(s, prec);
+    if (tc.cb > 0) {
+        sh.cb = (int)std::min<int64_t>(tc.cb, std::max<int64_t>(ddw, 1));
+        sh.bs = tc.bs;
+    }
     sh.shmem = (sh.lds ? (((size_t)sh.cb * colb + 15) & ~(size_t)15) : 0) +
                8 * (size_t)sh.cb * s->nimp;
     sh.grid = (unsigned)((ddw + sh.cb - 1) / sh.cb);
@@ -479,19 +546,168 @@ static int launch_t(const dmed_sector *s, const void *x, void *y, ChainScalars *
     return 0;
 }
 
+// ms per launch of `fn` on st: the best of three batches of `rep` launches,
+// after one untimed launch; -1 on a failure
+template <typename F>
+static float time_launches(F fn, int rep, hipStream_t st)
+{
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return -1.f;
+    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return -1.f; }
+    float best = -1.f;
+    bool ok = fn() == 0;
+    for (int b = 0; b < 3 && ok; ++b) {
+        float ms = 0.f;
+        ok = hipEventRecord(e0, st) == hipSuccess;
+        for (int r = 0; r < rep && ok; ++r) ok = fn() == 0;
+        ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (ok && (best < 0 || ms / rep < best)) best = ms / rep;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return ok ? best : -1.f;
+}
+
+// keep the fixed rule unless a candidate beats it by more than 3 %
+constexpr float kTuneMargin = 0.97f;
+
+static void tune_load_file(TuneTable &t)
+{
+    t.loaded = true;
+    const char *path = std::getenv("DMED_TUNE_FILE");
+    if (!path) return;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return;
+    long long dup, ddw;
+    int nhu, nhd, nimp, prec, cb, bs, wgs;
+    while (std::fscanf(f, "%lld %lld %d %d %d %d %d %d %d", &dup, &ddw, &nhu, &nhd, &nimp, &prec,
+                       &cb, &bs, &wgs) == 9)
+        t.tab[TuneKey{dup, ddw, nhu, nhd, nimp, prec}] = TuneChoice{cb, bs, wgs};
+    std::fclose(f);
+}
+
+// With DMED_TUNE=1, make sure this sector shape has a recorded launch shape,
+// timing the candidates on st the first time (not while st is capturing a
+// graph: then the fixed rules apply until a later eager call).
+void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
+{
+    if (!tune_enabled() || s->dim == 0) return;
+    prec = prec == DMED_FP32 ? DMED_FP32 : DMED_FP64;
+    TuneTable &t = tune_table();
+    const TuneKey key = tune_key(s, prec);
+    {
+        std::lock_guard<std::mutex> lk(t.mu);
+        if (!t.loaded) tune_load_file(t);
+        if (t.tab.count(key)) return;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
+    const size_t esz = prec == DMED_FP32 ? 4 : 8;
+    void *x = nullptr, *y = nullptr;
+    if (hipMalloc(&x, s->dim * esz) != hipSuccess || hipMalloc(&y, s->dim * esz) != hipSuccess) {
+        (void)hipGetLastError();
+        if (x) hipFree(x);
+        std::lock_guard<std::mutex> lk(t.mu);
+        t.tab[key] = TuneChoice{};
+        return;
+    }
+    hipMemsetAsync(x, 0, s->dim * esz, st);
+    TuneChoice best{};
+    float tbest = -1.f;
+    const bool f32 = prec == DMED_FP32;
+    const int rep = s->dim >= (int64_t)1 << 24 ? 3 : 20;
+    if (use_col_kernel(s, prec)) {
+        // workgroups per CU the registers and LDS allow: 1, and 2 if it fits
+        const int rpt = (int)((s->up.dim + 1023) / 1024);
+        const int ncu = std::max(8, ctx().num_cu);
+        int fit = 1;
+        const size_t shmem = col_lds_bytes(s->up.dim, esz);
+        switch (rpt) {
+#define DMED_FIT(R) \
+    case R: fit = f32 ? col_fit_per_cu<float, R>(shmem) : col_fit_per_cu<double, R>(shmem); break;
+            DMED_FIT(1) DMED_FIT(2) DMED_FIT(3) DMED_FIT(4) DMED_FIT(5) DMED_FIT(6) DMED_FIT(7)
+#undef DMED_FIT
+        default: fit = 1; break;
+        }
+        // the fixed rule first, then the other count
+        const int w0 = std::max(1, std::min(fit, 2));
+        for (int w : {w0, 3 - w0}) {
+            if (w > std::min(fit, 2)) continue;
+            unsigned g = (unsigned)((ncu / 8) * 8 * w);
+            while (g > 8 && g / 8 > (s->dw.dim + 7) / 8) g -= 8;
+            auto fn = [&]() {
+                return f32 ? launch_col_rpt<float, kPlain>(rpt, s, x, y, nullptr, nullptr, g, st)
+                           : launch_col_rpt<double, kPlain>(rpt, s, x, y, nullptr, nullptr, g, st);
+            };
+            const float ms = time_launches(fn, rep, st);
+            if (ms > 0 && (tbest < 0 || ms < kTuneMargin * tbest)) {
+                tbest = ms;
+                best = TuneChoice{0, 0, w};
+            }
+        }
+    } else {
+        LaunchShape base = pick_shape(s, prec);     // no recorded choice yet: the fixed rule
+        const int64_t dup = s->up.dim, ddw = s->dw.dim;
+        const size_t colb = (size_t)dup * esz;
+        std::vector<std::pair<int, int>> cand{{base.cb, base.bs}};
+        for (int cb = 1; cb <= 64 && cb <= ddw; cb *= 2)
+            for (int bs = 256; bs <= 1024; bs *= 2)
+                if (cb != base.cb || bs != base.bs) cand.push_back({cb, bs});
+        for (auto [cb, bs] : cand) {
+            {
+                LaunchShape sh = base;
+                sh.cb = cb;
+                sh.bs = bs;
+                if (sh.lds && (size_t)cb * colb + 16 + 8 * (size_t)cb * s->nimp > 64 * 1024) continue;
+                sh.shmem = (sh.lds ? (((size_t)cb * colb + 15) & ~(size_t)15) : 0) + 8 * (size_t)cb * s->nimp;
+                sh.grid = (unsigned)((ddw + cb - 1) / cb);
+                auto fn = [&]() {
+                    if (f32) return sh.lds ? launch_t<float, kPlain, true>(s, x, y, nullptr, nullptr, sh, st)
+                                           : launch_t<float, kPlain, false>(s, x, y, nullptr, nullptr, sh, st);
+                    return sh.lds ? launch_t<double, kPlain, true>(s, x, y, nullptr, nullptr, sh, st)
+                                  : launch_t<double, kPlain, false>(s, x, y, nullptr, nullptr, sh, st);
+                };
+                const float ms = time_launches(fn, rep, st);
+                if (ms > 0 && (tbest < 0 || ms < kTuneMargin * tbest)) {
+                    tbest = ms;
+                    best = TuneChoice{cb, bs, 0};
+                }
+            }
+        }
+    }
+    hipStreamSynchronize(st);
+    hipFree(x);
+    hipFree(y);
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (t.tab.count(key)) return;            // another thread got there first
+    t.tab[key] = best;
+    if (const char *path = std::getenv("DMED_TUNE_FILE")) {
+        if (FILE *f = std::fopen(path, "a")) {
+            std::fprintf(f, "%lld %lld %d %d %d %d %d %d %d\n", (long long)std::get<0>(key),
+                         (long long)std::get<1>(key), std::get<2>(key), std::get<3>(key),
+                         std::get<4>(key), std::get<5>(key), best.cb, best.bs, best.wgs);
+            std::fclose(f);
+        }
+    }
+}
+
 int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
                ChainScalars *cs, double *partials, int *nparts, hipStream_t st)
 {
     if (s->dim == 0) return 0;
     if (mode == kLanczos && partials == nullptr) return fail(DMED_EINVAL, "no partials");
+    tune_ensure(s, prec, st);
     if (use_col_kernel(s, prec)) {
-        if (nparts) *nparts = (int)col_grid(s, prec);
+        const unsigned grid = col_grid(s, prec);
+        if (nparts) *nparts = (int)grid;
         const int rpt = (int)((s->up.dim + 1023) / 1024);
         if (prec == DMED_FP32)
-            return mode == kPlain ? launch_col_rpt<float, kPlain>(rpt, s, x, y, cs, partials, st)
-                                  : launch_col_rpt<float, kLanczos>(rpt, s, x, y, cs, partials, st);
-        return mode == kPlain ? launch_col_rpt<double, kPlain>(rpt, s, x, y, cs, partials, st)
-                              : launch_col_rpt<double, kLanczos>(rpt, s, x, y, cs, partials, st);
+            return mode == kPlain ? launch_col_rpt<float, kPlain>(rpt, s, x, y, cs, partials, grid, st)
+                                  : launch_col_rpt<float, kLanczos>(rpt, s, x, y, cs, partials, grid, st);
+        return mode == kPlain ? launch_col_rpt<double, kPlain>(rpt, s, x, y, cs, partials, grid, st)
+                              : launch_col_rpt<double, kLanczos>(rpt, s, x, y, cs, partials, grid, st);
     }
     const LaunchShape sh = pick_shape(s, prec);
     if (nparts) *nparts = (int)sh.grid;
@@ -511,6 +727,7 @@ int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
 
 int hxv_nparts(const dmed_sector *s, int prec)
 {
+    tune_ensure(s, prec, ex().stream);
     if (use_col_kernel(s, prec)) return (int)col_grid(s, prec);
     return (int)pick_shape(s, prec).grid;
 }
