@@ -150,8 +150,6 @@ int dmed_set_model(const dmed_model *m)
     if (m->nbath < 0) return fail(DMED_EINVAL, "Nbath < 0");
     const int ns = (m->nbath + 1) * m->norb;
     if (ns > kMaxNs) return fail(DMED_EINVAL, "Ns too large for 32-bit spin strings");
-    if (m->jhflag && m->norb > 1)
-        return fail(DMED_EUNSUPP, "spin-exchange/pair-hopping (Jhflag) is not built on the GPU path");
     Model &md = ctx().model;
     md.norb = m->norb;
     md.nbath = m->nbath;
@@ -678,6 +676,54 @@ __global__ void __launch_bounds__(256) k_imp_hist(const T *__restrict__ psi, int
         partials[(int64_t)t * gridDim.x + blockIdx.x] = hist[t];
 }
 
+// <psi|O|psi> of the spin-exchange (O_x) and pair-hopping (O_p) operators,
+// sum over ordered impurity pairs a != b (ED_OBSERVABLES.f90:597-647): per
+// element, the up pair hop b -> a of its row with the down pair hop a -> b
+// (O_x) or b -> a (O_p) of its column, signs from the pair tables.
+// partials[block] (O_x) and partials[gridDim.x + block] (O_p).
+template <typename T>
+__global__ void __launch_bounds__(256) k_jterm_sums(const T *__restrict__ psi, int64_t dup,
+                                                    int64_t ddw, const int32_t *__restrict__ ju,
+                                                    const int32_t *__restrict__ jd, int no,
+                                                    double *__restrict__ partials)
+{
+    __shared__ double red[2][4];
+    double sx = 0.0, sp = 0.0;
+    const int64_t n = dup * ddw;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / dup, r = e - c * dup;
+        const double x = (double)psi[e];
+        for (int pa = 0; pa < no; ++pa)
+            for (int pb = 0; pb < no; ++pb) {
+                if (pa == pb) continue;
+                const int32_t wu = ju[(int64_t)(pa * no + pb) * dup + r];
+                if (wu < 0) continue;
+                const int64_t ru = wu >> 1;
+                const double su = (wu & 1) ? -1.0 : 1.0;
+                const int32_t wx = jd[(int64_t)(pb * no + pa) * ddw + c];
+                if (wx >= 0) sx += x * su * ((wx & 1) ? -1.0 : 1.0) * (double)psi[(int64_t)(wx >> 1) * dup + ru];
+                const int32_t wp = jd[(int64_t)(pa * no + pb) * ddw + c];
+                if (wp >= 0) sp += x * su * ((wp & 1) ? -1.0 : 1.0) * (double)psi[(int64_t)(wp >> 1) * dup + ru];
+            }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_down(sx, off, 64);
+        sp += __shfl_down(sp, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wv] = sx;
+        red[1][wv] = sp;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[threadIdx.x][w];
+        partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
 }  // namespace dmed
 
 extern "C" int dmed_observables(const dmed_vec *psi, double peso, double *dens, double *docc)
@@ -821,6 +867,32 @@ extern "C" int dmed_local_observables(const dmed_vec *psi, double peso, dmed_loc
         for (auto *v : cv) dmed_vec_free(v);
         dmed_sector_free(t);
         if (rc) return rc;
+    }
+    // spin exchange and pair hopping, per state without the Boltzmann weight
+    // (ED_OBSERVABLES.f90:597-647, as the reference accumulates them)
+    if (s->jterms) {
+        const unsigned gj = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (s->dim + 255) / 256));
+        if (int rc = ensure_partials((size_t)2 * gj + 16)) return rc;
+        if (psi->prec == DMED_FP64)
+            hipLaunchKernelGGL(dmed::k_jterm_sums<double>, dim3(gj), dim3(256), 0, st,
+                               (const double *)psi->data, s->up.dim, s->dw.dim, s->up.ipair,
+                               s->dw.ipair, no, ex().partials);
+        else
+            hipLaunchKernelGGL(dmed::k_jterm_sums<float>, dim3(gj), dim3(256), 0, st,
+                               (const float *)psi->data, s->up.dim, s->dw.dim, s->up.ipair,
+                               s->dw.ipair, no, ex().partials);
+        DMED_HIP(hipGetLastError());
+        std::vector<double> hj((size_t)2 * gj);
+        DMED_HIP(hipMemcpyAsync(hj.data(), ex().partials, hj.size() * 8, hipMemcpyDeviceToHost, st));
+        DMED_HIP(hipStreamSynchronize(st));
+        double ox = 0.0, op = 0.0;
+        for (unsigned b = 0; b < gj; ++b) {
+            ox += hj[b];
+            op += hj[gj + b];
+        }
+        o.dse += ox;
+        o.dph += op;
+        o.epot += md.jx * ox + md.jp * op;
     }
     double *dst = reinterpret_cast<double *>(acc);
     const double *src = reinterpret_cast<const double *>(&o);

@@ -62,7 +62,12 @@ struct SpinTables {
     int ncoef = 0;
     uint32_t *hpk = nullptr;            // [nhop][dim]
     double *coef = nullptr;             // [ncoef]
+    // impurity pair hops for the spin-exchange / pair-hopping terms
+    // (Jhflag, Norb > 1): ipair[(a*norb + b)*dim + r] = (rank << 1) | sign of
+    // c^+_a c_b applied to string r (b occupied, a empty), -1 otherwise
+    int32_t *ipair = nullptr;
     std::vector<uint32_t> h_str;        // host copies (small)
+    std::vector<int32_t> h_ipair;
     std::vector<double> h_diag;
     std::vector<uint8_t> h_imp;
     std::vector<int32_t> h_hidx;
@@ -79,6 +84,8 @@ struct dmed_sector {
     int model_generation = -1;
     dmed::SpinTables up, dw;
     int nimp = 1;                       // 2^Norb
+    int jterms = 0;                     // spin-exchange / pair-hopping present
+    double jx = 0.0, jp = 0.0;
     double *W = nullptr;                // device [nimp*nimp] interaction table W[iu*nimp+id]
     std::vector<double> h_W;
 };

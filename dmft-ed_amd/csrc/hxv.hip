@@ -47,6 +47,11 @@ struct HxvArgs {
     const double *__restrict__ W;         // [nimp][nimp]
     const ChainScalars *cs;
     double *__restrict__ partials;
+    // spin exchange / pair hopping (jterms): impurity pair-hop tables
+    const int32_t *__restrict__ ju;       // [norb*norb][dup]
+    const int32_t *__restrict__ jd;       // [norb*norb][ddw]
+    int32_t jterms, norb;
+    double jx, jp;
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -147,6 +152,29 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
         for (; h < a.nhd; ++h) {
             const int64_t jj = a.hd_idx[(int64_t)h * a.ddw + c];
             acc += a.hd_val[(int64_t)h * a.ddw + c] * a.x[jj * dup + i];
+        }
+        // spin exchange Jx c^+_a,up c_b,up c^+_b,dw c_a,dw and pair hopping
+        // Jp c^+_a,up c_b,up c^+_a,dw c_b,dw (HxVint.f90:48-97): an up hop
+        // b -> a on the row with a down hop a -> b (Jx) or b -> a (Jp) on
+        // the column
+        if (a.jterms) {
+            const int no = a.norb;
+            for (int pa = 0; pa < no; ++pa)
+                for (int pb = 0; pb < no; ++pb) {
+                    if (pa == pb) continue;
+                    const int32_t wu = a.ju[(int64_t)(pa * no + pb) * dup + i];
+                    if (wu < 0) continue;
+                    const int64_t iu2 = wu >> 1;
+                    const double su = (wu & 1) ? -1.0 : 1.0;
+                    const int32_t wx = a.jd[(int64_t)(pb * no + pa) * a.ddw + c];
+                    if (wx >= 0)
+                        acc += (T)(a.jx * su * ((wx & 1) ? -1.0 : 1.0)) *
+                               a.x[(int64_t)(wx >> 1) * dup + iu2];
+                    const int32_t wp = a.jd[(int64_t)(pa * no + pb) * a.ddw + c];
+                    if (wp >= 0)
+                        acc += (T)(a.jp * su * ((wp & 1) ? -1.0 : 1.0)) *
+                               a.x[(int64_t)(wp >> 1) * dup + iu2];
+                }
         }
         const int64_t o = c0 * dup + e;
         if (MODE == kPlain) {
@@ -328,7 +356,7 @@ static bool use_col_kernel(const dmed_sector *s, int prec)
     const int64_t dup = s->up.dim;
     const int rpt = (int)((dup + 1023) / 1024);
     const int maxr = prec == DMED_FP32 ? kColMaxRptF32 : kColMaxRptF64;
-    return s->up.packed && s->up.ncoef <= 128 && s->nimp <= 8 && dup >= 512 &&
+    return !s->jterms && s->up.packed && s->up.ncoef <= 128 && s->nimp <= 8 && dup >= 512 &&
            rpt <= maxr && col_lds_bytes(dup, esz) + 2048 <= c.lds_max && s->dw.dim >= 64;
 }
 
@@ -534,6 +562,12 @@ static int launch_t(const dmed_sector *s, const void *x, void *y, ChainScalars *
     a.impu = s->up.imp;
     a.impd = s->dw.imp;
     a.W = s->W;
+    a.ju = s->up.ipair;
+    a.jd = s->dw.ipair;
+    a.jterms = s->jterms;
+    a.norb = s->norb;
+    a.jx = s->jx;
+    a.jp = s->jp;
     a.cs = cs;
     a.partials = partials;
     auto kern = k_hxv<T, MODE, LDS>;

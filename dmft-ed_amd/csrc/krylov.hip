@@ -692,6 +692,26 @@ int dense_eigh(const dmed_sector *s, int neig, double *evals, dmed_vec **evecs, 
                 const int64_t j = iu + (int64_t)s->dw.h_hidx[(size_t)h * ddw + id] * dup;
                 H[(size_t)j * n + i] += s->dw.h_hval[(size_t)h * ddw + id];
             }
+            // spin exchange and pair hopping: up pair hop b -> a with down
+            // pair hop a -> b (Jx) or b -> a (Jp)
+            if (s->jterms) {
+                const int no = s->norb;
+                for (int pa = 0; pa < no; ++pa)
+                    for (int pb = 0; pb < no; ++pb) {
+                        if (pa == pb) continue;
+                        const int32_t wu = s->up.h_ipair[(size_t)(pa * no + pb) * dup + iu];
+                        if (wu < 0) continue;
+                        const double su = (wu & 1) ? -1.0 : 1.0;
+                        const int32_t wx = s->dw.h_ipair[(size_t)(pb * no + pa) * ddw + id];
+                        if (wx >= 0)
+                            H[(size_t)((wu >> 1) + (int64_t)(wx >> 1) * dup) * n + i] +=
+                                s->jx * su * ((wx & 1) ? -1.0 : 1.0);
+                        const int32_t wp = s->dw.h_ipair[(size_t)(pa * no + pb) * ddw + id];
+                        if (wp >= 0)
+                            H[(size_t)((wu >> 1) + (int64_t)(wp >> 1) * dup) * n + i] +=
+                                s->jp * su * ((wp & 1) ? -1.0 : 1.0);
+                    }
+            }
         }
     if (int rc = host_syev((int)n, H.data(), w.data())) return rc;
     for (int q = 0; q < neig; ++q) {

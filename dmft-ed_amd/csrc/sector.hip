@@ -133,6 +133,7 @@ static void release_spin(SpinTables &t)
     hipFree(t.hval32);
     hipFree(t.hpk);
     hipFree(t.coef);
+    hipFree(t.ipair);
     t = SpinTables();
 }
 
@@ -301,6 +302,41 @@ static double interaction(const Model &m, unsigned iu, unsigned id)
     return h;
 }
 
+// Impurity pair-hop tables of one spin (host strings -> device): for every
+// ordered pair (a, b) of impurity orbitals, a != b, the string with b
+// emptied and a filled, and the sign of c^+_a c_b on it (the fermionic signs
+// of c_b on the string, then of c^+_a on the result: ED_SETUP.f90:1055-1081).
+// The spin-exchange and pair-hopping terms of HxVint.f90:48-97 are products
+// of one such hop per spin: their two down-spin signs carry (-1)^n_up each,
+// which cancel.
+static int build_pairs(const Model &m, SpinTables &t)
+{
+    const int no = m.norb;
+    const int64_t dim = t.dim;
+    std::vector<int32_t> tab((size_t)no * no * dim, -1);
+    for (int a = 0; a < no; ++a)
+        for (int b = 0; b < no; ++b) {
+            if (a == b) continue;
+            int32_t *row = tab.data() + (size_t)(a * no + b) * dim;
+            for (int64_t r = 0; r < dim; ++r) {
+                const uint32_t u = t.h_str[r];
+                if (!((u >> b) & 1u) || ((u >> a) & 1u)) continue;
+                const uint32_t u1 = u & ~(1u << b);
+                const int n1 = __builtin_popcount(u & ((1u << b) - 1u));
+                const int n2 = __builtin_popcount(u1 & ((1u << a) - 1u));
+                const uint32_t u2 = u1 | (1u << a);
+                const auto it = std::lower_bound(t.h_str.begin(), t.h_str.end(), u2);
+                if (it == t.h_str.end() || *it != u2) return fail(DMED_EINVAL, "pair hop leaves the sector");
+                const int64_t rk = it - t.h_str.begin();
+                row[r] = (int32_t)((rk << 1) | ((n1 + n2) & 1));
+            }
+        }
+    DMED_HIP(hipMalloc(&t.ipair, tab.size() * 4));
+    DMED_HIP(hipMemcpy(t.ipair, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+    t.h_ipair = std::move(tab);
+    return 0;
+}
+
 int sector_build(dmed_sector *s, int nup, int ndw)
 {
     Context &c = ctx();
@@ -317,6 +353,13 @@ int sector_build(dmed_sector *s, int nup, int ndw)
     if (int rc = build_spin(m, 1, ndw, s->dw, ex().stream)) return rc;
     s->dim = s->up.dim * s->dw.dim;
     s->nimp = 1 << m.norb;
+    s->jterms = (m.jhflag && m.norb > 1 && (m.jx != 0.0 || m.jp != 0.0)) ? 1 : 0;
+    s->jx = m.jx;
+    s->jp = m.jp;
+    if (s->jterms) {
+        if (int rc = build_pairs(m, s->up)) return rc;
+        if (int rc = build_pairs(m, s->dw)) return rc;
+    }
     s->h_W.resize(s->nimp * s->nimp);
     for (int iu = 0; iu < s->nimp; ++iu)
         for (int id = 0; id < s->nimp; ++id)

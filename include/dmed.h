@@ -61,7 +61,8 @@ const char *dmed_last_error(void);
  *          Length 2*Nspin*Norb*Nbath = get_bath_dimension() (user_aux.f90:17-30).
  *   hloc : real part of impHloc(Nspin,Nspin,Norb,Norb), Fortran order.
  * Globals of ED_INPUT_VARS.f90:13-24 (Uloc, Ust, Jh, Jx, Jp, xmu, HFmode,
- * Jhflag).  Jhflag=1 with Norb>1 returns DMED_EUNSUPP on the GPU path.
+ * Jhflag).  Jhflag=1 with Norb>1 adds the spin-exchange (Jx) and
+ * pair-hopping (Jp) terms of HxVint.f90:48-97.
  * Replaces: set_dmft_bath + set_Hloc + the module globals read by
  * build_hxv.f90 (ED_HAMILTONIAN_MATVEC.f90:289-351). */
 typedef struct dmed_model {
@@ -190,8 +191,10 @@ int dmed_observables(const dmed_vec *psi, double peso, double *dens, double *doc
 /* Local observables and energies (observables_impurity, ED_OBSERVABLES.f90:74-150,
  * and local_energy_impurity, :450-669, normal mode).  Orbital arrays [a],
  * orbital pairs [a*3+b], Norb <= 3.  epot includes ehartree; eint is never
- * assigned by the reference and stays 0; dse and dph need Jhflag, which the
- * library does not support, and stay 0. */
+ * assigned by the reference and stays 0; dse and dph (spin exchange and pair
+ * hopping, Jhflag with Norb > 1) are summed without the state weight peso, as
+ * the reference does (ED_OBSERVABLES.f90:597-647), and epot adds Jx*dse +
+ * Jp*dph. */
 typedef struct dmed_local_obs {
     double dens[3], dens_up[3], dens_dw[3], docc[3], magz[3];
     double sz2[9], n2[9];

@@ -341,12 +341,12 @@ def _sector_eigs(smap, neigen, p, seed=1234567):
 
 
 def ed_solve(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, xmu=0.0,
-             hfmode=True, params=None, sectors=None):
+             hfmode=True, params=None, sectors=None, jx=0.0, jp=0.0, jhflag=False):
     """The reference's ed_solve_single sequence on the oracle (small Ns only).
     Returns a dict with egs, zeta, states, dens, docc, gmats, greal, smats, sreal
     (arrays indexed [ispin, iorb, freq])."""
     p = default_params() if params is None else params
-    ns = set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust, jh, 0.0, 0.0, xmu, hfmode)
+    ns = set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust, jh, jx, jp, xmu, hfmode, jhflag)
     finite_t = p["lanc_nstates_total"] != 1
     nstates_sector = p["lanc_nstates_sector"]
     nstates_total = p["lanc_nstates_total"]

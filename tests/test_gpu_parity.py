@@ -56,17 +56,7 @@ def set_cfg(c):
     return dmed.set_model(*args)
 
 
-GPU_GOLD = [n for n in GOLD if n != "m3_norb2_nb2_jh"]
-
-
-def test_jhflag_model_is_refused():
-    g = load("m3_norb2_nb2_jh")
-    with pytest.raises(dmed.DmedError) as e:
-        set_both(g)
-    assert e.value.code == -5
-
-
-@pytest.mark.parametrize("name", GPU_GOLD)
+@pytest.mark.parametrize("name", GOLD)
 def test_maps_and_hxv_vs_golden(name):
     g = load(name)
     ns = set_both(g)
@@ -167,7 +157,7 @@ def test_ns16_hxv_properties():
     assert np.abs(h32 - hx).max() <= 2e-6 * np.abs(hx).max()
 
 
-@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m4_norb3_nb1", "m5_norb1_nb5_T"])
+@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m3_norb2_nb2_jh", "m4_norb3_nb1", "m5_norb1_nb5_T"])
 def test_eigh_vs_golden_spectra(name):
     g = load(name)
     ns = set_both(g)
@@ -418,16 +408,19 @@ def test_apply_op_and_observables_vs_oracle(name):
                     assert np.isclose(n2, np.dot(ref, ref), rtol=1e-13)
 
 
-def _solve_both(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode=True, **kw):
+def _solve_both(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode=True, jx=0.0, jp=0.0,
+                jhflag=False, **kw):
     p = dmed.default_params(**kw)
     r = dmed.solve(norb, nspin, (nbath + 1) * norb, p)
     pr = edref.default_params(**{k: v for k, v in kw.items()
                                  if k not in ("prec_gf", "batched", "verbose")})
-    o = edref.ed_solve(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode, params=pr)
+    o = edref.ed_solve(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode, params=pr,
+                       jx=jx, jp=jp, jhflag=jhflag)
     return r, o
 
 
 @pytest.mark.parametrize("name,finite", [("m1_norb1_nb3", False), ("m2_norb2_nb1", False),
+                                          ("m3_norb2_nb2_jh", False),
                                           ("m4_norb3_nb1", False), ("m6_u0", False),
                                           ("m8_norb2_nb2_nohf", False),
                                           ("m5_norb1_nb5_T", True)])
@@ -442,7 +435,8 @@ def test_solve_vs_oracle(name, finite):
         kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
     say(f"[{name}] solve")
     r, o = _solve_both(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
-                       float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]), **kw)
+                       float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]),
+                       jx=float(g["jx"]), jp=float(g["jp"]), jhflag=bool(g["jhflag"]), **kw)
     assert abs(o["egs"] - float(g["egs"])) <= 1e-10 * abs(float(g["egs"]))
     assert abs(r["egs"] - o["egs"]) <= 1e-10 * abs(o["egs"])
     assert r["nstates"] == len(o["states"])
@@ -540,11 +534,12 @@ def test_vec_copy_converts_precision():
     assert np.array_equal(f.download(), x.astype(np.float32).astype(np.float64))
 
 
-@pytest.mark.parametrize("name", ["m2_norb2_nb1", "m4_norb3_nb1", "m5_norb1_nb5_T"])
+@pytest.mark.parametrize("name", ["m2_norb2_nb1", "m3_norb2_nb2_jh", "m4_norb3_nb1", "m5_norb1_nb5_T"])
 def test_local_observables_vs_oracle(name):
     """dmed_local_observables on GPU eigenvectors against the oracle's
     restatement on the same vectors: every field to 1e-12 (m2 has
-    off-diagonal Hloc, m4 three orbitals, m5 two spins)."""
+    off-diagonal Hloc, m3 spin exchange and pair hopping, m4 three orbitals,
+    m5 two spins)."""
     g = load(name)
     set_both(g)
     ns = (int(g["nbath"]) + 1) * int(g["norb"])

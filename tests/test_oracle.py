@@ -190,10 +190,12 @@ def _solve_golden(g, **pkw):
     norb = int(g["norb"])
     return edref.ed_solve(norb, int(g["nbath"]), int(g["nspin"]), g["bath"], g["hloc_f"],
                           g["uloc"][:norb], float(g["ust"]), float(g["jh"]),
-                          float(g["xmu"]), bool(g["hfmode"]), params=p)
+                          float(g["xmu"]), bool(g["hfmode"]), params=p, jx=float(g["jx"]),
+                          jp=float(g["jp"]), jhflag=bool(g["jhflag"]))
 
 
-@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m2_norb2_nb1", "m4_norb3_nb1", "m6_u0"])
+@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m2_norb2_nb1", "m3_norb2_nb2_jh", "m4_norb3_nb1",
+                                  "m6_u0"])
 def test_ed_solve_zero_temperature_matches_lehmann(name):
     g = load(name)
     r = _solve_golden(g, lanc_ngfiter=400)
