@@ -724,7 +724,89 @@ __global__ void __launch_bounds__(256) k_jterm_sums(const T *__restrict__ psi, i
     }
 }
 
+// out = S_z psi, S_z = (n_up - n_dw)/2 summed over the impurity orbitals in
+// `mask` (bit a = orbital a): the impurity bytes of the row and the column
+// give the occupations; partials of ||out||^2 per block
+template <typename T>
+__global__ void __launch_bounds__(256) k_apply_sz(const T *__restrict__ psi, int64_t dup,
+                                                  int64_t ddw, const uint8_t *__restrict__ impu,
+                                                  const uint8_t *__restrict__ impd, unsigned mask,
+                                                  T *__restrict__ out, double *__restrict__ partials)
+{
+    __shared__ double red[4];
+    double s2 = 0.0;
+    const int64_t n = dup * ddw;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / dup, r = e - c * dup;
+        const int sgn = __popc((unsigned)impu[r] & mask) - __popc((unsigned)impd[c] & mask);
+        const T v = (T)(0.5 * (double)sgn * (double)psi[e]);
+        out[e] = v;
+        s2 += (double)v * (double)v;
+    }
+    for (int off = 32; off > 0; off >>= 1) s2 += __shfl_down(s2, off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
 }  // namespace dmed
+
+extern "C" int dmed_apply_sz(const dmed_vec *psi, int32_t iorb, dmed_vec *out, double *norm2)
+{
+    REQUIRE_MODEL();
+    if (!psi || !out || !norm2) return fail(DMED_EINVAL, "null argument");
+    const dmed_sector *s = psi->sec;
+    if (int rc = check_sector(s)) return rc;
+    if (iorb < -1 || iorb >= s->norb) return fail(DMED_EINVAL, "iorb out of range");
+    if (out->sec != s || out->prec != psi->prec || out == psi)
+        return fail(DMED_EINVAL, "output vector must be another vector of psi's sector and precision");
+    hipStream_t st = ex().stream;
+    const unsigned mask = iorb < 0 ? (1u << s->norb) - 1u : 1u << iorb;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (s->dim + 255) / 256));
+    if (int rc = ensure_partials(4096)) return rc;
+    if (psi->prec == DMED_FP64)
+        hipLaunchKernelGGL(dmed::k_apply_sz<double>, dim3(g), dim3(256), 0, st,
+                           (const double *)psi->data, s->up.dim, s->dw.dim, s->up.imp, s->dw.imp,
+                           mask, (double *)out->data, ex().partials);
+    else
+        hipLaunchKernelGGL(dmed::k_apply_sz<float>, dim3(g), dim3(256), 0, st,
+                           (const float *)psi->data, s->up.dim, s->dw.dim, s->up.imp, s->dw.imp,
+                           mask, (float *)out->data, ex().partials);
+    DMED_HIP(hipGetLastError());
+    std::vector<double> h(g);
+    DMED_HIP(hipMemcpyAsync(h.data(), ex().partials, g * 8, hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipStreamSynchronize(st));
+    double t = 0.0;
+    for (unsigned b = 0; b < g; ++b) t += h[b];
+    *norm2 = t;
+    return 0;
+}
+
+extern "C" int dmed_chi_add_chain(double peso_f, double peso_bz, double ei, const double *alpha,
+                                  const double *beta_l, int32_t nlanc, double beta,
+                                  const double *vm, int32_t nv, const double *tau, int32_t ntau,
+                                  const double *wr, int32_t lreal, double eps, double *chi_iv,
+                                  double *chi_tau, double *chi_w)
+{
+    if (nlanc <= 0) return 0;
+    if (!alpha || !beta_l || (nv > 0 && (!vm || !chi_iv)) || (ntau > 0 && (!tau || !chi_tau)) ||
+        (lreal > 0 && (!wr || !chi_w)))
+        return fail(DMED_EINVAL, "null argument");
+    std::vector<double> d(alpha, alpha + nlanc), e(nlanc, 0.0), z(nlanc);
+    for (int i = 1; i < nlanc; ++i) e[i] = beta_l[i];
+    if (tql2_first_row(nlanc, d.data(), e.data(), z.data()))
+        return fail(DMED_ENOCONV, "tql2 did not converge");
+    for (auto &x : z) x = x * x;
+    chi_add_poles(peso_f, peso_bz, ei, d.data(), z.data(), nlanc, beta, vm, nv, tau, ntau, wr,
+                  lreal, eps, chi_iv, chi_tau, chi_w);
+    return 0;
+}
 
 extern "C" int dmed_observables(const dmed_vec *psi, double peso, double *dens, double *docc)
 {

@@ -216,5 +216,11 @@ int tql2_first_row(int n, double *d, double *e, double *z1);
 void gf_add_poles(double peso, double ei, const double *lam, const double *w2, int n,
                   int isign, const double *wm, int lmats, const double *wr, int lreal,
                   double eps, double *gm, double *gr);
+// add_to_lanczos_spinChi (build_chi_spin.f90:186-250) for isign = +1 then -1;
+// vm[0..nv), tau[0..ntau), wr[0..lreal); chi_iv, chi_w interleaved complex
+void chi_add_poles(double peso_f, double peso_bz, double ei, const double *lam, const double *w2,
+                   int n, double beta, const double *vm, int nv, const double *tau, int ntau,
+                   const double *wr, int lreal, double eps, double *chi_iv, double *chi_tau,
+                   double *chi_w);
 
 }  // namespace dmed

@@ -196,4 +196,40 @@ void gf_add_poles(double peso, double ei, const double *lam, const double *w2, i
     }
 }
 
+// add_to_lanczos_spinChi (build_chi_spin.f90:186-250).  S_z is hermitian,
+// so the particle (isign = +1) and hole (isign = -1) terms come from one
+// chain; per frequency the terms are added in pole order, +1 then -1.
+// Reciprocals written out: 1/(x + i y) = (x - i y)/(x^2 + y^2).
+void chi_add_poles(double peso_f, double peso_bz, double ei, const double *lam, const double *w2,
+                   int n, double beta, const double *vm, int nv, const double *tau, int ntau,
+                   const double *wr, int lreal, double eps, double *chi_iv, double *chi_tau,
+                   double *chi_w)
+{
+    for (int isign = 1; isign >= -1; isign -= 2)
+        for (int j = 0; j < n; ++j) {
+            const double de = lam[j] - ei;
+            const double peso = peso_f * w2[j] * peso_bz;
+            const double ex = std::exp(-beta * de);
+            if (nv > 0) chi_iv[0] += beta * de < 1e-1 ? peso * beta : peso * (1.0 - ex) / de;
+            // isign +1: peso*(ex - 1)/(z - de);  -1: peso*(1 - ex)/(z + de)
+            const double num = isign > 0 ? peso * (ex - 1.0) : peso * (1.0 - ex);
+            const double sde = isign > 0 ? de : -de;
+            for (int i = 1; i < nv; ++i) {
+                // z = i*vm: 1/(-sde + i vm) = (-sde - i vm)/(sde^2 + vm^2)
+                const double q = num / (sde * sde + vm[i] * vm[i]);
+                chi_iv[2 * i] -= sde * q;
+                chi_iv[2 * i + 1] -= vm[i] * q;
+            }
+            for (int i = 0; i < ntau; ++i)
+                chi_tau[i] += isign > 0 ? peso * std::exp(-tau[i] * de)
+                                        : peso * std::exp(-(beta - tau[i]) * de);
+            for (int i = 0; i < lreal; ++i) {
+                const double x = wr[i] - sde;
+                const double q = num / (x * x + eps * eps);
+                chi_w[2 * i] += x * q;
+                chi_w[2 * i + 1] -= eps * q;
+            }
+        }
+}
+
 }  // namespace dmed

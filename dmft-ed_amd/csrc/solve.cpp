@@ -202,6 +202,8 @@ extern "C" void dmed_solve_params_default(dmed_solve_params *p)
     p->verbose = 0;
     p->gf_fp64_head = kMixedHead;
     p->prec_gs = DMED_FP64;
+    p->chi_spin = 0;
+    p->ltau = 1000;
     p->pad_ = 0;
 }
 
@@ -502,6 +504,94 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
                 }
             }
         }
+    // ---------------------------------------------- spin susceptibility
+    // build_chi_spin (build_chi_spin.f90:4-18, ED_MAIN.f90:282): for every
+    // kept state, S_z of each orbital (rows 0..Norb-1) and, when Norb > 1, of
+    // all orbitals (row Norb) in the state's own sector, one chain each, the
+    // poles added for isign = +1 and -1; the weights carry 1/zeta and the
+    // sums are divided by zeta once more at the end, as the reference does.
+    // A zero S_z psi carries no weight (the reference divides by its norm).
+    if (!rc && p->chi_spin) {
+        const int ltau = std::max((int)beta, p->ltau);     // ED_INPUT_VARS.f90:191
+        const int nrow = norb + 1, nv = lm + 1, nt = ltau + 1;
+        std::vector<double> vm(nv), tau(nt);
+        for (int i = 0; i < nv; ++i) vm[i] = M_PI / beta * 2.0 * (double)i;
+        for (int i = 0; i < nt; ++i) tau[i] = ltau > 0 ? beta * (double)i / (double)ltau : 0.0;
+        struct ChiTask {
+            int state, row;
+            int nlanc = 0, rc = 0;
+            double norm2 = 0.0;
+            std::vector<double> lam, w2;
+            std::string err;
+        };
+        std::vector<ChiTask> tasks;
+        for (int k = 0; k < (int)sl.items.size(); ++k)
+            for (int row = 0; row < (norb > 1 ? nrow : norb); ++row) {
+                ChiTask t;
+                t.state = k;
+                t.row = row;
+                tasks.push_back(std::move(t));
+            }
+        auto run_chi = [&](int t) {
+            ChiTask &ck = tasks[t];
+            const State &st = sl.items[ck.state];
+            const dmed_sector *S = st.vec->sec;
+            dmed_vec v{S, DMED_FP64, S->dim, nullptr};
+            if ((ck.rc = exec_buf(ex(), kSlotStart, std::max<int64_t>(S->dim, 1) * 8, &v.data))) {
+                ck.err = dmed_last_error();
+                return;
+            }
+            ck.rc = dmed_apply_sz(st.vec, ck.row < norb ? ck.row : -1, &v, &ck.norm2);
+            if (!ck.rc && ck.norm2 > 0.0) {
+                ck.nlanc = (int)std::min<int64_t>(S->dim, ngf);
+                std::vector<double> al(ck.nlanc, 0.0), bl(ck.nlanc, 0.0);
+                int nsteps = 0;
+                ck.rc = chain_run(S, v.data, DMED_FP64, ck.nlanc, p->prec_gf, 1e-13, al.data(),
+                                  bl.data(), &nsteps, p->gf_fp64_head);
+                if (!ck.rc) {
+                    ck.lam = al;
+                    std::vector<double> e(ck.nlanc, 0.0);
+                    for (int i = 1; i < ck.nlanc; ++i) e[i] = bl[i];
+                    ck.w2.assign(ck.nlanc, 0.0);
+                    if (tql2_first_row(ck.nlanc, ck.lam.data(), e.data(), ck.w2.data()))
+                        ck.rc = fail(DMED_ENOCONV, "tql2 did not converge");
+                    for (auto &z : ck.w2) z *= z;
+                }
+            }
+            if (ck.rc) ck.err = dmed_last_error();
+        };
+        rc = run_tasks(p->batched, (int)tasks.size(), run_chi);
+        for (auto &ck : tasks)
+            if (!rc && ck.rc) rc = fail(ck.rc, ck.err);
+        if (!rc) {
+            std::vector<double> civ(2 * (size_t)nrow * nv, 0.0), ctau((size_t)nrow * nt, 0.0),
+                cw(2 * (size_t)nrow * lr, 0.0);
+            for (auto &ck : tasks) {
+                if (!(ck.norm2 > 0.0)) continue;
+                const State &st = sl.items[ck.state];
+                const double peso_f = ck.norm2 / zeta;
+                const double peso_bz = finite_t ? std::exp(-beta * (st.e - egs)) : 1.0;
+                chi_add_poles(peso_f, peso_bz, st.e, ck.lam.data(), ck.w2.data(), ck.nlanc, beta,
+                              vm.data(), nv, tau.data(), nt, wr.data(), lr, p->eps,
+                              civ.data() + 2 * (size_t)ck.row * nv, ctau.data() + (size_t)ck.row * nt,
+                              cw.data() + 2 * (size_t)ck.row * lr);
+            }
+            // Fortran order (Norb+1, L): element (row, i) at row + nrow*i
+            for (int row = 0; row < nrow; ++row) {
+                for (int i = 0; i < nv && out->spin_chi_iv; ++i) {
+                    out->spin_chi_iv[2 * ((size_t)row + nrow * i)] = civ[2 * ((size_t)row * nv + i)] / zeta;
+                    out->spin_chi_iv[2 * ((size_t)row + nrow * i) + 1] = civ[2 * ((size_t)row * nv + i) + 1] / zeta;
+                }
+                for (int i = 0; i < nt && out->spin_chi_tau; ++i)
+                    out->spin_chi_tau[(size_t)row + nrow * i] = ctau[(size_t)row * nt + i] / zeta;
+                for (int i = 0; i < lr && out->spin_chi_w; ++i) {
+                    out->spin_chi_w[2 * ((size_t)row + nrow * i)] = cw[2 * ((size_t)row * lr + i)] / zeta;
+                    out->spin_chi_w[2 * ((size_t)row + nrow * i) + 1] = cw[2 * ((size_t)row * lr + i) + 1] / zeta;
+                }
+            }
+            out->ltau_used = ltau;
+        }
+    }
     const double t2 = now();
     out->egs = egs;
     out->zeta = zeta;

@@ -46,7 +46,8 @@ class SolveParams(C.Structure):
                 ("wini", C.c_double), ("wfin", C.c_double), ("lmats", C.c_int32),
                 ("lreal", C.c_int32), ("prec_gf", C.c_int32), ("batched", C.c_int32),
                 ("verbose", C.c_int32), ("gf_fp64_head", C.c_int32),
-                ("prec_gs", C.c_int32), ("pad_", C.c_int32)]
+                ("prec_gs", C.c_int32), ("chi_spin", C.c_int32), ("ltau", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 class LocalObs(C.Structure):
@@ -76,7 +77,8 @@ class SolveOut(C.Structure):
                 ("greal", _dp), ("smats", _dp), ("sreal", _dp),
                 ("neigen_sector", _i32p), ("lanc_nstates_total", _i32p),
                 ("t_diag", C.c_double), ("t_gf", C.c_double),
-                ("obs", C.POINTER(LocalObs))]
+                ("obs", C.POINTER(LocalObs)), ("spin_chi_iv", _dp), ("spin_chi_tau", _dp),
+                ("spin_chi_w", _dp), ("ltau_used", C.c_int32), ("pad2_", C.c_int32)]
 
 
 _lib = None
@@ -114,6 +116,10 @@ SIGNATURES = [
                              _dp, _i32p]),
     ("dmed_gf_add_chain", C.c_int, [C.c_double, C.c_double, _dp, _dp, C.c_int32, C.c_int32,
                                     _dp, C.c_int32, _dp, C.c_int32, C.c_double, _dp, _dp]),
+    ("dmed_apply_sz", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, _dp]),
+    ("dmed_chi_add_chain", C.c_int, [C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int32,
+                                     C.c_double, _dp, C.c_int32, _dp, C.c_int32, _dp, C.c_int32,
+                                     C.c_double, _dp, _dp, _dp]),
     ("dmed_observables", C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
     ("dmed_solve_params_default", None, [C.POINTER(SolveParams)]),
     ("dmed_solve", C.c_int, [C.POINTER(SolveParams), C.POINTER(SolveOut)]),
@@ -356,10 +362,41 @@ def solve(norb, nspin, ns, params):
     o.lanc_nstates_total = C.pointer(ntot)
     obs = LocalObs()
     o.obs = C.pointer(obs)
+    chi = {}
+    if p.chi_spin:
+        # spinChi_* (Norb+1, 0:L) in Fortran order; Ltau raised to beta
+        ltau = max(int(p.beta), p.ltau)
+        chi["spin_chi_iv"] = np.zeros((norb + 1, p.lmats + 1), dtype=np.complex128, order="F")
+        chi["spin_chi_tau"] = np.zeros((norb + 1, ltau + 1), order="F")
+        chi["spin_chi_w"] = np.zeros((norb + 1, p.lreal), dtype=np.complex128, order="F")
+        for k, a in chi.items():
+            setattr(o, k, a.ctypes.data_as(_dp))
     _check(lib().dmed_solve(C.byref(p), C.byref(o)))
     return dict(egs=o.egs, zeta=o.zeta, nstates=o.nstates, numgs=o.numgs, dens=dens, docc=docc,
                 neigen_sector=neig, lanc_nstates_total=ntot.value, t_diag=o.t_diag,
-                t_gf=o.t_gf, obs=obs.as_dict(norb), **g)
+                t_gf=o.t_gf, obs=obs.as_dict(norb), **g, **chi)
+
+
+def apply_sz(psi, iorb):
+    """S_z psi (iorb = -1: all impurity orbitals) as a new vector, and its norm^2."""
+    out = Vec(psi.sector, psi.prec)
+    n2 = C.c_double()
+    _check(lib().dmed_apply_sz(psi.h, iorb, out.h, C.byref(n2)))
+    return out, n2.value
+
+
+def chi_add_chain(peso_f, peso_bz, ei, alpha, beta_l, beta, vm, tau, wr, eps, chi_iv, chi_tau,
+                  chi_w):
+    """add_to_lanczos_spinChi of one chain into chi_iv (complex), chi_tau, chi_w (complex)."""
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    beta_l = np.ascontiguousarray(beta_l, dtype=np.float64)
+    vm = np.ascontiguousarray(vm, dtype=np.float64)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    _check(lib().dmed_chi_add_chain(peso_f, peso_bz, ei, _d(alpha), _d(beta_l), len(alpha), beta,
+                                    _d(vm), len(vm), _d(tau), len(tau), _d(wr), len(wr), eps,
+                                    _d(chi_iv.view(np.float64)), _d(chi_tau),
+                                    _d(chi_w.view(np.float64))))
 
 
 def local_observables(psi, peso, acc=None):

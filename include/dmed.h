@@ -184,6 +184,29 @@ int dmed_gf_add_chain(double peso_bz, double ei, const double *alpha,
                       const double *wm, int32_t lmats, const double *wr,
                       int32_t lreal, double eps, double *gmats, double *greal);
 
+/* S_z |psi> into out (same sector, FP64), S_z = (n_a,up - n_a,dw)/2 of
+ * impurity orbital iorb (0-based), or of all impurity orbitals when
+ * iorb = -1; the start vector of lanc_ed_build_spinChi_c /
+ * lanc_ed_build_spinChi_tot_c (build_chi_spin.f90:55-60, 131-136).
+ * norm2 receives ||S_z psi||^2. */
+int dmed_apply_sz(const dmed_vec *psi, int32_t iorb, dmed_vec *out, double *norm2);
+
+/* add_to_lanczos_spinChi (build_chi_spin.f90:186-250), both isign = +1 and
+ * -1 of one chain: with (lambda, Z) the tql2 spectrum of the chain,
+ * dE_j = lambda_j - ei and peso_j = peso_f*peso_bz*Z(1,j)^2,
+ *   chi_iv[0] += peso_j*beta if beta*dE_j < 0.1, else peso_j*(1-exp(-beta dE_j))/dE_j
+ *   chi_iv[n] += peso_j*(exp(-beta dE_j)-1)/(i vm[n] - dE_j)
+ *              + peso_j*(1-exp(-beta dE_j))/(i vm[n] + dE_j),   n >= 1
+ *   chi_tau[n] += peso_j*(exp(-tau[n] dE_j) + exp(-(beta-tau[n]) dE_j))
+ *   chi_w[n]  += the two terms of chi_iv with i vm[n] -> wr[n] + i eps
+ * (the iv = 0 term is added once per isign).  vm: nv bosonic frequencies
+ * 2*pi*n/beta, n = 0..nv-1; chi_iv, chi_w interleaved complex. */
+int dmed_chi_add_chain(double peso_f, double peso_bz, double ei, const double *alpha,
+                       const double *beta_l, int32_t nlanc, double beta, const double *vm,
+                       int32_t nv, const double *tau, int32_t ntau, const double *wr,
+                       int32_t lreal, double eps, double *chi_iv, double *chi_tau,
+                       double *chi_w);
+
 /* dens/docc contribution of one state (ED_OBSERVABLES.f90:118-150):
  * dens[a] += peso*sum|psi|^2 (n_a,up + n_a,dw), docc[a] += peso*sum|psi|^2 n_a,up n_a,dw */
 int dmed_observables(const dmed_vec *psi, double peso, double *dens, double *docc);
@@ -231,6 +254,8 @@ typedef struct dmed_solve_params {
     int32_t verbose;
     int32_t gf_fp64_head;          /* 32: FP64 steps of a DMED_MIXED chain */
     int32_t prec_gs;               /* DMED_FP64 / DMED_MIXED plain-Lanczos ground states */
+    int32_t chi_spin;              /* 0; 1: spin susceptibility (CHIFLAG, ED_MAIN.f90:282) */
+    int32_t ltau;                  /* 1000: imaginary-time points, raised to beta (ED_INPUT_VARS.f90:191) */
     int32_t pad_;
 } dmed_solve_params;
 void dmed_solve_params_default(dmed_solve_params *p);
@@ -250,6 +275,12 @@ typedef struct dmed_solve_out {
     int32_t *lanc_nstates_total;
     double t_diag, t_gf;           /* seconds spent in the two phases */
     dmed_local_obs *obs;           /* local observables and energies, or NULL */
+    /* spin susceptibility (chi_spin = 1), rows iorb = 0..Norb-1 and Norb =
+     * the total (Norb > 1), Fortran order like spinChi_* (ED_VARS_GLOBAL.f90:
+     * 136-138): chi_iv (Norb+1, 0:Lmats) complex, chi_tau (Norb+1, 0:Ltau)
+     * real, chi_w (Norb+1, Lreal) complex; Ltau as raised; may be NULL */
+    double *spin_chi_iv, *spin_chi_tau, *spin_chi_w;
+    int32_t ltau_used, pad2_;
 } dmed_solve_out;
 
 /* ed_solve_single backend (ED_MAIN.f90:251-289): diagonalize_impurity

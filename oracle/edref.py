@@ -317,9 +317,41 @@ def default_params(**kw):
              lanc_niter=512, lanc_ngfiter=200, lanc_tolerance=1e-12,
              lanc_dim_threshold=256, lanc_ncv_factor=3, lanc_ncv_add=5,
              gs_threshold=1e-9, cutoff=1e-9, beta=1000.0, lmats=5000, lreal=5000,
-             wini=-5.0, wfin=5.0, eps=0.01, chain_threshold=1e-13)
+             wini=-5.0, wfin=5.0, eps=0.01, chain_threshold=1e-13, chiflag=False, ltau=1000)
     p.update(kw)
     return p
+
+
+def spin_chi_poles(peso_f, peso_bz, ei, lam, w2, beta, vm, tau, wr, eps, chi_iv, chi_tau,
+                   chi_w):
+    """add_to_lanczos_spinChi (build_chi_spin.f90:186-250) for isign = +1 and -1
+    of one chain with poles lam and weights w2 = Z(1,j)^2 (pure-Python loops
+    over the poles, numpy over the frequencies)."""
+    for isign in (1, -1):
+        for j in range(len(lam)):
+            de = lam[j] - ei
+            peso = peso_f * w2[j] * peso_bz
+            ex = np.exp(-beta * de)
+            chi_iv[0] += peso * beta if beta * de < 1e-1 else peso * (1.0 - ex) / de
+            if isign == 1:
+                chi_iv[1:] += peso * (ex - 1.0) / (1j * vm[1:] - de)
+                chi_tau += peso * np.exp(-tau * de)
+                chi_w += peso * (ex - 1.0) / (wr + 1j * eps - de)
+            else:
+                chi_iv[1:] += peso * (1.0 - ex) / (1j * vm[1:] + de)
+                chi_tau += peso * np.exp(-(beta - tau) * de)
+                chi_w += peso * (1.0 - ex) / (wr + 1j * eps + de)
+
+
+def sz_vector(smap, vec, ns, orbs):
+    """S_z |psi> with S_z = (1/2) sum_{a in orbs} (n_a,up - n_a,dw), from the
+    state codes (build_chi_spin.f90:55-60, 131-136; up bits 0..ns-1, down
+    bits ns..2ns-1)."""
+    smap = np.asarray(smap, dtype=np.int64)
+    sgn = np.zeros(len(smap))
+    for a in orbs:
+        sgn += ((smap >> a) & 1) - ((smap >> (a + ns)) & 1)
+    return 0.5 * sgn * np.asarray(vec)
 
 
 def _sector_eigs(smap, neigen, p, seed=1234567):
@@ -433,7 +465,37 @@ def ed_solve(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, xmu=0.0,
         for a in range(norb):
             sm[s, a] = sigma(s, a, 1j * wm, gm[s, a])
             sr[s, a] = sigma(s, a, wr + 1j * p["eps"], gr[s, a])
-    return dict(egs=egs, zeta=zeta, states=[(e, sec) for e, sec, _ in sl.items],
-                vectors=[v for _, _, v in sl.items], dens=dens, docc=docc,
-                obs=obs_dict(acc, norb), gmats=gm,
-                greal=gr, smats=sm, sreal=sr, wm=wm, wr=wr)
+    out = dict(egs=egs, zeta=zeta, states=[(e, sec) for e, sec, _ in sl.items],
+               vectors=[v for _, _, v in sl.items], dens=dens, docc=docc,
+               obs=obs_dict(acc, norb), gmats=gm,
+               greal=gr, smats=sm, sreal=sr, wm=wm, wr=wr)
+    if p["chiflag"]:
+        # build_chi_spin (build_chi_spin.f90:4-18): rows 0..Norb-1 per orbital,
+        # row Norb the total when Norb > 1; weights norm^2/zeta, then a second
+        # division by zeta at the end, as the reference does
+        ltau = max(int(beta), p["ltau"])
+        vm = np.pi / beta * 2.0 * np.arange(p["lmats"] + 1)
+        tau = np.linspace(0.0, beta, ltau + 1)
+        civ = np.zeros((norb + 1, p["lmats"] + 1), dtype=np.complex128)
+        ctau = np.zeros((norb + 1, ltau + 1))
+        cw = np.zeros((norb + 1, p["lreal"]), dtype=np.complex128)
+        rows = [[a] for a in range(norb)] + ([list(range(norb))] if norb > 1 else [])
+        for row, orbs in enumerate(rows):
+            for e, (nu, nd), vec in sl.items:
+                m = build_sector(nu, nd)
+                v = sz_vector(m, vec, ns, orbs)
+                norm2 = float(np.dot(v, v))
+                if norm2 == 0.0:
+                    continue
+                v = (v / np.sqrt(norm2)).astype(np.complex128)
+                nlanc = min(len(m), p["lanc_ngfiter"])
+                al, bl, _ = lanc_tridiag(m, v, nlanc, p["chain_threshold"])
+                dd, z, ierr = tql2(al, np.concatenate(([0.0], bl[1:])))
+                if ierr:
+                    raise RuntimeError("tql2 did not converge")
+                pbz = np.exp(-beta * (e - egs)) if finite_t else 1.0
+                spin_chi_poles(norm2 / zeta, pbz, e, dd, z[0, :] ** 2, beta, vm, tau, wr,
+                               p["eps"], civ[row], ctau[row], cw[row])
+        out.update(spin_chi_iv=civ / zeta, spin_chi_tau=ctau / zeta, spin_chi_w=cw / zeta,
+                   vm=vm, tau=tau)
+    return out

@@ -412,7 +412,7 @@ def _solve_both(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode=True,
                 jhflag=False, **kw):
     p = dmed.default_params(**kw)
     r = dmed.solve(norb, nspin, (nbath + 1) * norb, p)
-    pr = edref.default_params(**{k: v for k, v in kw.items()
+    pr = edref.default_params(**{("chiflag" if k == "chi_spin" else k): v for k, v in kw.items()
                                  if k not in ("prec_gf", "batched", "verbose")})
     o = edref.ed_solve(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode, params=pr,
                        jx=jx, jp=jp, jhflag=jhflag)
@@ -459,6 +459,36 @@ def test_solve_vs_oracle(name, finite):
                 return np.linalg.norm(x - y) / max(np.linalg.norm(y), np.sqrt(len(y)))
             assert rel(r["greal"][s, s, a, a], o["greal"][s, a]) < 1e-4
             assert rel(r["sreal"][s, s, a, a], o["sreal"][s, a]) < 1e-3
+
+
+@pytest.mark.parametrize("name,finite", [("m1_norb1_nb3", False), ("m2_norb2_nb1", False),
+                                          ("m7_atomic", False), ("m5_norb1_nb5_T", True)])
+def test_spin_chi_vs_oracle(name, finite):
+    """Spin susceptibility of the solve (chi_spin, build_chi_spin.f90) against
+    the oracle: chi(iv_n) and chi(tau) to 1e-8 absolute per orbital and, for
+    Norb = 2, the total; chi(w + i eps) in relative L2 over the grid (1e-4,
+    as for G on the real axis)."""
+    g = load(name)
+    set_both(g)
+    norb, nspin = int(g["norb"]), int(g["nspin"])
+    kw = dict(beta=float(g["beta"]), lmats=len(g["wm"]), lreal=len(g["wr"]),
+              wini=float(g["wr"][0]), wfin=float(g["wr"][-1]), eps=float(g["eps"]),
+              lanc_ngfiter=200, chi_spin=1, ltau=200)
+    if finite:
+        kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
+    say(f"[{name}] solve with spin chi")
+    r, o = _solve_both(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
+                       float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]),
+                       jx=float(g["jx"]), jp=float(g["jp"]), jhflag=bool(g["jhflag"]), **kw)
+    rows = norb + 1 if norb > 1 else norb
+    assert r["spin_chi_iv"].shape == o["spin_chi_iv"].shape
+    assert r["spin_chi_tau"].shape == o["spin_chi_tau"].shape
+    assert np.abs(r["spin_chi_iv"][:rows] - o["spin_chi_iv"][:rows]).max() < 1e-8
+    assert np.abs(r["spin_chi_tau"][:rows] - o["spin_chi_tau"][:rows]).max() < 1e-8
+    for row in range(rows):
+        x, y = r["spin_chi_w"][row], o["spin_chi_w"][row]
+        assert np.linalg.norm(x - y) / max(np.linalg.norm(y), np.sqrt(len(y))) < 1e-4
+    assert np.abs(r["spin_chi_iv"][rows:]).max(initial=0) == 0.0
 
 
 def test_solve_config1_vs_oracle():

@@ -326,3 +326,55 @@ def test_local_observables_match_jordan_wigner(name):
     ref = _golden_local_obs(g)
     for k, v in ref.items():
         assert np.allclose(r[k], v, atol=1e-10, rtol=0), (k, r[k], v)
+
+
+def _spin_chi_solve(name, **pkw):
+    g = load(name)
+    return g, _solve_golden(g, chiflag=True, **pkw)
+
+
+def test_spin_chi_atomic_limit_closed_form():
+    """V=0, half filling, T=0 (lanc_nstates_total=1): every kept state has the
+    impurity singly occupied, so S_z psi = +-psi/2 is an eigenvector and the
+    only pole sits at dE = 0.  With the reference's weights norm^2/zeta and
+    its final division by zeta (build_chi_spin.f90:14-16, 190),
+    chi(iv_0) = beta/(2 zeta), chi(iv_n>0) = 0 and chi(tau) = 1/(2 zeta) for
+    every tau -- the Curie values beta/4 and 1/4 when zeta = 2."""
+    g, r = _spin_chi_solve("m7_atomic", lanc_ngfiter=64)
+    beta, zeta = float(g["beta"]), r["zeta"]
+    assert zeta >= 2
+    assert abs(r["spin_chi_iv"][0, 0] - beta / (2 * zeta)) < 1e-12
+    assert np.abs(r["spin_chi_iv"][0, 1:]).max() < 1e-12
+    assert np.abs(r["spin_chi_tau"][0] - 1.0 / (2 * zeta)).max() < 1e-12
+    assert np.abs(r["spin_chi_w"][0]).max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m2_norb2_nb1"])
+def test_spin_chi_consistency(name):
+    """T=0 spin susceptibility, per orbital and (Norb=2) total:
+    - chi(tau) = chi(beta - tau) (particle and hole terms mirror each other);
+    - chi(iv_n) is real (S_z is hermitian);
+    - chi(tau = 0) = sum over kept states of (<S_z^2> + <S_z>^2)/zeta^2, from
+      the state vectors directly (the hole terms at tau = 0 add only the
+      dE = 0 pole; the others are below exp(-beta*gap));
+    - chi(iv_n) = int_0^beta exp(i v_n tau) chi(tau) dtau (trapezoid on the
+      tau grid), which ties the two formulas' signs and factors."""
+    g, r = _spin_chi_solve(name, lanc_ngfiter=400)
+    norb, beta, zeta = int(g["norb"]), float(g["beta"]), r["zeta"]
+    ns = (int(g["nbath"]) + 1) * norb
+    rows = [[a] for a in range(norb)] + ([list(range(norb))] if norb > 1 else [])
+    tau, vm = r["tau"], r["vm"]
+    for row, orbs in enumerate(rows):
+        ct, civ = r["spin_chi_tau"][row], r["spin_chi_iv"][row]
+        assert np.abs(ct - ct[::-1]).max() <= 1e-10 * np.abs(ct).max()
+        assert np.abs(civ.imag).max() <= 1e-12 * max(1.0, np.abs(civ.real).max())
+        exp0 = 0.0
+        for (e, (nu, nd)), vec in zip(r["states"], r["vectors"]):
+            m = np.asarray(edref.build_sector(nu, nd), dtype=np.int64)
+            sz = sum(0.5 * (((m >> a) & 1) - ((m >> (a + ns)) & 1)) for a in orbs)
+            p2 = np.abs(vec) ** 2
+            exp0 += (np.dot(p2, sz * sz) + np.dot(p2, sz) ** 2) / zeta ** 2
+        assert abs(ct[0] - exp0) <= 1e-9 * max(1.0, exp0)
+        for n in (0, 1, 2):
+            quad = np.trapezoid(np.exp(1j * vm[n] * tau) * ct, tau)
+            assert abs(quad - civ[n]) <= 5e-3 * abs(civ[n]) + 1e-12
