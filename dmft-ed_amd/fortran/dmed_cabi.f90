@@ -20,7 +20,8 @@ module DMED_CABI
      integer(c_int32_t) :: lanc_niter, lanc_ngfiter, lanc_ncv_factor, lanc_ncv_add
      integer(c_int32_t) :: lanc_dim_threshold
      real(c_double)     :: lanc_tolerance, gs_threshold, cutoff, beta, eps, wini, wfin
-     integer(c_int32_t) :: lmats, lreal, prec_gf, batched, verbose, gf_fp64_head, prec_gs, pad_
+     integer(c_int32_t) :: lmats, lreal, prec_gf, batched, verbose, gf_fp64_head, prec_gs
+     integer(c_int32_t) :: chi_spin, ltau, pad_
   end type dmed_solve_params_t
 
   !> dmed_local_obs: orbital arrays (3), orbital pairs (9, [a*3+b] in C)
@@ -38,6 +39,8 @@ module DMED_CABI
      type(c_ptr)        :: neigen_sector, lanc_nstates_total
      real(c_double)     :: t_diag, t_gf
      type(c_ptr)        :: obs
+     type(c_ptr)        :: spin_chi_iv, spin_chi_tau, spin_chi_w
+     integer(c_int32_t) :: ltau_used, pad2_
   end type dmed_solve_out_t
 
   public :: dmed_init, dmed_finalize, dmed_last_error, dmed_set_model, dmed_solve

@@ -58,6 +58,10 @@ module DMFT_ED
   real(8),public                :: ed_Epot=0d0, ed_Eint=0d0, ed_Ehartree=0d0, ed_Eknot=0d0
   real(8),public                :: ed_Dust=0d0, ed_Dund=0d0, ed_Dse=0d0, ed_Dph=0d0
   type(dmed_local_obs_t),target :: ed_obs
+  !> spin susceptibility of the last solve with CHIFLAG (build_chi_spin.f90):
+  !> rows 1..Norb per orbital, Norb+1 the total (Norb > 1)
+  real(8),allocatable,target,public    :: spinChi_tau(:,:)
+  complex(8),allocatable,target,public :: spinChi_iv(:,:), spinChi_w(:,:)
   real(8),allocatable,target    :: bath_e(:,:,:), bath_v(:,:,:)
   real(8),allocatable,target    :: impHloc(:,:,:,:)
   integer(c_int32_t),allocatable,target :: neigen_sector(:)
@@ -395,6 +399,8 @@ contains
     p%batched = merge(1,0,ed_gpu_batched)
     p%prec_gs = merge(DMED_MIXED,DMED_FP64,ed_gpu_mixed_gs)
     p%verbose = ed_verbose
+    p%chi_spin = merge(1,0,chiflag)
+    p%ltau = Ltau
     allocate(dens(Norb), docc(Norb))
     allocate(gm(Nspin,Nspin,Norb,Norb,Lmats), sm(Nspin,Nspin,Norb,Norb,Lmats))
     allocate(gr(Nspin,Nspin,Norb,Norb,Lreal), sr(Nspin,Nspin,Norb,Norb,Lreal))
@@ -404,6 +410,15 @@ contains
     o%neigen_sector = c_loc(neigen_sector)
     o%lanc_nstates_total = c_loc(ntot)
     o%obs = c_loc(ed_obs)
+    if(chiflag)then
+       ! Ltau=max(int(beta),Ltau) (ED_INPUT_VARS.f90:191)
+       if(allocated(spinChi_tau))deallocate(spinChi_tau,spinChi_iv,spinChi_w)
+       allocate(spinChi_tau(Norb+1,0:max(int(beta),Ltau)))
+       allocate(spinChi_iv(Norb+1,0:Lmats), spinChi_w(Norb+1,Lreal))
+       o%spin_chi_tau = c_loc(spinChi_tau)
+       o%spin_chi_iv = c_loc(spinChi_iv)
+       o%spin_chi_w = c_loc(spinChi_w)
+    endif
     call ed_check(dmed_solve(p,o),"dmed_solve")
     lanc_nstates_total = ntot
     impGmats = gm; impGreal = gr; impSmats = sm; impSreal = sr
@@ -419,7 +434,43 @@ contains
        write(LOGfile,"(A,10F18.12)")"docc=",ed_docc
     endif
     if(ed_print_Sigma)call print_impSigma()
+    if(chiflag)call print_chi_spin()
   end subroutine ed_solve
+
+  !> spinChi_l<a>_{tau,realw,iw}.ed and, for Norb > 1, spinChi_tot<Norb+1>_*.ed
+  !> (ED_IO/print_impChi.f90:4-18): columns x, y (real) or x, Im y, Re y
+  subroutine print_chi_spin()
+    integer :: a, i, unit, lt
+    real(8) :: pi, wr
+    character(len=:),allocatable :: stem
+    pi = acos(-1d0)
+    lt = ubound(spinChi_tau,2)
+    do a=1,Norb+1
+       if(a<=Norb)then
+          stem = "spinChi_l"//itoa(a)
+       else
+          if(Norb<=1)exit
+          stem = "spinChi_tot"//itoa(a)
+       endif
+       open(newunit=unit,file=stem//"_tau.ed")
+       do i=0,lt
+          write(unit,"(2ES24.15)")beta*dble(i)/dble(max(lt,1)),spinChi_tau(a,i)
+       enddo
+       close(unit)
+       open(newunit=unit,file=stem//"_realw.ed")
+       do i=1,Lreal
+          wr = wini
+          if(Lreal>1)wr = wini + (wfin-wini)*dble(i-1)/dble(Lreal-1)
+          write(unit,"(3ES24.15)")wr,aimag(spinChi_w(a,i)),dble(spinChi_w(a,i))
+       enddo
+       close(unit)
+       open(newunit=unit,file=stem//"_iw.ed")
+       do i=0,Lmats
+          write(unit,"(3ES24.15)")pi/beta*2d0*dble(i),aimag(spinChi_iv(a,i)),dble(spinChi_iv(a,i))
+       enddo
+       close(unit)
+    enddo
+  end subroutine print_chi_spin
 
   !> impSigma_l<a><b>_s<s>_iw.ed : columns wm, Im Sigma, Re Sigma (ED_IO/print_impSigma.f90)
   subroutine print_impSigma()

@@ -1,8 +1,9 @@
 """The Fortran DMFT_ED layer end to end on the GPU: the Bethe-lattice driver
 (ed_read_input -> ed_init_solver with a hamiltonian.restart bath -> ed_solve
 -> ed_get_sigma_matsubara -> local G -> ed_chi2_fitgf) for one DMFT
-iteration on workload 1, against the oracle's solve of the same bath.
-Bars: E0 1e-10 relative; dens, docc 1e-9; G_loc(iw) 1e-8 absolute."""
+iteration on workload 1 with CHIFLAG, against the oracle's solve of the same
+bath.  Bars: E0 1e-10 relative; dens, docc 1e-9; G_loc(iw) and the spin
+susceptibility files 1e-8 absolute."""
 import os
 import subprocess
 
@@ -39,7 +40,7 @@ def test_bethe_driver_one_iteration(tmp_path):
     (tmp_path / "inputED.in").write_text("\n".join([
         "NORB=1", "NBATH=7", "NSPIN=1", "ULOC=2.0", "XMU=0.0", f"BETA={c['beta']}",
         f"LMATS={lmats}", "LREAL=100", "NLOOP=1", "LANC_NGFITER=200",
-        "LANC_NSTATES_TOTAL=1", "HFMODE=T", "CG_SCHEME=weiss", ""]))
+        "LANC_NSTATES_TOTAL=1", "HFMODE=T", "CG_SCHEME=weiss", "CHIFLAG=T", "LTAU=200", ""]))
     configs.write_restart(str(tmp_path / "hamiltonian.restart"), c)
     bath = _read_restart(str(tmp_path / "hamiltonian.restart"), c["nbath"])
     print("[fortran driver] one DMFT iteration, workload 1", flush=True)
@@ -51,7 +52,8 @@ def test_bethe_driver_one_iteration(tmp_path):
     summ = {l.split()[0]: [float(x) for x in l.split()[1:]]
             for l in open(tmp_path / "dmft_summary.ed") if l.strip()}
     edref.set_model(1, c["nbath"], 1, bath, c["hloc"], [2.0], 0.0, 0.0, 0.0, 0.0, 0.0, True, False)
-    p = edref.default_params(beta=c["beta"], lmats=lmats, lreal=100, lanc_ngfiter=200)
+    p = edref.default_params(beta=c["beta"], lmats=lmats, lreal=100, lanc_ngfiter=200,
+                             chiflag=True, ltau=200)
     o = edref.ed_solve(1, c["nbath"], 1, bath, c["hloc"], [2.0], 0.0, 0.0, 0.0, True, params=p)
     assert abs(summ["egs"][0] - o["egs"]) <= 1e-10 * abs(o["egs"])
     assert abs(summ["dens"][0] - o["dens"][0]) <= 1e-9
@@ -72,6 +74,13 @@ def test_bethe_driver_one_iteration(tmp_path):
     zeta = 1j * wm - o["smats"][0, 0]
     gl = (wte[None, :] / (zeta[:, None] - e[None, :])).sum(1)
     assert np.abs(g[:, 2] + 1j * g[:, 1] - gl).max() <= 1e-8
+    # CHIFLAG: spinChi_l1_{iw,tau}.ed (columns v, Im, Re / tau, chi) against the oracle
+    ci = np.loadtxt(tmp_path / "spinChi_l1_iw.ed")
+    assert np.allclose(ci[:, 0], o["vm"], rtol=1e-14, atol=0)
+    assert np.abs(ci[:, 2] + 1j * ci[:, 1] - o["spin_chi_iv"][0]).max() <= 1e-8
+    ct = np.loadtxt(tmp_path / "spinChi_l1_tau.ed")
+    assert np.allclose(ct[:, 0], o["tau"], rtol=1e-14, atol=1e-14)
+    assert np.abs(ct[:, 1] - o["spin_chi_tau"][0]).max() <= 1e-8
     # the fit wrote a new bath
     new = _read_restart(str(tmp_path / "hamiltonian.restart"), c["nbath"])
     assert np.all(np.isfinite(new)) and not np.array_equal(new, bath)
