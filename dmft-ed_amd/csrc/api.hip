@@ -724,14 +724,17 @@ __global__ void __launch_bounds__(256) k_jterm_sums(const T *__restrict__ psi, i
     }
 }
 
-// out = S_z psi, S_z = (n_up - n_dw)/2 summed over the impurity orbitals in
-// `mask` (bit a = orbital a): the impurity bytes of the row and the column
-// give the occupations; partials of ||out||^2 per block
+// out = (wu*n_up + wd*n_dw) psi, occupations summed over the impurity
+// orbitals in `mask` (bit a = orbital a): S_z with (1/2, -1/2), the charge
+// N with (1, 1).  The impurity bytes of the row and the column give the
+// occupations; the factor is an exact small multiple of 1/2.  Partials of
+// ||out||^2 per block.
 template <typename T>
-__global__ void __launch_bounds__(256) k_apply_sz(const T *__restrict__ psi, int64_t dup,
-                                                  int64_t ddw, const uint8_t *__restrict__ impu,
-                                                  const uint8_t *__restrict__ impd, unsigned mask,
-                                                  T *__restrict__ out, double *__restrict__ partials)
+__global__ void __launch_bounds__(256) k_apply_occ(const T *__restrict__ psi, int64_t dup,
+                                                   int64_t ddw, const uint8_t *__restrict__ impu,
+                                                   const uint8_t *__restrict__ impd, unsigned mask,
+                                                   double wu, double wd, T *__restrict__ out,
+                                                   double *__restrict__ partials)
 {
     __shared__ double red[4];
     double s2 = 0.0;
@@ -739,8 +742,9 @@ __global__ void __launch_bounds__(256) k_apply_sz(const T *__restrict__ psi, int
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = e / dup, r = e - c * dup;
-        const int sgn = __popc((unsigned)impu[r] & mask) - __popc((unsigned)impd[c] & mask);
-        const T v = (T)(0.5 * (double)sgn * (double)psi[e]);
+        const double f = wu * (double)__popc((unsigned)impu[r] & mask) +
+                         wd * (double)__popc((unsigned)impd[c] & mask);
+        const T v = (T)(f * (double)psi[e]);
         out[e] = v;
         s2 += (double)v * (double)v;
     }
@@ -755,6 +759,33 @@ __global__ void __launch_bounds__(256) k_apply_sz(const T *__restrict__ psi, int
     }
 }
 
+// out = (wu*n_up + wd*n_dw)(mask) psi on the calling context's stream; psi
+// and out are vectors of sector s in precision prec (out may point into a
+// larger buffer, e.g. one half of a stacked vector); norm2 = ||out||^2
+int apply_occ(const dmed_sector *s, int prec, const void *psi, unsigned mask, double wu,
+              double wd, void *out, double *norm2)
+{
+    hipStream_t st = ex().stream;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (s->dim + 255) / 256));
+    if (int rc = ensure_partials(4096)) return rc;
+    if (prec == DMED_FP64)
+        hipLaunchKernelGGL(k_apply_occ<double>, dim3(g), dim3(256), 0, st, (const double *)psi,
+                           s->up.dim, s->dw.dim, s->up.imp, s->dw.imp, mask, wu, wd,
+                           (double *)out, ex().partials);
+    else
+        hipLaunchKernelGGL(k_apply_occ<float>, dim3(g), dim3(256), 0, st, (const float *)psi,
+                           s->up.dim, s->dw.dim, s->up.imp, s->dw.imp, mask, wu, wd,
+                           (float *)out, ex().partials);
+    DMED_HIP(hipGetLastError());
+    std::vector<double> h(g);
+    DMED_HIP(hipMemcpyAsync(h.data(), ex().partials, g * 8, hipMemcpyDeviceToHost, st));
+    DMED_HIP(hipStreamSynchronize(st));
+    double t = 0.0;
+    for (unsigned b = 0; b < g; ++b) t += h[b];
+    *norm2 = t;
+    return 0;
+}
+
 }  // namespace dmed
 
 extern "C" int dmed_apply_sz(const dmed_vec *psi, int32_t iorb, dmed_vec *out, double *norm2)
@@ -766,26 +797,20 @@ extern "C" int dmed_apply_sz(const dmed_vec *psi, int32_t iorb, dmed_vec *out, d
     if (iorb < -1 || iorb >= s->norb) return fail(DMED_EINVAL, "iorb out of range");
     if (out->sec != s || out->prec != psi->prec || out == psi)
         return fail(DMED_EINVAL, "output vector must be another vector of psi's sector and precision");
-    hipStream_t st = ex().stream;
     const unsigned mask = iorb < 0 ? (1u << s->norb) - 1u : 1u << iorb;
-    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (s->dim + 255) / 256));
-    if (int rc = ensure_partials(4096)) return rc;
-    if (psi->prec == DMED_FP64)
-        hipLaunchKernelGGL(dmed::k_apply_sz<double>, dim3(g), dim3(256), 0, st,
-                           (const double *)psi->data, s->up.dim, s->dw.dim, s->up.imp, s->dw.imp,
-                           mask, (double *)out->data, ex().partials);
-    else
-        hipLaunchKernelGGL(dmed::k_apply_sz<float>, dim3(g), dim3(256), 0, st,
-                           (const float *)psi->data, s->up.dim, s->dw.dim, s->up.imp, s->dw.imp,
-                           mask, (float *)out->data, ex().partials);
-    DMED_HIP(hipGetLastError());
-    std::vector<double> h(g);
-    DMED_HIP(hipMemcpyAsync(h.data(), ex().partials, g * 8, hipMemcpyDeviceToHost, st));
-    DMED_HIP(hipStreamSynchronize(st));
-    double t = 0.0;
-    for (unsigned b = 0; b < g; ++b) t += h[b];
-    *norm2 = t;
-    return 0;
+    return dmed::apply_occ(s, psi->prec, psi->data, mask, 0.5, -0.5, out->data, norm2);
+}
+
+extern "C" int dmed_apply_occ(const dmed_vec *psi, uint32_t mask, dmed_vec *out, double *norm2)
+{
+    REQUIRE_MODEL();
+    if (!psi || !out || !norm2) return fail(DMED_EINVAL, "null argument");
+    const dmed_sector *s = psi->sec;
+    if (int rc = check_sector(s)) return rc;
+    if (mask == 0 || mask >= (1u << s->norb)) return fail(DMED_EINVAL, "orbital mask out of range");
+    if (out->sec != s || out->prec != psi->prec || out == psi)
+        return fail(DMED_EINVAL, "output vector must be another vector of psi's sector and precision");
+    return dmed::apply_occ(s, psi->prec, psi->data, mask, 1.0, 1.0, out->data, norm2);
 }
 
 extern "C" int dmed_chi_add_chain(double peso_f, double peso_bz, double ei, const double *alpha,

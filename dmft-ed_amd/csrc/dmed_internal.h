@@ -116,6 +116,7 @@ constexpr int kSlotDeflY = 6;           // locked eigenvectors of a deflation ro
 constexpr int kSlotDeflX = 7;           // eigenvectors collected over rounds
 constexpr int kSlotOpTable = 8;         // c / c^+ source-target tables
 constexpr int kSlotStart = 9;           // GF chain start vector
+constexpr int kSlotMid = 10;            // intermediate vector of a two-operator start vector
 constexpr int kSlotChainLo = 12;        // 12..14: FP32 tail of a mixed chain
 constexpr int kExecSlots = 16;
 constexpr int kMixedHead = 32;          // FP64 steps at the head of a mixed chain
@@ -150,6 +151,16 @@ int ensure_partials(size_t n);          // on ex()
 
 // ------------------------------------------------------------- sectors
 int sector_build(dmed_sector *s, int nup, int ndw);
+// Sector (nup, ndw) stacked twice along the down-spin index: vectors of
+// 2*dim entries [x; y] and H*[x; y] = [H x; H y] (H (+) H).  Built from
+// sector_build with the down-spin tables repeated and their column indices
+// shifted by D_dw in the second copy; for chains of complex start vectors
+// x - i y under the real H, whose Lanczos coefficients are those of [x; y].
+int sector_build_stacked(dmed_sector *s, int nup, int ndw);
+// out = (wu*n_up + wd*n_dw)(orbitals in mask) psi on the calling context's
+// stream, norm2 = ||out||^2 (api.hip)
+int apply_occ(const dmed_sector *s, int prec, const void *psi, unsigned mask, double wu,
+              double wd, void *out, double *norm2);
 void sector_release(dmed_sector *s);
 int64_t binom(int n, int k);   // exact, n <= 62
 
@@ -216,6 +227,14 @@ int tql2_first_row(int n, double *d, double *e, double *z1);
 void gf_add_poles(double peso, double ei, const double *lam, const double *w2, int n,
                   int isign, const double *wm, int lmats, const double *wr, int lreal,
                   double eps, double *gm, double *gr);
+// one branch of the susceptibility pole sums (build_chi_spin.f90:186-250,
+// build_chi_dens.f90:505-733): complex weight pr + i pi, iv = 0 term signed
+// by iv0_sign; vm[0..nv), tau[0..ntau), wr[0..lreal); chi_iv, chi_w
+// interleaved complex, chi_tau real (takes the real part)
+void chi_add_branch(double pr, double pi, double peso_bz, double ei, const double *lam,
+                    const double *w2, int n, int isign, int iv0_sign, double beta,
+                    const double *vm, int nv, const double *tau, int ntau, const double *wr,
+                    int lreal, double eps, double *chi_iv, double *chi_tau, double *chi_w);
 // add_to_lanczos_spinChi (build_chi_spin.f90:186-250) for isign = +1 then -1;
 // vm[0..nv), tau[0..ntau), wr[0..lreal); chi_iv, chi_w interleaved complex
 void chi_add_poles(double peso_f, double peso_bz, double ei, const double *lam, const double *w2,

@@ -196,40 +196,66 @@ void gf_add_poles(double peso, double ei, const double *lam, const double *w2, i
     }
 }
 
+// One branch (isign = +1 or -1) of the susceptibility pole sums of the
+// reference: add_to_lanczos_spinChi (build_chi_spin.f90:186-250),
+// add_to_lanczos_densChi, _densChi_mix and _densChi_tot
+// (build_chi_dens.f90:505-733).  The weight peso_j = (pr + i pi)*w2_j*peso_bz
+// is complex for the (N_a -/+ i N_b) chains of the orbital-off-diagonal
+// charge susceptibility (cnorm2 = xi*norm0, build_chi_dens.f90:263, 300);
+// chi_tau is real in the reference, so it takes Re(peso_j).  The iv = 0
+// term is iv0_sign*(peso_j*beta if beta*dE < 0.1 else peso_j*(1-exp(-beta dE))/dE):
+// the spin sums use iv0_sign = +1 for both branches, the charge sums
+// iv0_sign = -isign (build_chi_dens.f90:539-542 against 560-563).
+// Reciprocals written out: 1/(x + i y) = (x - i y)/(x^2 + y^2).
+void chi_add_branch(double pr, double pi, double peso_bz, double ei, const double *lam,
+                    const double *w2, int n, int isign, int iv0_sign, double beta,
+                    const double *vm, int nv, const double *tau, int ntau, const double *wr,
+                    int lreal, double eps, double *chi_iv, double *chi_tau, double *chi_w)
+{
+    for (int j = 0; j < n; ++j) {
+        const double de = lam[j] - ei;
+        const double wr_ = pr * w2[j] * peso_bz, wi_ = pi * w2[j] * peso_bz;
+        const double ex = std::exp(-beta * de);
+        if (nv > 0) {
+            const double f = beta * de < 1e-1 ? beta : (1.0 - ex) / de;
+            chi_iv[0] += iv0_sign * wr_ * f;
+            chi_iv[1] += iv0_sign * wi_ * f;
+        }
+        // isign +1: peso*(ex - 1)/(z - de);  -1: peso*(1 - ex)/(z + de)
+        const double num = isign > 0 ? ex - 1.0 : 1.0 - ex;
+        const double sde = isign > 0 ? de : -de;
+        for (int i = 1; i < nv; ++i) {
+            // num/(i vm - sde) = num*(-sde - i vm)/(sde^2 + vm^2)
+            const double g = num / (sde * sde + vm[i] * vm[i]);
+            const double xr = -sde * g, xi = -vm[i] * g;
+            chi_iv[2 * i] += wr_ * xr - wi_ * xi;
+            chi_iv[2 * i + 1] += wr_ * xi + wi_ * xr;
+        }
+        for (int i = 0; i < ntau; ++i)
+            chi_tau[i] += isign > 0 ? wr_ * std::exp(-tau[i] * de)
+                                    : wr_ * std::exp(-(beta - tau[i]) * de);
+        for (int i = 0; i < lreal; ++i) {
+            // num/(wr + i eps - sde) = num*(x - i eps)/(x^2 + eps^2), x = wr - sde
+            const double x = wr[i] - sde;
+            const double g = num / (x * x + eps * eps);
+            const double xr = x * g, xi = -eps * g;
+            chi_w[2 * i] += wr_ * xr - wi_ * xi;
+            chi_w[2 * i + 1] += wr_ * xi + wi_ * xr;
+        }
+    }
+}
+
 // add_to_lanczos_spinChi (build_chi_spin.f90:186-250).  S_z is hermitian,
 // so the particle (isign = +1) and hole (isign = -1) terms come from one
-// chain; per frequency the terms are added in pole order, +1 then -1.
-// Reciprocals written out: 1/(x + i y) = (x - i y)/(x^2 + y^2).
+// chain, added +1 then -1.
 void chi_add_poles(double peso_f, double peso_bz, double ei, const double *lam, const double *w2,
                    int n, double beta, const double *vm, int nv, const double *tau, int ntau,
                    const double *wr, int lreal, double eps, double *chi_iv, double *chi_tau,
                    double *chi_w)
 {
     for (int isign = 1; isign >= -1; isign -= 2)
-        for (int j = 0; j < n; ++j) {
-            const double de = lam[j] - ei;
-            const double peso = peso_f * w2[j] * peso_bz;
-            const double ex = std::exp(-beta * de);
-            if (nv > 0) chi_iv[0] += beta * de < 1e-1 ? peso * beta : peso * (1.0 - ex) / de;
-            // isign +1: peso*(ex - 1)/(z - de);  -1: peso*(1 - ex)/(z + de)
-            const double num = isign > 0 ? peso * (ex - 1.0) : peso * (1.0 - ex);
-            const double sde = isign > 0 ? de : -de;
-            for (int i = 1; i < nv; ++i) {
-                // z = i*vm: 1/(-sde + i vm) = (-sde - i vm)/(sde^2 + vm^2)
-                const double q = num / (sde * sde + vm[i] * vm[i]);
-                chi_iv[2 * i] -= sde * q;
-                chi_iv[2 * i + 1] -= vm[i] * q;
-            }
-            for (int i = 0; i < ntau; ++i)
-                chi_tau[i] += isign > 0 ? peso * std::exp(-tau[i] * de)
-                                        : peso * std::exp(-(beta - tau[i]) * de);
-            for (int i = 0; i < lreal; ++i) {
-                const double x = wr[i] - sde;
-                const double q = num / (x * x + eps * eps);
-                chi_w[2 * i] += x * q;
-                chi_w[2 * i + 1] -= eps * q;
-            }
-        }
+        chi_add_branch(peso_f, 0.0, peso_bz, ei, lam, w2, n, isign, 1, beta, vm, nv, tau, ntau,
+                       wr, lreal, eps, chi_iv, chi_tau, chi_w);
 }
 
 }  // namespace dmed

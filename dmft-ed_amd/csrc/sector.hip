@@ -169,6 +169,40 @@ static void spin_terms(const Model &m, int sp, std::vector<HopTerm> &terms,
         }
 }
 
+// Packed hop words from the host tables: target rank (24 bits), index of
+// |coefficient| in t.coef (7 bits), sign; left unpacked (t.packed = false)
+// past 2^24 ranks or 128 distinct |coefficients|.
+static int pack_hops(SpinTables &t)
+{
+    const int64_t dim = t.dim;
+    std::vector<double> coef;
+    std::vector<uint32_t> pk((size_t)t.nhop * dim);
+    bool ok = dim < (1 << 24);
+    for (size_t q = 0; q < pk.size() && ok; ++q) {
+        const double a = std::fabs(t.h_hval[q]);
+        int id = -1;
+        for (size_t u = 0; u < coef.size(); ++u)
+            if (coef[u] == a) { id = (int)u; break; }
+        if (id < 0) {
+            if (coef.size() >= 128) { ok = false; break; }
+            coef.push_back(a);
+            id = (int)coef.size() - 1;
+        }
+        pk[q] = ((uint32_t)t.h_hidx[q] << 8) | ((uint32_t)id << 1) | (t.h_hval[q] < 0 ? 1u : 0u);
+    }
+    if (ok) {
+        if (coef.empty()) coef.push_back(0.0);
+        t.ncoef = (int)coef.size();
+        DMED_HIP(hipMalloc(&t.hpk, std::max<size_t>(pk.size(), 1) * 4));
+        DMED_HIP(hipMalloc(&t.coef, coef.size() * 8));
+        if (!pk.empty())
+            DMED_HIP(hipMemcpy(t.hpk, pk.data(), pk.size() * 4, hipMemcpyHostToDevice));
+        DMED_HIP(hipMemcpy(t.coef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice));
+        t.packed = true;
+    }
+    return 0;
+}
+
 static int build_spin(const Model &m, int sp, int n, SpinTables &t, hipStream_t st)
 {
     t.n = n;
@@ -240,33 +274,7 @@ static int build_spin(const Model &m, int sp, int n, SpinTables &t, hipStream_t 
     DMED_HIP(hipStreamSynchronize(st));
     hipFree(d_h);
     hipFree(d_i);
-    // packed hop words: target rank (24 bits), index of |coefficient| (7 bits), sign
-    std::vector<double> coef;
-    std::vector<uint32_t> pk((size_t)t.nhop * dim);
-    bool ok = dim < (1 << 24);
-    for (size_t q = 0; q < pk.size() && ok; ++q) {
-        const double a = std::fabs(t.h_hval[q]);
-        int id = -1;
-        for (size_t u = 0; u < coef.size(); ++u)
-            if (coef[u] == a) { id = (int)u; break; }
-        if (id < 0) {
-            if (coef.size() >= 128) { ok = false; break; }
-            coef.push_back(a);
-            id = (int)coef.size() - 1;
-        }
-        pk[q] = ((uint32_t)t.h_hidx[q] << 8) | ((uint32_t)id << 1) | (t.h_hval[q] < 0 ? 1u : 0u);
-    }
-    if (ok) {
-        if (coef.empty()) coef.push_back(0.0);
-        t.ncoef = (int)coef.size();
-        DMED_HIP(hipMalloc(&t.hpk, std::max<size_t>(pk.size(), 1) * 4));
-        DMED_HIP(hipMalloc(&t.coef, coef.size() * 8));
-        if (!pk.empty())
-            DMED_HIP(hipMemcpy(t.hpk, pk.data(), pk.size() * 4, hipMemcpyHostToDevice));
-        DMED_HIP(hipMemcpy(t.coef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice));
-        t.packed = true;
-    }
-    return 0;
+    return pack_hops(t);
 }
 
 // Interaction energy of the impurity occupations (HxVint.f90:1-43):
@@ -367,6 +375,88 @@ int sector_build(dmed_sector *s, int nup, int ndw)
     DMED_HIP(hipMalloc(&s->W, s->h_W.size() * sizeof(double)));
     DMED_HIP(hipMemcpy(s->W, s->h_W.data(), s->h_W.size() * sizeof(double),
                        hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Down-spin tables of the sector repeated twice (see sector_build_stacked):
+// column c + D_dw of the second copy is column c of the first, its hops
+// shifted by D_dw; rebuilt from the host copies and uploaded again.
+static int stack_spin(SpinTables &t)
+{
+    const int64_t d = t.dim, d2 = 2 * d;
+    if (d2 >= ((int64_t)1 << 30)) return fail(DMED_EINVAL, "stacked sector too large");
+    auto twice = [](auto &v) { v.insert(v.end(), v.begin(), v.end()); };
+    twice(t.h_str);
+    twice(t.h_diag);
+    twice(t.h_imp);
+    std::vector<int32_t> hidx((size_t)t.nhop * d2);
+    std::vector<double> hval((size_t)t.nhop * d2);
+    for (int h = 0; h < t.nhop; ++h)
+        for (int64_t r = 0; r < d; ++r) {
+            const size_t q = (size_t)h * d + r;
+            hidx[(size_t)h * d2 + r] = t.h_hidx[q];
+            hidx[(size_t)h * d2 + d + r] = t.h_hidx[q] + (int32_t)d;
+            hval[(size_t)h * d2 + r] = hval[(size_t)h * d2 + d + r] = t.h_hval[q];
+        }
+    t.h_hidx = std::move(hidx);
+    t.h_hval = std::move(hval);
+    if (!t.h_ipair.empty()) {
+        const size_t npair = t.h_ipair.size() / (size_t)std::max<int64_t>(d, 1);
+        std::vector<int32_t> ip(npair * d2);
+        for (size_t pr = 0; pr < npair; ++pr)
+            for (int64_t r = 0; r < d; ++r) {
+                const int32_t x = t.h_ipair[pr * d + r];
+                ip[pr * d2 + r] = x;
+                ip[pr * d2 + d + r] = x < 0 ? x : x + (int32_t)(2 * d);
+            }
+        t.h_ipair = std::move(ip);
+    }
+    hipFree(t.str);
+    hipFree(t.diag);
+    hipFree(t.imp);
+    hipFree(t.hidx);
+    hipFree(t.hval);
+    hipFree(t.hval32);
+    hipFree(t.hpk);
+    hipFree(t.coef);
+    hipFree(t.ipair);
+    t.str = nullptr; t.diag = nullptr; t.imp = nullptr; t.hidx = nullptr; t.hval = nullptr;
+    t.hval32 = nullptr; t.hpk = nullptr; t.coef = nullptr; t.ipair = nullptr;
+    t.packed = false;
+    t.ncoef = 0;
+    t.dim = d2;
+    const size_t hn = std::max<size_t>((size_t)t.nhop * d2, 1);
+    std::vector<float> h32(t.h_hval.size());
+    for (size_t q = 0; q < h32.size(); ++q) h32[q] = (float)t.h_hval[q];
+    DMED_HIP(hipMalloc(&t.str, std::max<int64_t>(d2, 1) * sizeof(uint32_t)));
+    DMED_HIP(hipMalloc(&t.diag, std::max<int64_t>(d2, 1) * sizeof(double)));
+    DMED_HIP(hipMalloc(&t.imp, std::max<int64_t>(d2, 1)));
+    DMED_HIP(hipMalloc(&t.hidx, hn * sizeof(int32_t)));
+    DMED_HIP(hipMalloc(&t.hval, hn * sizeof(double)));
+    DMED_HIP(hipMalloc(&t.hval32, hn * sizeof(float)));
+    if (d2) {
+        DMED_HIP(hipMemcpy(t.str, t.h_str.data(), d2 * 4, hipMemcpyHostToDevice));
+        DMED_HIP(hipMemcpy(t.diag, t.h_diag.data(), d2 * 8, hipMemcpyHostToDevice));
+        DMED_HIP(hipMemcpy(t.imp, t.h_imp.data(), d2, hipMemcpyHostToDevice));
+    }
+    if (!t.h_hidx.empty()) {
+        DMED_HIP(hipMemcpy(t.hidx, t.h_hidx.data(), t.h_hidx.size() * 4, hipMemcpyHostToDevice));
+        DMED_HIP(hipMemcpy(t.hval, t.h_hval.data(), t.h_hval.size() * 8, hipMemcpyHostToDevice));
+        DMED_HIP(hipMemcpy(t.hval32, h32.data(), h32.size() * 4, hipMemcpyHostToDevice));
+    }
+    if (!t.h_ipair.empty()) {
+        DMED_HIP(hipMalloc(&t.ipair, t.h_ipair.size() * 4));
+        DMED_HIP(hipMemcpy(t.ipair, t.h_ipair.data(), t.h_ipair.size() * 4, hipMemcpyHostToDevice));
+    }
+    return pack_hops(t);
+}
+
+int sector_build_stacked(dmed_sector *s, int nup, int ndw)
+{
+    if (int rc = sector_build(s, nup, ndw)) return rc;
+    DMED_HIP(hipStreamSynchronize(ex().stream));
+    if (int rc = stack_spin(s->dw)) return rc;
+    s->dim = s->up.dim * s->dw.dim;
     return 0;
 }
 

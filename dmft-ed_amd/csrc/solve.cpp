@@ -204,7 +204,7 @@ extern "C" void dmed_solve_params_default(dmed_solve_params *p)
     p->prec_gs = DMED_FP64;
     p->chi_spin = 0;
     p->ltau = 1000;
-    p->pad_ = 0;
+    p->chi_dens = 0;
 }
 
 extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
@@ -511,12 +511,22 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     // poles added for isign = +1 and -1; the weights carry 1/zeta and the
     // sums are divided by zeta once more at the end, as the reference does.
     // A zero S_z psi carries no weight (the reference divides by its norm).
-    if (!rc && p->chi_spin) {
-        const int ltau = std::max((int)beta, p->ltau);     // ED_INPUT_VARS.f90:191
-        const int nrow = norb + 1, nv = lm + 1, nt = ltau + 1;
-        std::vector<double> vm(nv), tau(nt);
+    // bosonic frequencies vm = 2 pi n/beta (n = 0..Lmats) and tau =
+    // linspace(0, beta, Ltau+1) of allocate_grids (ED_GREENS_FUNCTIONS.f90:281-296)
+    const int ltau = std::max((int)beta, p->ltau);         // ED_INPUT_VARS.f90:191
+    const int nv = lm + 1, nt = ltau + 1;
+    std::vector<double> vm, tau;
+    if (p->chi_spin || p->chi_dens) {
+        vm.resize(nv);
+        tau.resize(nt);
         for (int i = 0; i < nv; ++i) vm[i] = M_PI / beta * 2.0 * (double)i;
         for (int i = 0; i < nt; ++i) tau[i] = ltau > 0 ? beta * (double)i / (double)ltau : 0.0;
+    }
+    for (auto &it : sl.items)
+        if ((p->chi_spin || p->chi_dens) && !rc && it.vec->prec != DMED_FP64)
+            rc = fail(DMED_ESTATE, "susceptibility start vectors need FP64 states");
+    if (!rc && p->chi_spin) {
+        const int nrow = norb + 1;
         struct ChiTask {
             int state, row;
             int nlanc = 0, rc = 0;
@@ -588,6 +598,219 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
                     out->spin_chi_w[2 * ((size_t)row + nrow * i)] = cw[2 * ((size_t)row * lr + i)] / zeta;
                     out->spin_chi_w[2 * ((size_t)row + nrow * i) + 1] = cw[2 * ((size_t)row * lr + i) + 1] / zeta;
                 }
+            }
+            out->ltau_used = ltau;
+        }
+    }
+    // -------------------------------------------- charge susceptibility
+    // build_chi_dens (build_chi_dens.f90:4-41), chains in the order of the
+    // reference, every kept state in each:
+    //   diag    a:     N_a psi, isign +1 and -1 into dens(a,a)          (:49-110)
+    //   offdiag a < b: (N_a + N_b) psi, +1 and -1 into dens(a,b);
+    //                  (N_a - i N_b) psi, weight i*norm^2, +1 and
+    //                  (N_a + i N_b) psi, weight i*norm^2, -1           (:188-323)
+    //   mix     a, b:  c^+_{b,s} c_{a,s} psi, +1, then c^+_{a,s} c_{b,s} psi,
+    //                  -1, for each spin s < Nspin, into mix(a,b)      (:339-490)
+    //   tot:           N psi, +1 and -1 into tot                        (:118-178)
+    // with N_a = n_a,up + n_a,dw; offdiag, mix and tot only when Norb > 1.
+    // Under the real H the two complex chains have the Lanczos coefficients
+    // of the stacked real vector [N_a psi; N_b psi] under H (+) H (the
+    // Lanczos vectors are p_k(H) applied to both parts, with norms
+    // ||p_k(H) N_a psi||^2 + ||p_k(H) N_b psi||^2), so one chain on the stacked
+    // sector serves both.  The pole sums are add_to_lanczos_densChi(_mix,
+    // _tot) (:505-733); dens(a,b) real frequencies of a < b then become
+    // 0.5*(w(a,b) - (1+i) w(a,a) - (1+i) w(b,b)) (:19-23), and only the dens
+    // arrays are divided by zeta a second time (:36-38).  A zero start
+    // vector carries no weight (the reference divides by its norm), and a
+    // mix chain whose c_{a,s} leaves the sector range is skipped (the
+    // reference's getCsector returns 0 there and the loop has no valid
+    // vector).
+    if (!rc && p->chi_dens) {
+        enum { kOcc = 0, kStacked = 1, kMix = 2 };
+        enum { kDens = 0, kMixArr = 1, kTot = 2 };
+        struct DTask {
+            int kind, state, target, a, b;
+            int spin = 0, isign = 0;        // mix: operator spin; isign 0 = both branches
+            bool imag = false;              // weight i*norm^2
+            unsigned mask = 0;
+            dmed_sector *S = nullptr;       // sector of the chain
+            dmed_sector *M = nullptr;       // mix: sector after c_{.,s}
+            int nlanc = 0, rc = 0;
+            double norm2 = 0.0;
+            std::vector<double> lam, w2;
+            std::string err;
+        };
+        std::vector<DTask> dt;
+        std::map<int, dmed_sector *> stacked;
+        auto state_sector = [&](int k) { return const_cast<dmed_sector *>(sl.items[k].vec->sec); };
+        const int nst = (int)sl.items.size();
+        for (int a = 0; a < norb; ++a)
+            for (int k = 0; k < nst; ++k) {
+                DTask t{kOcc, k, kDens, a, a};
+                t.mask = 1u << a;
+                t.S = state_sector(k);
+                dt.push_back(std::move(t));
+            }
+        if (norb > 1) {
+            for (int a = 0; a < norb && !rc; ++a)
+                for (int b = a + 1; b < norb && !rc; ++b)
+                    for (int k = 0; k < nst && !rc; ++k) {
+                        DTask t{kOcc, k, kDens, a, b};
+                        t.mask = (1u << a) | (1u << b);
+                        t.S = state_sector(k);
+                        dt.push_back(std::move(t));
+                        DTask u{kStacked, k, kDens, a, b};
+                        u.imag = true;
+                        const int key = sl.items[k].sector;
+                        auto it = stacked.find(key);
+                        if (it == stacked.end()) {
+                            auto sec = std::make_unique<dmed_sector>();
+                            if ((rc = sector_build_stacked(sec.get(), key / (ns + 1), key % (ns + 1)))) {
+                                sector_release(sec.get());
+                                break;
+                            }
+                            it = stacked.emplace(key, sec.release()).first;
+                        }
+                        u.S = it->second;
+                        dt.push_back(std::move(u));
+                    }
+            for (int a = 0; a < norb && !rc; ++a)
+                for (int b = 0; b < norb && !rc; ++b)
+                    for (int k = 0; k < nst && !rc; ++k) {
+                        const int nu = sl.items[k].sector / (ns + 1), nd = sl.items[k].sector % (ns + 1);
+                        for (int pass = 0; pass < 2 && !rc; ++pass)
+                            for (int sp = 0; sp < nspin && !rc; ++sp) {
+                                const int mu = nu - (sp == 0), md = nd - (sp == 1);
+                                if (mu < 0 || md < 0) continue;
+                                // pass 0: c^+_b c_a, isign +1; pass 1: c^+_a c_b, isign -1
+                                DTask t{kMix, k, kMixArr, pass == 0 ? a : b, pass == 0 ? b : a};
+                                t.spin = sp;
+                                t.isign = pass == 0 ? 1 : -1;
+                                t.S = state_sector(k);
+                                if ((rc = cache.get(mu, md, ns, &t.M))) break;
+                                dt.push_back(std::move(t));
+                            }
+                    }
+            for (int k = 0; k < nst; ++k) {
+                DTask t{kOcc, k, kTot, 0, 0};
+                t.mask = (1u << norb) - 1u;
+                t.S = state_sector(k);
+                dt.push_back(std::move(t));
+            }
+        }
+        auto run_dchi = [&](int t) {
+            DTask &dk = dt[t];
+            const State &st = sl.items[dk.state];
+            const dmed_sector *I = st.vec->sec;
+            void *buf = nullptr;
+            if ((dk.rc = exec_buf(ex(), kSlotStart, std::max<int64_t>(dk.S->dim, 1) * 8, &buf))) {
+                dk.err = dmed_last_error();
+                return;
+            }
+            if (dk.kind == kOcc) {
+                dk.rc = apply_occ(I, DMED_FP64, st.vec->data, dk.mask, 1.0, 1.0, buf, &dk.norm2);
+            } else if (dk.kind == kStacked) {
+                double na = 0.0, nb = 0.0;
+                dk.rc = apply_occ(I, DMED_FP64, st.vec->data, 1u << dk.a, 1.0, 1.0, buf, &na);
+                if (!dk.rc)
+                    dk.rc = apply_occ(I, DMED_FP64, st.vec->data, 1u << dk.b, 1.0, 1.0,
+                                      (double *)buf + I->dim, &nb);
+                dk.norm2 = na + nb;
+            } else {
+                dmed_vec mid{dk.M, DMED_FP64, dk.M->dim, nullptr};
+                dmed_vec out{dk.S, DMED_FP64, dk.S->dim, buf};
+                if ((dk.rc = exec_buf(ex(), kSlotMid, std::max<int64_t>(dk.M->dim, 1) * 8, &mid.data))) {
+                    dk.err = dmed_last_error();
+                    return;
+                }
+                double n1 = 0.0;
+                dk.rc = dmed_apply_op(st.vec, dk.a + 1 + dk.spin * ns, 0, dk.M, &mid, &n1);
+                if (!dk.rc) dk.rc = dmed_apply_op(&mid, dk.b + 1 + dk.spin * ns, 1, dk.S, &out, &dk.norm2);
+            }
+            if (!dk.rc && dk.norm2 > 0.0) {
+                dk.nlanc = (int)std::min<int64_t>(dk.kind == kStacked ? I->dim : dk.S->dim, ngf);
+                std::vector<double> al(dk.nlanc, 0.0), bl(dk.nlanc, 0.0);
+                int nsteps = 0;
+                dk.rc = chain_run(dk.S, buf, DMED_FP64, dk.nlanc, p->prec_gf, 1e-13, al.data(),
+                                  bl.data(), &nsteps, p->gf_fp64_head);
+                if (!dk.rc) {
+                    dk.lam = al;
+                    std::vector<double> e(dk.nlanc, 0.0);
+                    for (int i = 1; i < dk.nlanc; ++i) e[i] = bl[i];
+                    dk.w2.assign(dk.nlanc, 0.0);
+                    if (tql2_first_row(dk.nlanc, dk.lam.data(), e.data(), dk.w2.data()))
+                        dk.rc = fail(DMED_ENOCONV, "tql2 did not converge");
+                    for (auto &z : dk.w2) z *= z;
+                }
+            }
+            if (dk.rc) dk.err = dmed_last_error();
+        };
+        if (!rc) rc = run_tasks(p->batched, (int)dt.size(), run_dchi);
+        for (auto &dk : dt)
+            if (!rc && dk.rc) rc = fail(dk.rc, dk.err);
+        for (auto &kv : stacked) dmed_sector_free(kv.second);
+        if (!rc) {
+            const size_t nab = (size_t)norb * norb;
+            // [target][a*norb+b][freq]; tot uses slot 0
+            std::vector<double> civ[3], ctau[3], cw[3];
+            for (int q = 0; q < 3; ++q) {
+                civ[q].assign(2 * nab * nv, 0.0);
+                ctau[q].assign(nab * nt, 0.0);
+                cw[q].assign(2 * nab * lr, 0.0);
+            }
+            for (auto &dk : dt) {
+                if (!(dk.norm2 > 0.0)) continue;
+                const State &st = sl.items[dk.state];
+                const double pf = dk.norm2 / zeta;
+                const double pbz = finite_t ? std::exp(-beta * (st.e - egs)) : 1.0;
+                // mix chains run c^+_{b} c_{a} with (a, b) swapped on the -1 pass
+                const size_t ab = dk.target == kTot ? 0
+                                  : dk.kind == kMix && dk.isign < 0 ? (size_t)dk.b * norb + dk.a
+                                                                   : (size_t)dk.a * norb + dk.b;
+                const int q = dk.target;
+                for (int isign = 1; isign >= -1; isign -= 2) {
+                    if (dk.isign != 0 && isign != dk.isign) continue;
+                    chi_add_branch(dk.imag ? 0.0 : pf, dk.imag ? pf : 0.0, pbz, st.e, dk.lam.data(),
+                                   dk.w2.data(), dk.nlanc, isign, -isign, beta, vm.data(), nv,
+                                   tau.data(), nt, wr.data(), lr, p->eps, civ[q].data() + 2 * ab * nv,
+                                   ctau[q].data() + ab * nt, cw[q].data() + 2 * ab * lr);
+                }
+            }
+            // w(a,b) = 0.5*(w(a,b) - (1+i) w(a,a) - (1+i) w(b,b)), a < b
+            for (int a = 0; a < norb; ++a)
+                for (int b = a + 1; b < norb; ++b) {
+                    double *wab = cw[kDens].data() + 2 * ((size_t)a * norb + b) * lr;
+                    const double *waa = cw[kDens].data() + 2 * ((size_t)a * norb + a) * lr;
+                    const double *wbb = cw[kDens].data() + 2 * ((size_t)b * norb + b) * lr;
+                    for (int i = 0; i < lr; ++i) {
+                        const double xr = waa[2 * i] + wbb[2 * i], xi = waa[2 * i + 1] + wbb[2 * i + 1];
+                        const double re = wab[2 * i] - (xr - xi), im = wab[2 * i + 1] - (xr + xi);
+                        wab[2 * i] = 0.5 * re;
+                        wab[2 * i + 1] = 0.5 * im;
+                    }
+                }
+            // Fortran order (Norb, Norb, L): element (a, b, i) at a + Norb*(b + Norb*i)
+            double *oiv[3] = {out->dens_chi_iv, out->dens_chi_mix_iv, out->dens_chi_tot_iv};
+            double *otau[3] = {out->dens_chi_tau, out->dens_chi_mix_tau, out->dens_chi_tot_tau};
+            double *ow[3] = {out->dens_chi_w, out->dens_chi_mix_w, out->dens_chi_tot_w};
+            for (int q = 0; q < 3; ++q) {
+                const double sc = q == kDens ? zeta : 1.0;
+                const int na = q == kTot ? 1 : norb;
+                for (int a = 0; a < na; ++a)
+                    for (int b = 0; b < na; ++b) {
+                        const size_t ab = (size_t)a * norb + b;
+                        const size_t fo = (size_t)a + (size_t)na * b, fs = (size_t)na * na;
+                        for (int i = 0; i < nv && oiv[q]; ++i) {
+                            oiv[q][2 * (fo + fs * i)] = civ[q][2 * (ab * nv + i)] / sc;
+                            oiv[q][2 * (fo + fs * i) + 1] = civ[q][2 * (ab * nv + i) + 1] / sc;
+                        }
+                        for (int i = 0; i < nt && otau[q]; ++i)
+                            otau[q][fo + fs * i] = ctau[q][ab * nt + i] / sc;
+                        for (int i = 0; i < lr && ow[q]; ++i) {
+                            ow[q][2 * (fo + fs * i)] = cw[q][2 * (ab * lr + i)] / sc;
+                            ow[q][2 * (fo + fs * i) + 1] = cw[q][2 * (ab * lr + i) + 1] / sc;
+                        }
+                    }
             }
             out->ltau_used = ltau;
         }

@@ -47,7 +47,7 @@ class SolveParams(C.Structure):
                 ("lreal", C.c_int32), ("prec_gf", C.c_int32), ("batched", C.c_int32),
                 ("verbose", C.c_int32), ("gf_fp64_head", C.c_int32),
                 ("prec_gs", C.c_int32), ("chi_spin", C.c_int32), ("ltau", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("chi_dens", C.c_int32)]
 
 
 class LocalObs(C.Structure):
@@ -78,7 +78,10 @@ class SolveOut(C.Structure):
                 ("neigen_sector", _i32p), ("lanc_nstates_total", _i32p),
                 ("t_diag", C.c_double), ("t_gf", C.c_double),
                 ("obs", C.POINTER(LocalObs)), ("spin_chi_iv", _dp), ("spin_chi_tau", _dp),
-                ("spin_chi_w", _dp), ("ltau_used", C.c_int32), ("pad2_", C.c_int32)]
+                ("spin_chi_w", _dp), ("ltau_used", C.c_int32), ("pad2_", C.c_int32),
+                ("dens_chi_iv", _dp), ("dens_chi_tau", _dp), ("dens_chi_w", _dp),
+                ("dens_chi_mix_iv", _dp), ("dens_chi_mix_tau", _dp), ("dens_chi_mix_w", _dp),
+                ("dens_chi_tot_iv", _dp), ("dens_chi_tot_tau", _dp), ("dens_chi_tot_w", _dp)]
 
 
 _lib = None
@@ -117,6 +120,7 @@ SIGNATURES = [
     ("dmed_gf_add_chain", C.c_int, [C.c_double, C.c_double, _dp, _dp, C.c_int32, C.c_int32,
                                     _dp, C.c_int32, _dp, C.c_int32, C.c_double, _dp, _dp]),
     ("dmed_apply_sz", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, _dp]),
+    ("dmed_apply_occ", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, _dp]),
     ("dmed_chi_add_chain", C.c_int, [C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int32,
                                      C.c_double, _dp, C.c_int32, _dp, C.c_int32, _dp, C.c_int32,
                                      C.c_double, _dp, _dp, _dp]),
@@ -363,14 +367,23 @@ def solve(norb, nspin, ns, params):
     obs = LocalObs()
     o.obs = C.pointer(obs)
     chi = {}
+    ltau = max(int(p.beta), p.ltau)     # Ltau raised to beta
     if p.chi_spin:
-        # spinChi_* (Norb+1, 0:L) in Fortran order; Ltau raised to beta
-        ltau = max(int(p.beta), p.ltau)
+        # spinChi_* (Norb+1, 0:L) in Fortran order
         chi["spin_chi_iv"] = np.zeros((norb + 1, p.lmats + 1), dtype=np.complex128, order="F")
         chi["spin_chi_tau"] = np.zeros((norb + 1, ltau + 1), order="F")
         chi["spin_chi_w"] = np.zeros((norb + 1, p.lreal), dtype=np.complex128, order="F")
-        for k, a in chi.items():
-            setattr(o, k, a.ctypes.data_as(_dp))
+    if p.chi_dens:
+        # densChi_*, densChi_mix_* (Norb, Norb, 0:L) and densChi_tot_* (0:L), Fortran order
+        for pre in ("dens_chi_", "dens_chi_mix_"):
+            chi[pre + "iv"] = np.zeros((norb, norb, p.lmats + 1), dtype=np.complex128, order="F")
+            chi[pre + "tau"] = np.zeros((norb, norb, ltau + 1), order="F")
+            chi[pre + "w"] = np.zeros((norb, norb, p.lreal), dtype=np.complex128, order="F")
+        chi["dens_chi_tot_iv"] = np.zeros(p.lmats + 1, dtype=np.complex128)
+        chi["dens_chi_tot_tau"] = np.zeros(ltau + 1)
+        chi["dens_chi_tot_w"] = np.zeros(p.lreal, dtype=np.complex128)
+    for k, a in chi.items():
+        setattr(o, k, a.ctypes.data_as(_dp))
     _check(lib().dmed_solve(C.byref(p), C.byref(o)))
     return dict(egs=o.egs, zeta=o.zeta, nstates=o.nstates, numgs=o.numgs, dens=dens, docc=docc,
                 neigen_sector=neig, lanc_nstates_total=ntot.value, t_diag=o.t_diag,
@@ -382,6 +395,17 @@ def apply_sz(psi, iorb):
     out = Vec(psi.sector, psi.prec)
     n2 = C.c_double()
     _check(lib().dmed_apply_sz(psi.h, iorb, out.h, C.byref(n2)))
+    return out, n2.value
+
+
+def apply_occ(psi, orbitals):
+    """(sum over a in orbitals of n_a,up + n_a,dw) psi as a new vector, and its norm^2."""
+    out = Vec(psi.sector, psi.prec)
+    n2 = C.c_double()
+    mask = 0
+    for a in orbitals:
+        mask |= 1 << a
+    _check(lib().dmed_apply_occ(psi.h, mask, out.h, C.byref(n2)))
     return out, n2.value
 
 

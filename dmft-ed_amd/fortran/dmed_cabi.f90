@@ -21,7 +21,7 @@ module DMED_CABI
      integer(c_int32_t) :: lanc_dim_threshold
      real(c_double)     :: lanc_tolerance, gs_threshold, cutoff, beta, eps, wini, wfin
      integer(c_int32_t) :: lmats, lreal, prec_gf, batched, verbose, gf_fp64_head, prec_gs
-     integer(c_int32_t) :: chi_spin, ltau, pad_
+     integer(c_int32_t) :: chi_spin, ltau, chi_dens
   end type dmed_solve_params_t
 
   !> dmed_local_obs: orbital arrays (3), orbital pairs (9, [a*3+b] in C)
@@ -41,6 +41,9 @@ module DMED_CABI
      type(c_ptr)        :: obs
      type(c_ptr)        :: spin_chi_iv, spin_chi_tau, spin_chi_w
      integer(c_int32_t) :: ltau_used, pad2_
+     type(c_ptr)        :: dens_chi_iv, dens_chi_tau, dens_chi_w
+     type(c_ptr)        :: dens_chi_mix_iv, dens_chi_mix_tau, dens_chi_mix_w
+     type(c_ptr)        :: dens_chi_tot_iv, dens_chi_tot_tau, dens_chi_tot_w
   end type dmed_solve_out_t
 
   public :: dmed_init, dmed_finalize, dmed_last_error, dmed_set_model, dmed_solve

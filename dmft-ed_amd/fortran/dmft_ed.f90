@@ -62,6 +62,13 @@ module DMFT_ED
   !> rows 1..Norb per orbital, Norb+1 the total (Norb > 1)
   real(8),allocatable,target,public    :: spinChi_tau(:,:)
   complex(8),allocatable,target,public :: spinChi_iv(:,:), spinChi_w(:,:)
+  !> charge susceptibility of the last solve with CHIFLAG (build_chi_dens.f90):
+  !> (Norb,Norb,0:Ltau) / (Norb,Norb,Lreal) / (Norb,Norb,0:Lmats), and the totals
+  real(8),allocatable,target,public    :: densChi_tau(:,:,:), densChi_mix_tau(:,:,:)
+  complex(8),allocatable,target,public :: densChi_iv(:,:,:), densChi_w(:,:,:)
+  complex(8),allocatable,target,public :: densChi_mix_iv(:,:,:), densChi_mix_w(:,:,:)
+  real(8),allocatable,target,public    :: densChi_tot_tau(:)
+  complex(8),allocatable,target,public :: densChi_tot_iv(:), densChi_tot_w(:)
   real(8),allocatable,target    :: bath_e(:,:,:), bath_v(:,:,:)
   real(8),allocatable,target    :: impHloc(:,:,:,:)
   integer(c_int32_t),allocatable,target :: neigen_sector(:)
@@ -369,6 +376,7 @@ contains
     real(8),allocatable,target :: bcopy(:), hcopy(:,:,:,:), dens(:), docc(:)
     complex(8),allocatable,target :: gm(:,:,:,:,:), gr(:,:,:,:,:), sm(:,:,:,:,:), sr(:,:,:,:,:)
     integer(c_int32_t),target :: ntot
+    integer                   :: lt
     if(isetup)stop "ed_solve: call ed_init_solver first"
     if(present(Hloc))call set_hloc(Hloc)
     if(size(bath)/=get_bath_dimension())stop "ED_SOLVE_SINGLE Error: wrong bath dimensions"
@@ -400,6 +408,7 @@ contains
     p%prec_gs = merge(DMED_MIXED,DMED_FP64,ed_gpu_mixed_gs)
     p%verbose = ed_verbose
     p%chi_spin = merge(1,0,chiflag)
+    p%chi_dens = merge(1,0,chiflag)
     p%ltau = Ltau
     allocate(dens(Norb), docc(Norb))
     allocate(gm(Nspin,Nspin,Norb,Norb,Lmats), sm(Nspin,Nspin,Norb,Norb,Lmats))
@@ -418,6 +427,19 @@ contains
        o%spin_chi_tau = c_loc(spinChi_tau)
        o%spin_chi_iv = c_loc(spinChi_iv)
        o%spin_chi_w = c_loc(spinChi_w)
+       lt = max(int(beta),Ltau)
+       if(allocated(densChi_tau))deallocate(densChi_tau,densChi_iv,densChi_w,densChi_mix_tau,&
+            densChi_mix_iv,densChi_mix_w,densChi_tot_tau,densChi_tot_iv,densChi_tot_w)
+       allocate(densChi_tau(Norb,Norb,0:lt), densChi_mix_tau(Norb,Norb,0:lt))
+       allocate(densChi_iv(Norb,Norb,0:Lmats), densChi_mix_iv(Norb,Norb,0:Lmats))
+       allocate(densChi_w(Norb,Norb,Lreal), densChi_mix_w(Norb,Norb,Lreal))
+       allocate(densChi_tot_tau(0:lt), densChi_tot_iv(0:Lmats), densChi_tot_w(Lreal))
+       o%dens_chi_tau = c_loc(densChi_tau); o%dens_chi_iv = c_loc(densChi_iv)
+       o%dens_chi_w = c_loc(densChi_w)
+       o%dens_chi_mix_tau = c_loc(densChi_mix_tau); o%dens_chi_mix_iv = c_loc(densChi_mix_iv)
+       o%dens_chi_mix_w = c_loc(densChi_mix_w)
+       o%dens_chi_tot_tau = c_loc(densChi_tot_tau); o%dens_chi_tot_iv = c_loc(densChi_tot_iv)
+       o%dens_chi_tot_w = c_loc(densChi_tot_w)
     endif
     call ed_check(dmed_solve(p,o),"dmed_solve")
     lanc_nstates_total = ntot
@@ -434,43 +456,70 @@ contains
        write(LOGfile,"(A,10F18.12)")"docc=",ed_docc
     endif
     if(ed_print_Sigma)call print_impSigma()
-    if(chiflag)call print_chi_spin()
+    if(chiflag)then
+       call print_chi_spin()
+       call print_chi_dens()
+    endif
   end subroutine ed_solve
 
-  !> spinChi_l<a>_{tau,realw,iw}.ed and, for Norb > 1, spinChi_tot<Norb+1>_*.ed
-  !> (ED_IO/print_impChi.f90:4-18): columns x, y (real) or x, Im y, Re y
-  subroutine print_chi_spin()
-    integer :: a, i, unit, lt
+  !> <stem>_tau.ed, <stem>_realw.ed, <stem>_iw.ed of one susceptibility
+  !> (splot of ED_IO/print_impChi.f90): columns x, y (real) or x, Im y, Re y
+  subroutine write_chi_files(stem, ctau, cw, civ)
+    character(len=*),intent(in) :: stem
+    real(8),intent(in)          :: ctau(0:)
+    complex(8),intent(in)       :: cw(:), civ(0:)
+    integer :: i, unit, lt
     real(8) :: pi, wr
-    character(len=:),allocatable :: stem
     pi = acos(-1d0)
-    lt = ubound(spinChi_tau,2)
-    do a=1,Norb+1
-       if(a<=Norb)then
-          stem = "spinChi_l"//itoa(a)
-       else
-          if(Norb<=1)exit
-          stem = "spinChi_tot"//itoa(a)
-       endif
-       open(newunit=unit,file=stem//"_tau.ed")
-       do i=0,lt
-          write(unit,"(2ES24.15)")beta*dble(i)/dble(max(lt,1)),spinChi_tau(a,i)
-       enddo
-       close(unit)
-       open(newunit=unit,file=stem//"_realw.ed")
-       do i=1,Lreal
-          wr = wini
-          if(Lreal>1)wr = wini + (wfin-wini)*dble(i-1)/dble(Lreal-1)
-          write(unit,"(3ES24.15)")wr,aimag(spinChi_w(a,i)),dble(spinChi_w(a,i))
-       enddo
-       close(unit)
-       open(newunit=unit,file=stem//"_iw.ed")
-       do i=0,Lmats
-          write(unit,"(3ES24.15)")pi/beta*2d0*dble(i),aimag(spinChi_iv(a,i)),dble(spinChi_iv(a,i))
-       enddo
-       close(unit)
+    lt = ubound(ctau,1)
+    open(newunit=unit,file=stem//"_tau.ed")
+    do i=0,lt
+       write(unit,"(2ES24.15)")beta*dble(i)/dble(max(lt,1)),ctau(i)
     enddo
+    close(unit)
+    open(newunit=unit,file=stem//"_realw.ed")
+    do i=1,Lreal
+       wr = wini
+       if(Lreal>1)wr = wini + (wfin-wini)*dble(i-1)/dble(Lreal-1)
+       write(unit,"(3ES24.15)")wr,aimag(cw(i)),dble(cw(i))
+    enddo
+    close(unit)
+    open(newunit=unit,file=stem//"_iw.ed")
+    do i=0,Lmats
+       write(unit,"(3ES24.15)")pi/beta*2d0*dble(i),aimag(civ(i)),dble(civ(i))
+    enddo
+    close(unit)
+  end subroutine write_chi_files
+
+  !> spinChi_l<a>_{tau,realw,iw}.ed and, for Norb > 1, spinChi_tot<Norb+1>_*.ed
+  !> (ED_IO/print_impChi.f90:4-18)
+  subroutine print_chi_spin()
+    integer :: a
+    do a=1,Norb
+       call write_chi_files("spinChi_l"//itoa(a),spinChi_tau(a,:),spinChi_w(a,:),spinChi_iv(a,:))
+    enddo
+    if(Norb>1)call write_chi_files("spinChi_tot"//itoa(Norb+1),spinChi_tau(Norb+1,:),&
+         spinChi_w(Norb+1,:),spinChi_iv(Norb+1,:))
   end subroutine print_chi_spin
+
+  !> densChi_l<a><b>_*.ed for a <= b, densChi_mix_l<a><b>_*.ed for all a, b
+  !> and densChi_tot_*.ed (ED_IO/print_impChi.f90:29-61)
+  subroutine print_chi_dens()
+    integer :: a, b
+    do a=1,Norb
+       do b=a,Norb
+          call write_chi_files("densChi_l"//itoa(a)//itoa(b),densChi_tau(a,b,:),&
+               densChi_w(a,b,:),densChi_iv(a,b,:))
+       enddo
+    enddo
+    do a=1,Norb
+       do b=1,Norb
+          call write_chi_files("densChi_mix_l"//itoa(a)//itoa(b),densChi_mix_tau(a,b,:),&
+               densChi_mix_w(a,b,:),densChi_mix_iv(a,b,:))
+       enddo
+    enddo
+    call write_chi_files("densChi_tot",densChi_tot_tau,densChi_tot_w,densChi_tot_iv)
+  end subroutine print_chi_dens
 
   !> impSigma_l<a><b>_s<s>_iw.ed : columns wm, Im Sigma, Re Sigma (ED_IO/print_impSigma.f90)
   subroutine print_impSigma()

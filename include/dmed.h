@@ -191,6 +191,13 @@ int dmed_gf_add_chain(double peso_bz, double ei, const double *alpha,
  * norm2 receives ||S_z psi||^2. */
 int dmed_apply_sz(const dmed_vec *psi, int32_t iorb, dmed_vec *out, double *norm2);
 
+/* N |psi> into out (same sector and precision), N = sum over the impurity
+ * orbitals a in mask (bit a = orbital a, 0-based) of n_a,up + n_a,dw; the
+ * start vectors of lanc_ed_build_densChi_diag_c (one orbital),
+ * _offdiag_c (two) and _tot_c (all) (build_chi_dens.f90:69-75, 214-221,
+ * 143-149).  norm2 receives ||N psi||^2. */
+int dmed_apply_occ(const dmed_vec *psi, uint32_t mask, dmed_vec *out, double *norm2);
+
 /* add_to_lanczos_spinChi (build_chi_spin.f90:186-250), both isign = +1 and
  * -1 of one chain: with (lambda, Z) the tql2 spectrum of the chain,
  * dE_j = lambda_j - ei and peso_j = peso_f*peso_bz*Z(1,j)^2,
@@ -256,7 +263,7 @@ typedef struct dmed_solve_params {
     int32_t prec_gs;               /* DMED_FP64 / DMED_MIXED plain-Lanczos ground states */
     int32_t chi_spin;              /* 0; 1: spin susceptibility (CHIFLAG, ED_MAIN.f90:282) */
     int32_t ltau;                  /* 1000: imaginary-time points, raised to beta (ED_INPUT_VARS.f90:191) */
-    int32_t pad_;
+    int32_t chi_dens;              /* 0; 1: charge susceptibility (CHIFLAG, build_chi_dens.f90:4-41) */
 } dmed_solve_params;
 void dmed_solve_params_default(dmed_solve_params *p);
 
@@ -281,6 +288,19 @@ typedef struct dmed_solve_out {
      * real, chi_w (Norb+1, Lreal) complex; Ltau as raised; may be NULL */
     double *spin_chi_iv, *spin_chi_tau, *spin_chi_w;
     int32_t ltau_used, pad2_;
+    /* charge susceptibility (chi_dens = 1), Fortran order like densChi_*
+     * (ED_VARS_GLOBAL.f90:143-157); each may be NULL:
+     *   dens_chi_iv/_mix_iv (Norb, Norb, 0:Lmats) complex,
+     *   dens_chi_tau/_mix_tau (Norb, Norb, 0:Ltau) real,
+     *   dens_chi_w/_mix_w (Norb, Norb, Lreal) complex,
+     *   dens_chi_tot_iv (0:Lmats), _tot_tau (0:Ltau), _tot_w (Lreal).
+     * Only a < b of dens and the diagonal are assigned by the reference's
+     * offdiag/diag builders; mix and tot are filled when Norb > 1 (zero
+     * otherwise); dens is divided by zeta twice, mix and tot once, as the
+     * reference does (build_chi_dens.f90:36-38). */
+    double *dens_chi_iv, *dens_chi_tau, *dens_chi_w;
+    double *dens_chi_mix_iv, *dens_chi_mix_tau, *dens_chi_mix_w;
+    double *dens_chi_tot_iv, *dens_chi_tot_tau, *dens_chi_tot_w;
 } dmed_solve_out;
 
 /* ed_solve_single backend (ED_MAIN.f90:251-289): diagonalize_impurity

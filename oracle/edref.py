@@ -354,6 +354,128 @@ def sz_vector(smap, vec, ns, orbs):
     return 0.5 * sgn * np.asarray(vec)
 
 
+def dens_chi_branch(peso, peso_bz, ei, lam, w2, isign, beta, vm, tau, wr, eps, chi_iv,
+                    chi_tau, chi_w):
+    """One isign branch of add_to_lanczos_densChi / _densChi_mix / _densChi_tot
+    (build_chi_dens.f90:505-733) for a chain with poles lam and weights
+    w2 = Z(1,j)^2; peso may be complex (the xi*norm0 weights of :263, 300).
+    chi_tau is a real array in the reference, so it takes the real part.  The
+    iv = 0 term of isign = +1 is -peso*beta or peso*(exp(-beta dE)-1)/dE, as
+    the reference writes it (:539-542)."""
+    for j in range(len(lam)):
+        de = lam[j] - ei
+        pj = peso * w2[j] * peso_bz
+        ex = np.exp(-beta * de)
+        if isign == 1:
+            chi_iv[0] += -pj * beta if beta * de < 1e-1 else pj * (ex - 1.0) / de
+            chi_iv[1:] += pj * (ex - 1.0) / (1j * vm[1:] - de)
+            chi_tau += np.real(pj * np.exp(-tau * de))
+            chi_w += pj * (ex - 1.0) / (wr + 1j * eps - de)
+        elif isign == -1:
+            chi_iv[0] += pj * beta if beta * de < 1e-1 else pj * (1.0 - ex) / de
+            chi_iv[1:] += pj * (1.0 - ex) / (1j * vm[1:] + de)
+            chi_tau += np.real(pj * np.exp(-(beta - tau) * de))
+            chi_w += pj * (1.0 - ex) / (wr + 1j * eps + de)
+        else:
+            raise ValueError("isign not in {-1, 1}")
+
+
+def n_vector(smap, vec, ns, orbs):
+    """N |psi> with N = sum_{a in orbs} (n_a,up + n_a,dw), from the state codes
+    (build_chi_dens.f90:69-75, 143-149, 214-221)."""
+    smap = np.asarray(smap, dtype=np.int64)
+    sgn = np.zeros(len(smap))
+    for a in orbs:
+        sgn += ((smap >> a) & 1) + ((smap >> (a + ns)) & 1)
+    return sgn * np.asarray(vec)
+
+
+def chi_chain(smap, v, p):
+    """sp_lanc_tridiag of the normalised start vector v/|v| and the tql2 poles
+    of add_to_lanczos_*Chi: (norm2, lam, w2), or None for a zero vector (the
+    reference divides by its norm there)."""
+    v = np.asarray(v, dtype=np.complex128)
+    norm2 = float(np.vdot(v, v).real)
+    if norm2 == 0.0:
+        return None
+    nlanc = min(len(smap), p["lanc_ngfiter"])
+    al, bl, _ = lanc_tridiag(smap, v / np.sqrt(norm2), nlanc, p["chain_threshold"])
+    dd, z, ierr = tql2(al, np.concatenate(([0.0], bl[1:])))
+    if ierr:
+        raise RuntimeError("tql2 did not converge")
+    return norm2, dd, z[0, :] ** 2
+
+
+def dens_chi(items, ns, norb, nspin, zeta, egs, finite_t, p, vm, tau, wr):
+    """build_chi_dens (build_chi_dens.f90:4-41) over the kept states
+    items = [(e, (nu, nd), vec)]: diag (:49-110), offdiag (:188-323, complex
+    start vectors N_a -/+ i N_b), the real-frequency combination (:19-23),
+    mix (:339-490) and tot (:118-178); dens divided by zeta once more (:36-38).
+    A mix chain whose c_{a,s} empties an empty spin species is skipped (the
+    reference has no valid vector there)."""
+    beta = p["beta"]
+    nv, nt, lr = len(vm), len(tau), len(wr)
+    d_iv = np.zeros((norb, norb, nv), dtype=np.complex128)
+    d_tau = np.zeros((norb, norb, nt))
+    d_w = np.zeros((norb, norb, lr), dtype=np.complex128)
+    m_iv, m_tau, m_w = np.zeros_like(d_iv), np.zeros_like(d_tau), np.zeros_like(d_w)
+    t_iv = np.zeros(nv, dtype=np.complex128)
+    t_tau = np.zeros(nt)
+    t_w = np.zeros(lr, dtype=np.complex128)
+
+    def add(ch, e, scale, isigns, civ, ctau, cw):
+        if ch is None:
+            return
+        norm2, lam, w2 = ch
+        pbz = np.exp(-beta * (e - egs)) if finite_t else 1.0
+        for isign in isigns:
+            dens_chi_branch(scale * norm2 / zeta, pbz, e, lam, w2, isign, beta, vm, tau, wr,
+                            p["eps"], civ, ctau, cw)
+
+    for a in range(norb):
+        for e, (nu, nd), vec in items:
+            m = build_sector(nu, nd)
+            add(chi_chain(m, n_vector(m, vec, ns, [a]), p), e, 1.0, (1, -1),
+                d_iv[a, a], d_tau[a, a], d_w[a, a])
+    if norb > 1:
+        for a in range(norb):
+            for b in range(a + 1, norb):
+                for e, (nu, nd), vec in items:
+                    m = build_sector(nu, nd)
+                    add(chi_chain(m, n_vector(m, vec, ns, [a, b]), p), e, 1.0, (1, -1),
+                        d_iv[a, b], d_tau[a, b], d_w[a, b])
+                    va = n_vector(m, vec, ns, [a])
+                    vb = n_vector(m, vec, ns, [b])
+                    add(chi_chain(m, va - 1j * vb, p), e, 1j, (1,), d_iv[a, b], d_tau[a, b],
+                        d_w[a, b])
+                    add(chi_chain(m, va + 1j * vb, p), e, 1j, (-1,), d_iv[a, b], d_tau[a, b],
+                        d_w[a, b])
+        for a in range(norb):
+            for b in range(a + 1, norb):
+                d_w[a, b] = 0.5 * (d_w[a, b] - (1 + 1j) * d_w[a, a] - (1 + 1j) * d_w[b, b])
+        for a in range(norb):
+            for b in range(norb):
+                for e, (nu, nd), vec in items:
+                    m = build_sector(nu, nd)
+                    for (i, j, isign) in ((a, b, 1), (b, a, -1)):
+                        for s in range(nspin):
+                            mu, md = nu - (s == 0), nd - (s == 1)
+                            if mu < 0 or md < 0:
+                                continue
+                            mm = build_sector(mu, md)
+                            t = apply_op(m, vec, mm, i + 1 + s * ns, False)
+                            v = apply_op(mm, t, m, j + 1 + s * ns, True)
+                            add(chi_chain(m, v, p), e, 1.0, (isign,), m_iv[a, b], m_tau[a, b],
+                                m_w[a, b])
+        for e, (nu, nd), vec in items:
+            m = build_sector(nu, nd)
+            add(chi_chain(m, n_vector(m, vec, ns, range(norb)), p), e, 1.0, (1, -1), t_iv, t_tau,
+                t_w)
+    return dict(dens_chi_iv=d_iv / zeta, dens_chi_tau=d_tau / zeta, dens_chi_w=d_w / zeta,
+                dens_chi_mix_iv=m_iv, dens_chi_mix_tau=m_tau, dens_chi_mix_w=m_w,
+                dens_chi_tot_iv=t_iv, dens_chi_tot_tau=t_tau, dens_chi_tot_w=t_w)
+
+
 def _sector_eigs(smap, neigen, p, seed=1234567):
     dim = len(smap)
     lanc = not (neigen == dim or dim <= p["lanc_dim_threshold"])
@@ -498,4 +620,5 @@ def ed_solve(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, xmu=0.0,
                                p["eps"], civ[row], ctau[row], cw[row])
         out.update(spin_chi_iv=civ / zeta, spin_chi_tau=ctau / zeta, spin_chi_w=cw / zeta,
                    vm=vm, tau=tau)
+        out.update(dens_chi(sl.items, ns, norb, nspin, zeta, egs, finite_t, p, vm, tau, wr))
     return out

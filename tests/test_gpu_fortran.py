@@ -2,8 +2,8 @@
 (ed_read_input -> ed_init_solver with a hamiltonian.restart bath -> ed_solve
 -> ed_get_sigma_matsubara -> local G -> ed_chi2_fitgf) for one DMFT
 iteration on workload 1 with CHIFLAG, against the oracle's solve of the same
-bath.  Bars: E0 1e-10 relative; dens, docc 1e-9; G_loc(iw) and the spin
-susceptibility files 1e-8 absolute."""
+bath.  Bars: E0 1e-10 relative; dens, docc 1e-9; G_loc(iw) and the spin and
+charge susceptibility files 1e-8 absolute."""
 import os
 import subprocess
 
@@ -81,6 +81,16 @@ def test_bethe_driver_one_iteration(tmp_path):
     ct = np.loadtxt(tmp_path / "spinChi_l1_tau.ed")
     assert np.allclose(ct[:, 0], o["tau"], rtol=1e-14, atol=1e-14)
     assert np.abs(ct[:, 1] - o["spin_chi_tau"][0]).max() <= 1e-8
+    # charge susceptibility: densChi_l11_*.ed against the oracle; with Norb = 1
+    # the mix and total files are written with zeros (ED_IO/print_impChi.f90:29-61)
+    di = np.loadtxt(tmp_path / "densChi_l11_iw.ed")
+    assert np.abs(di[:, 2] + 1j * di[:, 1] - o["dens_chi_iv"][0, 0]).max() <= 1e-8
+    dt = np.loadtxt(tmp_path / "densChi_l11_tau.ed")
+    assert np.abs(dt[:, 1] - o["dens_chi_tau"][0, 0]).max() <= 1e-8
+    for f in ("densChi_mix_l11_tau.ed", "densChi_mix_l11_iw.ed", "densChi_tot_tau.ed",
+              "densChi_tot_realw.ed"):
+        z = np.loadtxt(tmp_path / f)
+        assert not np.any(z[:, 1:]), f
     # the fit wrote a new bath
     new = _read_restart(str(tmp_path / "hamiltonian.restart"), c["nbath"])
     assert np.all(np.isfinite(new)) and not np.array_equal(new, bath)

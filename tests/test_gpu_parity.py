@@ -412,7 +412,8 @@ def _solve_both(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode=True,
                 jhflag=False, **kw):
     p = dmed.default_params(**kw)
     r = dmed.solve(norb, nspin, (nbath + 1) * norb, p)
-    pr = edref.default_params(**{("chiflag" if k == "chi_spin" else k): v for k, v in kw.items()
+    pr = edref.default_params(**{("chiflag" if k in ("chi_spin", "chi_dens") else k): v
+                                 for k, v in kw.items()
                                  if k not in ("prec_gf", "batched", "verbose")})
     o = edref.ed_solve(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode, params=pr,
                        jx=jx, jp=jp, jhflag=jhflag)
@@ -491,6 +492,62 @@ def test_spin_chi_vs_oracle(name, finite):
     assert np.abs(r["spin_chi_iv"][rows:]).max(initial=0) == 0.0
 
 
+@pytest.mark.parametrize("name,finite", [("m1_norb1_nb3", False), ("m2_norb2_nb1", False),
+                                          ("m3_norb2_nb2_jh", False), ("m4_norb3_nb1", False),
+                                          ("m7_atomic", False), ("m5_norb1_nb5_T", True)])
+def test_dens_chi_vs_oracle(name, finite):
+    """Charge susceptibility of the solve (chi_dens, build_chi_dens.f90):
+    densChi, densChi_mix and densChi_tot against the oracle -- chi(iv_n) and
+    chi(tau) to 1e-8 absolute, chi(w + i eps) in relative L2 per element
+    (1e-4, as for G on the real axis).  The off-diagonal elements come from a
+    chain on the stacked sector (H (+) H on [N_a psi; N_b psi]) where the
+    oracle runs the reference's complex chains; m3 has spin exchange and pair
+    hopping in the stacked tables and two spin species in the mix."""
+    g = load(name)
+    set_both(g)
+    norb, nspin = int(g["norb"]), int(g["nspin"])
+    kw = dict(beta=float(g["beta"]), lmats=len(g["wm"]), lreal=len(g["wr"]),
+              wini=float(g["wr"][0]), wfin=float(g["wr"][-1]), eps=float(g["eps"]),
+              lanc_ngfiter=200, chi_dens=1, ltau=200)
+    if finite:
+        kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
+    say(f"[{name}] solve with charge chi")
+    r, o = _solve_both(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
+                       float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]),
+                       jx=float(g["jx"]), jp=float(g["jp"]), jhflag=bool(g["jhflag"]), **kw)
+    for pre in ("dens_chi_", "dens_chi_mix_", "dens_chi_tot_"):
+        for suf in ("iv", "tau", "w"):
+            assert r[pre + suf].shape == o[pre + suf].shape, pre + suf
+        assert np.abs(r[pre + "iv"] - o[pre + "iv"]).max() < 1e-8, pre
+        assert np.abs(r[pre + "tau"] - o[pre + "tau"]).max() < 1e-8, pre
+        x = r[pre + "w"].reshape(-1, len(g["wr"]))
+        y = o[pre + "w"].reshape(-1, len(g["wr"]))
+        for k in range(len(x)):
+            assert np.linalg.norm(x[k] - y[k]) / max(np.linalg.norm(y[k]), np.sqrt(y.shape[1])) < 1e-4, (pre, k)
+    if norb > 1:
+        assert np.abs(r["dens_chi_mix_tau"]).max() > 0 and np.abs(r["dens_chi_iv"][0, 1]).max() > 0
+
+
+def test_apply_occ_vs_oracle():
+    """N psi for single orbitals, pairs and all orbitals (bit-exact: the
+    factor is a small integer) against the oracle's occupation vectors."""
+    g = load("m4_norb3_nb1")
+    set_both(g)
+    norb, ns = 3, 6
+    rng = np.random.default_rng(7)
+    s = dmed.Sector(3, 2)
+    smap = edref.build_sector(3, 2)
+    x = rng.standard_normal(len(smap))
+    psi = dmed.Vec(s, dmed.FP64)
+    psi.upload(x)
+    for orbs in ([0], [1], [2], [0, 2], [0, 1, 2]):
+        out, n2 = dmed.apply_occ(psi, orbs)
+        want = edref.n_vector(smap, x, ns, orbs)
+        got = out.download()
+        assert np.array_equal(got, want), orbs
+        assert abs(n2 - float(np.dot(want, want))) <= 1e-12 * n2
+
+
 def test_solve_config1_vs_oracle():
     """Workload 1 (Ns=8, the CPU-runnable case): whole ed_solve, T=0,
     Lmats=4096, chains of 200 steps; E0 1e-10 rel, G and Sigma 1e-8 abs."""
@@ -522,6 +579,7 @@ def test_batched_solve_identical_to_serial(name, finite):
     if finite:
         kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
     say(f"[{name}] serial vs batched")
+    kw.update(chi_spin=1, chi_dens=1, ltau=100)
     r0 = dmed.solve(norb, nspin, ns, dmed.default_params(batched=0, **kw))
     r1 = dmed.solve(norb, nspin, ns, dmed.default_params(batched=1, **kw))
     assert r0["nstates"] == r1["nstates"]
@@ -529,6 +587,9 @@ def test_batched_solve_identical_to_serial(name, finite):
     assert np.abs(r0["gmats"] - r1["gmats"]).max() <= 1e-13
     assert np.abs(r0["smats"] - r1["smats"]).max() <= 1e-12
     assert np.allclose(r0["dens"], r1["dens"], atol=1e-14)
+    for k in ("spin_chi_iv", "spin_chi_tau", "dens_chi_iv", "dens_chi_tau", "dens_chi_mix_iv",
+              "dens_chi_tot_iv"):
+        assert np.abs(r0[k] - r1[k]).max() <= 1e-13 * max(1.0, np.abs(r0[k]).max()), k
 
 
 def test_mixed_precision_chains_error_bounds():

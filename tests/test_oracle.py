@@ -378,3 +378,137 @@ def test_spin_chi_consistency(name):
         for n in (0, 1, 2):
             quad = np.trapezoid(np.exp(1j * vm[n] * tau) * ct, tau)
             assert abs(quad - civ[n]) <= 5e-3 * abs(civ[n]) + 1e-12
+
+
+def test_dens_chi_atomic_limit_closed_form():
+    """V=0, half filling, T=0: every kept state has the impurity singly
+    occupied, so N psi = psi and the one pole sits at dE = 0.  The particle
+    and hole terms then give chi(tau) = 2*sum_states 1/zeta^2 = 2/zeta for
+    every tau (weights norm^2/zeta, then the final division by zeta,
+    build_chi_dens.f90:36-38, 521), while the reference's iv = 0 terms of the
+    two branches (-peso*beta and +peso*beta, :540 and :561) cancel and every
+    other frequency vanishes with exp(-beta*dE) - 1 = 0.  With Norb = 1 the
+    mix and total arrays stay zero (:10-35)."""
+    g = load("m7_atomic")
+    r = _solve_golden(g, chiflag=True, lanc_ngfiter=64)
+    zeta = r["zeta"]
+    assert zeta >= 2
+    assert np.abs(r["dens_chi_tau"][0, 0] - 2.0 / zeta).max() < 1e-12
+    assert np.abs(r["dens_chi_iv"][0, 0]).max() < 1e-12
+    assert np.abs(r["dens_chi_w"][0, 0]).max() < 1e-12
+    for k in ("dens_chi_mix_iv", "dens_chi_mix_tau", "dens_chi_mix_w", "dens_chi_tot_iv",
+              "dens_chi_tot_tau", "dens_chi_tot_w"):
+        assert not np.any(r[k]), k
+
+
+def _quad(ct, tau, vm, n):
+    return np.trapezoid(np.exp(1j * vm[n] * tau) * ct, tau)
+
+
+@pytest.mark.parametrize("name", ["m1_norb1_nb3", "m2_norb2_nb1", "m8_norb2_nb2_nohf"])
+def test_dens_chi_diag_tot_mix_consistency(name):
+    """T=0 charge susceptibility, diagonal, total and mix:
+    - chi(iv_0) = 0 for the diagonal and total arrays: one chain feeds both
+      branches and the reference's iv = 0 terms have opposite signs
+      (build_chi_dens.f90:539-542 against 560-563);
+    - chi(iv_n) = int_0^beta exp(i v_n tau) chi(tau) dtau for n >= 1 (trapezoid),
+      which ties the tau and frequency formulas' signs and factors;
+    - the diagonal chi(tau) at tau = 0 is sum over kept states of
+      (<N_a^2> + <N_a>^2)/zeta^2 from the state vectors (the hole term at
+      tau = 0 keeps only the dE = 0 pole, the rest is below exp(-beta*gap));
+    - mix(b, a)(tau) = mix(a, b)(beta - tau): the particle chain of (a, b) is
+      the hole chain of (b, a) (:339-490)."""
+    g = load(name)
+    r = _solve_golden(g, chiflag=True, lanc_ngfiter=400)
+    norb, zeta = int(g["norb"]), r["zeta"]
+    ns = (int(g["nbath"]) + 1) * norb
+    tau, vm = r["tau"], r["vm"]
+    arrays = [(r["dens_chi_tau"][a, a], r["dens_chi_iv"][a, a]) for a in range(norb)]
+    if norb > 1:
+        arrays.append((r["dens_chi_tot_tau"], r["dens_chi_tot_iv"]))
+    for ct, civ in arrays:
+        assert abs(civ[0]) <= 1e-12 * np.abs(civ).max()
+        for n in (1, 2, 3):
+            assert abs(_quad(ct, tau, vm, n) - civ[n]) <= 5e-3 * abs(civ[n]) + 1e-12
+    for a in range(norb):
+        exp0 = 0.0
+        for (e, (nu, nd)), vec in zip(r["states"], r["vectors"]):
+            m = np.asarray(edref.build_sector(nu, nd), dtype=np.int64)
+            na = ((m >> a) & 1) + ((m >> (a + ns)) & 1)
+            p2 = np.abs(vec) ** 2
+            exp0 += (np.dot(p2, na * na) + np.dot(p2, na) ** 2) / zeta ** 2
+        ct = r["dens_chi_tau"][a, a]
+        assert abs(ct[0] - exp0) <= 1e-9 * max(1.0, exp0)
+    if norb > 1:
+        mt, miv = r["dens_chi_mix_tau"], r["dens_chi_mix_iv"]
+        assert np.abs(mt).max() > 0
+        for a in range(norb):
+            for b in range(norb):
+                assert np.abs(mt[b, a] - mt[a, b][::-1]).max() <= 1e-10 * np.abs(mt).max()
+                for n in (1, 2):
+                    q = _quad(mt[a, b], tau, vm, n)
+                    assert abs(q - miv[a, b, n]) <= 5e-3 * abs(miv[a, b, n]) + 1e-9
+
+
+@pytest.mark.parametrize("name", ["m2_norb2_nb1", "m8_norb2_nb2_nohf"])
+def test_dens_chi_offdiag_against_total_and_diagonal(name):
+    """Norb = 2, T=0, chains run to the end of each small sector.  The
+    (N_0 + N_1) chain of the off-diagonal element is the total chain; the two
+    complex chains (N_0 -/+ i N_1, weight i*norm^2, build_chi_dens.f90:238-
+    304) carry, under the real H, the spectral measure of N_0 psi plus that of
+    N_1 psi, i.e. the two diagonal chains.  So, with dens divided by zeta
+    twice and tot once:
+      dens_tau(0,1) = tot_tau/zeta                      (tau takes Re only)
+      dens_iv(0,1)  = tot_iv/zeta + i (dens_iv(0,0) + dens_iv(1,1))
+      dens_w(0,1)   = (tot_w/zeta - dens_w(0,0) - dens_w(1,1))/2
+    the last after the (1+i) combination of :19-23, in which the complex
+    chains cancel and the cross term <N_0; N_1> remains."""
+    g = load(name)
+    r = _solve_golden(g, chiflag=True, lanc_ngfiter=400)
+    zeta = r["zeta"]
+    dt, div, dw = r["dens_chi_tau"], r["dens_chi_iv"], r["dens_chi_w"]
+    tt, tiv, tw = r["dens_chi_tot_tau"], r["dens_chi_tot_iv"], r["dens_chi_tot_w"]
+    scale = np.abs(div).max()
+    assert np.abs(dt[0, 1] - tt / zeta).max() <= 1e-10 * np.abs(tt).max()
+    assert np.abs(div[0, 1] - (tiv / zeta + 1j * (div[0, 0] + div[1, 1]))).max() <= 1e-8 * scale
+    want = 0.5 * (tw / zeta - dw[0, 0] - dw[1, 1])
+    assert np.abs(dw[0, 1] - want).max() <= 1e-8 * np.abs(dw).max()
+    assert not np.any(dt[1, 0]) and not np.any(div[1, 0]) and not np.any(dw[1, 0])
+
+
+def test_dens_chi_mix_tau0_sum_rule():
+    """mix(a,b)(tau=0) at T=0: the particle chain of c^+_b c_a psi adds its
+    full weight sum ||c^+_b c_a psi||^2/zeta; the hole chain of c^+_a c_b psi
+    adds exp(-beta*dE) times its weights, which keeps only the overlaps with
+    the sector's ground states: |<gs'|c^+_a c_b|psi>|^2/zeta, from a dense
+    eigen-decomposition of the sector.  c and c^+ act through their matrices
+    built here from the Jordan-Wigner signs on the state codes."""
+    g = load("m2_norb2_nb1")
+    r = _solve_golden(g, chiflag=True, lanc_ngfiter=400)
+    norb, zeta, egs = int(g["norb"]), r["zeta"], r["egs"]
+    ns = (int(g["nbath"]) + 1) * norb
+
+    def op(m_from, m_to, pos, dagger):
+        idx = {int(x): i for i, x in enumerate(m_to)}
+        M = np.zeros((len(m_to), len(m_from)))
+        for j, x in enumerate(m_from):
+            x = int(x)
+            occ = (x >> pos) & 1
+            if occ == (0 if dagger else 1):
+                y = x ^ (1 << pos)
+                M[idx[y], j] = (-1) ** bin(x & ((1 << pos) - 1)).count("1")
+        return M
+
+    for a in range(norb):
+        for b in range(norb):
+            want = 0.0
+            for (e, (nu, nd)), vec in zip(r["states"], r["vectors"]):
+                m = edref.build_sector(nu, nd)
+                mm = edref.build_sector(nu - 1, nd)
+                w, U = np.linalg.eigh(edref.dense_h(m))
+                gsv = U[:, np.abs(w - egs) < 1e-9]
+                fwd = op(mm, m, b, True) @ op(m, mm, a, False) @ vec
+                bwd = op(mm, m, a, True) @ op(m, mm, b, False) @ vec
+                want += (fwd @ fwd + np.sum((gsv.T @ bwd) ** 2)) / zeta
+            got = r["dens_chi_mix_tau"][a, b, 0]
+            assert abs(got - want) <= 1e-8 * max(1.0, abs(want)), (a, b, got, want)
