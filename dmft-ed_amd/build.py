@@ -66,7 +66,8 @@ def build_lib(jobs=8):
 
 
 def build_fortran():
-    """DMFT_ED module (ISO_C_BINDING over libdmed.so) and the Bethe driver."""
+    """DMFT_ED module (ISO_C_BINDING over libdmed.so) and the Bethe drivers
+    (single site and inequivalent sites)."""
     os.makedirs(FBUILD, exist_ok=True)
     srcs = ["dmed_cabi.f90", "dmft_ed.f90"]
     objs = []
@@ -78,12 +79,14 @@ def build_fortran():
         _run([FLANG, "-O2", "-fPIC", "-J", FBUILD, "-c", src, "-o", obj])
         objs.append(obj)
     exe = None
-    drv = os.path.join(FDIR, "ed_hm_bethe.f90")
-    if os.path.exists(drv):
-        exe = os.path.join(FBUILD, "ed_hm_bethe")
-        _run([FLANG, "-O2", "-J", FBUILD, drv] + objs +
-             ["-o", exe, f"-L{HERE}", "-ldmed", f"-Wl,-rpath,{HERE}",
-              "-Wl,-rpath,$ORIGIN/../.."])
+    for name in ("ed_hm_bethe", "ed_hm_bethe_ineq"):
+        drv = os.path.join(FDIR, name + ".f90")
+        if os.path.exists(drv):
+            out = os.path.join(FBUILD, name)
+            _run([FLANG, "-O2", "-J", FBUILD, drv] + objs +
+                 ["-o", out, f"-L{HERE}", "-ldmed", f"-Wl,-rpath,{HERE}",
+                  "-Wl,-rpath,$ORIGIN/../.."])
+            exe = exe or out
     return exe
 
 

@@ -15,6 +15,8 @@
 
 #include "dmed_internal.h"
 
+#include <atomic>
+
 namespace dmed {
 
 Context &ctx()
@@ -25,9 +27,14 @@ Context &ctx()
 
 static thread_local Exec *t_exec = nullptr;
 static thread_local std::string t_error;
+static thread_local Model *t_model = nullptr;
+static std::atomic<int> g_generation{0};
 
 Exec &ex() { return t_exec ? *t_exec : ctx().exec; }
 void bind_exec(Exec *e) { t_exec = e; }
+Model &cur_model() { return t_model ? *t_model : ctx().model; }
+bool has_cur_model() { return t_model ? true : ctx().has_model; }
+void bind_model(Model *m) { t_model = m; }
 
 int fail(int code, const std::string &msg)
 {
@@ -100,7 +107,7 @@ using namespace dmed;
 #define REQUIRE_MODEL()                                                    \
     do {                                                                   \
         REQUIRE_INIT();                                                    \
-        if (!ctx().has_model) return fail(DMED_ESTATE, "dmed_set_model not called"); \
+        if (!has_cur_model()) return fail(DMED_ESTATE, "dmed_set_model not called"); \
     } while (0)
 
 extern "C" {
@@ -141,16 +148,20 @@ int dmed_finalize(void)
 
 const char *dmed_last_error(void) { return t_exec ? t_error.c_str() : ctx().last_error.c_str(); }
 
-int dmed_set_model(const dmed_model *m)
+}  // extern "C"
+
+namespace dmed {
+
+// Validate a model description and copy it into md under a new generation
+// (sectors remember the generation they were built for).
+int model_fill(const dmed_model *m, Model &md)
 {
-    REQUIRE_INIT();
     if (!m || !m->bath || !m->hloc) return fail(DMED_EINVAL, "null model");
     if (m->norb < 1 || m->norb > 3) return fail(DMED_EINVAL, "Norb must be 1..3 (ED_SETUP.f90:276)");
     if (m->nspin < 1 || m->nspin > 2) return fail(DMED_EINVAL, "Nspin must be 1..2 (ED_SETUP.f90:275)");
     if (m->nbath < 0) return fail(DMED_EINVAL, "Nbath < 0");
     const int ns = (m->nbath + 1) * m->norb;
     if (ns > kMaxNs) return fail(DMED_EINVAL, "Ns too large for 32-bit spin strings");
-    Model &md = ctx().model;
     md.norb = m->norb;
     md.nbath = m->nbath;
     md.nspin = m->nspin;
@@ -174,7 +185,18 @@ int dmed_set_model(const dmed_model *m)
     md.xmu = m->xmu;
     md.hfmode = m->hfmode;
     md.jhflag = m->jhflag;
-    md.generation++;
+    md.generation = ++g_generation;
+    return 0;
+}
+
+}  // namespace dmed
+
+extern "C" {
+
+int dmed_set_model(const dmed_model *m)
+{
+    REQUIRE_INIT();
+    if (int rc = model_fill(m, ctx().model)) return rc;
     ctx().has_model = true;
     return 0;
 }
@@ -182,7 +204,7 @@ int dmed_set_model(const dmed_model *m)
 int dmed_get_ns(int32_t *ns)
 {
     REQUIRE_MODEL();
-    *ns = ctx().model.ns;
+    *ns = cur_model().ns;
     return 0;
 }
 
@@ -323,7 +345,7 @@ int dmed_vec_download(const dmed_vec *v, double *host)
 static int check_sector(const dmed_sector *s)
 {
     if (!s) return fail(DMED_EINVAL, "null sector");
-    if (s->model_generation != ctx().model.generation)
+    if (s->model_generation != cur_model().generation)
         return fail(DMED_ESTATE, "sector was built for an older model; recreate it");
     return 0;
 }
@@ -876,7 +898,7 @@ extern "C" int dmed_local_observables(const dmed_vec *psi, double peso, dmed_loc
     if (!psi || !acc) return fail(DMED_EINVAL, "null argument");
     const dmed_sector *s = psi->sec;
     if (int rc = check_sector(s)) return rc;
-    const Model &md = ctx().model;
+    const Model &md = cur_model();
     const int no = md.norb, nimp = s->nimp, sd = md.nspin - 1;
     if (nimp > 8) return fail(DMED_EUNSUPP, "local observables: Norb <= 3");
     hipStream_t st = ex().stream;
@@ -1011,7 +1033,7 @@ extern "C" int dmed_local_observables(const dmed_vec *psi, double peso, dmed_loc
 extern "C" int dmed_delta(int32_t ispin, int32_t iorb, const double *z, int32_t L, double *delta)
 {
     REQUIRE_MODEL();
-    const Model &m = ctx().model;
+    const Model &m = cur_model();
     if (ispin < 0 || ispin >= m.nspin || iorb < 0 || iorb >= m.norb)
         return fail(DMED_EINVAL, "spin/orbital out of range");
     using cd = std::complex<double>;
@@ -1032,7 +1054,7 @@ extern "C" int dmed_sigma(int32_t ispin, int32_t iorb, const double *z, const do
                           double *sigma)
 {
     if (int rc = dmed_delta(ispin, iorb, z, L, sigma)) return rc;
-    const Model &m = ctx().model;
+    const Model &m = cur_model();
     using cd = std::complex<double>;
     const cd *zz = reinterpret_cast<const cd *>(z);
     const cd *gg = reinterpret_cast<const cd *>(g);

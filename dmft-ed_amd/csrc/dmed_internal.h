@@ -146,6 +146,13 @@ struct Context {
     std::string last_error;
 };
 Context &ctx();
+// Model of the calling thread: the one bound with bind_model (site-parallel
+// solves bind one per worker), else the context's (dmed_set_model).
+Model &cur_model();
+bool has_cur_model();
+void bind_model(Model *m);                // nullptr: back to the context's model
+int model_fill(const dmed_model *m, Model &md);
+int upload_binom();                       // binomial table for rank_of, once per process
 
 int ensure_partials(size_t n);          // on ex()
 

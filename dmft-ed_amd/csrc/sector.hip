@@ -32,7 +32,7 @@ int64_t binom(int n, int k)
     return r;
 }
 
-static int upload_binom()
+int upload_binom()
 {
     if (g_binom_ready) return 0;
     uint32_t h[32][32];
@@ -348,7 +348,7 @@ static int build_pairs(const Model &m, SpinTables &t)
 int sector_build(dmed_sector *s, int nup, int ndw)
 {
     Context &c = ctx();
-    const Model &m = c.model;
+    const Model &m = cur_model();
     if (nup < 0 || ndw < 0 || nup > m.ns || ndw > m.ns)
         return fail(DMED_EINVAL, "sector occupation out of range");
     if (int rc = upload_binom()) return rc;

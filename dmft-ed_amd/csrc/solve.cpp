@@ -211,13 +211,13 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
 {
     Context &c = ctx();
     if (!c.initialized) return fail(DMED_ESTATE, "dmed_init not called");
-    if (!c.has_model) return fail(DMED_ESTATE, "dmed_set_model not called");
+    if (!has_cur_model()) return fail(DMED_ESTATE, "dmed_set_model not called");
     if (!p || !out) return fail(DMED_EINVAL, "null argument");
     if (p->prec_gf != DMED_FP64 && p->prec_gf != DMED_FP32 && p->prec_gf != DMED_MIXED)
         return fail(DMED_EINVAL, "prec_gf must be DMED_FP64, DMED_FP32 or DMED_MIXED");
     if (p->prec_gs != DMED_FP64 && p->prec_gs != DMED_MIXED)
         return fail(DMED_EINVAL, "prec_gs must be DMED_FP64 or DMED_MIXED");
-    const Model &m = c.model;
+    const Model &m = cur_model();
     const int ns = m.ns, norb = m.norb, nspin = m.nspin;
     const int nsectors = (ns + 1) * (ns + 1);
     // init_ed_structure: finite T iff lanc_nstates_total != 1, and then the
@@ -829,4 +829,38 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     out->t_gf = t2 - t1;
     sl.free_all();
     return rc;
+}
+
+// ed_solve_lattice (ED_MAIN.f90:364-468): independent impurity problems of
+// inequivalent sites.  With batched set and more than one site, the sites
+// run concurrently on host workers, each with its own stream, scratch and
+// bound model, and each site's solve runs its chains serially; otherwise the
+// sites run one after another with the given params.  A site's results do
+// not depend on which worker ran it.  The context's model (dmed_set_model)
+// is left as it was.
+extern "C" int dmed_solve_sites(int32_t nsites, const dmed_model *models,
+                                const dmed_solve_params *p, dmed_solve_out *outs)
+{
+    if (!ctx().initialized) return fail(DMED_ESTATE, "dmed_init not called");
+    if (nsites < 1 || !models || !p || !outs) return fail(DMED_EINVAL, "null or empty argument");
+    std::vector<Model> ms(nsites);
+    for (int i = 0; i < nsites; ++i)
+        if (int rc = model_fill(&models[i], ms[i]))
+            return fail(rc, "site " + std::to_string(i) + ": " + dmed_last_error());
+    if (int rc = upload_binom()) return rc;
+    const bool conc = p->batched && nsites > 1;
+    dmed_solve_params q = *p;
+    if (conc) q.batched = 0;
+    std::vector<int> rcs(nsites, 0);
+    std::vector<std::string> errs(nsites);
+    int rc = run_tasks(conc ? 1 : 0, nsites, [&](int i) {
+        bind_model(&ms[i]);
+        rcs[i] = dmed_solve(&q, &outs[i]);
+        if (rcs[i]) errs[i] = dmed_last_error();
+        bind_model(nullptr);
+    });
+    if (rc) return rc;
+    for (int i = 0; i < nsites; ++i)
+        if (rcs[i]) return fail(rcs[i], "site " + std::to_string(i) + ": " + errs[i]);
+    return 0;
 }

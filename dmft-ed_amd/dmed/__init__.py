@@ -127,6 +127,8 @@ SIGNATURES = [
     ("dmed_observables", C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
     ("dmed_solve_params_default", None, [C.POINTER(SolveParams)]),
     ("dmed_solve", C.c_int, [C.POINTER(SolveParams), C.POINTER(SolveOut)]),
+    ("dmed_solve_sites", C.c_int, [C.c_int32, C.POINTER(Model), C.POINTER(SolveParams),
+                                   C.POINTER(SolveOut)]),
     ("dmed_sigma", C.c_int, [C.c_int32, C.c_int32, _dp, _dp, C.c_int32, _dp]),
     ("dmed_delta", C.c_int, [C.c_int32, C.c_int32, _dp, C.c_int32, _dp]),
     ("dmed_chi2_fit", C.c_int, [C.c_int32, C.c_int32, _dp, C.c_int32, C.c_double, C.c_double,
@@ -168,11 +170,11 @@ def finalize():
     _check(lib().dmed_finalize())
 
 
-def set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, jp=0.0,
-              xmu=0.0, hfmode=True, jhflag=False):
-    """bath: set_dmft_bath layout; hloc_f: real Hloc(Nspin,Nspin,Norb,Norb) in
+def make_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, jp=0.0,
+               xmu=0.0, hfmode=True, jhflag=False):
+    """A dmed_model struct and the arrays it points to (keep them alive).
+    bath: set_dmft_bath layout; hloc_f: real Hloc(Nspin,Nspin,Norb,Norb) in
     Fortran order (flattened) or a 4-d array."""
-    global _keep
     bath = np.ascontiguousarray(bath, dtype=np.float64)
     h = np.asarray(hloc_f, dtype=np.float64)
     if h.ndim == 4:
@@ -186,7 +188,15 @@ def set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, j
         m.uloc[i] = u[i]
     m.ust, m.jh, m.jx, m.jp, m.xmu = ust, jh, jx, jp, xmu
     m.hfmode, m.jhflag = int(bool(hfmode)), int(bool(jhflag))
-    _keep = (bath, h)
+    return m, (bath, h)
+
+
+def set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, jp=0.0,
+              xmu=0.0, hfmode=True, jhflag=False):
+    """dmed_set_model; returns Ns."""
+    global _keep
+    m, _keep = make_model(norb, nbath, nspin, bath, hloc_f, uloc, ust, jh, jx, jp, xmu, hfmode,
+                          jhflag)
     _check(lib().dmed_set_model(C.byref(m)))
     return (nbath + 1) * norb
 
@@ -347,47 +357,72 @@ def default_params(**kw):
     return p
 
 
-def solve(norb, nspin, ns, params):
-    """ed_solve on the current model.  Returns a dict with arrays shaped like
-    the reference's (Nspin,Nspin,Norb,Norb,L) complex outputs."""
-    p = params
+def _solve_out(norb, nspin, ns, p):
+    """A SolveOut wired to fresh result arrays (the reference's layouts)."""
     g = {}
     for k, L in (("gmats", p.lmats), ("greal", p.lreal), ("smats", p.lmats), ("sreal", p.lreal)):
         g[k] = np.zeros((nspin, nspin, norb, norb, L), dtype=np.complex128, order="F")
-    dens = np.zeros(norb)
-    docc = np.zeros(norb)
-    neig = np.zeros((ns + 1) ** 2, dtype=np.int32)
+    r = dict(dens=np.zeros(norb), docc=np.zeros(norb),
+             neigen_sector=np.zeros((ns + 1) ** 2, dtype=np.int32), **g)
     ntot = C.c_int32(p.lanc_nstates_total)
     o = SolveOut()
-    o.dens, o.docc = _d(dens), _d(docc)
-    for k in g:
-        setattr(o, k, g[k].ctypes.data_as(_dp))
-    o.neigen_sector = neig.ctypes.data_as(_i32p)
+    for k in ("dens", "docc", "gmats", "greal", "smats", "sreal"):
+        setattr(o, k, r[k].ctypes.data_as(_dp))
+    o.neigen_sector = r["neigen_sector"].ctypes.data_as(_i32p)
     o.lanc_nstates_total = C.pointer(ntot)
     obs = LocalObs()
     o.obs = C.pointer(obs)
-    chi = {}
     ltau = max(int(p.beta), p.ltau)     # Ltau raised to beta
     if p.chi_spin:
         # spinChi_* (Norb+1, 0:L) in Fortran order
-        chi["spin_chi_iv"] = np.zeros((norb + 1, p.lmats + 1), dtype=np.complex128, order="F")
-        chi["spin_chi_tau"] = np.zeros((norb + 1, ltau + 1), order="F")
-        chi["spin_chi_w"] = np.zeros((norb + 1, p.lreal), dtype=np.complex128, order="F")
+        r["spin_chi_iv"] = np.zeros((norb + 1, p.lmats + 1), dtype=np.complex128, order="F")
+        r["spin_chi_tau"] = np.zeros((norb + 1, ltau + 1), order="F")
+        r["spin_chi_w"] = np.zeros((norb + 1, p.lreal), dtype=np.complex128, order="F")
     if p.chi_dens:
         # densChi_*, densChi_mix_* (Norb, Norb, 0:L) and densChi_tot_* (0:L), Fortran order
         for pre in ("dens_chi_", "dens_chi_mix_"):
-            chi[pre + "iv"] = np.zeros((norb, norb, p.lmats + 1), dtype=np.complex128, order="F")
-            chi[pre + "tau"] = np.zeros((norb, norb, ltau + 1), order="F")
-            chi[pre + "w"] = np.zeros((norb, norb, p.lreal), dtype=np.complex128, order="F")
-        chi["dens_chi_tot_iv"] = np.zeros(p.lmats + 1, dtype=np.complex128)
-        chi["dens_chi_tot_tau"] = np.zeros(ltau + 1)
-        chi["dens_chi_tot_w"] = np.zeros(p.lreal, dtype=np.complex128)
-    for k, a in chi.items():
-        setattr(o, k, a.ctypes.data_as(_dp))
-    _check(lib().dmed_solve(C.byref(p), C.byref(o)))
-    return dict(egs=o.egs, zeta=o.zeta, nstates=o.nstates, numgs=o.numgs, dens=dens, docc=docc,
-                neigen_sector=neig, lanc_nstates_total=ntot.value, t_diag=o.t_diag,
-                t_gf=o.t_gf, obs=obs.as_dict(norb), **g, **chi)
+            r[pre + "iv"] = np.zeros((norb, norb, p.lmats + 1), dtype=np.complex128, order="F")
+            r[pre + "tau"] = np.zeros((norb, norb, ltau + 1), order="F")
+            r[pre + "w"] = np.zeros((norb, norb, p.lreal), dtype=np.complex128, order="F")
+        r["dens_chi_tot_iv"] = np.zeros(p.lmats + 1, dtype=np.complex128)
+        r["dens_chi_tot_tau"] = np.zeros(ltau + 1)
+        r["dens_chi_tot_w"] = np.zeros(p.lreal, dtype=np.complex128)
+    for k in r:
+        if k.startswith(("spin_chi_", "dens_chi_")):
+            setattr(o, k, r[k].ctypes.data_as(_dp))
+    return o, r, ntot, obs
+
+
+def _solve_result(o, r, ntot, obs, norb):
+    return dict(egs=o.egs, zeta=o.zeta, nstates=o.nstates, numgs=o.numgs,
+                lanc_nstates_total=ntot.value, t_diag=o.t_diag, t_gf=o.t_gf,
+                obs=obs.as_dict(norb), **r)
+
+
+def solve(norb, nspin, ns, params):
+    """ed_solve on the current model.  Returns a dict with arrays shaped like
+    the reference's (Nspin,Nspin,Norb,Norb,L) complex outputs."""
+    o, r, ntot, obs = _solve_out(norb, nspin, ns, params)
+    _check(lib().dmed_solve(C.byref(params), C.byref(o)))
+    return _solve_result(o, r, ntot, obs, norb)
+
+
+def solve_sites(sites, params):
+    """ed_solve_lattice: one dmed_solve per site, sites[i] the keyword
+    arguments of make_model; concurrent on the GPU when params.batched.
+    Returns a list of solve() dicts."""
+    ms = (Model * len(sites))()
+    keep, outs, res = [], (SolveOut * len(sites))(), []
+    for i, kw in enumerate(sites):
+        m, k = make_model(**kw)
+        ms[i] = m
+        keep.append(k)
+        norb, nspin = kw["norb"], kw["nspin"]
+        o, r, ntot, obs = _solve_out(norb, nspin, (kw["nbath"] + 1) * norb, params)
+        outs[i] = o
+        res.append((r, ntot, obs, norb))
+    _check(lib().dmed_solve_sites(len(sites), ms, C.byref(params), outs))
+    return [_solve_result(outs[i], r, ntot, obs, norb) for i, (r, ntot, obs, norb) in enumerate(res)]
 
 
 def apply_sz(psi, iorb):

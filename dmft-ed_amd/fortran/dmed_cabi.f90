@@ -47,6 +47,7 @@ module DMED_CABI
   end type dmed_solve_out_t
 
   public :: dmed_init, dmed_finalize, dmed_last_error, dmed_set_model, dmed_solve
+  public :: dmed_solve_sites
   public :: dmed_solve_params_default, dmed_sigma, dmed_delta, dmed_chi2_fit
   public :: c_string
 
@@ -74,6 +75,13 @@ module DMED_CABI
        type(dmed_solve_params_t),intent(in) :: p
        type(dmed_solve_out_t),intent(inout) :: o
      end function dmed_solve
+     integer(c_int) function dmed_solve_sites(nsites,models,p,outs) bind(C,name="dmed_solve_sites")
+       import :: c_int, c_int32_t, dmed_model_t, dmed_solve_params_t, dmed_solve_out_t
+       integer(c_int32_t),value             :: nsites
+       type(dmed_model_t),intent(in)        :: models(*)
+       type(dmed_solve_params_t),intent(in) :: p
+       type(dmed_solve_out_t),intent(inout) :: outs(*)
+     end function dmed_solve_sites
      integer(c_int) function dmed_sigma(ispin,iorb,z,g,L,sigma) bind(C,name="dmed_sigma")
        import :: c_int, c_int32_t, c_double_complex
        integer(c_int32_t),value :: ispin, iorb, L

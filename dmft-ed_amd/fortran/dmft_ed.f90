@@ -11,8 +11,13 @@
 !>   ed_chi2_fitgf            ED_CHI2FIT.f90:86-138 (normal bath)
 !>   get_bath_dimension       ED_BATH/user_aux.f90:11-30
 !>
-!> Scope: ed_mode="normal", bath_type="normal", single impurity site.  Other
-!> modes stop with a message, as the reference stops on unsupported input.
+!>   lattice forms (inequivalent sites, ED_MAIN.f90:170-195, 364-468;
+!>   ED_CHI2FIT.f90:413-452; ED_IO get_*_lattice): ed_init_solver,
+!>   ed_solve, ed_chi2_fitgf and the getters with a leading site index,
+!>   the sites solved concurrently on the GPU
+!>
+!> Scope: ed_mode="normal", bath_type="normal".  Other modes stop with a
+!> message, as the reference stops on unsupported input.
 module DMFT_ED
   use iso_c_binding
   use DMED_CABI
@@ -73,6 +78,51 @@ module DMFT_ED
   real(8),allocatable,target    :: impHloc(:,:,:,:)
   integer(c_int32_t),allocatable,target :: neigen_sector(:)
   logical                       :: isetup=.true.
+  !> file-name suffix of the site being handled (ed_file_suffix of the
+  !> reference): "" for a single site, "_ineq<4-digit site>" for the lattice
+  character(len=16)             :: ed_file_suffix=""
+  !> lattice results (ED_VARS_GLOBAL.f90 Smatsii, nii, ...): site index first
+  complex(8),allocatable        :: Smatsii(:,:,:,:,:,:), Srealii(:,:,:,:,:,:)
+  complex(8),allocatable        :: Gmatsii(:,:,:,:,:,:), Grealii(:,:,:,:,:,:)
+  real(8),allocatable           :: nii(:,:), dii(:,:), mii(:,:), eii(:,:), ddii(:,:)
+  integer(c_int32_t),allocatable,target :: neigen_sectorii(:,:), neigen_totalii(:)
+
+  interface ed_init_solver
+     module procedure ed_init_solver_single, ed_init_solver_lattice
+  end interface ed_init_solver
+  interface ed_solve
+     module procedure ed_solve_single, ed_solve_lattice
+  end interface ed_solve
+  interface ed_chi2_fitgf
+     module procedure ed_chi2_fitgf_single, ed_fit_bath_sites_normal
+  end interface ed_chi2_fitgf
+  interface ed_get_sigma_matsubara
+     module procedure ed_get_sigma_matsubara_1, ed_get_sigma_matsubara_lattice_1
+  end interface ed_get_sigma_matsubara
+  interface ed_get_sigma_real
+     module procedure ed_get_sigma_real_1, ed_get_sigma_real_lattice_1
+  end interface ed_get_sigma_real
+  interface ed_get_gimp_matsubara
+     module procedure ed_get_gimp_matsubara_1, ed_get_gimp_matsubara_lattice_1
+  end interface ed_get_gimp_matsubara
+  interface ed_get_gimp_real
+     module procedure ed_get_gimp_real_1, ed_get_gimp_real_lattice_1
+  end interface ed_get_gimp_real
+  interface ed_get_dens
+     module procedure ed_get_dens_1, ed_get_dens_lattice_1
+  end interface ed_get_dens
+  interface ed_get_docc
+     module procedure ed_get_docc_1, ed_get_docc_lattice_1
+  end interface ed_get_docc
+  interface ed_get_mag
+     module procedure ed_get_mag_1, ed_get_mag_lattice_1
+  end interface ed_get_mag
+  interface ed_get_eimp
+     module procedure ed_get_eimp_1, ed_get_eimp_lattice
+  end interface ed_get_eimp
+  interface ed_get_doubles
+     module procedure ed_get_doubles_1, ed_get_doubles_lattice
+  end interface ed_get_doubles
 
   public :: ed_read_input
   public :: get_bath_dimension
@@ -283,10 +333,10 @@ contains
     do i=1,Nbath
        bath_v(:,:,i) = max(0.1d0,1d0/sqrt(dble(Nbath)))
     enddo
-    inquire(file=trim(Hfile)//".restart",exist=exists)
+    inquire(file=trim(Hfile)//trim(ed_file_suffix)//".restart",exist=exists)
     if(exists)then
-       write(LOGfile,"(A)")"Reading bath from file "//trim(Hfile)//".restart"
-       open(newunit=unit,file=trim(Hfile)//".restart",status='old',action='read')
+       write(LOGfile,"(A)")"Reading bath from file "//trim(Hfile)//trim(ed_file_suffix)//".restart"
+       open(newunit=unit,file=trim(Hfile)//trim(ed_file_suffix)//".restart",status='old',action='read')
        read(unit,"(A)",iostat=ios)line
        do i=1,Nbath
           read(unit,*,iostat=ios)((bath_e(s,a,i),bath_v(s,a,i),a=1,Norb),s=1,Nspin)
@@ -302,7 +352,7 @@ contains
     integer :: unit, i, s, a
     character(len=16) :: ext
     ext = ".restart"; if(used)ext = ".used"
-    open(newunit=unit,file=trim(Hfile)//trim(ext))
+    open(newunit=unit,file=trim(Hfile)//trim(ed_file_suffix)//trim(ext))
     write(unit,"(90(A21,1X))")(("#Ek_l"//itoa(a)//"_s"//itoa(s),"Vk_l"//itoa(a)//"_s"//itoa(s),a=1,Norb),s=1,Nspin)
     do i=1,Nbath
        write(unit,"(90(F21.12,1X))")((bath_e(s,a,i),bath_v(s,a,i),a=1,Norb),s=1,Nspin)
@@ -335,7 +385,7 @@ contains
   end subroutine set_hloc
 
   !> ed_init_solver_single (ED_MAIN.f90:67-107)
-  subroutine ed_init_solver(bath,Hloc)
+  subroutine ed_init_solver_single(bath,Hloc)
     real(8),dimension(:),intent(inout) :: bath
     complex(8),intent(in)              :: Hloc(Nspin,Nspin,Norb,Norb)
     integer :: Ns
@@ -364,31 +414,11 @@ contains
     call init_bath()
     call get_bath(bath)
     isetup=.false.
-  end subroutine ed_init_solver
+  end subroutine ed_init_solver_single
 
-  !> ed_solve_single (ED_MAIN.f90:251-289)
-  subroutine ed_solve(bath,Hloc)
-    real(8),dimension(:),intent(in),target :: bath
-    complex(8),optional,intent(in)         :: Hloc(Nspin,Nspin,Norb,Norb)
-    type(dmed_model_t)        :: m
-    type(dmed_solve_params_t) :: p
-    type(dmed_solve_out_t)    :: o
-    real(8),allocatable,target :: bcopy(:), hcopy(:,:,:,:), dens(:), docc(:)
-    complex(8),allocatable,target :: gm(:,:,:,:,:), gr(:,:,:,:,:), sm(:,:,:,:,:), sr(:,:,:,:,:)
-    integer(c_int32_t),target :: ntot
-    integer                   :: lt
-    if(isetup)stop "ed_solve: call ed_init_solver first"
-    if(present(Hloc))call set_hloc(Hloc)
-    if(size(bath)/=get_bath_dimension())stop "ED_SOLVE_SINGLE Error: wrong bath dimensions"
-    call set_bath(bath)
-    call save_bath(used=.true.)
-    allocate(bcopy(size(bath))); bcopy = bath
-    allocate(hcopy(Nspin,Nspin,Norb,Norb)); hcopy = impHloc
-    m%norb = Norb; m%nbath = Nbath; m%nspin = Nspin
-    m%bath = c_loc(bcopy); m%hloc = c_loc(hcopy)
-    m%uloc = Uloc; m%ust = Ust; m%jh = Jh; m%jx = Jx; m%jp = Jp; m%xmu = xmu
-    m%hfmode = merge(1,0,HFmode); m%jhflag = merge(1,0,Jhflag)
-    call ed_check(dmed_set_model(m),"dmed_set_model")
+  !> dmed_solve_params from the input variables
+  subroutine fill_params(p)
+    type(dmed_solve_params_t),intent(out) :: p
     call dmed_solve_params_default(p)
     p%lanc_nstates_sector = lanc_nstates_sector
     p%lanc_nstates_total  = lanc_nstates_total
@@ -410,6 +440,32 @@ contains
     p%chi_spin = merge(1,0,chiflag)
     p%chi_dens = merge(1,0,chiflag)
     p%ltau = Ltau
+  end subroutine fill_params
+
+  !> ed_solve_single (ED_MAIN.f90:251-289)
+  subroutine ed_solve_single(bath,Hloc)
+    real(8),dimension(:),intent(in),target :: bath
+    complex(8),optional,intent(in)         :: Hloc(Nspin,Nspin,Norb,Norb)
+    type(dmed_model_t)        :: m
+    type(dmed_solve_params_t) :: p
+    type(dmed_solve_out_t)    :: o
+    real(8),allocatable,target :: bcopy(:), hcopy(:,:,:,:), dens(:), docc(:)
+    complex(8),allocatable,target :: gm(:,:,:,:,:), gr(:,:,:,:,:), sm(:,:,:,:,:), sr(:,:,:,:,:)
+    integer(c_int32_t),target :: ntot
+    integer                   :: lt
+    if(isetup)stop "ed_solve: call ed_init_solver first"
+    if(present(Hloc))call set_hloc(Hloc)
+    if(size(bath)/=get_bath_dimension())stop "ED_SOLVE_SINGLE Error: wrong bath dimensions"
+    call set_bath(bath)
+    call save_bath(used=.true.)
+    allocate(bcopy(size(bath))); bcopy = bath
+    allocate(hcopy(Nspin,Nspin,Norb,Norb)); hcopy = impHloc
+    m%norb = Norb; m%nbath = Nbath; m%nspin = Nspin
+    m%bath = c_loc(bcopy); m%hloc = c_loc(hcopy)
+    m%uloc = Uloc; m%ust = Ust; m%jh = Jh; m%jx = Jx; m%jp = Jp; m%xmu = xmu
+    m%hfmode = merge(1,0,HFmode); m%jhflag = merge(1,0,Jhflag)
+    call ed_check(dmed_set_model(m),"dmed_set_model")
+    call fill_params(p)
     allocate(dens(Norb), docc(Norb))
     allocate(gm(Nspin,Nspin,Norb,Norb,Lmats), sm(Nspin,Nspin,Norb,Norb,Lmats))
     allocate(gr(Nspin,Nspin,Norb,Norb,Lreal), sr(Nspin,Nspin,Norb,Norb,Lreal))
@@ -455,12 +511,115 @@ contains
        write(LOGfile,"(A,10F18.12)")"dens=",ed_dens
        write(LOGfile,"(A,10F18.12)")"docc=",ed_docc
     endif
-    if(ed_print_Sigma)call print_impSigma()
+    if(ed_print_Sigma)call print_impSigma(impSmats)
     if(chiflag)then
        call print_chi_spin()
        call print_chi_dens()
     endif
-  end subroutine ed_solve
+  end subroutine ed_solve_single
+
+  !> suffix of site ilat: ineq_site_suffix "_ineq" and site_indx_padding 4
+  !> (ED_VARS_GLOBAL.f90:214-215; ED_MAIN.f90:185)
+  function site_suffix(ilat) result(sfx)
+    integer,intent(in) :: ilat
+    character(len=16)  :: sfx
+    write(sfx,"(A,I4.4)")"_ineq",ilat
+  end function site_suffix
+
+  !> ed_init_solver_lattice (ED_MAIN.f90:170-195): every site initialised as a
+  !> single site (its bath from hamiltonian_ineq<site>.restart if present)
+  subroutine ed_init_solver_lattice(bath,Hloc)
+    real(8),dimension(:,:),intent(inout) :: bath
+    complex(8),intent(in)                :: Hloc(size(bath,1),Nspin,Nspin,Norb,Norb)
+    integer :: ilat, Nlat, Ns
+    Nlat = size(bath,1)
+    do ilat=1,Nlat
+       ed_file_suffix = site_suffix(ilat)
+       call ed_init_solver_single(bath(ilat,:),Hloc(ilat,:,:,:,:))
+    enddo
+    ed_file_suffix = ""
+    Ns = (Nbath+1)*Norb
+    if(allocated(neigen_sectorii))deallocate(neigen_sectorii,neigen_totalii)
+    allocate(neigen_sectorii((Ns+1)**2,Nlat), neigen_totalii(Nlat))
+    neigen_sectorii = 0
+    neigen_totalii = lanc_nstates_total
+  end subroutine ed_init_solver_lattice
+
+  !> ed_solve_lattice (ED_MAIN.f90:364-468): one impurity problem per site,
+  !> optional per-site Uloc, Ust, Jh; all sites in one dmed_solve_sites call,
+  !> which runs them concurrently on the GPU.  Per site the bath is saved to
+  !> hamiltonian_ineq<site>.used and Sigma to impSigma_..._ineq<site>.ed.
+  subroutine ed_solve_lattice(bath,Hloc,Uloc_ii,Ust_ii,Jh_ii)
+    real(8),intent(in)           :: bath(:,:)
+    complex(8),intent(in)        :: Hloc(size(bath,1),Nspin,Nspin,Norb,Norb)
+    real(8),optional,intent(in)  :: Uloc_ii(size(bath,1),Norb)
+    real(8),optional,intent(in)  :: Ust_ii(size(bath,1))
+    real(8),optional,intent(in)  :: Jh_ii(size(bath,1))
+    integer :: Nlat, ilat, Nb
+    type(dmed_model_t),allocatable        :: m(:)
+    type(dmed_solve_out_t),allocatable    :: o(:)
+    type(dmed_solve_params_t)             :: p
+    type(dmed_local_obs_t),allocatable,target :: obs(:)
+    real(8),allocatable,target    :: bcopy(:,:), hcopy(:,:,:,:,:), dens(:,:), docc(:,:)
+    complex(8),allocatable,target :: gm(:,:,:,:,:,:), gr(:,:,:,:,:,:), sm(:,:,:,:,:,:), sr(:,:,:,:,:,:)
+    if(isetup)stop "ed_solve_lattice: call ed_init_solver first"
+    if(.not.allocated(neigen_sectorii))stop "ed_solve_lattice: call ed_init_solver with the lattice bath first"
+    if(chiflag)stop "ed_solve_lattice: CHIFLAG with inequivalent sites is not built"
+    Nlat = size(bath,1)
+    Nb = get_bath_dimension()
+    if(size(bath,2)/=Nb)stop "init_lattice_bath: wrong bath size dimension 1 or 2 "
+    if(size(neigen_totalii)<Nlat)stop "ed_solve_lattice error: size(neigen_totalii,1)<Nsites"
+    allocate(m(Nlat), o(Nlat), obs(Nlat))
+    allocate(bcopy(Nb,Nlat), hcopy(Nspin,Nspin,Norb,Norb,Nlat), dens(Norb,Nlat), docc(Norb,Nlat))
+    allocate(gm(Nspin,Nspin,Norb,Norb,Lmats,Nlat), sm(Nspin,Nspin,Norb,Norb,Lmats,Nlat))
+    allocate(gr(Nspin,Nspin,Norb,Norb,Lreal,Nlat), sr(Nspin,Nspin,Norb,Norb,Lreal,Nlat))
+    call fill_params(p)
+    p%batched = 1
+    do ilat=1,Nlat
+       if(any(abs(aimag(Hloc(ilat,:,:,:,:)))>0d0))stop "DMFT_ED: complex Hloc is not supported on the GPU path (H must be real)"
+       bcopy(:,ilat) = bath(ilat,:)
+       hcopy(:,:,:,:,ilat) = dble(Hloc(ilat,:,:,:,:))
+       m(ilat)%norb = Norb; m(ilat)%nbath = Nbath; m(ilat)%nspin = Nspin
+       m(ilat)%bath = c_loc(bcopy(1,ilat)); m(ilat)%hloc = c_loc(hcopy(1,1,1,1,ilat))
+       m(ilat)%uloc = Uloc; m(ilat)%ust = Ust; m(ilat)%jh = Jh
+       if(present(Uloc_ii))m(ilat)%uloc(1:Norb) = Uloc_ii(ilat,1:Norb)
+       if(present(Ust_ii))m(ilat)%ust = Ust_ii(ilat)
+       if(present(Jh_ii))m(ilat)%jh = Jh_ii(ilat)
+       m(ilat)%jx = Jx; m(ilat)%jp = Jp; m(ilat)%xmu = xmu
+       m(ilat)%hfmode = merge(1,0,HFmode); m(ilat)%jhflag = merge(1,0,Jhflag)
+       o(ilat)%dens = c_loc(dens(1,ilat)); o(ilat)%docc = c_loc(docc(1,ilat))
+       o(ilat)%gmats = c_loc(gm(1,1,1,1,1,ilat)); o(ilat)%greal = c_loc(gr(1,1,1,1,1,ilat))
+       o(ilat)%smats = c_loc(sm(1,1,1,1,1,ilat)); o(ilat)%sreal = c_loc(sr(1,1,1,1,1,ilat))
+       o(ilat)%neigen_sector = c_loc(neigen_sectorii(1,ilat))
+       o(ilat)%lanc_nstates_total = c_loc(neigen_totalii(ilat))
+       o(ilat)%obs = c_loc(obs(ilat))
+       o(ilat)%spin_chi_iv = c_null_ptr; o(ilat)%spin_chi_tau = c_null_ptr; o(ilat)%spin_chi_w = c_null_ptr
+       o(ilat)%dens_chi_iv = c_null_ptr; o(ilat)%dens_chi_tau = c_null_ptr; o(ilat)%dens_chi_w = c_null_ptr
+       o(ilat)%dens_chi_mix_iv = c_null_ptr; o(ilat)%dens_chi_mix_tau = c_null_ptr
+       o(ilat)%dens_chi_mix_w = c_null_ptr; o(ilat)%dens_chi_tot_iv = c_null_ptr
+       o(ilat)%dens_chi_tot_tau = c_null_ptr; o(ilat)%dens_chi_tot_w = c_null_ptr
+    enddo
+    call ed_check(dmed_solve_sites(int(Nlat,c_int32_t),m,p,o),"dmed_solve_sites")
+    if(allocated(Smatsii))deallocate(Smatsii,Srealii,Gmatsii,Grealii,nii,dii,mii,eii,ddii)
+    allocate(Smatsii(Nlat,Nspin,Nspin,Norb,Norb,Lmats), Srealii(Nlat,Nspin,Nspin,Norb,Norb,Lreal))
+    allocate(Gmatsii(Nlat,Nspin,Nspin,Norb,Norb,Lmats), Grealii(Nlat,Nspin,Nspin,Norb,Norb,Lreal))
+    allocate(nii(Nlat,Norb), dii(Nlat,Norb), mii(Nlat,Norb), eii(Nlat,4), ddii(Nlat,4))
+    do ilat=1,Nlat
+       Smatsii(ilat,:,:,:,:,:) = sm(:,:,:,:,:,ilat); Srealii(ilat,:,:,:,:,:) = sr(:,:,:,:,:,ilat)
+       Gmatsii(ilat,:,:,:,:,:) = gm(:,:,:,:,:,ilat); Grealii(ilat,:,:,:,:,:) = gr(:,:,:,:,:,ilat)
+       nii(ilat,:) = dens(:,ilat); dii(ilat,:) = docc(:,ilat)
+       mii(ilat,:) = obs(ilat)%dens_up(1:Norb) - obs(ilat)%dens_dw(1:Norb)
+       eii(ilat,:) = [obs(ilat)%epot,obs(ilat)%eint,obs(ilat)%ehartree,obs(ilat)%eknot]
+       ddii(ilat,:) = [obs(ilat)%dust,obs(ilat)%dund,obs(ilat)%dse,obs(ilat)%dph]
+       ed_file_suffix = site_suffix(ilat)
+       call set_bath(bath(ilat,:))
+       call save_bath(used=.true.)
+       if(ed_print_Sigma)call print_impSigma(sm(:,:,:,:,:,ilat))
+       if(ed_verbose>=1)write(LOGfile,"(A,I4,A,F20.12,A,I4)")"site",ilat,"  Egs =",o(ilat)%egs,&
+            "  states:",o(ilat)%nstates
+    enddo
+    ed_file_suffix = ""
+  end subroutine ed_solve_lattice
 
   !> <stem>_tau.ed, <stem>_realw.ed, <stem>_iw.ed of one susceptibility
   !> (splot of ED_IO/print_impChi.f90): columns x, y (real) or x, Im y, Re y
@@ -521,16 +680,19 @@ contains
     call write_chi_files("densChi_tot",densChi_tot_tau,densChi_tot_w,densChi_tot_iv)
   end subroutine print_chi_dens
 
-  !> impSigma_l<a><b>_s<s>_iw.ed : columns wm, Im Sigma, Re Sigma (ED_IO/print_impSigma.f90)
-  subroutine print_impSigma()
+  !> impSigma_l<a><b>_s<s>_iw<suffix>.ed : columns wm, Im Sigma, Re Sigma
+  !> (ED_IO/print_impSigma.f90:43)
+  subroutine print_impSigma(Sm)
+    complex(8),intent(in) :: Sm(Nspin,Nspin,Norb,Norb,Lmats)
     integer :: unit, s, a, i
     real(8) :: wm
     do s=1,Nspin
        do a=1,Norb
-          open(newunit=unit,file="impSigma_l"//itoa(a)//itoa(a)//"_s"//itoa(s)//"_iw.ed")
+          open(newunit=unit,file="impSigma_l"//itoa(a)//itoa(a)//"_s"//itoa(s)//"_iw"//&
+               trim(ed_file_suffix)//".ed")
           do i=1,Lmats
              wm = acos(-1d0)/beta*dble(2*i-1)
-             write(unit,"(3ES24.15)")wm,aimag(impSmats(s,s,a,a,i)),dble(impSmats(s,s,a,a,i))
+             write(unit,"(3ES24.15)")wm,aimag(Sm(s,s,a,a,i)),dble(Sm(s,s,a,a,i))
           enddo
           close(unit)
        enddo
@@ -538,35 +700,35 @@ contains
   end subroutine print_impSigma
 
   !------------------------------------------------------------ getters
-  subroutine ed_get_sigma_matsubara(Smats)
+  subroutine ed_get_sigma_matsubara_1(Smats)
     complex(8),dimension(Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: Smats
     Smats = impSmats
-  end subroutine ed_get_sigma_matsubara
+  end subroutine ed_get_sigma_matsubara_1
 
-  subroutine ed_get_sigma_real(Sreal)
+  subroutine ed_get_sigma_real_1(Sreal)
     complex(8),dimension(Nspin,Nspin,Norb,Norb,Lreal),intent(inout) :: Sreal
     Sreal = impSreal
-  end subroutine ed_get_sigma_real
+  end subroutine ed_get_sigma_real_1
 
-  subroutine ed_get_gimp_matsubara(Gmats)
+  subroutine ed_get_gimp_matsubara_1(Gmats)
     complex(8),dimension(Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: Gmats
     Gmats = impGmats
-  end subroutine ed_get_gimp_matsubara
+  end subroutine ed_get_gimp_matsubara_1
 
-  subroutine ed_get_gimp_real(Greal)
+  subroutine ed_get_gimp_real_1(Greal)
     complex(8),dimension(Nspin,Nspin,Norb,Norb,Lreal),intent(inout) :: Greal
     Greal = impGreal
-  end subroutine ed_get_gimp_real
+  end subroutine ed_get_gimp_real_1
 
-  subroutine ed_get_dens(dens)
+  subroutine ed_get_dens_1(dens)
     real(8),dimension(Norb) :: dens
     dens = ed_dens
-  end subroutine ed_get_dens
+  end subroutine ed_get_dens_1
 
-  subroutine ed_get_docc(docc)
+  subroutine ed_get_docc_1(docc)
     real(8),dimension(Norb) :: docc
     docc = ed_docc
-  end subroutine ed_get_docc
+  end subroutine ed_get_docc_1
 
   !> ED_IO/get_dens.f90, get_mag.f90 (ed_get_mag_1: dens_up - dens_dw)
   subroutine ed_get_dens_up(dens)
@@ -579,16 +741,16 @@ contains
     dens = ed_dens_dw
   end subroutine ed_get_dens_dw
 
-  subroutine ed_get_mag(mag)
+  subroutine ed_get_mag_1(mag)
     real(8),dimension(Norb) :: mag
     mag = ed_dens_up - ed_dens_dw
-  end subroutine ed_get_mag
+  end subroutine ed_get_mag_1
 
   !> ED_IO/get_eimp.f90: [Epot, Eint, Ehartree, Eknot]
-  subroutine ed_get_eimp(eimp)
+  subroutine ed_get_eimp_1(eimp)
     real(8),dimension(4) :: eimp
     eimp = [ed_Epot,ed_Eint,ed_Ehartree,ed_Eknot]
-  end subroutine ed_get_eimp
+  end subroutine ed_get_eimp_1
 
   subroutine ed_get_epot(e)
     real(8) :: e
@@ -611,10 +773,10 @@ contains
   end subroutine ed_get_eknot
 
   !> ED_IO/get_doubles.f90: [Dust, Dund, Dse, Dph]
-  subroutine ed_get_doubles(d)
+  subroutine ed_get_doubles_1(d)
     real(8),dimension(4) :: d
     d = [ed_Dust,ed_Dund,ed_Dse,ed_Dph]
-  end subroutine ed_get_doubles
+  end subroutine ed_get_doubles_1
 
   subroutine ed_get_dust(d)
     real(8) :: d
@@ -641,7 +803,7 @@ contains
   !> fitgf_normal_normal.f90:18-156): per orbital, fit (e_k, V_k) of spin ispin
   !> to fg on the first Lfit Matsubara points, scheme cg_scheme (weiss/delta),
   !> weights cg_weight; then save the bath (hamiltonian.restart).
-  subroutine ed_chi2_fitgf(fg,bath,ispin)
+  subroutine ed_chi2_fitgf_single(fg,bath,ispin)
     complex(8),dimension(:,:,:,:,:) :: fg
     real(8),dimension(:)            :: bath
     integer,optional                :: ispin
@@ -670,6 +832,111 @@ contains
     endif
     call save_bath(used=.false.)
     call get_bath(bath)
-  end subroutine ed_chi2_fitgf
+  end subroutine ed_chi2_fitgf_single
+
+  !> lattice getters (ED_IO/get_sigma_matsubara.f90:34-38 and the *_lattice
+  !> forms of get_dens, get_docc, get_mag, get_eimp, get_doubles)
+  subroutine ed_get_sigma_matsubara_lattice_1(Smats,Nsites)
+    integer :: Nsites
+    complex(8),dimension(Nsites,Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: Smats
+    Smats(1:Nsites,:,:,:,:,:) = Smatsii(1:Nsites,:,:,:,:,:)
+  end subroutine ed_get_sigma_matsubara_lattice_1
+
+  subroutine ed_get_sigma_real_lattice_1(Sreal,Nsites)
+    integer :: Nsites
+    complex(8),dimension(Nsites,Nspin,Nspin,Norb,Norb,Lreal),intent(inout) :: Sreal
+    Sreal(1:Nsites,:,:,:,:,:) = Srealii(1:Nsites,:,:,:,:,:)
+  end subroutine ed_get_sigma_real_lattice_1
+
+  subroutine ed_get_gimp_matsubara_lattice_1(Gmats,Nsites)
+    integer :: Nsites
+    complex(8),dimension(Nsites,Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: Gmats
+    Gmats(1:Nsites,:,:,:,:,:) = Gmatsii(1:Nsites,:,:,:,:,:)
+  end subroutine ed_get_gimp_matsubara_lattice_1
+
+  subroutine ed_get_gimp_real_lattice_1(Greal,Nsites)
+    integer :: Nsites
+    complex(8),dimension(Nsites,Nspin,Nspin,Norb,Norb,Lreal),intent(inout) :: Greal
+    Greal(1:Nsites,:,:,:,:,:) = Grealii(1:Nsites,:,:,:,:,:)
+  end subroutine ed_get_gimp_real_lattice_1
+
+  subroutine ed_get_dens_lattice_1(yii,Nlat)
+    integer :: Nlat
+    real(8),dimension(Nlat,Norb) :: yii
+    yii = 0d0
+    if(allocated(nii))then
+       if(Nlat>size(nii,1))stop "ed_get_dens error: required N_sites > evaluated N_sites"
+       yii = nii(1:Nlat,:)
+    endif
+  end subroutine ed_get_dens_lattice_1
+
+  subroutine ed_get_docc_lattice_1(yii,Nlat)
+    integer :: Nlat
+    real(8),dimension(Nlat,Norb) :: yii
+    yii = 0d0
+    if(allocated(dii))then
+       if(Nlat>size(dii,1))stop "ed_get_docc error: required N_sites > evaluated N_sites"
+       yii = dii(1:Nlat,:)
+    endif
+  end subroutine ed_get_docc_lattice_1
+
+  subroutine ed_get_mag_lattice_1(yii,Nlat)
+    integer :: Nlat
+    real(8),dimension(Nlat,Norb) :: yii
+    yii = 0d0
+    if(allocated(mii))then
+       if(Nlat>size(mii,1))stop "ed_get_mag error: required N_sites > evaluated N_sites"
+       yii = mii(1:Nlat,:)
+    endif
+  end subroutine ed_get_mag_lattice_1
+
+  subroutine ed_get_eimp_lattice(yii,Nlat)
+    integer :: Nlat
+    real(8),dimension(Nlat,4) :: yii
+    yii = 0d0
+    if(allocated(eii))then
+       if(Nlat>size(eii,1))stop "ed_get_eimp error: required N_sites > evaluated N_sites"
+       yii = eii(1:Nlat,:)
+    endif
+  end subroutine ed_get_eimp_lattice
+
+  subroutine ed_get_doubles_lattice(yii,Nlat)
+    integer :: Nlat
+    real(8),dimension(Nlat,4) :: yii
+    yii = 0d0
+    if(allocated(ddii))then
+       if(Nlat>size(ddii,1))stop "ed_get_doubles error: required N_sites > evaluated N_sites"
+       yii = ddii(1:Nlat,:)
+    endif
+  end subroutine ed_get_doubles_lattice
+
+  !> ed_fit_bath_sites_normal (ED_CHI2FIT.f90:413-452): the single-site fit
+  !> on every site with its own Hloc, bath file hamiltonian_ineq<site>.restart
+  subroutine ed_fit_bath_sites_normal(bath,Delta,Hloc,ispin)
+    real(8),intent(inout)    :: bath(:,:)
+    complex(8),intent(inout) :: Delta(size(bath,1),Nspin,Nspin,Norb,Norb,Lmats)
+    complex(8)               :: Hloc(size(bath,1),Nspin,Nspin,Norb,Norb)
+    integer,optional         :: ispin
+    integer                  :: ilat
+    real(8),allocatable      :: b(:)
+    complex(8),allocatable   :: fg(:,:,:,:,:)
+    if(present(ispin))then
+       if(ispin>Nspin)stop "ed_fit_bath_sites error: required spin index > Nspin"
+    endif
+    allocate(b(size(bath,2)), fg(Nspin,Nspin,Norb,Norb,Lmats))
+    do ilat=1,size(bath,1)
+       call set_hloc(Hloc(ilat,:,:,:,:))
+       ed_file_suffix = site_suffix(ilat)
+       b = bath(ilat,:)
+       fg = Delta(ilat,:,:,:,:,:)
+       if(present(ispin))then
+          call ed_chi2_fitgf_single(fg,b,ispin)
+       else
+          call ed_chi2_fitgf_single(fg,b)
+       endif
+       bath(ilat,:) = b
+    enddo
+    ed_file_suffix = ""
+  end subroutine ed_fit_bath_sites_normal
 
 end module DMFT_ED

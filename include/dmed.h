@@ -308,6 +308,17 @@ typedef struct dmed_solve_out {
  * (build_gf_normal + get_sigma_normal), on the current model. */
 int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out);
 
+/* ed_solve_lattice (ED_MAIN.f90:364-468): dmed_solve for nsites independent
+ * impurity models (inequivalent sites; per-site baths, Hloc and
+ * interactions in models[i]), results in outs[i] (each with its own
+ * neigen_sector / lanc_nstates_total in/out, as neigen_sectorii and
+ * neigen_totalii of the reference).  With p->batched and nsites > 1 the
+ * sites run concurrently on host workers with their own streams (each
+ * site's chains serially); otherwise one after another.  Results do not
+ * depend on the schedule.  The model set by dmed_set_model is unchanged. */
+int dmed_solve_sites(int32_t nsites, const dmed_model *models, const dmed_solve_params *p,
+                     dmed_solve_out *outs);
+
 /* Sigma = G0^-1 - G^-1 of the normal bath on arbitrary points z
  * (get_sigma_normal.f90:27-38; invg0_bath_mats, ED_BATH_FUNCTIONS.f90:1784-1807).
  * ispin, iorb: 0-based.  z, g, sigma: interleaved complex of length L. */

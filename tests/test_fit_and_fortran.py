@@ -53,8 +53,9 @@ def test_fortran_driver_and_modules_built():
     exe = os.path.join(b, "ed_hm_bethe")
     if not os.path.exists(exe):
         pytest.skip("Fortran layer not built (run __graft_entry__.build())")
-    out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
-    assert "libdmed.so" in out and "not found" not in out.split("libdmed.so")[1].split("\n")[0]
+    for e in (exe, os.path.join(b, "ed_hm_bethe_ineq")):
+        out = subprocess.run(["ldd", e], capture_output=True, text=True).stdout
+        assert "libdmed.so" in out and "not found" not in out.split("libdmed.so")[1].split("\n")[0]
     mod = open(os.path.join(b, "dmft_ed.mod"), "rb").read().decode("latin-1").lower()
     for name in ("ed_read_input", "ed_init_solver", "ed_solve", "ed_get_sigma_matsubara",
                  "ed_chi2_fitgf", "get_bath_dimension"):

@@ -528,6 +528,46 @@ def test_dens_chi_vs_oracle(name, finite):
         assert np.abs(r["dens_chi_mix_tau"]).max() > 0 and np.abs(r["dens_chi_iv"][0, 1]).max() > 0
 
 
+def test_solve_sites_matches_single_solves():
+    """ed_solve_lattice (dmed_solve_sites): three inequivalent sites (the m1
+    bath at U = 1, 2, 3.5) solved concurrently on one GPU give the same
+    results as three dmed_solve calls one after another (bit for bit: each
+    site runs the serial engine either way), site 2 matches the oracle, and
+    the context's model is left untouched."""
+    g = load("m1_norb1_nb3")
+    norb, nspin, nbath = 1, 1, int(g["nbath"])
+    kw = dict(beta=float(g["beta"]), lmats=len(g["wm"]), lreal=len(g["wr"]),
+              wini=float(g["wr"][0]), wfin=float(g["wr"][-1]), eps=float(g["eps"]),
+              lanc_ngfiter=200)
+    sites = [dict(norb=norb, nbath=nbath, nspin=nspin, bath=g["bath"], hloc_f=g["hloc_f"],
+                  uloc=[u], xmu=float(g["xmu"])) for u in (1.0, 2.0, 3.5)]
+    say("[sites] concurrent lattice solve")
+    conc = dmed.solve_sites(sites, dmed.default_params(batched=1, **kw))
+    seq = dmed.solve_sites(sites, dmed.default_params(batched=0, **kw))
+    single = []
+    for st in sites:
+        dmed.set_model(**st)
+        single.append(dmed.solve(norb, nspin, (nbath + 1) * norb, dmed.default_params(batched=0, **kw)))
+    for a, b, c in zip(conc, seq, single):
+        for k in ("gmats", "smats", "greal", "dens", "docc"):
+            assert np.array_equal(a[k], c[k]) and np.array_equal(b[k], c[k]), k
+        assert a["egs"] == c["egs"] and a["nstates"] == c["nstates"]
+    assert len({float(r["egs"]) for r in conc}) == 3
+    # the context keeps the model set before
+    dmed.set_model(**sites[0])
+    before = dmed.solve(norb, nspin, (nbath + 1) * norb, dmed.default_params(**kw))
+    dmed.solve_sites(sites[1:], dmed.default_params(batched=1, **kw))
+    after = dmed.solve(norb, nspin, (nbath + 1) * norb, dmed.default_params(**kw))
+    assert np.array_equal(before["gmats"], after["gmats"])
+    # site 2 against the oracle
+    set_both(g)
+    pr = edref.default_params(**kw)
+    o = edref.ed_solve(norb, nbath, nspin, g["bath"], g["hloc_f"], [3.5], 0.0, 0.0,
+                       float(g["xmu"]), True, params=pr)
+    assert abs(conc[2]["egs"] - o["egs"]) <= 1e-10 * abs(o["egs"])
+    assert np.abs(conc[2]["smats"][0, 0, 0, 0] - o["smats"][0, 0]).max() < 1e-8
+
+
 def test_apply_occ_vs_oracle():
     """N psi for single orbitals, pairs and all orbitals (bit-exact: the
     factor is a small integer) against the oracle's occupation vectors."""
