@@ -160,15 +160,20 @@ int model_fill(const dmed_model *m, Model &md)
     if (m->norb < 1 || m->norb > 3) return fail(DMED_EINVAL, "Norb must be 1..3 (ED_SETUP.f90:276)");
     if (m->nspin < 1 || m->nspin > 2) return fail(DMED_EINVAL, "Nspin must be 1..2 (ED_SETUP.f90:275)");
     if (m->nbath < 0) return fail(DMED_EINVAL, "Nbath < 0");
-    const int ns = (m->nbath + 1) * m->norb;
+    if (m->bath_type != DMED_BATH_NORMAL && m->bath_type != DMED_BATH_HYBRID)
+        return fail(DMED_EINVAL, "bath_type must be DMED_BATH_NORMAL or DMED_BATH_HYBRID");
+    const int hyb = m->bath_type == DMED_BATH_HYBRID;
+    const int ns = hyb ? m->nbath + m->norb : (m->nbath + 1) * m->norb;
     if (ns > kMaxNs) return fail(DMED_EINVAL, "Ns too large for 32-bit spin strings");
     md.norb = m->norb;
     md.nbath = m->nbath;
     md.nspin = m->nspin;
     md.ns = ns;
-    const int nb = m->nspin * m->norb * m->nbath;
-    md.e.assign(m->bath, m->bath + nb);
-    md.v.assign(m->bath + nb, m->bath + 2 * nb);
+    md.hybrid = hyb;
+    const int nv = m->nspin * m->norb * m->nbath;
+    const int ne = hyb ? m->nspin * m->nbath : nv;
+    md.e.assign(m->bath, m->bath + ne);
+    md.v.assign(m->bath + ne, m->bath + ne + nv);
     std::memset(md.hloc, 0, sizeof md.hloc);
     for (int s = 0; s < m->nspin; ++s)
         for (int a = 0; a < m->norb; ++a)
@@ -1055,6 +1060,8 @@ extern "C" int dmed_sigma(int32_t ispin, int32_t iorb, const double *z, const do
 {
     if (int rc = dmed_delta(ispin, iorb, z, L, sigma)) return rc;
     const Model &m = cur_model();
+    if (m.hybrid && m.norb > 1)
+        return fail(DMED_EUNSUPP, "hybrid bath with Norb > 1: Sigma is an orbital matrix (dmed_solve)");
     using cd = std::complex<double>;
     const cd *zz = reinterpret_cast<const cd *>(z);
     const cd *gg = reinterpret_cast<const cd *>(g);

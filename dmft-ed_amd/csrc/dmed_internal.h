@@ -31,10 +31,16 @@ struct Model {
     double hloc[2][3][3] = {};          // spin-diagonal real Hloc
     double uloc[3] = {}, ust = 0, jh = 0, jx = 0, jp = 0, xmu = 0;
     int hfmode = 1, jhflag = 0;
+    int hybrid = 0;                     // bath_type: levels shared by all orbitals
     int generation = 0;                 // bumped by every dmed_set_model
-    double E(int s, int a, int k) const { return e[(s * norb + a) * nbath + k]; }
+    // e: [spin][orb][k] (normal) or [spin][k] (hybrid); v: [spin][orb][k]
+    double E(int s, int a, int k) const
+    {
+        return hybrid ? e[s * nbath + k] : e[(s * norb + a) * nbath + k];
+    }
     double V(int s, int a, int k) const { return v[(s * norb + a) * nbath + k]; }
-    int bath_bit(int a, int k) const { return norb + a * nbath + k; }   // 0-based
+    // 0-based site of bath level k of orbital a (getBathStride, ED_SETUP.f90:446-455)
+    int bath_bit(int a, int k) const { return hybrid ? norb + k : norb + a * nbath + k; }
 };
 
 // one-body hop term c^+_i c_j (bit positions, i != j) with coefficient t

@@ -146,14 +146,16 @@ void sector_release(dmed_sector *s)
 }
 
 // one-body terms and onsite energies of spin `sp` (0 up, 1 down) for the
-// normal bath: HxVimp.f90:2-49, HxVbath.f90:1-14, HxVimp_bath.f90:1-38
+// normal or hybrid bath (bath_bit): HxVimp.f90:2-49, HxVbath.f90:1-14,
+// HxVimp_bath.f90:1-38
 static void spin_terms(const Model &m, int sp, std::vector<HopTerm> &terms,
                        std::vector<double> &onsite)
 {
     const int s = (m.nspin == 2) ? sp : 0;     // Nspin=1 reuses spin 1 for the down spin
     onsite.assign(m.ns, 0.0);
     for (int a = 0; a < m.norb; ++a) onsite[a] = -m.xmu + m.hloc[s][a][a];
-    for (int a = 0; a < m.norb; ++a)
+    // the hybrid bath has one set of levels (HxVbath.f90:4-10 loops over size(e,2) = 1)
+    for (int a = 0; a < (m.hybrid ? 1 : m.norb); ++a)
         for (int k = 0; k < m.nbath; ++k) onsite[m.bath_bit(a, k)] = m.E(s, a, k);
     terms.clear();
     for (int a = 0; a < m.norb; ++a)

@@ -13,6 +13,7 @@
 #include <string>
 #include <thread>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <map>
 #include <memory>
@@ -146,6 +147,37 @@ int run_tasks(int batched, int ntask, const std::function<void(int)> &fn)
     for (auto &t : th) t.join();
     for (auto &e : execs) exec_destroy(e);
     return 0;
+}
+
+// In-place inverse of a small complex n x n matrix (row-major), Gauss-Jordan
+// with partial pivoting; false if singular.
+bool invert_small(int n, std::complex<double> *A)
+{
+    using cd = std::complex<double>;
+    std::vector<cd> M(2 * (size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) M[i * 2 * n + j] = A[i * n + j];
+        M[i * 2 * n + n + i] = 1.0;
+    }
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::abs(M[r * 2 * n + c]) > std::abs(M[piv * 2 * n + c])) piv = r;
+        if (std::abs(M[piv * 2 * n + c]) == 0.0) return false;
+        if (piv != c)
+            for (int j = 0; j < 2 * n; ++j) std::swap(M[c * 2 * n + j], M[piv * 2 * n + j]);
+        const cd inv = 1.0 / M[c * 2 * n + c];
+        for (int j = 0; j < 2 * n; ++j) M[c * 2 * n + j] *= inv;
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const cd f = M[r * 2 * n + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < 2 * n; ++j) M[r * 2 * n + j] -= f * M[c * 2 * n + j];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) A[i * n + j] = M[i * 2 * n + n + j];
+    return true;
 }
 
 // Host-only parallel loop (no device context): tasks write disjoint outputs,
@@ -379,9 +411,15 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     const int ngf = std::max(p->lanc_ngfiter, 1);
     // one task per (spin, orbital, state, particle/hole) chain, in the order
     // of lanc_build_gf_normal_c (build_gf_normal.f90:83-176)
+    // kind 0: c_a psi (build_gf_normal.f90:83-176); for the hybrid bath and
+    // a < b also (lanc_build_gf_normal_mix_c, :192-420) kind 1: (c_a + c_b) psi
+    // and kind 2: the complex (c_a -/+ i c_b) psi, run as the stacked real
+    // vector [c_a psi; c_b psi] on the stacked target sector
     struct Chain {
         int s, a, state, dagger, isign;
+        int kind = 0, b = -1;
         dmed_sector *J = nullptr;
+        dmed_sector *J0 = nullptr;      // kind 2: the unstacked target sector
         int nlanc = 0, rc = 0;
         double norm2 = 0.0;
         std::vector<double> al, bl;
@@ -409,6 +447,46 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
                     chains.push_back(std::move(ch));
                 }
             }
+    const bool offdiag = m.hybrid && norb > 1;
+    std::map<int, dmed_sector *> stacked_gf;
+    for (int s = 0; s < nspin && !rc && offdiag; ++s)
+        for (int a = 0; a < norb && !rc; ++a)
+            for (int b = a + 1; b < norb && !rc; ++b)
+                for (int k = 0; k < (int)sl.items.size() && !rc; ++k) {
+                    const int nu = sl.items[k].sector / (ns + 1), nd = sl.items[k].sector % (ns + 1);
+                    // (c^+_a + c^+_b) +1, (c_a + c_b) -1, (c^+_a + i c^+_b) +1, (c_a - i c_b) -1
+                    for (int pass = 0; pass < 4 && !rc; ++pass) {
+                        const int dagger = (pass % 2 == 0) ? 1 : 0;
+                        const int ju = nu + (s == 0 ? (dagger ? 1 : -1) : 0);
+                        const int jd = nd + (s == 1 ? (dagger ? 1 : -1) : 0);
+                        if (ju < 0 || jd < 0 || ju > ns || jd > ns) continue;
+                        Chain ch;
+                        ch.s = s;
+                        ch.a = a;
+                        ch.b = b;
+                        ch.state = k;
+                        ch.dagger = dagger;
+                        ch.isign = dagger ? 1 : -1;
+                        ch.kind = pass < 2 ? 1 : 2;
+                        if ((rc = cache.get(ju, jd, ns, &ch.J0))) break;
+                        ch.J = ch.J0;
+                        if (ch.kind == 2) {
+                            const int key = ju * (ns + 1) + jd;
+                            auto it = stacked_gf.find(key);
+                            if (it == stacked_gf.end()) {
+                                auto sec = std::make_unique<dmed_sector>();
+                                if ((rc = sector_build_stacked(sec.get(), ju, jd))) {
+                                    sector_release(sec.get());
+                                    break;
+                                }
+                                it = stacked_gf.emplace(key, sec.release()).first;
+                            }
+                            ch.J = it->second;
+                        }
+                        ch.nlanc = (int)std::min<int64_t>(ch.J0->dim, ngf);
+                        chains.push_back(std::move(ch));
+                    }
+                }
     auto run_chain = [&](int t) {
         Chain &ch = chains[t];
         const State &st = sl.items[ch.state];
@@ -419,7 +497,36 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
             return;
         }
         const int isite = ch.a + 1 + ch.s * ns;
-        ch.rc = dmed_apply_op(st.vec, isite, ch.dagger, ch.J, &v, &ch.norm2);
+        if (ch.kind == 0) {
+            ch.rc = dmed_apply_op(st.vec, isite, ch.dagger, ch.J, &v, &ch.norm2);
+        } else {
+            // c_a psi and c_b psi into the two halves of a 2*D buffer
+            const int64_t d0 = ch.J0->dim;
+            void *two = nullptr;
+            if (ch.kind == 1) {
+                if ((ch.rc = exec_buf(ex(), kSlotMid, std::max<int64_t>(2 * d0, 1) * 8, &two))) {
+                    ch.err = dmed_last_error();
+                    return;
+                }
+            } else {
+                two = v.data;       // the stacked start vector itself (J->dim = 2*d0)
+            }
+            dmed_vec h1{ch.J0, DMED_FP64, d0, two};
+            dmed_vec h2{ch.J0, DMED_FP64, d0, (double *)two + d0};
+            double n1 = 0.0, n2 = 0.0;
+            ch.rc = dmed_apply_op(st.vec, isite, ch.dagger, ch.J0, &h1, &n1);
+            if (!ch.rc)
+                ch.rc = dmed_apply_op(st.vec, ch.b + 1 + ch.s * ns, ch.dagger, ch.J0, &h2, &n2);
+            if (!ch.rc && ch.kind == 1) {
+                hipStream_t cs = ex().stream;
+                ch.rc = launch_scale_copy(v.data, DMED_FP64, h1.data, DMED_FP64, d0, 1.0, cs);
+                if (!ch.rc) ch.rc = launch_axpy(v.data, DMED_FP64, h2.data, DMED_FP64, d0, 1.0, cs);
+                if (!ch.rc) ch.rc = ensure_partials(1 << 14);
+                if (!ch.rc) ch.rc = dot_host((const double *)v.data, (const double *)v.data, d0, &ch.norm2);
+            } else if (!ch.rc) {
+                ch.norm2 = n1 + n2;
+            }
+        }
         if (!ch.rc && ch.norm2 > 0.0) {
             // zero norm: the operator annihilates the state and the chain
             // carries no weight (the reference divides by 0 here)
@@ -444,6 +551,10 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
     if (!rc) rc = run_tasks(p->batched, (int)chains.size(), run_chain);
     for (auto &ch : chains)
         if (!rc && ch.rc) rc = fail(ch.rc, ch.err);
+    for (auto &kv : stacked_gf) dmed_sector_free(kv.second);
+    // full orbital matrices G[s][a][b] (hybrid bath), interleaved complex
+    std::vector<double> gfm(offdiag ? 2 * (size_t)nspin * norb * norb * lm : 0),
+        gfr(offdiag ? 2 * (size_t)nspin * norb * norb * lr : 0);
     for (int s = 0; s < nspin && !rc; ++s)
         for (int a = 0; a < norb && !rc; ++a) {
             std::fill(gm.begin(), gm.end(), 0.0);
@@ -453,7 +564,7 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
             struct Term { const Chain *ch; double peso, e; };
             std::vector<Term> terms;
             for (auto &ch : chains) {
-                if (ch.s != s || ch.a != a || !(ch.norm2 > 0.0)) continue;
+                if (ch.kind != 0 || ch.s != s || ch.a != a || !(ch.norm2 > 0.0)) continue;
                 const State &st = sl.items[ch.state];
                 double peso = 0.0;
                 if (finite_t) {
@@ -483,6 +594,12 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
                 out->greal[2 * o] = gr[2 * l];
                 out->greal[2 * o + 1] = gr[2 * l + 1];
             }
+            if (offdiag) {
+                // Sigma is an orbital matrix: after the off-diagonal elements
+                std::copy(gm.begin(), gm.end(), gfm.begin() + 2 * (size_t)lm * ((s * norb + a) * norb + a));
+                std::copy(gr.begin(), gr.end(), gfr.begin() + 2 * (size_t)lr * ((s * norb + a) * norb + a));
+                continue;
+            }
             // Sigma = G0^-1 - G^-1 on both axes
             std::vector<double> z(2 * (size_t)std::max(lm, lr)), sg(2 * (size_t)std::max(lm, lr));
             if (out->smats) {
@@ -504,6 +621,102 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
                 }
             }
         }
+    // ------------------------- hybrid bath: off-diagonal G and matrix Sigma
+    // add_to_lanczos_gf_normal (build_gf_normal.f90:442-488) of the kind-1
+    // chains with weight norm^2 and of the kind-2 chains with weight
+    // -i*norm^2, then G_ab = (G_ab - (1-i) G_aa - (1-i) G_bb)/2, G_ba = G_ab
+    // (:40-53), and Sigma = G0^-1 - G^-1 as Norb x Norb matrices
+    // (get_sigma_normal.f90:40-56) with G0^-1 = (z + xmu) 1 - Hloc - Delta,
+    // Delta_ab = sum_k V_a(k) V_b(k)/(z - e_k) (ED_BATH_FUNCTIONS.f90:1836-1848, 294-310).
+    for (int s = 0; s < nspin && !rc && offdiag; ++s) {
+        for (int a = 0; a < norb; ++a)
+            for (int b = a + 1; b < norb; ++b) {
+                std::vector<double> am(2 * (size_t)lm, 0.0), ar(2 * (size_t)lr, 0.0),
+                    tm(2 * (size_t)lm), tr(2 * (size_t)lr);
+                for (auto &ch : chains) {
+                    if (ch.kind == 0 || ch.s != s || ch.a != a || ch.b != b || !(ch.norm2 > 0.0))
+                        continue;
+                    const State &st = sl.items[ch.state];
+                    double peso = 0.0;
+                    if (finite_t) {
+                        if (beta * (st.e - egs) < 200) peso = ch.norm2 * std::exp(-beta * (st.e - egs)) / zeta;
+                    } else {
+                        peso = ch.norm2 / zeta;
+                    }
+                    if (ch.kind == 1) {
+                        gf_add_poles(peso, st.e, ch.lam.data(), ch.w2.data(), ch.nlanc, ch.isign,
+                                     wm.data(), lm, wr.data(), lr, p->eps, am.data(), ar.data());
+                        continue;
+                    }
+                    // weight -i*peso: (x + i y) += -i (tx + i ty) = ty - i tx
+                    std::fill(tm.begin(), tm.end(), 0.0);
+                    std::fill(tr.begin(), tr.end(), 0.0);
+                    gf_add_poles(peso, st.e, ch.lam.data(), ch.w2.data(), ch.nlanc, ch.isign,
+                                 wm.data(), lm, wr.data(), lr, p->eps, tm.data(), tr.data());
+                    for (int l = 0; l < lm; ++l) { am[2 * l] += tm[2 * l + 1]; am[2 * l + 1] -= tm[2 * l]; }
+                    for (int l = 0; l < lr; ++l) { ar[2 * l] += tr[2 * l + 1]; ar[2 * l + 1] -= tr[2 * l]; }
+                }
+                auto combine = [&](std::vector<double> &g, const std::vector<double> &full, int L) {
+                    const double *ga = full.data() + 2 * (size_t)L * ((s * norb + a) * norb + a);
+                    const double *gb = full.data() + 2 * (size_t)L * ((s * norb + b) * norb + b);
+                    for (int l = 0; l < L; ++l) {
+                        // (1 - i)(x + i y) = (x + y) + i (y - x)
+                        const double x = ga[2 * l] + gb[2 * l], y = ga[2 * l + 1] + gb[2 * l + 1];
+                        g[2 * l] = 0.5 * (g[2 * l] - (x + y));
+                        g[2 * l + 1] = 0.5 * (g[2 * l + 1] - (y - x));
+                    }
+                };
+                combine(am, gfm, lm);
+                combine(ar, gfr, lr);
+                for (int pr = 0; pr < 2; ++pr) {
+                    const int i = pr ? b : a, j = pr ? a : b;
+                    std::copy(am.begin(), am.end(), gfm.begin() + 2 * (size_t)lm * ((s * norb + i) * norb + j));
+                    std::copy(ar.begin(), ar.end(), gfr.begin() + 2 * (size_t)lr * ((s * norb + i) * norb + j));
+                    for (int l = 0; l < lm && out->gmats; ++l) {
+                        const size_t o = gidx(s, s, i, j, l, nspin, norb);
+                        out->gmats[2 * o] = am[2 * l];
+                        out->gmats[2 * o + 1] = am[2 * l + 1];
+                    }
+                    for (int l = 0; l < lr && out->greal; ++l) {
+                        const size_t o = gidx(s, s, i, j, l, nspin, norb);
+                        out->greal[2 * o] = ar[2 * l];
+                        out->greal[2 * o + 1] = ar[2 * l + 1];
+                    }
+                }
+            }
+        using cd = std::complex<double>;
+        const int sb = m.nspin == 2 ? s : 0;
+        auto sigma_axis = [&](const std::vector<double> &full, int L, bool mats, double *dst) {
+            std::vector<cd> G(norb * norb), S(norb * norb);
+            for (int l = 0; l < L; ++l) {
+                const cd z = mats ? cd(0.0, wm[l]) : cd(wr[l], p->eps);
+                for (int a = 0; a < norb; ++a)
+                    for (int b = 0; b < norb; ++b) {
+                        const double *g = full.data() + 2 * ((size_t)L * ((s * norb + a) * norb + b) + l);
+                        G[a * norb + b] = cd(g[0], g[1]);
+                    }
+                if (!invert_small(norb, G.data())) {
+                    rc = fail(DMED_ENOCONV, "singular G matrix in Sigma = G0^-1 - G^-1");
+                    return;
+                }
+                for (int a = 0; a < norb; ++a)
+                    for (int b = 0; b < norb; ++b) {
+                        cd d = 0.0;
+                        for (int k = 0; k < m.nbath; ++k)
+                            d += m.V(sb, a, k) * m.V(sb, b, k) / (z - m.E(sb, a, k));
+                        const cd g0 = (a == b ? z + m.xmu : cd(0.0)) - m.hloc[sb][a][b] - d;
+                        const cd sg = g0 - G[a * norb + b];
+                        if (dst) {
+                            const size_t o = gidx(s, s, a, b, l, nspin, norb);
+                            dst[2 * o] = sg.real();
+                            dst[2 * o + 1] = sg.imag();
+                        }
+                    }
+            }
+        };
+        sigma_axis(gfm, lm, true, out->smats);
+        if (!rc) sigma_axis(gfr, lr, false, out->sreal);
+    }
     // ---------------------------------------------- spin susceptibility
     // build_chi_spin (build_chi_spin.f90:4-18, ED_MAIN.f90:282): for every
     // kept state, S_z of each orbital (rows 0..Norb-1) and, when Norb > 1, of

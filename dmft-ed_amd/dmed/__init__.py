@@ -33,7 +33,7 @@ class Model(C.Structure):
                 ("bath", _dp), ("hloc", _dp), ("uloc", C.c_double * 3),
                 ("ust", C.c_double), ("jh", C.c_double), ("jx", C.c_double),
                 ("jp", C.c_double), ("xmu", C.c_double), ("hfmode", C.c_int32),
-                ("jhflag", C.c_int32)]
+                ("jhflag", C.c_int32), ("bath_type", C.c_int32), ("pad_", C.c_int32)]
 
 
 class SolveParams(C.Structure):
@@ -170,11 +170,19 @@ def finalize():
     _check(lib().dmed_finalize())
 
 
+BATH_TYPES = {"normal": 0, "hybrid": 1}
+
+
+def nsites(norb, nbath, bath_type="normal"):
+    """Ns of the model: (Nbath+1)*Norb, or Nbath+Norb for the hybrid bath."""
+    return nbath + norb if bath_type == "hybrid" else (nbath + 1) * norb
+
+
 def make_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, jp=0.0,
-               xmu=0.0, hfmode=True, jhflag=False):
+               xmu=0.0, hfmode=True, jhflag=False, bath_type="normal"):
     """A dmed_model struct and the arrays it points to (keep them alive).
-    bath: set_dmft_bath layout; hloc_f: real Hloc(Nspin,Nspin,Norb,Norb) in
-    Fortran order (flattened) or a 4-d array."""
+    bath: set_dmft_bath layout of the bath type; hloc_f: real
+    Hloc(Nspin,Nspin,Norb,Norb) in Fortran order (flattened) or a 4-d array."""
     bath = np.ascontiguousarray(bath, dtype=np.float64)
     h = np.asarray(hloc_f, dtype=np.float64)
     if h.ndim == 4:
@@ -188,17 +196,18 @@ def make_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, 
         m.uloc[i] = u[i]
     m.ust, m.jh, m.jx, m.jp, m.xmu = ust, jh, jx, jp, xmu
     m.hfmode, m.jhflag = int(bool(hfmode)), int(bool(jhflag))
+    m.bath_type = BATH_TYPES[bath_type]
     return m, (bath, h)
 
 
 def set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, jp=0.0,
-              xmu=0.0, hfmode=True, jhflag=False):
+              xmu=0.0, hfmode=True, jhflag=False, bath_type="normal"):
     """dmed_set_model; returns Ns."""
     global _keep
     m, _keep = make_model(norb, nbath, nspin, bath, hloc_f, uloc, ust, jh, jx, jp, xmu, hfmode,
-                          jhflag)
+                          jhflag, bath_type)
     _check(lib().dmed_set_model(C.byref(m)))
-    return (nbath + 1) * norb
+    return nsites(norb, nbath, bath_type)
 
 
 _keep = None
@@ -418,7 +427,8 @@ def solve_sites(sites, params):
         ms[i] = m
         keep.append(k)
         norb, nspin = kw["norb"], kw["nspin"]
-        o, r, ntot, obs = _solve_out(norb, nspin, (kw["nbath"] + 1) * norb, params)
+        ns = nsites(norb, kw["nbath"], kw.get("bath_type", "normal"))
+        o, r, ntot, obs = _solve_out(norb, nspin, ns, params)
         outs[i] = o
         res.append((r, ntot, obs, norb))
     _check(lib().dmed_solve_sites(len(sites), ms, C.byref(params), outs))

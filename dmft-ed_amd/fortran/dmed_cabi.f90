@@ -13,6 +13,7 @@ module DMED_CABI
      real(c_double)     :: uloc(3)
      real(c_double)     :: ust, jh, jx, jp, xmu
      integer(c_int32_t) :: hfmode, jhflag
+     integer(c_int32_t) :: bath_type = 0, pad_ = 0
   end type dmed_model_t
 
   type,bind(C),public :: dmed_solve_params_t

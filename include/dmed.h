@@ -65,6 +65,8 @@ const char *dmed_last_error(void);
  * pair-hopping (Jp) terms of HxVint.f90:48-97.
  * Replaces: set_dmft_bath + set_Hloc + the module globals read by
  * build_hxv.f90 (ED_HAMILTONIAN_MATVEC.f90:289-351). */
+#define DMED_BATH_NORMAL 0   /* Nbath levels per orbital, Ns = (Nbath+1)*Norb */
+#define DMED_BATH_HYBRID 1   /* Nbath levels shared by all orbitals, Ns = Nbath+Norb */
 typedef struct dmed_model {
     int32_t norb, nbath, nspin;
     const double *bath;
@@ -72,10 +74,17 @@ typedef struct dmed_model {
     double uloc[3];
     double ust, jh, jx, jp, xmu;
     int32_t hfmode, jhflag;
+    /* bath_type (ED_INPUT_VARS bath_type).  DMED_BATH_HYBRID uses the layout
+     * of dmft_aux.f90:572-590: e(ispin,1,k) at k + (ispin-1)*Nbath, then
+     * v(ispin,iorb,k) at Nspin*Nbath + k + (iorb-1)*Nbath + (ispin-1)*Norb*Nbath,
+     * length Nspin*(Nbath + Norb*Nbath); level k sits on site Norb+k and
+     * couples to every orbital (getBathStride, ED_SETUP.f90:452-455). */
+    int32_t bath_type, pad_;
 } dmed_model;
 int dmed_set_model(const dmed_model *m);
 
-/* Ns = (Nbath+1)*Norb (setup_ed_dimensions, ED_SETUP.f90:81-128) */
+/* Ns = (Nbath+1)*Norb, or Nbath+Norb for the hybrid bath
+ * (setup_ed_dimensions, ED_SETUP.f90:81-128) */
 int dmed_get_ns(int32_t *ns);
 
 /* binomial(n1,n2) with the reference's rounding (ED_SETUP.f90:1258-1275) */

@@ -56,11 +56,15 @@ static double g_v[2][3][64];     /* dmft_bath%v(ispin,iorb,k) */
 static double g_hloc[2][2][3][3];/* real(impHloc(ispin,jspin,iorb,jorb)) */
 static double g_uloc[3], g_ust, g_jh, g_jx, g_jp, g_xmu;
 static int g_hfmode = 1, g_jhflag = 0;
+static int g_hybrid = 0;          /* bath_type: 0 normal, 1 hybrid */
 
 /* bath: flat array in the set_dmft_bath layout (dmft_aux.f90:489-512):
  *   e(ispin,iorb,i) at io = i + (iorb-1)*Nbath + (ispin-1)*Nbath*Norb
  *   v(...)          at io + Nspin*Norb*Nbath
  * hloc: Fortran-ordered real array Hloc(Nspin,Nspin,Norb,Norb). */
+/* bath_type of the next ref_set_model: 0 normal, 1 hybrid (ED_INPUT_VARS bath_type). */
+void ref_set_bath_type(int hybrid) { g_hybrid = hybrid ? 1 : 0; }
+
 int ref_set_model(int norb, int nbath, int nspin, const double *bath,
                   const double *hloc_f, const double *uloc, double ust,
                   double jh, double jx, double jp, double xmu, int hfmode,
@@ -69,16 +73,30 @@ int ref_set_model(int norb, int nbath, int nspin, const double *bath,
     if (norb < 1 || norb > 3 || nspin < 1 || nspin > 2 || nbath < 0 || nbath > 64)
         return -1;
     g_norb = norb; g_nbath = nbath; g_nspin = nspin;
-    g_ns = (nbath + 1) * norb;                 /* setup_ed_dimensions, ED_SETUP.f90:86 */
+    /* setup_ed_dimensions, ED_SETUP.f90:86-88 */
+    g_ns = g_hybrid ? nbath + norb : (nbath + 1) * norb;
     if (g_ns > 30) return -2;
-    int stride = nspin * norb * nbath;
-    for (int s = 0; s < nspin; ++s)
-        for (int a = 0; a < norb; ++a)
-            for (int k = 0; k < nbath; ++k) {
-                int io = k + a * nbath + s * nbath * norb;
-                g_e[s][a][k] = bath[io];
-                g_v[s][a][k] = bath[stride + io];
-            }
+    memset(g_e, 0, sizeof g_e);
+    memset(g_v, 0, sizeof g_v);
+    if (g_hybrid) {
+        /* hybrid layout (dmft_aux.f90:572-590): e(ispin,1,i) at i + (ispin-1)*Nbath,
+         * v(ispin,iorb,i) at Nspin*Nbath + i + (iorb-1)*Nbath + (ispin-1)*Norb*Nbath */
+        for (int s = 0; s < nspin; ++s) {
+            for (int k = 0; k < nbath; ++k) g_e[s][0][k] = bath[k + s * nbath];
+            for (int a = 0; a < norb; ++a)
+                for (int k = 0; k < nbath; ++k)
+                    g_v[s][a][k] = bath[nspin * nbath + k + a * nbath + s * norb * nbath];
+        }
+    } else {
+        int stride = nspin * norb * nbath;
+        for (int s = 0; s < nspin; ++s)
+            for (int a = 0; a < norb; ++a)
+                for (int k = 0; k < nbath; ++k) {
+                    int io = k + a * nbath + s * nbath * norb;
+                    g_e[s][a][k] = bath[io];
+                    g_v[s][a][k] = bath[stride + io];
+                }
+    }
     memset(g_hloc, 0, sizeof g_hloc);
     for (int s1 = 0; s1 < nspin; ++s1)
         for (int s2 = 0; s2 < nspin; ++s2)
@@ -181,7 +199,12 @@ long long ref_binary_search(const int64_t *a, long long n, int64_t value)
 }
 
 /* getBathStride(iorb,k) = Norb + (iorb-1)*Nbath + k (1-based), ED_SETUP.f90:447-451 */
-static inline int bath_stride(int a0, int k0) { return g_norb + a0 * g_nbath + k0 + 1; }
+/* getBathStride (ED_SETUP.f90:446-455), 1-based site of bath level k0 of
+ * orbital a0: Norb + (a0)*Nbath + k0 + 1 (normal), Norb + k0 + 1 (hybrid) */
+static inline int bath_stride(int a0, int k0)
+{
+    return g_hybrid ? g_norb + k0 + 1 : g_norb + a0 * g_nbath + k0 + 1;
+}
 
 /* ----------------------------------------------------------- H * v */
 static int g_abort_flag = 0;
@@ -268,9 +291,10 @@ static void hxv_rows(const int64_t *map, long long dim, const zdouble *vin,
                     }
                 }
         }
-        /* HxVbath.f90:1-14 -- diagonal bath energies (normal bath) */
+        /* HxVbath.f90:1-14 -- diagonal bath energies; the orbital loop runs
+         * over size(dmft_bath%e,2), which is 1 for the hybrid bath */
         htmp = 0.0;
-        for (int a = 0; a < Norb; ++a)
+        for (int a = 0; a < (g_hybrid ? 1 : Norb); ++a)
             for (int k = 0; k < Nbath; ++k) {
                 int alfa = bath_stride(a, k);
                 htmp += g_e[0][a][k] * ib[alfa - 1];
@@ -681,6 +705,25 @@ void ref_delta(int ispin, int iorb, const double *z, int L, double *out)
         zdouble delta = 0;
         for (int k = 0; k < g_nbath; ++k)
             delta += g_v[ispin][iorb][k] * g_v[ispin][iorb][k] / (zz[i] - g_e[ispin][iorb][k]);
+        o[i] = delta;
+    }
+}
+
+/* delta_bath_mats, any bath: Delta_ab(z) = sum_k V_a(k) V_b(k)/(z - e(k)),
+ * e(k) of the hybrid bath shared by the orbitals (ED_BATH_FUNCTIONS.f90:
+ * 294-310), of orbital a for the normal bath (zero for a != b). */
+void ref_delta_ab(int ispin, int a, int b, const double *z, int L, double *out)
+{
+    const zdouble *zz = (const zdouble *)z;
+    zdouble *o = (zdouble *)out;
+    for (int i = 0; i < L; ++i) {
+        zdouble delta = 0;
+        if (g_hybrid)
+            for (int k = 0; k < g_nbath; ++k)
+                delta += g_v[ispin][a][k] * g_v[ispin][b][k] / (zz[i] - g_e[ispin][0][k]);
+        else if (a == b)
+            for (int k = 0; k < g_nbath; ++k)
+                delta += g_v[ispin][a][k] * g_v[ispin][a][k] / (zz[i] - g_e[ispin][a][k]);
         o[i] = delta;
     }
 }

@@ -57,6 +57,8 @@ def lib():
         L.ref_set_model.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double,
                                     C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_int, C.c_int]
+        L.ref_set_bath_type.argtypes = [C.c_int]
+        L.ref_delta_ab.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int, _dp]
         L.ref_binomial.restype = C.c_longlong
         L.ref_binomial.argtypes = [C.c_int, C.c_int]
         L.ref_sector_dim.restype = C.c_longlong
@@ -101,16 +103,20 @@ def _i(a):
 
 # ------------------------------------------------------------ wrappers
 def set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, jx=0.0, jp=0.0,
-              xmu=0.0, hfmode=True, jhflag=False):
+              xmu=0.0, hfmode=True, jhflag=False, bath_type="normal"):
+    """bath_type "normal" or "hybrid" (Nbath levels shared by all orbitals);
+    returns Ns."""
     bath = np.ascontiguousarray(bath, dtype=np.float64)
     hloc_f = np.ascontiguousarray(hloc_f, dtype=np.float64)
     u = np.zeros(3)
     u[:len(uloc)] = uloc
+    hyb = bath_type == "hybrid"
+    lib().ref_set_bath_type(int(hyb))
     rc = lib().ref_set_model(norb, nbath, nspin, _d(bath), _d(hloc_f), _d(u), ust, jh,
                              jx, jp, xmu, int(bool(hfmode)), int(bool(jhflag)))
     if rc != 0:
         raise ValueError(f"ref_set_model failed ({rc})")
-    return (nbath + 1) * norb
+    return nbath + norb if hyb else (nbath + 1) * norb
 
 
 def binomial(n1, n2):
@@ -217,6 +223,28 @@ def delta(ispin, iorb, z):
     z = np.ascontiguousarray(z, dtype=np.complex128)
     out = np.empty_like(z)
     lib().ref_delta(ispin, iorb, _d(z.view(np.float64)), len(z), _d(out.view(np.float64)))
+    return out
+
+
+def delta_ab(ispin, a, b, z):
+    """Delta_ab(z) of the current bath (delta_bath_mats, ED_BATH_FUNCTIONS.f90:221-310)."""
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.empty_like(z)
+    lib().ref_delta_ab(ispin, a, b, _d(z.view(np.float64)), len(z), _d(out.view(np.float64)))
+    return out
+
+
+def sigma_matrix(ispin, z, g, hloc, xmu):
+    """get_sigma_normal for the hybrid bath (get_sigma_normal.f90:40-56):
+    Sigma = G0^-1 - G^-1 as Norb x Norb matrices at every point, with
+    G0^-1 = (z + xmu) 1 - Hloc - Delta (ED_BATH_FUNCTIONS.f90:1836-1848).
+    g: (Norb, Norb, L); hloc: (Norb, Norb) real."""
+    norb, L = g.shape[0], g.shape[2]
+    D = np.array([[delta_ab(ispin, a, b, z) for b in range(norb)] for a in range(norb)])
+    out = np.empty((norb, norb, L), dtype=np.complex128)
+    for i in range(L):
+        g0inv = (z[i] + xmu) * np.eye(norb) - hloc - D[:, :, i]
+        out[:, :, i] = g0inv - np.linalg.inv(g[:, :, i])
     return out
 
 
@@ -476,6 +504,64 @@ def dens_chi(items, ns, norb, nspin, zeta, egs, finite_t, p, vm, tau, wr):
                 dens_chi_tot_iv=t_iv, dens_chi_tot_tau=t_tau, dens_chi_tot_w=t_w)
 
 
+def hybrid_offdiag_gf(sl, ns, norb, nspin, zeta, egs, finite_t, p, wm, wr, gm, gr):
+    """build_gf_normal for bath_type /= normal (build_gf_normal.f90:18-55):
+    for a < b, lanc_build_gf_normal_mix_c (:192-420) adds, per kept state,
+    the chains of (c^+_a + c^+_b) psi (+1) and (c_a + c_b) psi (-1) with
+    weight norm^2, then (c^+_a + i c^+_b) psi (+1) and (c_a - i c_b) psi (-1)
+    with weight -i*norm^2; then G_ab = (G_ab - (1-i) G_aa - (1-i) G_bb)/2 and
+    G_ba = G_ab.  Returns the full (Nspin, Norb, Norb, L) matrices with the
+    diagonal from gm, gr."""
+    beta = p["beta"]
+    gmf = np.zeros((nspin, norb, norb, len(wm)), dtype=np.complex128)
+    grf = np.zeros((nspin, norb, norb, len(wr)), dtype=np.complex128)
+    for s in range(nspin):
+        for a in range(norb):
+            gmf[s, a, a] = gm[s, a]
+            grf[s, a, a] = gr[s, a]
+    for s in range(nspin):
+        for a in range(norb):
+            for b in range(a + 1, norb):
+                for e, (nu, nd), vec in sl.items:
+                    mapI = build_sector(nu, nd)
+                    if finite_t:
+                        f = np.exp(-beta * (e - egs)) / zeta if beta * (e - egs) < 200 else 0.0
+                    else:
+                        f = 1.0 / zeta
+                    for k, (dagger, isign, cb) in enumerate(((True, 1, 1.0), (False, -1, 1.0),
+                                                             (True, 1, 1j), (False, -1, -1j))):
+                        ju = nu + (1 if dagger else -1) * (s == 0)
+                        jd = nd + (1 if dagger else -1) * (s == 1)
+                        if ju < 0 or jd < 0 or ju > ns or jd > ns:
+                            continue
+                        mapJ = build_sector(ju, jd)
+                        va = apply_op(mapI, vec.astype(np.complex128), mapJ, a + 1 + s * ns, dagger)
+                        vb = apply_op(mapI, vec.astype(np.complex128), mapJ, b + 1 + s * ns, dagger)
+                        v = va + cb * vb
+                        norm2 = float(np.vdot(v, v).real)
+                        if norm2 == 0.0:
+                            continue
+                        nlanc = min(len(mapJ), p["lanc_ngfiter"])
+                        al, bl, _ = lanc_tridiag(mapJ, v / np.sqrt(norm2), nlanc,
+                                                 p["chain_threshold"])
+                        if k < 2:
+                            add_to_gf(norm2 * f, e, al, bl, isign, wm, wr, p["eps"],
+                                      gmf[s, a, b], grf[s, a, b])
+                        else:
+                            tm = np.zeros(len(wm), dtype=np.complex128)
+                            tr = np.zeros(len(wr), dtype=np.complex128)
+                            add_to_gf(-norm2 * f, e, al, bl, isign, wm, wr, p["eps"], tm, tr)
+                            gmf[s, a, b] += 1j * tm
+                            grf[s, a, b] += 1j * tr
+        for a in range(norb):
+            for b in range(a + 1, norb):
+                gmf[s, a, b] = 0.5 * (gmf[s, a, b] - (1 - 1j) * gmf[s, a, a] - (1 - 1j) * gmf[s, b, b])
+                grf[s, a, b] = 0.5 * (grf[s, a, b] - (1 - 1j) * grf[s, a, a] - (1 - 1j) * grf[s, b, b])
+                gmf[s, b, a] = gmf[s, a, b]
+                grf[s, b, a] = grf[s, a, b]
+    return gmf, grf
+
+
 def _sector_eigs(smap, neigen, p, seed=1234567):
     dim = len(smap)
     lanc = not (neigen == dim or dim <= p["lanc_dim_threshold"])
@@ -495,12 +581,14 @@ def _sector_eigs(smap, neigen, p, seed=1234567):
 
 
 def ed_solve(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, xmu=0.0,
-             hfmode=True, params=None, sectors=None, jx=0.0, jp=0.0, jhflag=False):
+             hfmode=True, params=None, sectors=None, jx=0.0, jp=0.0, jhflag=False,
+             bath_type="normal"):
     """The reference's ed_solve_single sequence on the oracle (small Ns only).
     Returns a dict with egs, zeta, states, dens, docc, gmats, greal, smats, sreal
     (arrays indexed [ispin, iorb, freq])."""
     p = default_params() if params is None else params
-    ns = set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust, jh, jx, jp, xmu, hfmode, jhflag)
+    ns = set_model(norb, nbath, nspin, bath, hloc_f, uloc, ust, jh, jx, jp, xmu, hfmode, jhflag,
+                   bath_type)
     finite_t = p["lanc_nstates_total"] != 1
     nstates_sector = p["lanc_nstates_sector"]
     nstates_total = p["lanc_nstates_total"]
@@ -583,14 +671,30 @@ def ed_solve(norb, nbath, nspin, bath, hloc_f, uloc, ust=0.0, jh=0.0, xmu=0.0,
                     add_to_gf(peso, e, al, bl, isign, wm, wr, p["eps"], gm[s, a], gr[s, a])
     sm = np.zeros_like(gm)
     sr = np.zeros_like(gr)
-    for s in range(nspin):
-        for a in range(norb):
-            sm[s, a] = sigma(s, a, 1j * wm, gm[s, a])
-            sr[s, a] = sigma(s, a, wr + 1j * p["eps"], gr[s, a])
+    hybrid = bath_type == "hybrid"
+    if hybrid:
+        gmf, grf = hybrid_offdiag_gf(sl, ns, norb, nspin, zeta, egs, finite_t, p, wm, wr,
+                                     gm, gr)
+        hl = np.asarray(hloc_f, dtype=np.float64).reshape((nspin, nspin, norb, norb), order="F")
+        smf = np.zeros_like(gmf)
+        srf = np.zeros_like(grf)
+        for s in range(nspin):
+            smf[s] = sigma_matrix(s, 1j * wm, gmf[s], hl[s, s], xmu)
+            srf[s] = sigma_matrix(s, wr + 1j * p["eps"], grf[s], hl[s, s], xmu)
+            for a in range(norb):
+                sm[s, a] = smf[s, a, a]
+                sr[s, a] = srf[s, a, a]
+    else:
+        for s in range(nspin):
+            for a in range(norb):
+                sm[s, a] = sigma(s, a, 1j * wm, gm[s, a])
+                sr[s, a] = sigma(s, a, wr + 1j * p["eps"], gr[s, a])
     out = dict(egs=egs, zeta=zeta, states=[(e, sec) for e, sec, _ in sl.items],
                vectors=[v for _, _, v in sl.items], dens=dens, docc=docc,
                obs=obs_dict(acc, norb), gmats=gm,
                greal=gr, smats=sm, sreal=sr, wm=wm, wr=wr)
+    if hybrid:
+        out.update(gmats_full=gmf, greal_full=grf, smats_full=smf, sreal_full=srf)
     if p["chiflag"]:
         # build_chi_spin (build_chi_spin.f90:4-18): rows 0..Norb-1 per orbital,
         # row Norb the total when Norb > 1; weights norm^2/zeta, then a second

@@ -50,10 +50,36 @@ def fock_ops(nmodes):
     return ops
 
 
-def model_hamiltonian(mdl):
-    """Full Fock-space H (sparse) of the normal-bath Anderson model."""
+def nsites(mdl):
+    """Ns: Norb*(Nbath+1) for the normal bath, Norb+Nbath for the hybrid bath
+    (ED_SETUP.f90:55-59)."""
+    if mdl.get("bath_type", "normal") == "hybrid":
+        return mdl["nbath"] + mdl["norb"]
+    return (mdl["nbath"] + 1) * mdl["norb"]
+
+
+def bath_levels(mdl):
+    """[(site, e_up, e_dw)] and [(orbital, site, v_up, v_dw)]: a normal bath
+    has Nbath levels per orbital at sites Norb + a*Nbath + k, each coupled to
+    its orbital only; a hybrid bath has Nbath levels at sites Norb + k, each
+    coupled to every orbital (getBathStride, ED_SETUP.f90:446-455)."""
     norb, nbath, nspin = mdl["norb"], mdl["nbath"], mdl["nspin"]
-    ns = (nbath + 1) * norb
+    e, v, sd = mdl["e"], mdl["v"], nspin - 1
+    if mdl.get("bath_type", "normal") == "hybrid":
+        lev = [(norb + k, e[0][k], e[sd][k]) for k in range(nbath)]
+        hyb = [(a, norb + k, v[0][a][k], v[sd][a][k]) for a in range(norb) for k in range(nbath)]
+    else:
+        lev = [(norb + a * nbath + k, e[0][a][k], e[sd][a][k])
+               for a in range(norb) for k in range(nbath)]
+        hyb = [(a, norb + a * nbath + k, v[0][a][k], v[sd][a][k])
+               for a in range(norb) for k in range(nbath)]
+    return lev, hyb
+
+
+def model_hamiltonian(mdl):
+    """Full Fock-space H (sparse) of the Anderson model (normal or hybrid bath)."""
+    norb, nbath, nspin = mdl["norb"], mdl["nbath"], mdl["nspin"]
+    ns = nsites(mdl)
     c = fock_ops(2 * ns)
     cd = [op.T.tocsr() for op in c]
     n = [cd[p] @ c[p] for p in range(2 * ns)]
@@ -69,9 +95,6 @@ def model_hamiltonian(mdl):
 
     def dw(a):
         return a + ns
-
-    def bath(a, k):
-        return norb + a * nbath + k
 
     sd = nspin - 1
     for a in range(norb):
@@ -108,11 +131,12 @@ def model_hamiltonian(mdl):
                 H = H + Jx * (cd[up(a)] @ cd[dw(b)] @ c[dw(a)] @ c[up(b)])
                 # pair hopping   Jp c^+_{a up} c^+_{a dw} c_{b dw} c_{b up}
                 H = H + Jp * (cd[up(a)] @ cd[dw(a)] @ c[dw(b)] @ c[up(b)])
-    for a in range(norb):
-        for k in range(nbath):
-            H = H + e[0][a][k] * n[up(bath(a, k))] + e[sd][a][k] * n[dw(bath(a, k))]
-            H = H + v[0][a][k] * (cd[up(a)] @ c[up(bath(a, k))] + cd[up(bath(a, k))] @ c[up(a)])
-            H = H + v[sd][a][k] * (cd[dw(a)] @ c[dw(bath(a, k))] + cd[dw(bath(a, k))] @ c[dw(a)])
+    lev, hyb = bath_levels(mdl)
+    for m, eu, ed in lev:
+        H = H + eu * n[up(m)] + ed * n[dw(m)]
+    for a, m, vu, vd in hyb:
+        H = H + vu * (cd[up(a)] @ c[up(m)] + cd[up(m)] @ c[up(a)])
+        H = H + vd * (cd[dw(a)] @ c[dw(m)] + cd[dw(m)] @ c[dw(a)])
     return H.tocsr(), c, ns
 
 
@@ -138,7 +162,6 @@ def local_operators(mdl, c, ns):
     sd = nspin - 1
     up = lambda a: a
     dw = lambda a: a + ns
-    bath = lambda a, k: norb + a * nbath + k
     ops = {}
     ops["hloc_diag"] = sum((hloc[0][a][a] * n[up(a)] + hloc[sd][a][a] * n[dw(a)]
                             for a in range(norb)), zero)
@@ -172,11 +195,11 @@ def local_operators(mdl, c, ns):
                     se = se + cd[up(a)] @ cd[dw(b)] @ c[dw(a)] @ c[up(b)]
                     ph = ph + cd[up(a)] @ cd[dw(a)] @ c[dw(b)] @ c[up(b)]
     ops["dse"], ops["dph"] = se, ph
-    ops["bath"] = sum((e[0][a][k] * n[up(bath(a, k))] + e[sd][a][k] * n[dw(bath(a, k))]
-                       for a in range(norb) for k in range(nbath)), zero)
-    ops["hyb"] = sum((v[0][a][k] * (cd[up(a)] @ c[up(bath(a, k))] + cd[up(bath(a, k))] @ c[up(a)]) +
-                      v[sd][a][k] * (cd[dw(a)] @ c[dw(bath(a, k))] + cd[dw(bath(a, k))] @ c[dw(a)])
-                      for a in range(norb) for k in range(nbath)), zero)
+    lev, hyb = bath_levels(mdl)
+    ops["bath"] = sum((eu * n[up(m)] + ed * n[dw(m)] for m, eu, ed in lev), zero)
+    ops["hyb"] = sum((vu * (cd[up(a)] @ c[up(m)] + cd[up(m)] @ c[up(a)]) +
+                      vd * (cd[dw(a)] @ c[dw(m)] + cd[dw(m)] @ c[dw(a)])
+                      for a, m, vu, vd in hyb), zero)
     ops["n_up"] = [n[up(a)] for a in range(norb)]
     ops["n_dw"] = [n[dw(a)] for a in range(norb)]
     return ops
@@ -212,8 +235,18 @@ def sector_codes(ns, nup, ndw):
 
 
 def bath_vector(mdl):
-    """Flat bath in the set_dmft_bath layout (dmft_aux.f90:489-512)."""
+    """Flat bath in the set_dmft_bath layout (dmft_aux.f90:489-512; hybrid:
+    e(ispin,1,i) then v(ispin,iorb,i), :572-590)."""
     norb, nbath, nspin = mdl["norb"], mdl["nbath"], mdl["nspin"]
+    if mdl.get("bath_type", "normal") == "hybrid":
+        out = np.zeros(nspin * nbath + nspin * norb * nbath)
+        for s in range(nspin):
+            for k in range(nbath):
+                out[k + s * nbath] = mdl["e"][s][k]
+            for a in range(norb):
+                for k in range(nbath):
+                    out[nspin * nbath + k + a * nbath + s * norb * nbath] = mdl["v"][s][a][k]
+        return out
     out = np.zeros(2 * nspin * norb * nbath)
     st = nspin * norb * nbath
     for s in range(nspin):
@@ -256,7 +289,8 @@ def make_model(rng, norb, nbath, nspin, uloc, ust=0.0, jh=0.0, jx=0.0, jp=0.0,
 
 def lehmann_gf(H, c, ns, mdl, beta, wm, wr, eps, finite_t, gs_threshold=1e-9,
                cutoff_states=None):
-    """Exact G_{a sigma}(iw_n), G(w+i eps) and E per sector via dense eigh."""
+    """Exact G_{a sigma}(iw_n), G(w+i eps), the full orbital matrix G_{ab sigma}
+    and E per sector via dense eigh."""
     norb, nspin = mdl["norb"], mdl["nspin"]
     spectra = {}
     for nu in range(ns + 1):
@@ -280,6 +314,9 @@ def lehmann_gf(H, c, ns, mdl, beta, wm, wr, eps, finite_t, gs_threshold=1e-9,
         zeta = float(len(states))
     gm = np.zeros((nspin, norb, len(wm)), dtype=complex)
     gr = np.zeros((nspin, norb, len(wr)), dtype=complex)
+    # full orbital matrix G_ab (used for the hybrid bath)
+    gmf = np.zeros((nspin, norb, norb, len(wm)), dtype=complex)
+    grf = np.zeros((nspin, norb, norb, len(wr)), dtype=complex)
     dens = np.zeros(norb)
     docc = np.zeros(norb)
     ops = local_operators(mdl, c, ns)
@@ -301,6 +338,24 @@ def lehmann_gf(H, c, ns, mdl, beta, wm, wr, eps, finite_t, gs_threshold=1e-9,
         obs_peso.append(peso)
         obs_vals.append(state_observables(ops, norb, full))
         for s in range(nspin):
+            tgt = (nu + 1, nd) if s == 0 else (nu, nd + 1)
+            if tgt in spectra:
+                jdx, wj, Uj = spectra[tgt]
+                amp = [Uj.T @ (c[a + s * ns].T @ full)[jdx] for a in range(norb)]
+                for a in range(norb):
+                    for b in range(norb):
+                        wt = peso * amp[a] * amp[b]
+                        gmf[s, a, b] += (wt[None, :] / (1j * wm[:, None] - (wj - E)[None, :])).sum(1)
+                        grf[s, a, b] += (wt[None, :] / (wr[:, None] + 1j * eps - (wj - E)[None, :])).sum(1)
+            tgt = (nu - 1, nd) if s == 0 else (nu, nd - 1)
+            if tgt in spectra:
+                jdx, wj, Uj = spectra[tgt]
+                amp = [Uj.T @ (c[a + s * ns] @ full)[jdx] for a in range(norb)]
+                for a in range(norb):
+                    for b in range(norb):
+                        wt = peso * amp[a] * amp[b]
+                        gmf[s, a, b] += (wt[None, :] / (1j * wm[:, None] + (wj - E)[None, :])).sum(1)
+                        grf[s, a, b] += (wt[None, :] / (wr[:, None] + 1j * eps + (wj - E)[None, :])).sum(1)
             for a in range(norb):
                 p = a + s * ns
                 # particle: c^+_p |E> into sector (nu+1,nd) or (nu,nd+1)
@@ -321,7 +376,7 @@ def lehmann_gf(H, c, ns, mdl, beta, wm, wr, eps, finite_t, gs_threshold=1e-9,
                         gm[s, a] += wt / (1j * wm + (wj[k] - E))
                         gr[s, a] += wt / (wr + 1j * eps + (wj[k] - E))
     return (egs, zeta, len(states), gm, gr, dens, docc, spectra, np.array(obs_peso),
-            np.array(obs_vals))
+            np.array(obs_vals), gmf, grf)
 
 
 def write_model(name, mdl, rng, beta=50.0, lmats=64, lreal=48, eps=0.05,
@@ -329,7 +384,7 @@ def write_model(name, mdl, rng, beta=50.0, lmats=64, lreal=48, eps=0.05,
     H, c, ns = model_hamiltonian(mdl)
     wm = np.pi / beta * (2 * np.arange(1, lmats + 1) - 1)
     wr = np.linspace(-4.0, 4.0, lreal)
-    egs, zeta, nstates, gm, gr, dens, docc, spectra, opeso, ovals = lehmann_gf(
+    egs, zeta, nstates, gm, gr, dens, docc, spectra, opeso, ovals, gmf, grf = lehmann_gf(
         H, c, ns, mdl, beta, wm, wr, eps, finite_t, cutoff_states=cutoff_states)
     out = dict(norb=mdl["norb"], nbath=mdl["nbath"], nspin=mdl["nspin"],
                bath=bath_vector(mdl), hloc_f=hloc_fortran(mdl),
@@ -341,6 +396,8 @@ def write_model(name, mdl, rng, beta=50.0, lmats=64, lreal=48, eps=0.05,
                obs_peso=opeso, obs_vals=ovals)
     if cutoff_states is not None:
         out["cutoff_states"] = cutoff_states
+    if mdl.get("bath_type", "normal") == "hybrid":
+        out.update(bath_type="hybrid", gmats_full=gmf, greal_full=grf)
     # per-sector map, seeded H*x, lowest eigenvalues
     nsec = (ns + 1) ** 2
     maps, xs, hxs, evs = [], [], [], []
@@ -360,6 +417,26 @@ def write_model(name, mdl, rng, beta=50.0, lmats=64, lreal=48, eps=0.05,
                eig_offsets=eoffs, eigs=np.concatenate(evs), nsectors=nsec)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print(f"{name}: Ns={ns} sectors={nsec} egs={egs:.12f} nstates={nstates}")
+
+
+def make_hybrid_model(rng, norb, nbath, nspin, uloc, ust=0.0, jh=0.0, xmu=0.0, hfmode=1,
+                      offdiag=0.0, crystal=None):
+    """Hybrid bath: Nbath levels e[s][k] shared by all orbitals, couplings v[s][a][k]."""
+    e = rng.uniform(-1.5, 1.5, size=(nspin, nbath))
+    v = rng.uniform(0.2, 0.7, size=(nspin, norb, nbath))
+    mdl = make_model(rng, norb, 1, nspin, uloc, ust=ust, jh=jh, xmu=xmu, hfmode=hfmode,
+                     offdiag=offdiag, crystal=crystal)
+    mdl.update(nbath=nbath, e=e.tolist(), v=v.tolist(), bath_type="hybrid")
+    return mdl
+
+
+def main_hybrid():
+    """Hybrid-bath fixtures (own seed, so the fixtures above are unchanged)."""
+    rng = np.random.default_rng(20261203)
+    # two orbitals sharing Nbath=3 levels (Ns=5), Hloc off-diagonal, T=0
+    write_model("m9_hybrid_norb2_nb3", make_hybrid_model(rng, 2, 3, 1, [2.5, 2.0], ust=1.5,
+                                                         jh=0.25, offdiag=0.1,
+                                                         crystal=[-0.1, 0.1]), rng)
 
 
 def main():
@@ -389,4 +466,9 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    if "--hybrid-only" in sys.argv:
+        main_hybrid()
+    else:
+        main()
+        main_hybrid()

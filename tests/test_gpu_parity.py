@@ -39,12 +39,16 @@ def load(name):
     return dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
 
 
+def bath_type(g):
+    return str(g["bath_type"]) if "bath_type" in g else "normal"
+
+
 def set_both(g):
     """Same model in the GPU library and in the oracle."""
     norb = int(g["norb"])
     args = (norb, int(g["nbath"]), int(g["nspin"]), g["bath"], g["hloc_f"],
             list(g["uloc"][:3]), float(g["ust"]), float(g["jh"]), float(g["jx"]),
-            float(g["jp"]), float(g["xmu"]), bool(g["hfmode"]), bool(g["jhflag"]))
+            float(g["jp"]), float(g["xmu"]), bool(g["hfmode"]), bool(g["jhflag"]), bath_type(g))
     edref.set_model(*args)
     return dmed.set_model(*args)
 
@@ -566,6 +570,42 @@ def test_solve_sites_matches_single_solves():
                        float(g["xmu"]), True, params=pr)
     assert abs(conc[2]["egs"] - o["egs"]) <= 1e-10 * abs(o["egs"])
     assert np.abs(conc[2]["smats"][0, 0, 0, 0] - o["smats"][0, 0]).max() < 1e-8
+
+
+def test_hybrid_bath_solve_vs_oracle():
+    """Hybrid bath (two orbitals sharing three bath levels, off-diagonal Hloc):
+    the full orbital matrices G_ab and Sigma_ab of the solve against the
+    oracle -- G(iw) and Sigma(iw) to 1e-8 in every element, real axis in
+    relative L2 (1e-4 for G, 1e-3 for Sigma).  The off-diagonal elements come
+    from the (c_a + c_b) chains and the complex chains, which run on the
+    stacked sector.  Also the fixture's Lehmann G_ab(iw), to 1e-8."""
+    g = load("m9_hybrid_norb2_nb3")
+    set_both(g)
+    norb, nspin = int(g["norb"]), int(g["nspin"])
+    kw = dict(beta=float(g["beta"]), lmats=len(g["wm"]), lreal=len(g["wr"]),
+              wini=float(g["wr"][0]), wfin=float(g["wr"][-1]), eps=float(g["eps"]),
+              lanc_ngfiter=200)
+    say("[m9_hybrid] solve")
+    p = dmed.default_params(**kw)
+    r = dmed.solve(norb, nspin, int(g["nbath"]) + norb, p)
+    pr = edref.default_params(**kw)
+    o = edref.ed_solve(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
+                       float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]),
+                       params=pr, bath_type="hybrid")
+    def rel(x, y):
+        return np.linalg.norm(x - y) / max(np.linalg.norm(y), np.sqrt(len(y)))
+
+    assert abs(r["egs"] - o["egs"]) <= 1e-10 * abs(o["egs"])
+    assert np.allclose(r["dens"], o["dens"], atol=1e-9)
+    for s in range(nspin):
+        assert np.abs(r["gmats"][s, s] - o["gmats_full"][s]).max() < 1e-8
+        assert np.abs(r["smats"][s, s] - o["smats_full"][s]).max() < 1e-8
+        assert np.abs(r["gmats"][s, s] - g["gmats_full"][s]).max() < 1e-8
+        for a in range(norb):
+            for b in range(norb):
+                assert rel(r["greal"][s, s, a, b], o["greal_full"][s, a, b]) < 1e-4
+                assert rel(r["sreal"][s, s, a, b], o["sreal_full"][s, a, b]) < 1e-3
+    assert np.abs(r["gmats"][0, 0, 0, 1]).max() > 0.05
 
 
 def test_apply_occ_vs_oracle():

@@ -21,11 +21,15 @@ def load(name):
     return dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
 
 
+def bath_type(g):
+    return str(g["bath_type"]) if "bath_type" in g else "normal"
+
+
 def set_model_from(g):
     return edref.set_model(int(g["norb"]), int(g["nbath"]), int(g["nspin"]), g["bath"],
                            g["hloc_f"], g["uloc"], float(g["ust"]), float(g["jh"]),
                            float(g["jx"]), float(g["jp"]), float(g["xmu"]),
-                           int(g["hfmode"]), int(g["jhflag"]))
+                           int(g["hfmode"]), int(g["jhflag"]), bath_type(g))
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -191,7 +195,7 @@ def _solve_golden(g, **pkw):
     return edref.ed_solve(norb, int(g["nbath"]), int(g["nspin"]), g["bath"], g["hloc_f"],
                           g["uloc"][:norb], float(g["ust"]), float(g["jh"]),
                           float(g["xmu"]), bool(g["hfmode"]), params=p, jx=float(g["jx"]),
-                          jp=float(g["jp"]), jhflag=bool(g["jhflag"]))
+                          jp=float(g["jp"]), jhflag=bool(g["jhflag"]), bath_type=bath_type(g))
 
 
 @pytest.mark.parametrize("name", ["m1_norb1_nb3", "m2_norb2_nb1", "m3_norb2_nb2_jh", "m4_norb3_nb1",
@@ -512,3 +516,31 @@ def test_dens_chi_mix_tau0_sum_rule():
                 want += (fwd @ fwd + np.sum((gsv.T @ bwd) ** 2)) / zeta
             got = r["dens_chi_mix_tau"][a, b, 0]
             assert abs(got - want) <= 1e-8 * max(1.0, abs(want)), (a, b, got, want)
+
+
+def test_ed_solve_hybrid_matches_lehmann():
+    """Hybrid bath (Nbath levels shared by the two orbitals, Ns = Nbath+Norb,
+    off-diagonal Hloc): E0, n, d and the full orbital matrix G_ab(iw_n) of the
+    oracle's solve -- diagonal chains plus the (c_a + c_b), (c_a -/+ i c_b)
+    chains and the (1-i) combination of build_gf_normal.f90:18-55 -- against
+    the Jordan-Wigner Lehmann sum of the fixture.  The real axis carries the
+    ghost-pole error of the plain chain (see the Lehmann test above)."""
+    g = load("m9_hybrid_norb2_nb3")
+    r = _solve_golden(g, lanc_ngfiter=400)
+    assert abs(r["egs"] - g["egs"]) <= 1e-10 * abs(g["egs"])
+    assert np.allclose(r["dens"], g["dens"], atol=1e-10)
+    assert np.allclose(r["docc"], g["docc"], atol=1e-10)
+    assert np.abs(r["gmats_full"] - g["gmats_full"]).max() < 1e-8
+    assert np.abs(g["gmats_full"][0, 0, 1]).max() > 0.05
+    assert np.abs(r["greal_full"] - g["greal_full"]).max() < 2e-3
+
+
+def test_hybrid_sigma_vanishes_without_interaction():
+    """U = Ust = Jh = 0 on the hybrid bath: G is the non-interacting G0, so the
+    matrix Sigma = G0^-1 - G^-1 (get_sigma_normal.f90:40-56, with Delta_ab
+    and the off-diagonal Hloc in G0^-1) must vanish in every orbital element."""
+    g = load("m9_hybrid_norb2_nb3")
+    g.update(uloc=np.zeros(3), ust=np.array(0.0), jh=np.array(0.0))
+    r = _solve_golden(g, lanc_ngfiter=400)
+    assert np.abs(r["gmats_full"][0, 0, 1]).max() > 0.01
+    assert np.abs(r["smats_full"]).max() < 1e-8
