@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <complex>
 #include <string>
 #include <vector>
 
@@ -235,6 +236,8 @@ int ground_state(const dmed_sector *s, int prec, int maxit, double tol, double *
 // symmetric eigen-decomposition of a dense n x n matrix (column-major, in
 // place -> eigenvectors), eigenvalues ascending
 int host_syev(int n, double *a, double *w);
+// in-place inverse of a small complex n x n matrix (row-major); false if singular
+bool invert_small(int n, std::complex<double> *A);
 // tql2 of the reference restricted to the first row of Z (the pole weights)
 int tql2_first_row(int n, double *d, double *e, double *z1);
 void gf_add_poles(double peso, double ei, const double *lam, const double *w2, int n,

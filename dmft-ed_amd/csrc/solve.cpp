@@ -149,6 +149,32 @@ int run_tasks(int batched, int ntask, const std::function<void(int)> &fn)
     return 0;
 }
 
+// Host-only parallel loop (no device context): tasks write disjoint outputs,
+// so the result does not depend on the thread count.
+void host_parallel(int ntask, const std::function<void(int)> &fn)
+{
+    int nw = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    nw = std::min(nw, ntask);
+    if (nw <= 1) {
+        for (int k = 0; k < ntask; ++k) fn(k);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int w = 0; w < nw; ++w)
+        th.emplace_back([&]() {
+            for (int k; (k = next.fetch_add(1)) < ntask;) fn(k);
+        });
+    for (auto &t : th) t.join();
+}
+
+double now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
 // In-place inverse of a small complex n x n matrix (row-major), Gauss-Jordan
 // with partial pivoting; false if singular.
 bool invert_small(int n, std::complex<double> *A)
@@ -179,32 +205,6 @@ bool invert_small(int n, std::complex<double> *A)
         for (int j = 0; j < n; ++j) A[i * n + j] = M[i * 2 * n + n + j];
     return true;
 }
-
-// Host-only parallel loop (no device context): tasks write disjoint outputs,
-// so the result does not depend on the thread count.
-void host_parallel(int ntask, const std::function<void(int)> &fn)
-{
-    int nw = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-    nw = std::min(nw, ntask);
-    if (nw <= 1) {
-        for (int k = 0; k < ntask; ++k) fn(k);
-        return;
-    }
-    std::atomic<int> next(0);
-    std::vector<std::thread> th;
-    for (int w = 0; w < nw; ++w)
-        th.emplace_back([&]() {
-            for (int k; (k = next.fetch_add(1)) < ntask;) fn(k);
-        });
-    for (auto &t : th) t.join();
-}
-
-double now()
-{
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-}  // namespace
 
 }  // namespace dmed
 

@@ -131,6 +131,9 @@ SIGNATURES = [
                                    C.POINTER(SolveOut)]),
     ("dmed_sigma", C.c_int, [C.c_int32, C.c_int32, _dp, _dp, C.c_int32, _dp]),
     ("dmed_delta", C.c_int, [C.c_int32, C.c_int32, _dp, C.c_int32, _dp]),
+    ("dmed_chi2_fit_hybrid", C.c_int, [C.c_int32, C.c_int32, _dp, C.c_int32, C.c_double,
+                                       C.c_double, _dp, C.c_int32, C.c_int32, _dp, _dp,
+                                       C.c_int32, C.c_double, _dp, _i32p]),
     ("dmed_chi2_fit", C.c_int, [C.c_int32, C.c_int32, _dp, C.c_int32, C.c_double, C.c_double,
                                 C.c_double, C.c_int32, _dp, _dp, C.c_int32, C.c_double, _dp,
                                 _i32p]),
@@ -487,3 +490,21 @@ def chi2_fit(fg, beta, e, v, scheme="weiss", weight=0, xmu=0.0, hloc=0.0, niter=
                                len(fg), beta, xmu, hloc, len(e), _d(e), _d(v), niter, ftol,
                                C.byref(chi), C.byref(it)))
     return e, v, chi.value, it.value
+
+
+def chi2_fit_hybrid(fg, beta, e, v, hloc, scheme="weiss", weight=0, xmu=0.0, niter=500,
+                    ftol=1e-5):
+    """Fit one spin of a hybrid bath: fg (Norb, Norb, L) complex, e (Nbath,),
+    v (Norb, Nbath), hloc (Norb, Norb) real.  Host-only."""
+    fg = np.asfortranarray(fg, dtype=np.complex128)
+    norb, nb = v.shape
+    e = np.array(e, dtype=np.float64)
+    vv = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(norb * nb))
+    h = np.asfortranarray(hloc, dtype=np.float64)
+    chi = C.c_double()
+    it = C.c_int32()
+    _check(lib().dmed_chi2_fit_hybrid(0 if scheme == "weiss" else 1, weight,
+                                      fg.ctypes.data_as(_dp), fg.shape[2], beta, xmu,
+                                      h.ctypes.data_as(_dp), norb, nb, _d(e), _d(vv), niter, ftol,
+                                      C.byref(chi), C.byref(it)))
+    return e, vv.reshape(norb, nb), chi.value, it.value

@@ -6,6 +6,7 @@ module DMED_CABI
   private
 
   integer(c_int),parameter,public :: DMED_FP64 = 0, DMED_FP32 = 1, DMED_MIXED = 2
+  integer(c_int),parameter,public :: DMED_BATH_NORMAL = 0, DMED_BATH_HYBRID = 1
 
   type,bind(C),public :: dmed_model_t
      integer(c_int32_t) :: norb, nbath, nspin
@@ -50,6 +51,7 @@ module DMED_CABI
   public :: dmed_init, dmed_finalize, dmed_last_error, dmed_set_model, dmed_solve
   public :: dmed_solve_sites
   public :: dmed_solve_params_default, dmed_sigma, dmed_delta, dmed_chi2_fit
+  public :: dmed_chi2_fit_hybrid
   public :: c_string
 
   interface
@@ -105,6 +107,17 @@ module DMED_CABI
        real(c_double),intent(out) :: chi2
        integer(c_int32_t),intent(out) :: iter
      end function dmed_chi2_fit
+     integer(c_int) function dmed_chi2_fit_hybrid(scheme,weight,fg,L,beta,xmu,hloc,norb,nbath,&
+          e,v,niter,ftol,chi2,iter) bind(C,name="dmed_chi2_fit_hybrid")
+       import :: c_int, c_int32_t, c_double, c_double_complex
+       integer(c_int32_t),value :: scheme, weight, L, norb, nbath, niter
+       complex(c_double_complex),intent(in) :: fg(*)
+       real(c_double),value :: beta, xmu, ftol
+       real(c_double),intent(in) :: hloc(*)
+       real(c_double),intent(inout) :: e(*), v(*)
+       real(c_double),intent(out) :: chi2
+       integer(c_int32_t),intent(out) :: iter
+     end function dmed_chi2_fit_hybrid
   end interface
 
 contains

@@ -260,22 +260,44 @@ contains
     complex(8),optional,intent(in) :: Hloc_nn(:,:,:,:)
     integer,optional               :: ispin_
     integer                        :: bath_size
-    if(bath_type/='normal' .or. ed_mode/='normal')stop "get_bath_dimension: only bath_type=normal, ed_mode=normal"
-    bath_size = Norb*Nbath + Norb*Nbath
+    if(ed_mode/='normal')stop "get_bath_dimension: only ed_mode=normal"
+    select case(trim(bath_type))
+    case('normal')
+       bath_size = Norb*Nbath + Norb*Nbath
+    case('hybrid')
+       !(e [Nspin][1][Nbath] + v [Nspin][Norb][Nbath]) (ED_BATH/user_aux.f90:31-35)
+       bath_size = Nbath + Norb*Nbath
+    case default
+       stop "get_bath_dimension: only bath_type=normal or hybrid"
+    end select
     if(.not.present(ispin_))bath_size=Nspin*bath_size
   end function get_bath_dimension
 
   subroutine check_modes()
     if(ed_mode/='normal')stop "DMFT_ED: only ed_mode=normal is built on the GPU path"
-    if(bath_type/='normal')stop "DMFT_ED: only bath_type=normal is built on the GPU path"
+    if(bath_type/='normal' .and. bath_type/='hybrid')stop "DMFT_ED: only bath_type=normal or hybrid is built on the GPU path"
     if(Nspin>2)stop "ED ERROR: Nspin > 2 is currently not supported"
     if(Norb>3)stop "ED ERROR: Norb > 3 is currently not supported"
   end subroutine check_modes
 
-  !> set_dmft_bath (dmft_aux.f90:489-512): flat bath -> e(ispin,iorb,k), v(...)
+  !> set_dmft_bath (dmft_aux.f90:489-512; hybrid :572-590): flat bath ->
+  !> e(ispin,iorb,k), v(...); the hybrid bath keeps its levels in e(ispin,1,k)
   subroutine set_bath(bath)
     real(8),intent(in) :: bath(:)
     integer :: s, a, k, io, st
+    if(trim(bath_type)=='hybrid')then
+       do s=1,Nspin
+          do k=1,Nbath
+             bath_e(s,:,k) = bath(k + (s-1)*Nbath)
+          enddo
+          do a=1,Norb
+             do k=1,Nbath
+                bath_v(s,a,k) = bath(Nspin*Nbath + k + (a-1)*Nbath + (s-1)*Norb*Nbath)
+             enddo
+          enddo
+       enddo
+       return
+    endif
     st = Nspin*Norb*Nbath
     do s=1,Nspin
        do a=1,Norb
@@ -291,6 +313,19 @@ contains
   subroutine get_bath(bath)
     real(8),intent(out) :: bath(:)
     integer :: s, a, k, io, st
+    if(trim(bath_type)=='hybrid')then
+       do s=1,Nspin
+          do k=1,Nbath
+             bath(k + (s-1)*Nbath) = bath_e(s,1,k)
+          enddo
+          do a=1,Norb
+             do k=1,Nbath
+                bath(Nspin*Nbath + k + (a-1)*Nbath + (s-1)*Norb*Nbath) = bath_v(s,a,k)
+             enddo
+          enddo
+       enddo
+       return
+    endif
     st = Nspin*Norb*Nbath
     do s=1,Nspin
        do a=1,Norb
@@ -339,7 +374,15 @@ contains
        open(newunit=unit,file=trim(Hfile)//trim(ed_file_suffix)//".restart",status='old',action='read')
        read(unit,"(A)",iostat=ios)line
        do i=1,Nbath
-          read(unit,*,iostat=ios)((bath_e(s,a,i),bath_v(s,a,i),a=1,Norb),s=1,Nspin)
+          if(trim(bath_type)=='hybrid')then
+             ! (e(s,1,i), (v(s,a,i), a=1,Norb), s=1,Nspin) (dmft_aux.f90:196-205)
+             read(unit,*,iostat=ios)(bath_e(s,1,i),(bath_v(s,a,i),a=1,Norb),s=1,Nspin)
+             do s=1,Nspin
+                bath_e(s,:,i) = bath_e(s,1,i)
+             enddo
+          else
+             read(unit,*,iostat=ios)((bath_e(s,a,i),bath_v(s,a,i),a=1,Norb),s=1,Nspin)
+          endif
           if(ios/=0)exit
        enddo
        close(unit)
@@ -353,10 +396,18 @@ contains
     character(len=16) :: ext
     ext = ".restart"; if(used)ext = ".used"
     open(newunit=unit,file=trim(Hfile)//trim(ed_file_suffix)//trim(ext))
-    write(unit,"(90(A21,1X))")(("#Ek_l"//itoa(a)//"_s"//itoa(s),"Vk_l"//itoa(a)//"_s"//itoa(s),a=1,Norb),s=1,Nspin)
-    do i=1,Nbath
-       write(unit,"(90(F21.12,1X))")((bath_e(s,a,i),bath_v(s,a,i),a=1,Norb),s=1,Nspin)
-    enddo
+    if(trim(bath_type)=='hybrid')then
+       ! write_dmft_bath, hybrid (dmft_aux.f90:366-377)
+       write(unit,"(90(A21,1X))")("#Ek_s"//itoa(s),("Vk_l"//itoa(a)//"_s"//itoa(s),a=1,Norb),s=1,Nspin)
+       do i=1,Nbath
+          write(unit,"(90(F21.12,1X))")(bath_e(s,1,i),(bath_v(s,a,i),a=1,Norb),s=1,Nspin)
+       enddo
+    else
+       write(unit,"(90(A21,1X))")(("#Ek_l"//itoa(a)//"_s"//itoa(s),"Vk_l"//itoa(a)//"_s"//itoa(s),a=1,Norb),s=1,Nspin)
+       do i=1,Nbath
+          write(unit,"(90(F21.12,1X))")((bath_e(s,a,i),bath_v(s,a,i),a=1,Norb),s=1,Nspin)
+       enddo
+    endif
     close(unit)
   end subroutine save_bath
 
@@ -403,6 +454,7 @@ contains
        allocate(ed_dens_up(Norb), ed_dens_dw(Norb)); ed_dens_up = 0d0; ed_dens_dw = 0d0
        allocate(bath_e(Nspin,Norb,Nbath), bath_v(Nspin,Norb,Nbath))
        Ns = (Nbath+1)*Norb
+       if(trim(bath_type)=='hybrid')Ns = Nbath+Norb
        allocate(neigen_sector((Ns+1)**2)); neigen_sector = 0
        if(lanc_nstates_total/=1)then
           if(mod(lanc_nstates_sector,2)/=0)lanc_nstates_sector=lanc_nstates_sector+1
@@ -464,6 +516,7 @@ contains
     m%bath = c_loc(bcopy); m%hloc = c_loc(hcopy)
     m%uloc = Uloc; m%ust = Ust; m%jh = Jh; m%jx = Jx; m%jp = Jp; m%xmu = xmu
     m%hfmode = merge(1,0,HFmode); m%jhflag = merge(1,0,Jhflag)
+    m%bath_type = merge(DMED_BATH_HYBRID,DMED_BATH_NORMAL,trim(bath_type)=='hybrid')
     call ed_check(dmed_set_model(m),"dmed_set_model")
     call fill_params(p)
     allocate(dens(Norb), docc(Norb))
@@ -539,6 +592,7 @@ contains
     enddo
     ed_file_suffix = ""
     Ns = (Nbath+1)*Norb
+    if(trim(bath_type)=='hybrid')Ns = Nbath+Norb
     if(allocated(neigen_sectorii))deallocate(neigen_sectorii,neigen_totalii)
     allocate(neigen_sectorii((Ns+1)**2,Nlat), neigen_totalii(Nlat))
     neigen_sectorii = 0
@@ -587,6 +641,7 @@ contains
        if(present(Jh_ii))m(ilat)%jh = Jh_ii(ilat)
        m(ilat)%jx = Jx; m(ilat)%jp = Jp; m(ilat)%xmu = xmu
        m(ilat)%hfmode = merge(1,0,HFmode); m(ilat)%jhflag = merge(1,0,Jhflag)
+       m(ilat)%bath_type = merge(DMED_BATH_HYBRID,DMED_BATH_NORMAL,trim(bath_type)=='hybrid')
        o(ilat)%dens = c_loc(dens(1,ilat)); o(ilat)%docc = c_loc(docc(1,ilat))
        o(ilat)%gmats = c_loc(gm(1,1,1,1,1,ilat)); o(ilat)%greal = c_loc(gr(1,1,1,1,1,ilat))
        o(ilat)%smats = c_loc(sm(1,1,1,1,1,ilat)); o(ilat)%sreal = c_loc(sr(1,1,1,1,1,ilat))
@@ -684,17 +739,21 @@ contains
   !> (ED_IO/print_impSigma.f90:43)
   subroutine print_impSigma(Sm)
     complex(8),intent(in) :: Sm(Nspin,Nspin,Norb,Norb,Lmats)
-    integer :: unit, s, a, i
+    integer :: unit, s, a, b, i
     real(8) :: wm
     do s=1,Nspin
        do a=1,Norb
-          open(newunit=unit,file="impSigma_l"//itoa(a)//itoa(a)//"_s"//itoa(s)//"_iw"//&
-               trim(ed_file_suffix)//".ed")
-          do i=1,Lmats
-             wm = acos(-1d0)/beta*dble(2*i-1)
-             write(unit,"(3ES24.15)")wm,aimag(Sm(s,s,a,a,i)),dble(Sm(s,s,a,a,i))
+          do b=1,Norb
+             ! all orbital pairs for the hybrid bath (print_impSigma.f90:88-103)
+             if(b/=a .and. trim(bath_type)/='hybrid')cycle
+             open(newunit=unit,file="impSigma_l"//itoa(a)//itoa(b)//"_s"//itoa(s)//"_iw"//&
+                  trim(ed_file_suffix)//".ed")
+             do i=1,Lmats
+                wm = acos(-1d0)/beta*dble(2*i-1)
+                write(unit,"(3ES24.15)")wm,aimag(Sm(s,s,a,b,i)),dble(Sm(s,s,a,b,i))
+             enddo
+             close(unit)
           enddo
-          close(unit)
        enddo
     enddo
   end subroutine print_impSigma
@@ -816,6 +875,15 @@ contains
     if(size(fg,1)/=Nspin .or. size(fg,3)/=Norb)stop "chi2_fitgf_generic_normal: wrong fg shape"
     call set_bath(bath)
     L = min(Lfit,size(fg,5))
+    if(trim(bath_type)=='hybrid')then
+       call fit_hybrid_spin(fg,ispin_,L)
+       if(Nspin==2 .and. .not.present(ispin))then
+          bath_e(2,:,:) = bath_e(1,:,:); bath_v(2,:,:) = bath_v(1,:,:)
+       endif
+       call save_bath(used=.false.)
+       call get_bath(bath)
+       return
+    endif
     allocate(e(Nbath), v(Nbath), g(L))
     scheme = 0; if(trim(cg_scheme)=='delta')scheme = 1
     do a=1,Norb
@@ -833,6 +901,34 @@ contains
     call save_bath(used=.false.)
     call get_bath(bath)
   end subroutine ed_chi2_fitgf_single
+
+  !> chi2_fitgf_hybrid_normal (fitgf_hybrid_normal.f90:18-156): all orbital
+  !> pairs a <= b of spin ispin at once, through dmed_chi2_fit_hybrid
+  subroutine fit_hybrid_spin(fg,ispin,L)
+    complex(8),dimension(:,:,:,:,:),intent(in) :: fg
+    integer,intent(in)                         :: ispin, L
+    complex(8),allocatable :: g(:,:,:)
+    real(8),allocatable    :: e(:), v(:), h(:,:)
+    real(8)                :: chi2
+    integer(c_int32_t)     :: iter, scheme
+    integer                :: a
+    allocate(g(Norb,Norb,L), e(Nbath), v(Norb*Nbath), h(Norb,Norb))
+    g = fg(ispin,ispin,:,:,1:L)
+    h = impHloc(ispin,ispin,:,:)
+    e = bath_e(ispin,1,:)
+    do a=1,Norb
+       v((a-1)*Nbath+1:a*Nbath) = bath_v(ispin,a,:)
+    enddo
+    scheme = 0; if(trim(cg_scheme)=='delta')scheme = 1
+    call ed_check(dmed_chi2_fit_hybrid(scheme,int(cg_weight,c_int32_t),g,int(L,c_int32_t),beta,xmu,&
+         h,int(Norb,c_int32_t),int(Nbath,c_int32_t),e,v,int(cg_niter,c_int32_t),cg_ftol,chi2,iter),&
+         "dmed_chi2_fit_hybrid")
+    do a=1,Norb
+       bath_e(ispin,a,:) = e
+       bath_v(ispin,a,:) = v((a-1)*Nbath+1:a*Nbath)
+    enddo
+    write(LOGfile,"(A,ES18.9,A,I5,A)")"chi^2|iter= ",chi2," | ",iter,"  <--  all Orbs, Spin"//itoa(ispin)
+  end subroutine fit_hybrid_spin
 
   !> lattice getters (ED_IO/get_sigma_matsubara.f90:34-38 and the *_lattice
   !> forms of get_dens, get_docc, get_mag, get_eimp, get_doubles)

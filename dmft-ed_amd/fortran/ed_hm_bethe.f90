@@ -39,6 +39,8 @@ program ed_hm_bethe
 
   allocate(Weiss(Nspin,Nspin,Norb,Norb,Lmats), Weiss_old(Nspin,Nspin,Norb,Norb,Lmats))
   allocate(Smats(Nspin,Nspin,Norb,Norb,Lmats), Gloc(Nspin,Nspin,Norb,Norb,Lmats))
+  ! orbital-diagonal self-consistency: the off-diagonal Weiss elements stay 0
+  Weiss = (0d0,0d0); Gloc = (0d0,0d0)
   allocate(Hloc(Nspin,Nspin,Norb,Norb)); Hloc = (0d0,0d0)
   Nb = get_bath_dimension()
   allocate(bath(Nb), bath_(Nb))

@@ -42,6 +42,7 @@ program ed_hm_bethe_ineq
 
   allocate(Weiss(Nlat,Nspin,Nspin,Norb,Norb,Lmats), Smats(Nlat,Nspin,Nspin,Norb,Norb,Lmats))
   allocate(Gloc(Nlat,Nspin,Nspin,Norb,Norb,Lmats))
+  Weiss = (0d0,0d0); Gloc = (0d0,0d0)
   allocate(Hloc(Nlat,Nspin,Nspin,Norb,Norb)); Hloc = (0d0,0d0)
   Nb = get_bath_dimension()
   allocate(bath(Nlat,Nb), bath_(Nlat,Nb))

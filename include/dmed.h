@@ -337,6 +337,19 @@ int dmed_sigma(int32_t ispin, int32_t iorb, const double *z, const double *g,
  * ispin, iorb 0-based */
 int dmed_delta(int32_t ispin, int32_t iorb, const double *z, int32_t L, double *delta);
 
+/* chi^2 fit of one spin of a hybrid bath (e_k; V_ak for all orbitals), in
+ * place, to the orbital matrix fg(Norb,Norb,L) (Fortran order, complex) on
+ * the first L Matsubara points, summed over a <= b: scheme 0 fits
+ * G0 = [(iw + xmu) 1 - Hloc - Delta]^-1, 1 fits Delta, with
+ * Delta_ab = sum_k V_ak V_bk/(iw - e_k); hloc(Norb,Norb) real, Fortran order;
+ * e[Nbath], v[k + a*Nbath]; weights as dmed_chi2_fit.  Host-only.  Replaces
+ * chi2_fitgf_hybrid_normal (ED_CHI2FIT/fitgf_hybrid_normal.f90:18-156) with
+ * the same minimiser as dmed_chi2_fit (parity of the optimiser unpinned). */
+int dmed_chi2_fit_hybrid(int32_t scheme, int32_t weight, const double *fg, int32_t L,
+                         double beta, double xmu, const double *hloc, int32_t norb,
+                         int32_t nbath, double *e, double *v, int32_t niter, double ftol,
+                         double *chi2, int32_t *iter);
+
 /* chi^2 fit of one orbital's normal bath (e_k, V_k), in place, to fg on the
  * first L Matsubara points w_n = pi/beta*(2n-1): scheme 0 fits
  * G0 = 1/(iw + xmu - hloc_aa - Delta) ("weiss"), 1 fits Delta ("delta");

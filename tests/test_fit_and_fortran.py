@@ -1,4 +1,5 @@
-"""CPU tests of the bath fit (dmed_chi2_fit) and of the Fortran layer build.
+"""CPU tests of the bath fits (dmed_chi2_fit, dmed_chi2_fit_hybrid) and of the
+Fortran layer build.
 
 The reference's minimiser (SciFortran fmin_cg) is not available, so the fit's
 parity is unpinned; these tests pin what the mathematics fixes: a bath whose
@@ -36,6 +37,43 @@ def test_fit_recovers_exact_bath(scheme, weight):
                                   xmu=xmu, hloc=hloc, niter=2000, ftol=1e-14)
     fit = delta(1j * wm, e, v)
     fit = fit if scheme == "delta" else 1.0 / (1j * wm + xmu - hloc - fit)
+    assert np.abs(fit - fg).max() < 2e-4, (chi, it)
+    assert chi < 1e-6
+
+
+def hybrid_target(z, e, v, hloc, xmu, scheme):
+    """Delta_ab(z) = sum_k V_ak V_bk/(z - e_k), or G0 = [(z + xmu) 1 - Hloc - Delta]^-1."""
+    D = np.einsum("ak,bk,nk->abn", v, v, 1.0 / (z[:, None] - e[None, :]))
+    if scheme == "delta":
+        return D
+    no = v.shape[0]
+    out = np.empty_like(D)
+    for n in range(len(z)):
+        out[:, :, n] = np.linalg.inv((z[n] + xmu) * np.eye(no) - hloc - D[:, :, n])
+    return out
+
+
+@pytest.mark.parametrize("scheme", ["delta", "weiss"])
+@pytest.mark.parametrize("weight", [0, 3])
+def test_hybrid_fit_recovers_exact_bath(scheme, weight):
+    """dmed_chi2_fit_hybrid (chi2_fitgf_hybrid_normal): two orbitals sharing
+    four levels, off-diagonal Hloc; the target is the bath's own Delta or G0
+    matrix, the start a perturbed bath.  The fit must return to chi^2 ~ 0 and
+    reproduce every orbital element, which needs the analytic gradient of
+    both schemes (including G0 dDelta G0) to be right."""
+    beta, L = 50.0, 300
+    wm = np.pi / beta * (2 * np.arange(1, L + 1) - 1)
+    e0 = np.array([-1.0, -0.3, 0.35, 1.2])
+    v0 = np.array([[0.4, 0.25, 0.3, 0.5], [0.2, 0.45, 0.35, 0.3]])
+    hloc = np.array([[-0.1, 0.08], [0.08, 0.05]])
+    xmu = 0.1
+    fg = hybrid_target(1j * wm, e0, v0, hloc, xmu, scheme)
+    rng = np.random.default_rng(10 + weight)
+    e_start = e0 + 0.05 * rng.standard_normal(4)
+    v_start = v0 * (1 + 0.05 * rng.standard_normal(v0.shape))
+    e, v, chi, it = dmed.chi2_fit_hybrid(fg, beta, e_start, v_start, hloc, scheme=scheme,
+                                         weight=weight, xmu=xmu, niter=3000, ftol=1e-14)
+    fit = hybrid_target(1j * wm, e, v, hloc, xmu, scheme)
     assert np.abs(fit - fg).max() < 2e-4, (chi, it)
     assert chi < 1e-6
 

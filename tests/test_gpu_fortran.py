@@ -148,3 +148,60 @@ def test_bethe_ineq_driver_one_iteration(tmp_path):
         assert np.abs(s[:, 2] + 1j * s[:, 1] - o["smats"][0, 0]).max() <= 1e-8
         new = _read_restart(str(tmp_path / f"hamiltonian_ineq{i + 1:04d}.restart"), nb)
         assert np.all(np.isfinite(new)) and not np.array_equal(new, baths[i])
+
+
+def _write_hybrid_restart(path, e, v):
+    """hamiltonian.restart of a hybrid bath, one spin (dmft_aux.f90:366-377):
+    header, then per level e_k and V_ak for a = 1..Norb."""
+    norb, nb = v.shape
+    with open(path, "w") as f:
+        f.write(" ".join(["#Ek_s1"] + [f"Vk_l{a + 1}_s1" for a in range(norb)]) + "\n")
+        for k in range(nb):
+            f.write(" ".join(f"{x:21.12f}" for x in [e[k]] + [v[a, k] for a in range(norb)]) + "\n")
+
+
+def _read_hybrid_restart(path, norb):
+    rows = np.array([list(map(float, l.split())) for l in open(path).read().splitlines()[1:]
+                     if l.strip()])
+    return rows[:, 0], rows[:, 1:1 + norb].T
+
+
+def test_bethe_driver_hybrid_bath(tmp_path):
+    """BATH_TYPE=hybrid through the Fortran layer: two orbitals sharing three
+    bath levels read from a hybrid-format hamiltonian.restart; the solve's
+    E0, n, d and all four Sigma_ab(iw) files (off-diagonal included) against
+    the oracle's hybrid solve, and the hybrid fit writes a new bath."""
+    assert os.path.exists(EXE), "Fortran driver not built (__graft_entry__.build())"
+    g = dict(np.load(os.path.join(ROOT, "tests", "golden", "m9_hybrid_norb2_nb3.npz"),
+                     allow_pickle=False))
+    norb, nb = 2, 3
+    e = g["bath"][:nb]
+    v = g["bath"][nb:].reshape(norb, nb)
+    lmats = 256
+    (tmp_path / "inputED.in").write_text("\n".join([
+        "NORB=2", "NBATH=3", "NSPIN=1", "BATH_TYPE=hybrid", "ULOC=2.5,2.0,0.0", "UST=1.5",
+        "JH=0.25", "XMU=0.0", "BETA=50.0", f"LMATS={lmats}", "LREAL=64", "NLOOP=1",
+        "LANC_NGFITER=200", "LANC_NSTATES_TOTAL=1", "HFMODE=T", "CG_SCHEME=weiss", ""]))
+    _write_hybrid_restart(str(tmp_path / "hamiltonian.restart"), e, v)
+    print("[fortran driver] hybrid bath, one iteration", flush=True)
+    r = subprocess.run([EXE, "inputED.in"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    eu, vu = _read_hybrid_restart(str(tmp_path / "hamiltonian.used"), norb)
+    assert np.allclose(eu, e, atol=1e-11) and np.allclose(vu, v, atol=1e-11)
+    bath = np.concatenate([eu, vu.reshape(-1)])
+    summ = {l.split()[0]: [float(x) for x in l.split()[1:]]
+            for l in open(tmp_path / "dmft_summary.ed") if l.strip()}
+    p = edref.default_params(beta=50.0, lmats=lmats, lreal=64, lanc_ngfiter=200)
+    o = edref.ed_solve(norb, nb, 1, bath, np.zeros(4), [2.5, 2.0], 1.5, 0.25, 0.0, True,
+                       params=p, bath_type="hybrid")
+    assert abs(summ["egs"][0] - o["egs"]) <= 1e-10 * abs(o["egs"])
+    assert np.allclose(summ["dens"][:norb], o["dens"], atol=1e-9)
+    assert np.allclose(summ["docc"][:norb], o["docc"], atol=1e-9)
+    for a in range(norb):
+        for b in range(norb):
+            s = np.loadtxt(tmp_path / f"impSigma_l{a + 1}{b + 1}_s1_iw.ed")
+            assert np.abs(s[:, 2] + 1j * s[:, 1] - o["smats_full"][0, a, b]).max() <= 1e-8, (a, b)
+    assert np.abs(o["smats_full"][0, 0, 1]).max() > 1e-3
+    en, vn = _read_hybrid_restart(str(tmp_path / "hamiltonian.restart"), norb)
+    assert np.all(np.isfinite(vn)) and not np.allclose(vn, v)
