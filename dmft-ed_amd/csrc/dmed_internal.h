@@ -197,6 +197,12 @@ struct ChainScalars {      // device-resident (inside Context::dscal)
     int32_t pad;
 };
 
+// Lanczos-step graph capture on the calling thread (default on).  Workers
+// that share the device with threads allocating, freeing or building sectors
+// turn it off: those calls fail or break a capture in flight on another
+// thread.
+void set_thread_graphs(bool on);
+
 // mode kLanczos: y = sx*H*x with the block partials of <x, y> (cs->sx)
 int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
                ChainScalars *cs, double *partials, int *nparts, hipStream_t st);

@@ -1066,10 +1066,14 @@ extern "C" int dmed_solve_sites(int32_t nsites, const dmed_model *models,
     if (conc) q.batched = 0;
     std::vector<int> rcs(nsites, 0);
     std::vector<std::string> errs(nsites);
+    // concurrent sites build sectors and allocate while others run chains:
+    // their chains launch step by step (no stream capture)
     int rc = run_tasks(conc ? 1 : 0, nsites, [&](int i) {
         bind_model(&ms[i]);
+        if (conc) set_thread_graphs(false);
         rcs[i] = dmed_solve(&q, &outs[i]);
         if (rcs[i]) errs[i] = dmed_last_error();
+        set_thread_graphs(true);
         bind_model(nullptr);
     });
     if (rc) return rc;
