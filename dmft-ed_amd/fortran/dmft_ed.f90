@@ -135,6 +135,8 @@ module DMFT_ED
   public :: ed_get_eimp, ed_get_epot, ed_get_eint, ed_get_ehartree, ed_get_eknot
   public :: ed_get_doubles, ed_get_dust, ed_get_dund, ed_get_dse, ed_get_dph
   public :: ed_chi2_fitgf
+  public :: ed_print_impSigma, ed_print_impG, ed_print_impG0, ed_print_impChi
+  public :: ed_read_impSigma
 
 contains
 
@@ -564,7 +566,9 @@ contains
        write(LOGfile,"(A,10F18.12)")"dens=",ed_dens
        write(LOGfile,"(A,10F18.12)")"docc=",ed_docc
     endif
-    if(ed_print_Sigma)call print_impSigma(impSmats)
+    if(ed_print_Sigma)call ed_print_impSigma()
+    if(ed_print_G)call ed_print_impG()
+    if(ed_print_G0)call ed_print_impG0()
     if(chiflag)then
        call print_chi_spin()
        call print_chi_dens()
@@ -668,8 +672,11 @@ contains
        ddii(ilat,:) = [obs(ilat)%dust,obs(ilat)%dund,obs(ilat)%dse,obs(ilat)%dph]
        ed_file_suffix = site_suffix(ilat)
        call set_bath(bath(ilat,:))
+       call set_hloc(Hloc(ilat,:,:,:,:))
        call save_bath(used=.true.)
-       if(ed_print_Sigma)call print_impSigma(sm(:,:,:,:,:,ilat))
+       if(ed_print_Sigma)call write_gf_files("impSigma",sm(:,:,:,:,:,ilat),sr(:,:,:,:,:,ilat))
+       if(ed_print_G)call write_gf_files("impG",gm(:,:,:,:,:,ilat),gr(:,:,:,:,:,ilat))
+       if(ed_print_G0)call ed_print_impG0()
        if(ed_verbose>=1)write(LOGfile,"(A,I4,A,F20.12,A,I4)")"site",ilat,"  Egs =",o(ilat)%egs,&
             "  states:",o(ilat)%nstates
     enddo
@@ -735,28 +742,177 @@ contains
     call write_chi_files("densChi_tot",densChi_tot_tau,densChi_tot_w,densChi_tot_iv)
   end subroutine print_chi_dens
 
-  !> impSigma_l<a><b>_s<s>_iw<suffix>.ed : columns wm, Im Sigma, Re Sigma
-  !> (ED_IO/print_impSigma.f90:43)
-  subroutine print_impSigma(Sm)
-    complex(8),intent(in) :: Sm(Nspin,Nspin,Norb,Norb,Lmats)
+  !> <stem>_l<a><b>_s<s>_iw<suffix>.ed and _realw: columns x, Im F, Re F
+  !> (splot), for a = b, or a <= b with the hybrid bath
+  !> (ED_IO/print_impSigma.f90:1-50, print_impG.f90:1-51)
+  subroutine write_gf_files(stem,Fm,Fr)
+    character(len=*),intent(in) :: stem
+    complex(8),intent(in)       :: Fm(Nspin,Nspin,Norb,Norb,Lmats), Fr(Nspin,Nspin,Norb,Norb,Lreal)
     integer :: unit, s, a, b, i
-    real(8) :: wm
+    real(8) :: x
+    character(len=:),allocatable :: sfx
     do s=1,Nspin
        do a=1,Norb
-          do b=1,Norb
-             ! all orbital pairs for the hybrid bath (print_impSigma.f90:88-103)
+          do b=a,Norb
              if(b/=a .and. trim(bath_type)/='hybrid')cycle
-             open(newunit=unit,file="impSigma_l"//itoa(a)//itoa(b)//"_s"//itoa(s)//"_iw"//&
-                  trim(ed_file_suffix)//".ed")
+             sfx = "_l"//itoa(a)//itoa(b)//"_s"//itoa(s)
+             open(newunit=unit,file=stem//sfx//"_iw"//trim(ed_file_suffix)//".ed")
              do i=1,Lmats
-                wm = acos(-1d0)/beta*dble(2*i-1)
-                write(unit,"(3ES24.15)")wm,aimag(Sm(s,s,a,b,i)),dble(Sm(s,s,a,b,i))
+                x = acos(-1d0)/beta*dble(2*i-1)
+                write(unit,"(3ES24.15)")x,aimag(Fm(s,s,a,b,i)),dble(Fm(s,s,a,b,i))
+             enddo
+             close(unit)
+             open(newunit=unit,file=stem//sfx//"_realw"//trim(ed_file_suffix)//".ed")
+             do i=1,Lreal
+                x = wini
+                if(Lreal>1)x = wini + (wfin-wini)*dble(i-1)/dble(Lreal-1)
+                write(unit,"(3ES24.15)")x,aimag(Fr(s,s,a,b,i)),dble(Fr(s,s,a,b,i))
              enddo
              close(unit)
           enddo
        enddo
     enddo
-  end subroutine print_impSigma
+  end subroutine write_gf_files
+
+  !> g0and_bath_mats / _real of the current bath and Hloc: [(z + xmu) - Hloc -
+  !> Delta]^-1, orbital-diagonal for the normal bath, the full matrix for the
+  !> hybrid bath (ED_BATH_FUNCTIONS.f90:1784-1848; Delta :221-310)
+  subroutine compute_g0(G0m,G0r)
+    complex(8),intent(out) :: G0m(Nspin,Nspin,Norb,Norb,Lmats), G0r(Nspin,Nspin,Norb,Norb,Lreal)
+    integer :: s, i
+    complex(8) :: z
+    G0m = (0d0,0d0); G0r = (0d0,0d0)
+    do s=1,Nspin
+       do i=1,Lmats
+          z = cmplx(0d0,acos(-1d0)/beta*dble(2*i-1),8)
+          call g0_at(s,z,G0m(s,s,:,:,i))
+       enddo
+       do i=1,Lreal
+          z = cmplx(wini,eps,8)
+          if(Lreal>1)z = cmplx(wini + (wfin-wini)*dble(i-1)/dble(Lreal-1),eps,8)
+          call g0_at(s,z,G0r(s,s,:,:,i))
+       enddo
+    enddo
+  end subroutine compute_g0
+
+  subroutine g0_at(s,z,g)
+    integer,intent(in)     :: s
+    complex(8),intent(in)  :: z
+    complex(8),intent(out) :: g(Norb,Norb)
+    complex(8) :: m(Norb,Norb)
+    integer :: a, b, k
+    m = (0d0,0d0)
+    do a=1,Norb
+       m(a,a) = z + xmu
+       do b=1,Norb
+          m(a,b) = m(a,b) - impHloc(s,s,a,b)
+          if(trim(bath_type)=='hybrid')then
+             do k=1,Nbath
+                m(a,b) = m(a,b) - bath_v(s,a,k)*bath_v(s,b,k)/(z - bath_e(s,1,k))
+             enddo
+          elseif(a==b)then
+             do k=1,Nbath
+                m(a,a) = m(a,a) - bath_v(s,a,k)**2/(z - bath_e(s,a,k))
+             enddo
+          endif
+       enddo
+    enddo
+    g = (0d0,0d0)
+    if(trim(bath_type)=='hybrid')then
+       call invert_cmat(m,g)
+    else
+       do a=1,Norb
+          g(a,a) = 1d0/m(a,a)
+       enddo
+    endif
+  end subroutine g0_at
+
+  !> inverse of a small complex matrix (Gauss-Jordan with partial pivoting)
+  subroutine invert_cmat(m,inv)
+    complex(8),intent(in)  :: m(:,:)
+    complex(8),intent(out) :: inv(size(m,1),size(m,1))
+    complex(8) :: w(size(m,1),2*size(m,1)), f, row(2*size(m,1))
+    integer :: n, c, r, p
+    n = size(m,1)
+    w = (0d0,0d0)
+    w(:,1:n) = m
+    do r=1,n
+       w(r,n+r) = (1d0,0d0)
+    enddo
+    do c=1,n
+       p = c
+       do r=c+1,n
+          if(abs(w(r,c))>abs(w(p,c)))p = r
+       enddo
+       if(abs(w(p,c))==0d0)stop "DMFT_ED: singular matrix in G0"
+       if(p/=c)then
+          row = w(c,:); w(c,:) = w(p,:); w(p,:) = row
+       endif
+       w(c,:) = w(c,:)/w(c,c)
+       do r=1,n
+          if(r==c)cycle
+          f = w(r,c)
+          w(r,:) = w(r,:) - f*w(c,:)
+       enddo
+    enddo
+    inv = w(:,n+1:2*n)
+  end subroutine invert_cmat
+
+  !> ed_print_impSigma, ed_print_impG, ed_print_impG0 (ED_IO.f90:266-297;
+  !> ED_GREENS_FUNCTIONS.f90:161-165).  As in the reference, impG0 is written
+  !> to the impG_* file names (print_impG0.f90:43-44), so with ED_PRINT_G0=T
+  !> those files hold G0 after a solve.
+  subroutine ed_print_impSigma()
+    call write_gf_files("impSigma",impSmats,impSreal)
+  end subroutine ed_print_impSigma
+
+  subroutine ed_print_impG()
+    call write_gf_files("impG",impGmats,impGreal)
+  end subroutine ed_print_impG
+
+  subroutine ed_print_impG0()
+    complex(8),allocatable :: G0m(:,:,:,:,:), G0r(:,:,:,:,:)
+    allocate(G0m(Nspin,Nspin,Norb,Norb,Lmats), G0r(Nspin,Nspin,Norb,Norb,Lreal))
+    call compute_g0(G0m,G0r)
+    call write_gf_files("impG",G0m,G0r)
+  end subroutine ed_print_impG0
+
+  !> ed_print_impChi (ED_IO.f90:299-305): the spin and charge files
+  subroutine ed_print_impChi()
+    if(allocated(spinChi_tau))call print_chi_spin()
+    if(allocated(densChi_tau))call print_chi_dens()
+  end subroutine ed_print_impChi
+
+  !> ed_read_impSigma (ED_IO/read_impSigma.f90:1-50): impSmats / impSreal from
+  !> the impSigma_*_iw / _realw files written by ed_print_impSigma
+  subroutine ed_read_impSigma()
+    integer :: unit, s, a, b, i, ios
+    real(8) :: x, im, re
+    character(len=:),allocatable :: sfx
+    if(.not.allocated(impSmats))stop "ed_read_impSigma: call ed_init_solver first"
+    do s=1,Nspin
+       do a=1,Norb
+          do b=a,Norb
+             if(b/=a .and. trim(bath_type)/='hybrid')cycle
+             sfx = "_l"//itoa(a)//itoa(b)//"_s"//itoa(s)
+             open(newunit=unit,file="impSigma"//sfx//"_iw"//trim(ed_file_suffix)//".ed",status='old',action='read')
+             do i=1,Lmats
+                read(unit,*,iostat=ios)x,im,re
+                if(ios/=0)exit
+                impSmats(s,s,a,b,i) = cmplx(re,im,8)
+             enddo
+             close(unit)
+             open(newunit=unit,file="impSigma"//sfx//"_realw"//trim(ed_file_suffix)//".ed",status='old',action='read')
+             do i=1,Lreal
+                read(unit,*,iostat=ios)x,im,re
+                if(ios/=0)exit
+                impSreal(s,s,a,b,i) = cmplx(re,im,8)
+             enddo
+             close(unit)
+          enddo
+       enddo
+    enddo
+  end subroutine ed_read_impSigma
 
   !------------------------------------------------------------ getters
   subroutine ed_get_sigma_matsubara_1(Smats)

@@ -74,6 +74,18 @@ def test_bethe_driver_one_iteration(tmp_path):
     zeta = 1j * wm - o["smats"][0, 0]
     gl = (wte[None, :] / (zeta[:, None] - e[None, :])).sum(1)
     assert np.abs(g[:, 2] + 1j * g[:, 1] - gl).max() <= 1e-8
+    # Sigma on the real axis; impG_* files hold G0 = 1/(z + xmu - Delta) (the
+    # reference prints G0 under the impG names after G, print_impG0.f90:43-44)
+    wr = np.linspace(-5.0, 5.0, 100)
+    sr = np.loadtxt(tmp_path / "impSigma_l11_s1_realw.ed")
+    assert np.allclose(sr[:, 0], wr, rtol=0, atol=1e-12)
+    x, y = sr[:, 2] + 1j * sr[:, 1], o["sreal"][0, 0]
+    assert np.linalg.norm(x - y) / max(np.linalg.norm(y), np.sqrt(len(y))) < 1e-3
+    e_b, v_b = bath[:c["nbath"]], bath[c["nbath"]:]
+    for name, z in (("impG_l11_s1_iw.ed", 1j * wm), ("impG_l11_s1_realw.ed", wr + 0.01j)):
+        gf = np.loadtxt(tmp_path / name)
+        g0 = 1.0 / (z - (v_b[None, :] ** 2 / (z[:, None] - e_b[None, :])).sum(1))
+        assert np.abs(gf[:, 2] + 1j * gf[:, 1] - g0).max() <= 1e-12 * np.abs(g0).max() + 1e-13, name
     # CHIFLAG: spinChi_l1_{iw,tau}.ed (columns v, Im, Re / tau, chi) against the oracle
     ci = np.loadtxt(tmp_path / "spinChi_l1_iw.ed")
     assert np.allclose(ci[:, 0], o["vm"], rtol=1e-14, atol=0)
@@ -169,7 +181,7 @@ def _read_hybrid_restart(path, norb):
 def test_bethe_driver_hybrid_bath(tmp_path):
     """BATH_TYPE=hybrid through the Fortran layer: two orbitals sharing three
     bath levels read from a hybrid-format hamiltonian.restart; the solve's
-    E0, n, d and all four Sigma_ab(iw) files (off-diagonal included) against
+    E0, n, d and the Sigma_ab(iw) files of a <= b (off-diagonal included) against
     the oracle's hybrid solve, and the hybrid fit writes a new bath."""
     assert os.path.exists(EXE), "Fortran driver not built (__graft_entry__.build())"
     g = dict(np.load(os.path.join(ROOT, "tests", "golden", "m9_hybrid_norb2_nb3.npz"),
@@ -199,9 +211,10 @@ def test_bethe_driver_hybrid_bath(tmp_path):
     assert np.allclose(summ["dens"][:norb], o["dens"], atol=1e-9)
     assert np.allclose(summ["docc"][:norb], o["docc"], atol=1e-9)
     for a in range(norb):
-        for b in range(norb):
+        for b in range(a, norb):      # the reference prints a <= b (print_impSigma.f90:21-31)
             s = np.loadtxt(tmp_path / f"impSigma_l{a + 1}{b + 1}_s1_iw.ed")
             assert np.abs(s[:, 2] + 1j * s[:, 1] - o["smats_full"][0, a, b]).max() <= 1e-8, (a, b)
+    assert not os.path.exists(tmp_path / "impSigma_l21_s1_iw.ed")
     assert np.abs(o["smats_full"][0, 0, 1]).max() > 1e-3
     en, vn = _read_hybrid_restart(str(tmp_path / "hamiltonian.restart"), norb)
     assert np.all(np.isfinite(vn)) and not np.allclose(vn, v)

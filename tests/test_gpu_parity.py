@@ -413,14 +413,14 @@ def test_apply_op_and_observables_vs_oracle(name):
 
 
 def _solve_both(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode=True, jx=0.0, jp=0.0,
-                jhflag=False, **kw):
+                jhflag=False, bath_type="normal", **kw):
     p = dmed.default_params(**kw)
-    r = dmed.solve(norb, nspin, (nbath + 1) * norb, p)
+    r = dmed.solve(norb, nspin, dmed.nsites(norb, nbath, bath_type), p)
     pr = edref.default_params(**{("chiflag" if k in ("chi_spin", "chi_dens") else k): v
                                  for k, v in kw.items()
                                  if k not in ("prec_gf", "batched", "verbose")})
     o = edref.ed_solve(norb, nbath, nspin, bath, hloc, uloc, ust, jh, xmu, hfmode, params=pr,
-                       jx=jx, jp=jp, jhflag=jhflag)
+                       jx=jx, jp=jp, jhflag=jhflag, bath_type=bath_type)
     return r, o
 
 
@@ -498,7 +498,8 @@ def test_spin_chi_vs_oracle(name, finite):
 
 @pytest.mark.parametrize("name,finite", [("m1_norb1_nb3", False), ("m2_norb2_nb1", False),
                                           ("m3_norb2_nb2_jh", False), ("m4_norb3_nb1", False),
-                                          ("m7_atomic", False), ("m5_norb1_nb5_T", True)])
+                                          ("m7_atomic", False), ("m5_norb1_nb5_T", True),
+                                          ("m9_hybrid_norb2_nb3", False)])
 def test_dens_chi_vs_oracle(name, finite):
     """Charge susceptibility of the solve (chi_dens, build_chi_dens.f90):
     densChi, densChi_mix and densChi_tot against the oracle -- chi(iv_n) and
@@ -506,7 +507,8 @@ def test_dens_chi_vs_oracle(name, finite):
     (1e-4, as for G on the real axis).  The off-diagonal elements come from a
     chain on the stacked sector (H (+) H on [N_a psi; N_b psi]) where the
     oracle runs the reference's complex chains; m3 has spin exchange and pair
-    hopping in the stacked tables and two spin species in the mix."""
+    hopping in the stacked tables and two spin species in the mix; m9 has a
+    hybrid bath.  The spin susceptibility of the same solve is checked too."""
     g = load(name)
     set_both(g)
     norb, nspin = int(g["norb"]), int(g["nspin"])
@@ -516,9 +518,14 @@ def test_dens_chi_vs_oracle(name, finite):
     if finite:
         kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
     say(f"[{name}] solve with charge chi")
+    kw.update(chi_spin=1)
     r, o = _solve_both(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
                        float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]),
-                       jx=float(g["jx"]), jp=float(g["jp"]), jhflag=bool(g["jhflag"]), **kw)
+                       jx=float(g["jx"]), jp=float(g["jp"]), jhflag=bool(g["jhflag"]),
+                       bath_type=bath_type(g), **kw)
+    rows = norb + 1 if norb > 1 else norb
+    assert np.abs(r["spin_chi_iv"][:rows] - o["spin_chi_iv"][:rows]).max() < 1e-8
+    assert np.abs(r["spin_chi_tau"][:rows] - o["spin_chi_tau"][:rows]).max() < 1e-8
     for pre in ("dens_chi_", "dens_chi_mix_", "dens_chi_tot_"):
         for suf in ("iv", "tau", "w"):
             assert r[pre + suf].shape == o[pre + suf].shape, pre + suf
