@@ -437,6 +437,11 @@ def main_hybrid():
     write_model("m9_hybrid_norb2_nb3", make_hybrid_model(rng, 2, 3, 1, [2.5, 2.0], ust=1.5,
                                                          jh=0.25, offdiag=0.1,
                                                          crystal=[-0.1, 0.1]), rng)
+    # three orbitals sharing Nbath=2 levels (Ns=5), two spin species with their
+    # own bath, no Hartree shift, xmu != 0, T=0
+    write_model("m10_hybrid_norb3_nb2_spin2",
+                make_hybrid_model(rng, 3, 2, 2, [2.0, 2.2, 1.8], ust=1.2, jh=0.2, xmu=0.3,
+                                  hfmode=0, offdiag=0.08, crystal=[-0.1, 0.0, 0.1]), rng)
 
 
 def main():

@@ -499,7 +499,8 @@ def test_spin_chi_vs_oracle(name, finite):
 @pytest.mark.parametrize("name,finite", [("m1_norb1_nb3", False), ("m2_norb2_nb1", False),
                                           ("m3_norb2_nb2_jh", False), ("m4_norb3_nb1", False),
                                           ("m7_atomic", False), ("m5_norb1_nb5_T", True),
-                                          ("m9_hybrid_norb2_nb3", False)])
+                                          ("m9_hybrid_norb2_nb3", False),
+                                          ("m10_hybrid_norb3_nb2_spin2", False)])
 def test_dens_chi_vs_oracle(name, finite):
     """Charge susceptibility of the solve (chi_dens, build_chi_dens.f90):
     densChi, densChi_mix and densChi_tot against the oracle -- chi(iv_n) and
@@ -579,26 +580,28 @@ def test_solve_sites_matches_single_solves():
     assert np.abs(conc[2]["smats"][0, 0, 0, 0] - o["smats"][0, 0]).max() < 1e-8
 
 
-def test_hybrid_bath_solve_vs_oracle():
+@pytest.mark.parametrize("name", ["m9_hybrid_norb2_nb3", "m10_hybrid_norb3_nb2_spin2"])
+def test_hybrid_bath_solve_vs_oracle(name):
     """Hybrid bath (two orbitals sharing three bath levels, off-diagonal Hloc):
     the full orbital matrices G_ab and Sigma_ab of the solve against the
-    oracle -- G(iw) and Sigma(iw) to 1e-8 in every element, real axis in
-    relative L2 (1e-4 for G, 1e-3 for Sigma).  The off-diagonal elements come
+    oracle -- G(iw) and Sigma(iw) to 1e-8 in every element; the real axis
+    to 1e-8 in relative L2 on 8-step chains (see below).  The off-diagonal elements come
     from the (c_a + c_b) chains and the complex chains, which run on the
     stacked sector.  Also the fixture's Lehmann G_ab(iw), to 1e-8."""
-    g = load("m9_hybrid_norb2_nb3")
+    g = load(name)
     set_both(g)
     norb, nspin = int(g["norb"]), int(g["nspin"])
     kw = dict(beta=float(g["beta"]), lmats=len(g["wm"]), lreal=len(g["wr"]),
               wini=float(g["wr"][0]), wfin=float(g["wr"][-1]), eps=float(g["eps"]),
               lanc_ngfiter=200)
-    say("[m9_hybrid] solve")
+    say(f"[{name}] solve")
     p = dmed.default_params(**kw)
     r = dmed.solve(norb, nspin, int(g["nbath"]) + norb, p)
     pr = edref.default_params(**kw)
     o = edref.ed_solve(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
                        float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]),
                        params=pr, bath_type="hybrid")
+
     def rel(x, y):
         return np.linalg.norm(x - y) / max(np.linalg.norm(y), np.sqrt(len(y)))
 
@@ -608,11 +611,22 @@ def test_hybrid_bath_solve_vs_oracle():
         assert np.abs(r["gmats"][s, s] - o["gmats_full"][s]).max() < 1e-8
         assert np.abs(r["smats"][s, s] - o["smats_full"][s]).max() < 1e-8
         assert np.abs(r["gmats"][s, s] - g["gmats_full"][s]).max() < 1e-8
+    assert np.abs(r["gmats"][0, 0, 0, 1]).max() > 0.05
+    # real axis: with chains that run past the Krylov dimension of these small
+    # sectors (D <= 100 < lanc_ngfiter) both sides carry ghost poles whose
+    # weights depend on round-off, so the real axis is compared on chains of 8
+    # steps, which both sides build from the same few Krylov vectors
+    kw8 = dict(kw, lanc_ngfiter=8)
+    r8 = dmed.solve(norb, nspin, int(g["nbath"]) + norb, dmed.default_params(**kw8))
+    o8 = edref.ed_solve(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
+                        float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]),
+                        params=edref.default_params(**kw8), bath_type="hybrid")
+    for s in range(nspin):
+        assert np.abs(r8["smats"][s, s] - o8["smats_full"][s]).max() < 1e-8
         for a in range(norb):
             for b in range(norb):
-                assert rel(r["greal"][s, s, a, b], o["greal_full"][s, a, b]) < 1e-4
-                assert rel(r["sreal"][s, s, a, b], o["sreal_full"][s, a, b]) < 1e-3
-    assert np.abs(r["gmats"][0, 0, 0, 1]).max() > 0.05
+                assert rel(r8["greal"][s, s, a, b], o8["greal_full"][s, a, b]) < 1e-8
+                assert rel(r8["sreal"][s, s, a, b], o8["sreal_full"][s, a, b]) < 1e-8
 
 
 def test_apply_occ_vs_oracle():
