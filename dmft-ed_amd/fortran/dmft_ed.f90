@@ -90,6 +90,36 @@ module DMFT_ED
   interface ed_init_solver
      module procedure ed_init_solver_single, ed_init_solver_lattice
   end interface ed_init_solver
+  interface set_Hloc
+     module procedure set_hloc_nn, set_hloc_so
+  end interface set_Hloc
+  interface lso2nnn_reshape
+     module procedure d_nlso2nnn, c_nlso2nnn
+  end interface lso2nnn_reshape
+  interface nnn2lso_reshape
+     module procedure d_nnn2nlso, c_nnn2nlso
+  end interface nnn2lso_reshape
+  interface so2nn_reshape
+     module procedure d_nso2nn, c_nso2nn
+  end interface so2nn_reshape
+  interface nn2so_reshape
+     module procedure d_nn2nso, c_nn2nso
+  end interface nn2so_reshape
+  interface spin_symmetrize_bath
+     module procedure spin_symmetrize_bath_site, spin_symmetrize_bath_lattice
+  end interface spin_symmetrize_bath
+  interface break_symmetry_bath
+     module procedure break_symmetry_bath_site, break_symmetry_bath_lattice
+  end interface break_symmetry_bath
+  interface ph_symmetrize_bath
+     module procedure ph_symmetrize_bath_site, ph_symmetrize_bath_lattice
+  end interface ph_symmetrize_bath
+  interface ph_trans_bath
+     module procedure ph_trans_bath_site, ph_trans_bath_lattice
+  end interface ph_trans_bath
+  interface enforce_normal_bath
+     module procedure enforce_normal_bath_site, enforce_normal_bath_lattice
+  end interface enforce_normal_bath
   interface ed_solve
      module procedure ed_solve_single, ed_solve_lattice
   end interface ed_solve
@@ -137,6 +167,9 @@ module DMFT_ED
   public :: ed_chi2_fitgf
   public :: ed_print_impSigma, ed_print_impG, ed_print_impG0, ed_print_impChi
   public :: ed_read_impSigma
+  public :: set_Hloc, lso2nnn_reshape, nnn2lso_reshape, so2nn_reshape, nn2so_reshape
+  public :: spin_symmetrize_bath, break_symmetry_bath, ph_symmetrize_bath, ph_trans_bath
+  public :: enforce_normal_bath, search_chemical_potential, ed_get_self_matsubara
 
 contains
 
@@ -431,11 +464,344 @@ contains
     endif
   end subroutine ed_check
 
-  subroutine set_hloc(Hloc)
+  !> set_Hloc_nn / set_Hloc_so (ED_AUX_FUNX.f90:135-153)
+  subroutine set_hloc_nn(Hloc)
     complex(8),intent(in) :: Hloc(Nspin,Nspin,Norb,Norb)
     if(any(abs(aimag(Hloc))>0d0))stop "DMFT_ED: complex Hloc is not supported on the GPU path (H must be real)"
+    if(.not.allocated(impHloc))allocate(impHloc(Nspin,Nspin,Norb,Norb))
     impHloc = dble(Hloc)
-  end subroutine set_hloc
+  end subroutine set_hloc_nn
+
+  subroutine set_hloc_so(Hloc)
+    complex(8),intent(in) :: Hloc(Nspin*Norb,Nspin*Norb)
+    call set_hloc_nn(so2nn_reshape(Hloc,Nspin,Norb))
+  end subroutine set_hloc_so
+
+  !-------------------------------------------- reshapes (ED_AUX_FUNX.f90:177-351)
+  !> (Nlat*Nspin*Norb)^2 <-> (Nlat,Nspin,Nspin,Norb,Norb), index
+  !> io = iorb + (ispin-1)*Norb + (ilat-1)*Norb*Nspin, site-diagonal blocks
+  function d_nlso2nnn(Hlso,Nlat,Nspin_,Norb_) result(Hnnn)
+    integer :: Nlat, Nspin_, Norb_
+    real(8),dimension(Nlat*Nspin_*Norb_,Nlat*Nspin_*Norb_) :: Hlso
+    real(8),dimension(Nlat,Nspin_,Nspin_,Norb_,Norb_)      :: Hnnn
+    integer :: l, s1, s2, a, b
+    do l=1,Nlat
+       do s1=1,Nspin_
+          do s2=1,Nspin_
+             do a=1,Norb_
+                do b=1,Norb_
+                   Hnnn(l,s1,s2,a,b) = Hlso(a+(s1-1)*Norb_+(l-1)*Norb_*Nspin_, b+(s2-1)*Norb_+(l-1)*Norb_*Nspin_)
+                enddo
+             enddo
+          enddo
+       enddo
+    enddo
+  end function d_nlso2nnn
+
+  function c_nlso2nnn(Hlso,Nlat,Nspin_,Norb_) result(Hnnn)
+    integer :: Nlat, Nspin_, Norb_
+    complex(8),dimension(Nlat*Nspin_*Norb_,Nlat*Nspin_*Norb_) :: Hlso
+    complex(8),dimension(Nlat,Nspin_,Nspin_,Norb_,Norb_)      :: Hnnn
+    integer :: l, s1, s2, a, b
+    do l=1,Nlat
+       do s1=1,Nspin_
+          do s2=1,Nspin_
+             do a=1,Norb_
+                do b=1,Norb_
+                   Hnnn(l,s1,s2,a,b) = Hlso(a+(s1-1)*Norb_+(l-1)*Norb_*Nspin_, b+(s2-1)*Norb_+(l-1)*Norb_*Nspin_)
+                enddo
+             enddo
+          enddo
+       enddo
+    enddo
+  end function c_nlso2nnn
+
+  function d_nnn2nlso(Hnnn,Nlat,Nspin_,Norb_) result(Hlso)
+    integer :: Nlat, Nspin_, Norb_
+    real(8),dimension(Nlat,Nspin_,Nspin_,Norb_,Norb_)      :: Hnnn
+    real(8),dimension(Nlat*Nspin_*Norb_,Nlat*Nspin_*Norb_) :: Hlso
+    integer :: l, s1, s2, a, b
+    Hlso = 0d0
+    do l=1,Nlat
+       do s1=1,Nspin_
+          do s2=1,Nspin_
+             do a=1,Norb_
+                do b=1,Norb_
+                   Hlso(a+(s1-1)*Norb_+(l-1)*Norb_*Nspin_, b+(s2-1)*Norb_+(l-1)*Norb_*Nspin_) = Hnnn(l,s1,s2,a,b)
+                enddo
+             enddo
+          enddo
+       enddo
+    enddo
+  end function d_nnn2nlso
+
+  function c_nnn2nlso(Hnnn,Nlat,Nspin_,Norb_) result(Hlso)
+    integer :: Nlat, Nspin_, Norb_
+    complex(8),dimension(Nlat,Nspin_,Nspin_,Norb_,Norb_)      :: Hnnn
+    complex(8),dimension(Nlat*Nspin_*Norb_,Nlat*Nspin_*Norb_) :: Hlso
+    integer :: l, s1, s2, a, b
+    Hlso = (0d0,0d0)
+    do l=1,Nlat
+       do s1=1,Nspin_
+          do s2=1,Nspin_
+             do a=1,Norb_
+                do b=1,Norb_
+                   Hlso(a+(s1-1)*Norb_+(l-1)*Norb_*Nspin_, b+(s2-1)*Norb_+(l-1)*Norb_*Nspin_) = Hnnn(l,s1,s2,a,b)
+                enddo
+             enddo
+          enddo
+       enddo
+    enddo
+  end function c_nnn2nlso
+
+  function d_nso2nn(Hso,Nspin_,Norb_) result(Hnn)
+    integer :: Nspin_, Norb_
+    real(8),dimension(Nspin_*Norb_,Nspin_*Norb_) :: Hso
+    real(8),dimension(Nspin_,Nspin_,Norb_,Norb_) :: Hnn
+    real(8) :: t(1,Nspin_,Nspin_,Norb_,Norb_)
+    t = d_nlso2nnn(Hso,1,Nspin_,Norb_)
+    Hnn = t(1,:,:,:,:)
+  end function d_nso2nn
+
+  function c_nso2nn(Hso,Nspin_,Norb_) result(Hnn)
+    integer :: Nspin_, Norb_
+    complex(8),dimension(Nspin_*Norb_,Nspin_*Norb_) :: Hso
+    complex(8),dimension(Nspin_,Nspin_,Norb_,Norb_) :: Hnn
+    complex(8) :: t(1,Nspin_,Nspin_,Norb_,Norb_)
+    t = c_nlso2nnn(Hso,1,Nspin_,Norb_)
+    Hnn = t(1,:,:,:,:)
+  end function c_nso2nn
+
+  function d_nn2nso(Hnn,Nspin_,Norb_) result(Hso)
+    integer :: Nspin_, Norb_
+    real(8),dimension(Nspin_,Nspin_,Norb_,Norb_) :: Hnn
+    real(8),dimension(Nspin_*Norb_,Nspin_*Norb_) :: Hso
+    real(8) :: t(1,Nspin_,Nspin_,Norb_,Norb_)
+    t(1,:,:,:,:) = Hnn
+    Hso = d_nnn2nlso(t,1,Nspin_,Norb_)
+  end function d_nn2nso
+
+  function c_nn2nso(Hnn,Nspin_,Norb_) result(Hso)
+    integer :: Nspin_, Norb_
+    complex(8),dimension(Nspin_,Nspin_,Norb_,Norb_) :: Hnn
+    complex(8),dimension(Nspin_*Norb_,Nspin_*Norb_) :: Hso
+    complex(8) :: t(1,Nspin_,Nspin_,Norb_,Norb_)
+    t(1,:,:,:,:) = Hnn
+    Hso = c_nnn2nlso(t,1,Nspin_,Norb_)
+  end function c_nn2nso
+
+  !----------------------------------- bath symmetries (ED_BATH/user_aux.f90:1071-1270)
+  subroutine bath_arrays()
+    if(.not.allocated(bath_e))allocate(bath_e(Nspin,Norb,Nbath), bath_v(Nspin,Norb,Nbath))
+  end subroutine bath_arrays
+
+  !> spin-down bath := spin-up bath (Nspin = 2)
+  subroutine spin_symmetrize_bath_site(bath,save)
+    real(8),dimension(:) :: bath
+    logical,optional     :: save
+    if(Nspin==1)then
+       write(LOGfile,"(A)")"spin_symmetrize_bath: Nspin=1 nothing to symmetrize"
+       return
+    endif
+    call bath_arrays(); call set_bath(bath)
+    bath_e(Nspin,:,:) = bath_e(1,:,:)
+    bath_v(Nspin,:,:) = bath_v(1,:,:)
+    call finish_bath(bath,save)
+  end subroutine spin_symmetrize_bath_site
+
+  !> e_up += sign*field, e_dw -= sign*field
+  subroutine break_symmetry_bath_site(bath,field,sign,save)
+    real(8),dimension(:) :: bath
+    real(8)              :: field, sign
+    logical,optional     :: save
+    call bath_arrays(); call set_bath(bath)
+    bath_e(1,:,:) = bath_e(1,:,:) + sign*field
+    bath_e(Nspin,:,:) = bath_e(Nspin,:,:) - sign*field
+    call finish_bath(bath,save)
+  end subroutine break_symmetry_bath_site
+
+  !> e(Nbath+1-i) = -e(i), V(Nbath+1-i) = V(i); the middle level of an odd bath at 0
+  subroutine ph_symmetrize_bath_site(bath,save)
+    real(8),dimension(:) :: bath
+    logical,optional     :: save
+    integer :: i
+    if(Nbath==1)return
+    call bath_arrays(); call set_bath(bath)
+    do i=1,Nbath/2
+       bath_e(:,:,Nbath+1-i) = -bath_e(:,:,i)
+       bath_v(:,:,Nbath+1-i) =  bath_v(:,:,i)
+    enddo
+    if(mod(Nbath,2)/=0)bath_e(:,:,(Nbath-1)/2+1) = 0d0
+    call finish_bath(bath,save)
+  end subroutine ph_symmetrize_bath_site
+
+  !> particle-hole transformation of the normal bath: Norb = 1: e -> -e;
+  !> Norb = 2: e_1 <-> -e_2, V_1 <-> V_2; other Norb unchanged
+  subroutine ph_trans_bath_site(bath,save)
+    real(8),dimension(:) :: bath
+    logical,optional     :: save
+    real(8),allocatable  :: te(:,:), tv(:,:)
+    if(Nbath==1)return
+    if(trim(bath_type)/='normal')stop "ph_trans_bath: normal bath only"
+    call bath_arrays(); call set_bath(bath)
+    select case(Norb)
+    case(1)
+       bath_e = -bath_e
+    case(2)
+       allocate(te(Nspin,Nbath), tv(Nspin,Nbath))
+       te = bath_e(:,2,:); tv = bath_v(:,2,:)
+       bath_e(:,2,:) = -bath_e(:,1,:); bath_v(:,2,:) = bath_v(:,1,:)
+       bath_e(:,1,:) = -te; bath_v(:,1,:) = tv
+    end select
+    call finish_bath(bath,save)
+  end subroutine ph_trans_bath_site
+
+  !> no pairing amplitudes in the normal mode: the bath is saved unchanged
+  subroutine enforce_normal_bath_site(bath,save)
+    real(8),dimension(:) :: bath
+    logical,optional     :: save
+    call bath_arrays(); call set_bath(bath)
+    call finish_bath(bath,save)
+  end subroutine enforce_normal_bath_site
+
+  subroutine finish_bath(bath,save)
+    real(8),dimension(:) :: bath
+    logical,optional     :: save
+    logical :: save_
+    save_ = .true.; if(present(save))save_ = save
+    if(save_)call save_bath(used=.false.)
+    call get_bath(bath)
+  end subroutine finish_bath
+
+  !> the lattice forms loop over sites with the suffix "_site<4-digit site>"
+  subroutine spin_symmetrize_bath_lattice(bath,save)
+    real(8),dimension(:,:) :: bath
+    logical,optional       :: save
+    integer :: l
+    do l=1,size(bath,1)
+       write(ed_file_suffix,"(A,I4.4)")"_site",l
+       call spin_symmetrize_bath_site(bath(l,:),save)
+    enddo
+    ed_file_suffix = ""
+  end subroutine spin_symmetrize_bath_lattice
+
+  subroutine break_symmetry_bath_lattice(bath,field,sign,save)
+    real(8),dimension(:,:) :: bath
+    real(8)                :: field, sign
+    logical,optional       :: save
+    integer :: l
+    do l=1,size(bath,1)
+       write(ed_file_suffix,"(A,I4.4)")"_site",l
+       call break_symmetry_bath_site(bath(l,:),field,sign,save)
+    enddo
+    ed_file_suffix = ""
+  end subroutine break_symmetry_bath_lattice
+
+  subroutine ph_symmetrize_bath_lattice(bath,save)
+    real(8),dimension(:,:) :: bath
+    logical,optional       :: save
+    integer :: l
+    do l=1,size(bath,1)
+       write(ed_file_suffix,"(A,I4.4)")"_site",l
+       call ph_symmetrize_bath_site(bath(l,:),save)
+    enddo
+    ed_file_suffix = ""
+  end subroutine ph_symmetrize_bath_lattice
+
+  subroutine ph_trans_bath_lattice(bath,save)
+    real(8),dimension(:,:) :: bath
+    logical,optional       :: save
+    integer :: l
+    do l=1,size(bath,1)
+       write(ed_file_suffix,"(A,I4.4)")"_site",l
+       call ph_trans_bath_site(bath(l,:),save)
+    enddo
+    ed_file_suffix = ""
+  end subroutine ph_trans_bath_lattice
+
+  subroutine enforce_normal_bath_lattice(bath,save)
+    real(8),dimension(:,:) :: bath
+    logical,optional       :: save
+    integer :: l
+    do l=1,size(bath,1)
+       write(ed_file_suffix,"(A,I4.4)")"_site",l
+       call enforce_normal_bath_site(bath(l,:),save)
+    enddo
+    ed_file_suffix = ""
+  end subroutine enforce_normal_bath_lattice
+
+  !> search_chemical_potential (ED_AUX_FUNX.f90:459-564): one step of the
+  !> density-driven search of var towards nread with the reference's step
+  !> halving and threshold tightening; nindex_old starts at 0 (the reference
+  !> leaves it unset), niter is the reference's unset module integer (0)
+  subroutine search_chemical_potential(var,ntmp,converged)
+    real(8),intent(inout) :: var
+    real(8),intent(in)    :: ntmp
+    logical,intent(inout) :: converged
+    logical               :: bool
+    real(8)               :: ndiff, ndelta_old, nratio
+    integer,save          :: count=0, nindex=0, nindex_old(3)=0
+    integer,save          :: nth_magnitude=-2, nth_magnitude_old=-2
+    real(8),save          :: nth=1d-2
+    logical,save          :: ireduce=.true.
+    integer,parameter     :: niter=0
+    integer :: unit, i
+    ndiff = ntmp-nread
+    nratio = 0.5d0
+    count = count+1
+    if(count>2)then
+       do i=1,2
+          nindex_old(i+1) = nindex_old(i)
+       enddo
+    endif
+    nindex_old(1) = nindex
+    if(ndiff>=nth)then
+       nindex = -1
+    elseif(ndiff<=-nth)then
+       nindex = 1
+    else
+       nindex = 0
+    endif
+    ndelta_old = ndelta
+    bool = nindex/=0 .and. ((nindex+nindex_old(1)==0) .or. (nindex+sum(nindex_old(:))==0))
+    if(bool)then
+       ndelta = ndelta_old*nratio
+    else
+       ndelta = ndelta_old
+    endif
+    if(ndelta_old<1d-9)then
+       ndelta_old = 0d0
+       nindex = 0
+    endif
+    var = var + dble(nindex)*ndelta
+    write(LOGfile,"(A,f16.9,A,f15.9)")"n    = ",ntmp," /",nread
+    write(LOGfile,"(A,f15.9)")"var  = ",var
+    open(newunit=unit,file="search_mu_iteration"//trim(ed_file_suffix)//".ed",position="append")
+    write(unit,*)var,ntmp,ndiff
+    close(unit)
+    bool = ireduce .and. (abs(ndiff)<nth) .and. converged .and. (nth>nerr)
+    if(bool)then
+       nth_magnitude_old = nth_magnitude
+       nth_magnitude = nth_magnitude_old-1
+       nth = max(nerr,10d0**(nth_magnitude))
+       count = 0
+       converged = .false.
+       ndelta = ndelta_old*nratio
+    endif
+    if(abs(ntmp-nread)>nth)converged = .false.
+    bool = ireduce .and. (count>niter) .and. (.not.converged)
+    if(bool)then
+       ireduce = .false.
+       nth = 10d0**(nth_magnitude_old)
+    endif
+  end subroutine search_chemical_potential
+
+  !> ed_get_self_matsubara (ED_IO/get_self_matsubara.f90): the anomalous
+  !> self-energy, identically zero in the normal mode
+  subroutine ed_get_self_matsubara(SAmats)
+    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: SAmats
+    SAmats = (0d0,0d0)
+  end subroutine ed_get_self_matsubara
 
   !> ed_init_solver_single (ED_MAIN.f90:67-107)
   subroutine ed_init_solver_single(bath,Hloc)

@@ -98,3 +98,137 @@ def test_fortran_driver_and_modules_built():
     for name in ("ed_read_input", "ed_init_solver", "ed_solve", "ed_get_sigma_matsubara",
                  "ed_chi2_fitgf", "get_bath_dimension"):
         assert name in mod, name
+
+
+UTIL_CHECK = """program util_check
+  use DMFT_ED
+  implicit none
+  real(8),allocatable :: b(:), bl(:,:)
+  real(8)    :: hso(4,4), hnn(2,2,2,2), hl(8,8), hnnn(2,2,2,2,2), var
+  logical    :: conv
+  integer    :: i, nb
+  call ed_read_input("inputED.in")
+  nb = get_bath_dimension()
+  allocate(b(nb), bl(2,nb))
+  b = [(0.1d0*i, i=1,nb)]
+  call spin_symmetrize_bath(b,save=.false.)
+  write(*,"(A,100ES24.15)")"spin_sym ",b
+  b = [(0.1d0*i, i=1,nb)]
+  call break_symmetry_bath(b,0.05d0,1d0,save=.false.)
+  write(*,"(A,100ES24.15)")"break ",b
+  b = [(0.1d0*i, i=1,nb)]
+  call ph_symmetrize_bath(b,save=.false.)
+  write(*,"(A,100ES24.15)")"ph_sym ",b
+  b = [(0.1d0*i, i=1,nb)]
+  call ph_trans_bath(b,save=.false.)
+  write(*,"(A,100ES24.15)")"ph_trans ",b
+  bl(1,:) = [(0.1d0*i, i=1,nb)]; bl(2,:) = -bl(1,:)
+  call spin_symmetrize_bath(bl,save=.false.)
+  write(*,"(A,100ES24.15)")"spin_sym_lat ",bl(2,:)
+  hso = reshape([(dble(i), i=1,16)],[4,4])
+  hnn = so2nn_reshape(hso,2,2)
+  write(*,"(A,100ES24.15)")"so2nn ",hnn
+  write(*,"(A,100ES24.15)")"nn2so ",nn2so_reshape(hnn,2,2)
+  hl = reshape([(dble(i), i=1,64)],[8,8])
+  hnnn = lso2nnn_reshape(hl,2,2,2)
+  write(*,"(A,100ES24.15)")"lso2nnn ",hnnn
+  write(*,"(A,100ES24.15)")"nnn2lso ",nnn2lso_reshape(hnnn,2,2,2)
+  var = 0.5d0; conv = .true.
+  call search_chemical_potential(var,0.8d0,conv)
+  write(*,"(A,ES24.15,L2)")"mu ",var,conv
+end program util_check
+"""
+
+
+def test_fortran_bath_utilities(tmp_path):
+    """The host-side helpers of the Fortran API (no GPU call): the bath
+    symmetry operations of ED_BATH/user_aux.f90:1071-1270 (site and lattice
+    forms), the spin/orbital/site reshapes of ED_AUX_FUNX.f90:177-351 and one
+    step of search_chemical_potential (:459-564), compiled against the
+    module and checked against the index rules written out here."""
+    b = os.path.join(ROOT, "dmft-ed_amd", "fortran", "build")
+    if not os.path.exists(os.path.join(b, "dmft_ed.o")):
+        pytest.skip("Fortran layer not built (run __graft_entry__.build())")
+    import shutil
+    flang = shutil.which("amdflang") or "/opt/rocm/llvm/bin/amdflang"
+    (tmp_path / "check.f90").write_text(UTIL_CHECK)
+    lib = os.path.join(ROOT, "dmft-ed_amd")
+    subprocess.run([flang, "-O0", "-I", b, "check.f90", os.path.join(b, "dmed_cabi.o"),
+                    os.path.join(b, "dmft_ed.o"), "-L" + lib, "-ldmed", "-Wl,-rpath," + lib,
+                    "-o", "check"], cwd=str(tmp_path), check=True, capture_output=True)
+    (tmp_path / "inputED.in").write_text("NORB=2\nNBATH=3\nNSPIN=2\nNREAD=0.6\nNDELTA=0.1\n")
+    out = subprocess.run(["./check"], cwd=str(tmp_path), capture_output=True, text=True,
+                         timeout=60, check=True).stdout
+    res = {}
+    for line in out.splitlines():
+        parts = line.split()
+        if parts and parts[0] in ("spin_sym", "break", "ph_sym", "ph_trans", "spin_sym_lat",
+                                  "so2nn", "nn2so", "lso2nnn", "nnn2lso", "mu"):
+            res[parts[0]] = parts[1:]
+    nspin, norb, nb = 2, 2, 3
+    st = nspin * norb * nb
+    b0 = 0.1 * np.arange(1, 2 * st + 1)
+
+    def ev(x):      # e(s,a,k), v(s,a,k) of the set_dmft_bath layout
+        e = np.zeros((nspin, norb, nb))
+        v = np.zeros_like(e)
+        for s_ in range(nspin):
+            for a in range(norb):
+                for k in range(nb):
+                    io = k + a * nb + s_ * nb * norb
+                    e[s_, a, k], v[s_, a, k] = x[io], x[st + io]
+        return e, v
+
+    def flat(e, v):
+        x = np.zeros(2 * st)
+        for s_ in range(nspin):
+            for a in range(norb):
+                for k in range(nb):
+                    io = k + a * nb + s_ * nb * norb
+                    x[io], x[st + io] = e[s_, a, k], v[s_, a, k]
+        return x
+
+    got = {k: np.array(list(map(float, v))) for k, v in res.items() if k != "mu"}
+    e, v = ev(b0)
+    e1, v1 = e.copy(), v.copy()
+    e1[1], v1[1] = e[0], v[0]
+    assert np.allclose(got["spin_sym"], flat(e1, v1), rtol=0, atol=1e-14)
+    e1 = e.copy()
+    e1[0] += 0.05
+    e1[1] -= 0.05
+    assert np.allclose(got["break"], flat(e1, v), rtol=0, atol=1e-14)
+    e1, v1 = e.copy(), v.copy()
+    e1[:, :, 2], v1[:, :, 2], e1[:, :, 1] = -e[:, :, 0], v[:, :, 0], 0.0
+    assert np.allclose(got["ph_sym"], flat(e1, v1), rtol=0, atol=1e-14)
+    e1, v1 = e.copy(), v.copy()
+    e1[:, 0], e1[:, 1] = -e[:, 1], -e[:, 0]
+    v1[:, 0], v1[:, 1] = v[:, 1], v[:, 0]
+    assert np.allclose(got["ph_trans"], flat(e1, v1), rtol=0, atol=1e-14)
+    e2, v2 = ev(-b0)
+    e2[1], v2[1] = e2[0], v2[0]
+    assert np.allclose(got["spin_sym_lat"], flat(e2, v2), rtol=0, atol=1e-14)
+    hso = np.arange(1, 17, dtype=float).reshape(4, 4, order="F")
+    hnn = np.zeros((2, 2, 2, 2))
+    for s1 in range(2):
+        for s2 in range(2):
+            for a in range(2):
+                for b_ in range(2):
+                    hnn[s1, s2, a, b_] = hso[a + 2 * s1, b_ + 2 * s2]
+    assert np.array_equal(got["so2nn"], hnn.flatten(order="F"))
+    assert np.array_equal(got["nn2so"], hso.flatten(order="F"))
+    hl = np.arange(1, 65, dtype=float).reshape(8, 8, order="F")
+    hnnn = np.zeros((2, 2, 2, 2, 2))
+    back = np.zeros((8, 8))
+    for l in range(2):
+        for s1 in range(2):
+            for s2 in range(2):
+                for a in range(2):
+                    for b_ in range(2):
+                        i, j = a + 2 * s1 + 4 * l, b_ + 2 * s2 + 4 * l
+                        hnnn[l, s1, s2, a, b_] = hl[i, j]
+                        back[i, j] = hl[i, j]
+    assert np.array_equal(got["lso2nnn"], hnnn.flatten(order="F"))
+    assert np.array_equal(got["nnn2lso"], back.flatten(order="F"))
+    # n = 0.8 above nread = 0.6: mu steps down by ndelta, not converged
+    assert abs(float(res["mu"][0]) - 0.4) < 1e-14 and res["mu"][1] == "F"
+
