@@ -127,25 +127,29 @@ module DMFT_ED
      module procedure ed_chi2_fitgf_single, ed_fit_bath_sites_normal
   end interface ed_chi2_fitgf
   interface ed_get_sigma_matsubara
-     module procedure ed_get_sigma_matsubara_1, ed_get_sigma_matsubara_lattice_1
+     module procedure ed_get_sigma_matsubara_1, ed_get_sigma_matsubara_2, ed_get_sigma_matsubara_3
+     module procedure ed_get_sigma_matsubara_lattice_1, ed_get_sigma_matsubara_lattice_2, ed_get_sigma_matsubara_lattice_3
   end interface ed_get_sigma_matsubara
   interface ed_get_sigma_real
-     module procedure ed_get_sigma_real_1, ed_get_sigma_real_lattice_1
+     module procedure ed_get_sigma_real_1, ed_get_sigma_real_2, ed_get_sigma_real_3
+     module procedure ed_get_sigma_real_lattice_1, ed_get_sigma_real_lattice_2, ed_get_sigma_real_lattice_3
   end interface ed_get_sigma_real
   interface ed_get_gimp_matsubara
-     module procedure ed_get_gimp_matsubara_1, ed_get_gimp_matsubara_lattice_1
+     module procedure ed_get_gimp_matsubara_1, ed_get_gimp_matsubara_2, ed_get_gimp_matsubara_3
+     module procedure ed_get_gimp_matsubara_lattice_1, ed_get_gimp_matsubara_lattice_2, ed_get_gimp_matsubara_lattice_3
   end interface ed_get_gimp_matsubara
   interface ed_get_gimp_real
-     module procedure ed_get_gimp_real_1, ed_get_gimp_real_lattice_1
+     module procedure ed_get_gimp_real_1, ed_get_gimp_real_2, ed_get_gimp_real_3
+     module procedure ed_get_gimp_real_lattice_1, ed_get_gimp_real_lattice_2, ed_get_gimp_real_lattice_3
   end interface ed_get_gimp_real
   interface ed_get_dens
-     module procedure ed_get_dens_1, ed_get_dens_lattice_1
+     module procedure ed_get_dens_1, ed_get_dens_2, ed_get_dens_lattice_1, ed_get_dens_lattice_2
   end interface ed_get_dens
   interface ed_get_docc
-     module procedure ed_get_docc_1, ed_get_docc_lattice_1
+     module procedure ed_get_docc_1, ed_get_docc_2, ed_get_docc_lattice_1, ed_get_docc_lattice_2
   end interface ed_get_docc
   interface ed_get_mag
-     module procedure ed_get_mag_1, ed_get_mag_lattice_1
+     module procedure ed_get_mag_1, ed_get_mag_2, ed_get_mag_lattice_1, ed_get_mag_lattice_2
   end interface ed_get_mag
   interface ed_get_eimp
      module procedure ed_get_eimp_1, ed_get_eimp_lattice
@@ -1556,5 +1560,234 @@ contains
     enddo
     ed_file_suffix = ""
   end subroutine ed_fit_bath_sites_normal
+
+  !> the (Nspin*Norb)^2 and single-element forms of the getters
+  !> (ED_IO/get_sigma_matsubara.f90:7-31 and alike; io = iorb + (ispin-1)*Norb)
+
+  subroutine ed_get_sigma_matsubara_2(F)
+    complex(8),dimension(Nspin*Norb,Nspin*Norb,Lmats),intent(inout) :: F
+    integer :: s1, s2, a, b
+    do s1=1,Nspin
+       do s2=1,Nspin
+          do a=1,Norb
+             do b=1,Norb
+                F(a+(s1-1)*Norb,b+(s2-1)*Norb,:) = impSmats(s1,s2,a,b,:)
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_sigma_matsubara_2
+
+  subroutine ed_get_sigma_matsubara_3(F,ispin,jspin,iorb,jorb)
+    complex(8),dimension(Lmats),intent(inout) :: F
+    integer :: ispin, jspin, iorb, jorb
+    F = impSmats(ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_sigma_matsubara_3
+
+  subroutine ed_get_sigma_matsubara_lattice_2(F,Nsites)
+    integer :: Nsites
+    complex(8),dimension(Nsites,Nspin*Norb,Nspin*Norb,Lmats),intent(inout) :: F
+    integer :: l, s1, s2, a, b
+    do l=1,Nsites
+       do s1=1,Nspin
+          do s2=1,Nspin
+             do a=1,Norb
+                do b=1,Norb
+                   F(l,a+(s1-1)*Norb,b+(s2-1)*Norb,:) = Smatsii(l,s1,s2,a,b,:)
+                enddo
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_sigma_matsubara_lattice_2
+
+  subroutine ed_get_sigma_matsubara_lattice_3(F,Nsites,ispin,jspin,iorb,jorb)
+    integer :: Nsites, ispin, jspin, iorb, jorb
+    complex(8),dimension(Nsites,Lmats),intent(inout) :: F
+    F(1:Nsites,:) = Smatsii(1:Nsites,ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_sigma_matsubara_lattice_3
+
+  subroutine ed_get_sigma_real_2(F)
+    complex(8),dimension(Nspin*Norb,Nspin*Norb,Lreal),intent(inout) :: F
+    integer :: s1, s2, a, b
+    do s1=1,Nspin
+       do s2=1,Nspin
+          do a=1,Norb
+             do b=1,Norb
+                F(a+(s1-1)*Norb,b+(s2-1)*Norb,:) = impSreal(s1,s2,a,b,:)
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_sigma_real_2
+
+  subroutine ed_get_sigma_real_3(F,ispin,jspin,iorb,jorb)
+    complex(8),dimension(Lreal),intent(inout) :: F
+    integer :: ispin, jspin, iorb, jorb
+    F = impSreal(ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_sigma_real_3
+
+  subroutine ed_get_sigma_real_lattice_2(F,Nsites)
+    integer :: Nsites
+    complex(8),dimension(Nsites,Nspin*Norb,Nspin*Norb,Lreal),intent(inout) :: F
+    integer :: l, s1, s2, a, b
+    do l=1,Nsites
+       do s1=1,Nspin
+          do s2=1,Nspin
+             do a=1,Norb
+                do b=1,Norb
+                   F(l,a+(s1-1)*Norb,b+(s2-1)*Norb,:) = Srealii(l,s1,s2,a,b,:)
+                enddo
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_sigma_real_lattice_2
+
+  subroutine ed_get_sigma_real_lattice_3(F,Nsites,ispin,jspin,iorb,jorb)
+    integer :: Nsites, ispin, jspin, iorb, jorb
+    complex(8),dimension(Nsites,Lreal),intent(inout) :: F
+    F(1:Nsites,:) = Srealii(1:Nsites,ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_sigma_real_lattice_3
+
+  subroutine ed_get_gimp_matsubara_2(F)
+    complex(8),dimension(Nspin*Norb,Nspin*Norb,Lmats),intent(inout) :: F
+    integer :: s1, s2, a, b
+    do s1=1,Nspin
+       do s2=1,Nspin
+          do a=1,Norb
+             do b=1,Norb
+                F(a+(s1-1)*Norb,b+(s2-1)*Norb,:) = impGmats(s1,s2,a,b,:)
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_gimp_matsubara_2
+
+  subroutine ed_get_gimp_matsubara_3(F,ispin,jspin,iorb,jorb)
+    complex(8),dimension(Lmats),intent(inout) :: F
+    integer :: ispin, jspin, iorb, jorb
+    F = impGmats(ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_gimp_matsubara_3
+
+  subroutine ed_get_gimp_matsubara_lattice_2(F,Nsites)
+    integer :: Nsites
+    complex(8),dimension(Nsites,Nspin*Norb,Nspin*Norb,Lmats),intent(inout) :: F
+    integer :: l, s1, s2, a, b
+    do l=1,Nsites
+       do s1=1,Nspin
+          do s2=1,Nspin
+             do a=1,Norb
+                do b=1,Norb
+                   F(l,a+(s1-1)*Norb,b+(s2-1)*Norb,:) = Gmatsii(l,s1,s2,a,b,:)
+                enddo
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_gimp_matsubara_lattice_2
+
+  subroutine ed_get_gimp_matsubara_lattice_3(F,Nsites,ispin,jspin,iorb,jorb)
+    integer :: Nsites, ispin, jspin, iorb, jorb
+    complex(8),dimension(Nsites,Lmats),intent(inout) :: F
+    F(1:Nsites,:) = Gmatsii(1:Nsites,ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_gimp_matsubara_lattice_3
+
+  subroutine ed_get_gimp_real_2(F)
+    complex(8),dimension(Nspin*Norb,Nspin*Norb,Lreal),intent(inout) :: F
+    integer :: s1, s2, a, b
+    do s1=1,Nspin
+       do s2=1,Nspin
+          do a=1,Norb
+             do b=1,Norb
+                F(a+(s1-1)*Norb,b+(s2-1)*Norb,:) = impGreal(s1,s2,a,b,:)
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_gimp_real_2
+
+  subroutine ed_get_gimp_real_3(F,ispin,jspin,iorb,jorb)
+    complex(8),dimension(Lreal),intent(inout) :: F
+    integer :: ispin, jspin, iorb, jorb
+    F = impGreal(ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_gimp_real_3
+
+  subroutine ed_get_gimp_real_lattice_2(F,Nsites)
+    integer :: Nsites
+    complex(8),dimension(Nsites,Nspin*Norb,Nspin*Norb,Lreal),intent(inout) :: F
+    integer :: l, s1, s2, a, b
+    do l=1,Nsites
+       do s1=1,Nspin
+          do s2=1,Nspin
+             do a=1,Norb
+                do b=1,Norb
+                   F(l,a+(s1-1)*Norb,b+(s2-1)*Norb,:) = Grealii(l,s1,s2,a,b,:)
+                enddo
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_gimp_real_lattice_2
+
+  subroutine ed_get_gimp_real_lattice_3(F,Nsites,ispin,jspin,iorb,jorb)
+    integer :: Nsites, ispin, jspin, iorb, jorb
+    complex(8),dimension(Nsites,Lreal),intent(inout) :: F
+    F(1:Nsites,:) = Grealii(1:Nsites,ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_gimp_real_lattice_3
+
+  subroutine ed_get_dens_2(y,iorb)
+    real(8) :: y
+    integer :: iorb
+    if(iorb>Norb)stop "ed_get_dens error: orbital index > N_orbital"
+    y = ed_dens(iorb)
+  end subroutine ed_get_dens_2
+
+  subroutine ed_get_dens_lattice_2(yii,Nlat,iorb)
+    integer :: Nlat, iorb
+    real(8),dimension(Nlat) :: yii
+    if(iorb>Norb)stop "ed_get_dens error: orbital index > N_orbital"
+    yii = 0d0
+    if(allocated(nii))then
+       if(Nlat>size(nii,1))stop "ed_get_dens error: required N_sites > evaluated N_sites"
+       yii = nii(1:Nlat,iorb)
+    endif
+  end subroutine ed_get_dens_lattice_2
+
+  subroutine ed_get_docc_2(y,iorb)
+    real(8) :: y
+    integer :: iorb
+    if(iorb>Norb)stop "ed_get_docc error: orbital index > N_orbital"
+    y = ed_docc(iorb)
+  end subroutine ed_get_docc_2
+
+  subroutine ed_get_docc_lattice_2(yii,Nlat,iorb)
+    integer :: Nlat, iorb
+    real(8),dimension(Nlat) :: yii
+    if(iorb>Norb)stop "ed_get_docc error: orbital index > N_orbital"
+    yii = 0d0
+    if(allocated(dii))then
+       if(Nlat>size(dii,1))stop "ed_get_docc error: required N_sites > evaluated N_sites"
+       yii = dii(1:Nlat,iorb)
+    endif
+  end subroutine ed_get_docc_lattice_2
+
+  subroutine ed_get_mag_2(y,iorb)
+    real(8) :: y
+    integer :: iorb
+    if(iorb>Norb)stop "ed_get_mag error: orbital index > N_orbital"
+    y = (ed_dens_up(iorb)-ed_dens_dw(iorb))
+  end subroutine ed_get_mag_2
+
+  subroutine ed_get_mag_lattice_2(yii,Nlat,iorb)
+    integer :: Nlat, iorb
+    real(8),dimension(Nlat) :: yii
+    if(iorb>Norb)stop "ed_get_mag error: orbital index > N_orbital"
+    yii = 0d0
+    if(allocated(mii))then
+       if(Nlat>size(mii,1))stop "ed_get_mag error: required N_sites > evaluated N_sites"
+       yii = mii(1:Nlat,iorb)
+    endif
+  end subroutine ed_get_mag_lattice_2
 
 end module DMFT_ED

@@ -17,7 +17,8 @@ program ed_hm_bethe
   real(8),allocatable      :: mag(:)
   real(8),allocatable      :: bath(:), bath_(:), He(:), Wte(:), wm(:)
   complex(8),allocatable   :: Hloc(:,:,:,:), Weiss(:,:,:,:,:), Weiss_old(:,:,:,:,:)
-  complex(8),allocatable   :: Smats(:,:,:,:,:), Gloc(:,:,:,:,:)
+  complex(8),allocatable   :: Smats(:,:,:,:,:), Gloc(:,:,:,:,:), S2(:,:,:), S3(:)
+  real(8)                  :: d2
   character(len=256)       :: finput
 
   pi = acos(-1d0)
@@ -83,6 +84,14 @@ program ed_hm_bethe
   write(unit,"(A,4ES24.15)")"eimp ",eimp
   write(unit,"(A,4ES24.15)")"doubles ",dbl
   write(unit,"(A,10ES24.15)")"mag ",mag
+  ! the (Nspin*Norb)^2 and single-element getter forms agree with the full one
+  call ed_get_sigma_matsubara(Smats)
+  allocate(S2(Nspin*Norb,Nspin*Norb,Lmats), S3(Lmats))
+  call ed_get_sigma_matsubara(S2)
+  call ed_get_sigma_matsubara(S3,1,1,1,1)
+  call ed_get_dens(d2,1)
+  write(unit,"(A,3ES24.15)")"getters ",maxval(abs(S2(1,1,:)-Smats(1,1,1,1,:))),&
+       maxval(abs(S3-Smats(1,1,1,1,:))),abs(d2-ed_dens(1))
   close(unit)
 
 contains

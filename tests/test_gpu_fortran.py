@@ -65,6 +65,7 @@ def test_bethe_driver_one_iteration(tmp_path):
     assert np.allclose(summ["doubles"], [ob["dust"], ob["dund"], ob["dse"], ob["dph"]],
                        atol=1e-9, rtol=0)
     assert np.allclose(summ["mag"], ob["magz"], atol=1e-9, rtol=0)
+    assert summ["getters"] == [0.0, 0.0, 0.0]     # matrix / element / orbital getter forms
     # G_loc of the semicircular DOS (half-bandwidth 1, 500 points) from Sigma
     g = np.loadtxt(tmp_path / "Gloc_iw.ed")
     wm = np.pi / c["beta"] * (2 * np.arange(lmats) + 1)
