@@ -216,6 +216,8 @@ int launch_scale_copy(void *dst, int dprec, const void *src, int sprec, int64_t 
                       double a, hipStream_t st);
 int launch_axpy(void *y, int yprec, const void *x, int xprec, int64_t n, double a,
                 hipStream_t st);
+// dst (cols x rows) = transpose of src (rows x cols), column-major FP64
+int launch_transpose(const double *src, int64_t rows, int64_t cols, double *dst, hipStream_t st);
 
 int dot_host(const double *x, const double *y, int64_t n, double *out);
 

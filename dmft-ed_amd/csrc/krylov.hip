@@ -416,6 +416,36 @@ int launch_fill_random(void *x, int64_t n, int prec, uint64_t seed, hipStream_t 
     return 0;
 }
 
+// dst (cols x rows) = transpose of src (rows x cols), both column-major FP64:
+// 64 x 64 tiles through LDS (one padding column against bank conflicts);
+// tiles on the edges are guarded per element.
+__global__ void __launch_bounds__(256) k_transpose(const double *__restrict__ src, int64_t rows,
+                                                   int64_t cols, double *__restrict__ dst)
+{
+    __shared__ double tile[64][65];
+    const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;     // 64 x 4 threads
+    for (int k = ty; k < 64; k += 4) {
+        const int64_t r = r0 + tx, c = c0 + k;
+        if (r < rows && c < cols) tile[k][tx] = src[r + rows * c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) {
+        const int64_t c = c0 + tx, r = r0 + k;                  // dst(c, r)
+        if (r < rows && c < cols) dst[c + cols * r] = tile[tx][k];
+    }
+}
+
+int launch_transpose(const double *src, int64_t rows, int64_t cols, double *dst, hipStream_t st)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    const dim3 grid((unsigned)((rows + 63) / 64), (unsigned)((cols + 63) / 64));
+    if (grid.y > 65535u) return fail(DMED_EINVAL, "transpose: too many column tiles");
+    hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, st, src, rows, cols, dst);
+    DMED_HIP(hipGetLastError());
+    return 0;
+}
+
 int launch_scale_copy(void *dst, int dprec, const void *src, int sprec, int64_t n, double a,
                       hipStream_t st)
 {

@@ -320,14 +320,51 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
                          ne, r.nout,
                          1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0s).count());
     };
+    // Nspin = 1 makes H symmetric under the spin flip, which maps sector
+    // (nu, nd) onto (nd, nu) and an eigenvector psi(iu, id) onto the transposed
+    // array (times the global sign (-1)^(nu*nd), immaterial for an
+    // eigenvector).  A sector with nu < nd then takes the eigenvalues of its
+    // twin and the transposed vectors instead of being diagonalised; this is
+    // the twin-sector symmetry the reference offers behind ed_twin
+    // (ED_SETUP.f90:429-436, ED_EIGENSPACE.f90:182-188), here always on for
+    // Nspin = 1 unless DMED_TWIN=0.  Both sectors keep their own entries in
+    // the state list, in the reference's sector order.
+    const char *tw_env = std::getenv("DMED_TWIN");
+    const bool twin = nspin == 1 && !(tw_env && tw_env[0] == '0');
+    auto twin_of = [&](int is) { return (is % (ns + 1)) * (ns + 1) + is / (ns + 1); };
+    std::vector<char> derived(nsectors, 0);
+    for (int is = 0; is < nsectors && twin; ++is)
+        if (is / (ns + 1) < is % (ns + 1) && neigen[is] == neigen[twin_of(is)]) derived[is] = 1;
     {
-        std::vector<int> order(nsectors);
-        for (int i = 0; i < nsectors; ++i) order[i] = i;
+        std::vector<int> order;
+        for (int i = 0; i < nsectors; ++i)
+            if (!derived[i]) order.push_back(i);
         std::stable_sort(order.begin(), order.end(),
                          [&](int x, int y) { return secs[x]->dim > secs[y]->dim; });
-        if ((rc = run_tasks(p->batched, nsectors, [&](int k) { solve_sector(order[k]); })))
+        if ((rc = run_tasks(p->batched, (int)order.size(), [&](int k) { solve_sector(order[k]); })))
             return rc;
     }
+    for (int is = 0; is < nsectors && !rc; ++is) {
+        if (!derived[is]) continue;
+        const int it = twin_of(is);
+        const SectorRes &t = sres[it];
+        SectorRes &r = sres[is];
+        if (t.rc) continue;                     // reported with the twin below
+        r.nout = t.nout;
+        r.ev = t.ev;
+        r.vecs.assign(t.vecs.size(), nullptr);
+        for (int i = 0; i < t.nout && !rc; ++i) {
+            const dmed_sector *T = secs[it];
+            if (t.vecs[i]->prec != DMED_FP64) {
+                rc = fail(DMED_ESTATE, "twin sector: FP64 eigenvectors expected");
+                break;
+            }
+            if ((rc = dmed_vec_create(secs[is], DMED_FP64, &r.vecs[i]))) break;
+            rc = launch_transpose((const double *)t.vecs[i]->data, T->up.dim, T->dw.dim,
+                                  (double *)r.vecs[i]->data, ex().stream);
+        }
+    }
+    if (!rc && twin) DMED_HIP(hipStreamSynchronize(ex().stream));
     for (int is = 0; is < nsectors && !rc; ++is) {
         SectorRes &r = sres[is];
         if (r.rc) {

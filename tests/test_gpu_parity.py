@@ -629,6 +629,59 @@ def test_hybrid_bath_solve_vs_oracle(name):
                 assert rel(r8["sreal"][s, s, a, b], o8["sreal_full"][s, a, b]) < 1e-8
 
 
+@pytest.mark.parametrize("case", ["m2", "m1_finite", "m9_hybrid", "workload4"])
+def test_twin_sectors_match_full_diagonalisation(case):
+    """Nspin = 1: sectors (nu, nd) with nu < nd take the transposed
+    eigenvectors of their twin (nd, nu).  The solve must agree with the one
+    that diagonalises every sector (DMED_TWIN=0): same kept states, E0, n, d,
+    G and Sigma on both axes.  m2 has its degenerate ground states in the
+    twin sectors (3,1) and (1,3); workload 4 is the Ns = 12 all-sector solve."""
+    from dmed import configs
+    if case == "workload4":
+        c = configs.config(4)
+        set_cfg(c)
+        norb, nspin, ns = 1, 1, 12
+        kw = dict(beta=c["beta"], lmats=512, lreal=64, lanc_ngfiter=100)
+    else:
+        name = {"m2": "m2_norb2_nb1", "m1_finite": "m1_norb1_nb3",
+                "m9_hybrid": "m9_hybrid_norb2_nb3"}[case]
+        g = load(name)
+        ns = set_both(g)
+        norb, nspin = int(g["norb"]), int(g["nspin"])
+        kw = dict(beta=float(g["beta"]), lmats=len(g["wm"]), lreal=len(g["wr"]),
+                  wini=float(g["wr"][0]), wfin=float(g["wr"][-1]), eps=float(g["eps"]),
+                  lanc_ngfiter=200)
+        if case == "m1_finite":
+            kw.update(beta=10.0, lanc_nstates_sector=6, lanc_nstates_total=20)
+    say(f"[{case}] twin sectors on/off")
+    import time
+    os.environ["DMED_TWIN"] = "0"
+    try:
+        t = time.time()
+        full = dmed.solve(norb, nspin, ns, dmed.default_params(**kw))
+        t_full = time.time() - t
+    finally:
+        del os.environ["DMED_TWIN"]
+    t = time.time()
+    tw = dmed.solve(norb, nspin, ns, dmed.default_params(**kw))
+    t_tw = time.time() - t
+    say(f"[{case}] all sectors {t_full:.3f} s, twin reuse {t_tw:.3f} s")
+    assert tw["nstates"] == full["nstates"]
+    assert abs(tw["egs"] - full["egs"]) <= 1e-12 * abs(full["egs"])
+    # finite T keeps the lanc_nstates_total lowest states: at the cut, twin
+    # copies with exactly equal energies and independently computed ones
+    # (equal to round-off) can keep a different member of a twin pair, which
+    # moves G by that state's Boltzmann weight times G_up - G_dw (~1e-8 here)
+    tol = 1e-6 if case == "m1_finite" else 1e-9
+    assert np.allclose(tw["dens"], full["dens"], atol=tol)
+    assert np.allclose(tw["docc"], full["docc"], atol=tol)
+    assert np.abs(tw["gmats"] - full["gmats"]).max() < tol
+    assert np.abs(tw["smats"] - full["smats"]).max() < 10 * tol
+    # real axis in relative L2: chains run past the Krylov dimension of the
+    # small sectors carry round-off dependent ghost poles
+    assert np.linalg.norm(tw["greal"] - full["greal"]) / np.linalg.norm(full["greal"]) < 1e-3
+
+
 def test_apply_occ_vs_oracle():
     """N psi for single orbitals, pairs and all orbitals (bit-exact: the
     factor is a small integer) against the oracle's occupation vectors."""
