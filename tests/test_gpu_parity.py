@@ -629,7 +629,7 @@ def test_hybrid_bath_solve_vs_oracle(name):
                 assert rel(r8["sreal"][s, s, a, b], o8["sreal_full"][s, a, b]) < 1e-8
 
 
-@pytest.mark.parametrize("case", ["m2", "m1_finite", "m9_hybrid", "workload4"])
+@pytest.mark.parametrize("case", ["m2", "m1_finite", "m9_hybrid", "m3_jh_nspin1", "workload4"])
 def test_twin_sectors_match_full_diagonalisation(case):
     """Nspin = 1: sectors (nu, nd) with nu < nd take the transposed
     eigenvectors of their twin (nd, nu).  The solve must agree with the one
@@ -642,6 +642,18 @@ def test_twin_sectors_match_full_diagonalisation(case):
         set_cfg(c)
         norb, nspin, ns = 1, 1, 12
         kw = dict(beta=c["beta"], lmats=512, lreal=64, lanc_ngfiter=100)
+    elif case == "m3_jh_nspin1":
+        # the spin-up half of m3 as an Nspin = 1 model with spin exchange and
+        # pair hopping, which are symmetric under the spin flip too
+        g = load("m3_norb2_nb2_jh")
+        norb, nspin, nb = 2, 1, int(g["nbath"])
+        st = 2 * norb * nb
+        bath = np.concatenate([g["bath"][:norb * nb], g["bath"][st:st + norb * nb]])
+        hl = np.asarray(g["hloc_f"]).reshape((2, 2, norb, norb), order="F")[0, 0]
+        ns = dmed.set_model(norb, nb, 1, bath, hl.flatten(order="F"), list(g["uloc"][:3]),
+                            float(g["ust"]), float(g["jh"]), float(g["jx"]), float(g["jp"]),
+                            float(g["xmu"]), bool(g["hfmode"]), True)
+        kw = dict(beta=50.0, lmats=64, lreal=48, lanc_ngfiter=200)
     else:
         name = {"m2": "m2_norb2_nb1", "m1_finite": "m1_norb1_nb3",
                 "m9_hybrid": "m9_hybrid_norb2_nb3"}[case]
