@@ -54,8 +54,8 @@ def hybrid_target(z, e, v, hloc, xmu, scheme):
 
 
 @pytest.mark.parametrize("scheme", ["delta", "weiss"])
-@pytest.mark.parametrize("weight", [0, 3])
-def test_hybrid_fit_recovers_exact_bath(scheme, weight):
+@pytest.mark.parametrize("weight,norb", [(0, 2), (3, 2), (1, 3)])
+def test_hybrid_fit_recovers_exact_bath(scheme, weight, norb):
     """dmed_chi2_fit_hybrid (chi2_fitgf_hybrid_normal): two orbitals sharing
     four levels, off-diagonal Hloc; the target is the bath's own Delta or G0
     matrix, the start a perturbed bath.  The fit must return to chi^2 ~ 0 and
@@ -64,8 +64,8 @@ def test_hybrid_fit_recovers_exact_bath(scheme, weight):
     beta, L = 50.0, 300
     wm = np.pi / beta * (2 * np.arange(1, L + 1) - 1)
     e0 = np.array([-1.0, -0.3, 0.35, 1.2])
-    v0 = np.array([[0.4, 0.25, 0.3, 0.5], [0.2, 0.45, 0.35, 0.3]])
-    hloc = np.array([[-0.1, 0.08], [0.08, 0.05]])
+    v0 = np.array([[0.4, 0.25, 0.3, 0.5], [0.2, 0.45, 0.35, 0.3], [0.3, 0.2, 0.5, 0.25]])[:norb]
+    hloc = np.array([[-0.1, 0.08, 0.02], [0.08, 0.05, -0.03], [0.02, -0.03, 0.12]])[:norb, :norb]
     xmu = 0.1
     fg = hybrid_target(1j * wm, e0, v0, hloc, xmu, scheme)
     rng = np.random.default_rng(10 + weight)
