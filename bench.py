@@ -251,7 +251,7 @@ def main():
                          "plain FP64 vs mixed precision")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
+        print("bench.py measures one GPU: the solver runs the hot path on a single MI355X "
               "(no multi-GPU sharding); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
