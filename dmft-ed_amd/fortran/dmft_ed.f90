@@ -990,9 +990,13 @@ contains
     type(dmed_local_obs_t),allocatable,target :: obs(:)
     real(8),allocatable,target    :: bcopy(:,:), hcopy(:,:,:,:,:), dens(:,:), docc(:,:)
     complex(8),allocatable,target :: gm(:,:,:,:,:,:), gr(:,:,:,:,:,:), sm(:,:,:,:,:,:), sr(:,:,:,:,:,:)
+    ! per-site susceptibilities (CHIFLAG), site index last
+    real(8),allocatable,target    :: cst(:,:,:), cdt(:,:,:,:), cmt(:,:,:,:), ctt(:,:)
+    complex(8),allocatable,target :: csi(:,:,:), csw(:,:,:), cdi(:,:,:,:), cdw(:,:,:,:)
+    complex(8),allocatable,target :: cmi(:,:,:,:), cmw(:,:,:,:), cti(:,:), ctw(:,:)
+    integer :: lt
     if(isetup)stop "ed_solve_lattice: call ed_init_solver first"
     if(.not.allocated(neigen_sectorii))stop "ed_solve_lattice: call ed_init_solver with the lattice bath first"
-    if(chiflag)stop "ed_solve_lattice: CHIFLAG with inequivalent sites is not built"
     Nlat = size(bath,1)
     Nb = get_bath_dimension()
     if(size(bath,2)/=Nb)stop "init_lattice_bath: wrong bath size dimension 1 or 2 "
@@ -1028,6 +1032,23 @@ contains
        o(ilat)%dens_chi_mix_w = c_null_ptr; o(ilat)%dens_chi_tot_iv = c_null_ptr
        o(ilat)%dens_chi_tot_tau = c_null_ptr; o(ilat)%dens_chi_tot_w = c_null_ptr
     enddo
+    if(chiflag)then
+       lt = max(int(beta),Ltau)
+       allocate(cst(Norb+1,0:lt,Nlat), csi(Norb+1,0:Lmats,Nlat), csw(Norb+1,Lreal,Nlat))
+       allocate(cdt(Norb,Norb,0:lt,Nlat), cdi(Norb,Norb,0:Lmats,Nlat), cdw(Norb,Norb,Lreal,Nlat))
+       allocate(cmt(Norb,Norb,0:lt,Nlat), cmi(Norb,Norb,0:Lmats,Nlat), cmw(Norb,Norb,Lreal,Nlat))
+       allocate(ctt(0:lt,Nlat), cti(0:Lmats,Nlat), ctw(Lreal,Nlat))
+       do ilat=1,Nlat
+          o(ilat)%spin_chi_tau = c_loc(cst(1,0,ilat)); o(ilat)%spin_chi_iv = c_loc(csi(1,0,ilat))
+          o(ilat)%spin_chi_w = c_loc(csw(1,1,ilat))
+          o(ilat)%dens_chi_tau = c_loc(cdt(1,1,0,ilat)); o(ilat)%dens_chi_iv = c_loc(cdi(1,1,0,ilat))
+          o(ilat)%dens_chi_w = c_loc(cdw(1,1,1,ilat))
+          o(ilat)%dens_chi_mix_tau = c_loc(cmt(1,1,0,ilat)); o(ilat)%dens_chi_mix_iv = c_loc(cmi(1,1,0,ilat))
+          o(ilat)%dens_chi_mix_w = c_loc(cmw(1,1,1,ilat))
+          o(ilat)%dens_chi_tot_tau = c_loc(ctt(0,ilat)); o(ilat)%dens_chi_tot_iv = c_loc(cti(0,ilat))
+          o(ilat)%dens_chi_tot_w = c_loc(ctw(1,ilat))
+       enddo
+    endif
     call ed_check(dmed_solve_sites(int(Nlat,c_int32_t),m,p,o),"dmed_solve_sites")
     if(allocated(Smatsii))deallocate(Smatsii,Srealii,Gmatsii,Grealii,nii,dii,mii,eii,ddii)
     allocate(Smatsii(Nlat,Nspin,Nspin,Norb,Norb,Lmats), Srealii(Nlat,Nspin,Nspin,Norb,Norb,Lreal))
@@ -1047,6 +1068,25 @@ contains
        if(ed_print_Sigma)call write_gf_files("impSigma",sm(:,:,:,:,:,ilat),sr(:,:,:,:,:,ilat))
        if(ed_print_G)call write_gf_files("impG",gm(:,:,:,:,:,ilat),gr(:,:,:,:,:,ilat))
        if(ed_print_G0)call ed_print_impG0()
+       if(chiflag)then
+          ! the site's susceptibilities into the module arrays and its files
+          ! (buildChi_impurity inside ed_solve_single, ED_MAIN.f90:282)
+          if(allocated(spinChi_tau))deallocate(spinChi_tau,spinChi_iv,spinChi_w)
+          if(allocated(densChi_tau))deallocate(densChi_tau,densChi_iv,densChi_w,densChi_mix_tau,&
+               densChi_mix_iv,densChi_mix_w,densChi_tot_tau,densChi_tot_iv,densChi_tot_w)
+          allocate(spinChi_tau(Norb+1,0:lt), spinChi_iv(Norb+1,0:Lmats), spinChi_w(Norb+1,Lreal))
+          allocate(densChi_tau(Norb,Norb,0:lt), densChi_mix_tau(Norb,Norb,0:lt))
+          allocate(densChi_iv(Norb,Norb,0:Lmats), densChi_mix_iv(Norb,Norb,0:Lmats))
+          allocate(densChi_w(Norb,Norb,Lreal), densChi_mix_w(Norb,Norb,Lreal))
+          allocate(densChi_tot_tau(0:lt), densChi_tot_iv(0:Lmats), densChi_tot_w(Lreal))
+          spinChi_tau = cst(:,:,ilat); spinChi_iv = csi(:,:,ilat); spinChi_w = csw(:,:,ilat)
+          densChi_tau = cdt(:,:,:,ilat); densChi_iv = cdi(:,:,:,ilat); densChi_w = cdw(:,:,:,ilat)
+          densChi_mix_tau = cmt(:,:,:,ilat); densChi_mix_iv = cmi(:,:,:,ilat)
+          densChi_mix_w = cmw(:,:,:,ilat)
+          densChi_tot_tau = ctt(:,ilat); densChi_tot_iv = cti(:,ilat); densChi_tot_w = ctw(:,ilat)
+          call print_chi_spin()
+          call print_chi_dens()
+       endif
        if(ed_verbose>=1)write(LOGfile,"(A,I4,A,F20.12,A,I4)")"site",ilat,"  Egs =",o(ilat)%egs,&
             "  states:",o(ilat)%nstates
     enddo
@@ -1063,19 +1103,19 @@ contains
     real(8) :: pi, wr
     pi = acos(-1d0)
     lt = ubound(ctau,1)
-    open(newunit=unit,file=stem//"_tau.ed")
+    open(newunit=unit,file=stem//"_tau"//trim(ed_file_suffix)//".ed")
     do i=0,lt
        write(unit,"(2ES24.15)")beta*dble(i)/dble(max(lt,1)),ctau(i)
     enddo
     close(unit)
-    open(newunit=unit,file=stem//"_realw.ed")
+    open(newunit=unit,file=stem//"_realw"//trim(ed_file_suffix)//".ed")
     do i=1,Lreal
        wr = wini
        if(Lreal>1)wr = wini + (wfin-wini)*dble(i-1)/dble(Lreal-1)
        write(unit,"(3ES24.15)")wr,aimag(cw(i)),dble(cw(i))
     enddo
     close(unit)
-    open(newunit=unit,file=stem//"_iw.ed")
+    open(newunit=unit,file=stem//"_iw"//trim(ed_file_suffix)//".ed")
     do i=0,Lmats
        write(unit,"(3ES24.15)")pi/beta*2d0*dble(i),aimag(civ(i)),dble(civ(i))
     enddo

@@ -118,15 +118,16 @@ def test_bethe_ineq_driver_one_iteration(tmp_path):
     three Ns=8 sites with their own baths (hamiltonian_ineq<site>.restart)
     and U = 2.0, 2.5, 3.0, solved concurrently by dmed_solve_sites; per site
     the used bath, dens, docc, energies and Sigma(iw) match the oracle's solve
-    of that site (1e-9 for observables, 1e-8 for Sigma), and the fit writes a
-    new bath per site."""
+    of that site (1e-9 for observables, 1e-8 for Sigma and the CHIFLAG
+    susceptibility files), and the fit writes a new bath per site."""
     assert os.path.exists(EXE_INEQ), "Fortran driver not built (__graft_entry__.build())"
     c = configs.config(1)
     lmats, nlat = 512, 3
     (tmp_path / "inputED.in").write_text("\n".join([
         "NORB=1", "NBATH=7", "NSPIN=1", "ULOC=2.0", "XMU=0.0", f"BETA={c['beta']}",
         f"LMATS={lmats}", "LREAL=64", "NLOOP=1", "LANC_NGFITER=200",
-        "LANC_NSTATES_TOTAL=1", "HFMODE=T", "CG_SCHEME=weiss", f"NLAT={nlat}", "DU=0.5", ""]))
+        "LANC_NSTATES_TOTAL=1", "HFMODE=T", "CG_SCHEME=weiss", f"NLAT={nlat}", "DU=0.5",
+        "CHIFLAG=T", "LTAU=100", ""]))
     baths = []
     nb = c["nbath"]
     for i in range(nlat):
@@ -143,7 +144,8 @@ def test_bethe_ineq_driver_one_iteration(tmp_path):
     summ = [list(map(float, l.split()[2:])) for l in open(tmp_path / "dmft_summary_ineq.ed")
             if l.strip()]
     assert len(summ) == nlat
-    p = edref.default_params(beta=c["beta"], lmats=lmats, lreal=64, lanc_ngfiter=200)
+    p = edref.default_params(beta=c["beta"], lmats=lmats, lreal=64, lanc_ngfiter=200,
+                             chiflag=True, ltau=100)
     for i in range(nlat):
         used = _read_restart(str(tmp_path / f"hamiltonian_ineq{i + 1:04d}.used"), nb)
         assert np.array_equal(used, baths[i])
@@ -161,6 +163,12 @@ def test_bethe_ineq_driver_one_iteration(tmp_path):
         assert np.abs(s[:, 2] + 1j * s[:, 1] - o["smats"][0, 0]).max() <= 1e-8
         new = _read_restart(str(tmp_path / f"hamiltonian_ineq{i + 1:04d}.restart"), nb)
         assert np.all(np.isfinite(new)) and not np.array_equal(new, baths[i])
+        # CHIFLAG: the site's spin and charge susceptibility files
+        sfx = f"_ineq{i + 1:04d}"
+        ci = np.loadtxt(tmp_path / f"spinChi_l1_iw{sfx}.ed")
+        assert np.abs(ci[:, 2] + 1j * ci[:, 1] - o["spin_chi_iv"][0]).max() <= 1e-8
+        dt = np.loadtxt(tmp_path / f"densChi_l11_tau{sfx}.ed")
+        assert np.abs(dt[:, 1] - o["dens_chi_tau"][0, 0]).max() <= 1e-8
 
 
 def _write_hybrid_restart(path, e, v):
