@@ -173,7 +173,8 @@ module DMFT_ED
   public :: ed_read_impSigma
   public :: set_Hloc, lso2nnn_reshape, nnn2lso_reshape, so2nn_reshape, nn2so_reshape
   public :: spin_symmetrize_bath, break_symmetry_bath, ph_symmetrize_bath, ph_trans_bath
-  public :: enforce_normal_bath, search_chemical_potential, ed_get_self_matsubara
+  public :: enforce_normal_bath, search_chemical_potential, search_chempot
+  public :: ed_get_self_matsubara
 
 contains
 
@@ -799,6 +800,37 @@ contains
        nth = 10d0**(nth_magnitude_old)
     endif
   end subroutine search_chemical_potential
+
+  !> search_chempot (ED_AUX_FUNX.f90:572-668): converged when |n - nread| <=
+  !> nerr; otherwise xmu -= 0.2*(n - nread), 0.4 after ten attempts.  The
+  !> reference's bracketing branch needs its ilarge/ismall counters, which it
+  !> never sets, so it is never taken.
+  subroutine search_chempot(xmu_tmp,dens_tmp,converged_)
+    real(8),intent(in)    :: dens_tmp
+    real(8),intent(inout) :: xmu_tmp
+    logical,intent(inout) :: converged_
+    real(8)               :: diffdens, delta_xmu
+    integer,save          :: inotbound=0, iattempt=1
+    integer               :: unit
+    diffdens = dens_tmp-nread
+    delta_xmu = 0.2d0
+    if(abs(diffdens)<=nerr)then
+       converged_ = .true.
+       inotbound = 0
+       open(newunit=unit,file="search_mu_iteration"//trim(ed_file_suffix)//".ed",position="append")
+       write(unit,'(3F25.12)')xmu_tmp,dens_tmp,diffdens
+       close(unit)
+    else
+       converged_ = .false.
+       inotbound = inotbound+1
+       if(inotbound>=10)delta_xmu = 0.4d0
+       xmu_tmp = xmu_tmp - delta_xmu*diffdens
+       open(newunit=unit,file="search_mu_iteration"//trim(ed_file_suffix)//".ed",position="append")
+       write(unit,'(3F25.12,1I5)')xmu_tmp,dens_tmp,diffdens,iattempt
+       close(unit)
+    endif
+    iattempt = iattempt+1
+  end subroutine search_chempot
 
   !> ed_get_self_matsubara (ED_IO/get_self_matsubara.f90): the anomalous
   !> self-energy, identically zero in the normal mode

@@ -136,6 +136,9 @@ UTIL_CHECK = """program util_check
   var = 0.5d0; conv = .true.
   call search_chemical_potential(var,0.8d0,conv)
   write(*,"(A,ES24.15,L2)")"mu ",var,conv
+  var = 0.5d0; conv = .true.
+  call search_chempot(var,0.7d0,conv)
+  write(*,"(A,ES24.15,L2)")"chempot ",var,conv
 end program util_check
 """
 
@@ -163,7 +166,7 @@ def test_fortran_bath_utilities(tmp_path):
     for line in out.splitlines():
         parts = line.split()
         if parts and parts[0] in ("spin_sym", "break", "ph_sym", "ph_trans", "spin_sym_lat",
-                                  "so2nn", "nn2so", "lso2nnn", "nnn2lso", "mu"):
+                                  "so2nn", "nn2so", "lso2nnn", "nnn2lso", "mu", "chempot"):
             res[parts[0]] = parts[1:]
     nspin, norb, nb = 2, 2, 3
     st = nspin * norb * nb
@@ -188,7 +191,7 @@ def test_fortran_bath_utilities(tmp_path):
                     x[io], x[st + io] = e[s_, a, k], v[s_, a, k]
         return x
 
-    got = {k: np.array(list(map(float, v))) for k, v in res.items() if k != "mu"}
+    got = {k: np.array(list(map(float, v))) for k, v in res.items() if k not in ("mu", "chempot")}
     e, v = ev(b0)
     e1, v1 = e.copy(), v.copy()
     e1[1], v1[1] = e[0], v[0]
@@ -231,4 +234,6 @@ def test_fortran_bath_utilities(tmp_path):
     assert np.array_equal(got["nnn2lso"], back.flatten(order="F"))
     # n = 0.8 above nread = 0.6: mu steps down by ndelta, not converged
     assert abs(float(res["mu"][0]) - 0.4) < 1e-14 and res["mu"][1] == "F"
+    # search_chempot: xmu -= 0.2*(n - nread) = 0.5 - 0.2*0.1
+    assert abs(float(res["chempot"][0]) - 0.48) < 1e-14 and res["chempot"][1] == "F"
 
