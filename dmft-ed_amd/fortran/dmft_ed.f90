@@ -174,7 +174,7 @@ module DMFT_ED
   public :: set_Hloc, lso2nnn_reshape, nnn2lso_reshape, so2nn_reshape, nn2so_reshape
   public :: spin_symmetrize_bath, break_symmetry_bath, ph_symmetrize_bath, ph_trans_bath
   public :: enforce_normal_bath, search_chemical_potential, search_chempot
-  public :: ed_get_self_matsubara
+  public :: ed_get_self_matsubara, ed_get_density_matrix
 
 contains
 
@@ -1861,5 +1861,154 @@ contains
        yii = mii(1:Nlat,iorb)
     endif
   end subroutine ed_get_mag_lattice_2
+
+  ! The impurity density matrix in the basis custom_rot (ED_IO.f90:400-515),
+  ! written to imp_density_matrix.dat in the same layout. In the normal mode
+  ! with the normal bath the matrix <c+_{a,s} c_{b,s'}> of ED_OBSERVABLES.f90
+  ! :229-282 is diagonal, its entries being the orbital densities per spin of
+  ! the last solve: rho_diag(io,io) = n(a,s), io = a + (s-1)*Norb, and
+  ! rho = R rho_diag R^+. With dm_eig_ and dm_rot_ present, rho is also
+  ! diagonalised (eigenvalues ascending, eigenvectors in the columns). For
+  ! the hybrid bath the reference fills neither matrix, and neither does this.
+  subroutine ed_get_density_matrix(dm_,custom_rot,dm_eig_,dm_rot_)
+    complex(8),allocatable,intent(out)          :: dm_(:,:)
+    complex(8),allocatable,intent(in)           :: custom_rot(:,:)
+    real(8),allocatable,intent(out),optional    :: dm_eig_(:)
+    complex(8),allocatable,intent(out),optional :: dm_rot_(:,:)
+    complex(8),allocatable                      :: dm_diag(:,:)
+    integer :: n, io, jo, iorb, ispin, unit
+    if(.not.allocated(ed_dens_up))then
+       write(LOGfile,"(A)") "imp_density_matrix is not allocated"
+       stop
+    endif
+    n = Nspin*Norb
+    if(present(dm_eig_))then
+       allocate(dm_eig_(n)); dm_eig_ = 0d0
+    endif
+    if(present(dm_rot_))then
+       allocate(dm_rot_(n,n)); dm_rot_ = (0d0,0d0)
+    endif
+    allocate(dm_(n,n), dm_diag(n,n))
+    dm_ = (0d0,0d0); dm_diag = (0d0,0d0)
+    if(trim(bath_type)=="normal")then
+       do iorb=1,Norb
+          do ispin=1,Nspin
+             io = iorb + (ispin-1)*Norb
+             if(ispin==1)then
+                dm_diag(io,io) = cmplx(ed_dens_up(iorb),0d0,kind=8)
+             else
+                dm_diag(io,io) = cmplx(ed_dens_dw(iorb),0d0,kind=8)
+             endif
+          enddo
+       enddo
+       dm_ = matmul(custom_rot,matmul(dm_diag,transpose(conjg(custom_rot))))
+       if(present(dm_eig_).and.present(dm_rot_))then
+          dm_rot_ = dm_
+          call herm_eig(dm_rot_,dm_eig_)
+       endif
+    endif
+    open(newunit=unit,file="imp_density_matrix.dat",action="write",position="rewind",status="unknown")
+    write(unit,"(A30)")"# Re{rho_asbs'}:"
+    do io=1,n
+       write(unit,"(90(F15.9,1X))") (real(dm_(io,jo)),jo=1,n)
+    enddo
+    write(unit,*)
+    write(unit,"(A30)")"# Im{rho_asbs'}:"
+    do io=1,n
+       write(unit,"(90(F15.9,1X))") (aimag(dm_(io,jo)),jo=1,n)
+    enddo
+    write(unit,*)
+    if(present(dm_eig_).and.present(dm_rot_))then
+       write(unit,"(A30)")"# rho_tilda"
+       write(unit,'(10F22.12)') dm_eig_
+       write(unit,*)
+       write(unit,"(A30)")"# Re{theta}:"
+       do io=1,n
+          write(unit,"(90(F15.9,1X))") (real(dm_rot_(io,jo)),jo=1,n)
+       enddo
+       write(unit,*)
+       write(unit,"(A30)")"# Im{theta}:"
+       do io=1,n
+          write(unit,"(90(F15.9,1X))") (aimag(dm_rot_(io,jo)),jo=1,n)
+       enddo
+       write(unit,*)
+    endif
+    write(unit,"(A30)")"# Re{rho_Jj}"
+    do io=1,n
+       write(unit,"(90(F15.9,1X))") (real(dm_diag(io,jo)),jo=1,n)
+    enddo
+    write(unit,*)
+    write(unit,"(A30)")"# Im{rho_Jj}"
+    do io=1,n
+       write(unit,"(90(F15.9,1X))") (aimag(dm_diag(io,jo)),jo=1,n)
+    enddo
+    write(unit,*)
+    write(unit,"(A30)")"# J basis densities"
+    write(unit,"(90(F15.9,1X))") (real(dm_diag(io,io)),io=1,n)
+    close(unit)
+  end subroutine ed_get_density_matrix
+
+  ! Eigenvalues (ascending) and eigenvectors (columns of a on return) of a
+  ! small Hermitian matrix by cyclic Jacobi rotations. For the pair (p,q) the
+  ! phase of a(p,q) is first moved onto basis vector q, which leaves a real
+  ! symmetric 2x2 block that the real rotation with t = tan(phi),
+  ! t^2 + 2 t theta - 1 = 0, theta = (a_qq - a_pp)/(2 a_pq), zeroes.
+  subroutine herm_eig(a,w)
+    complex(8),intent(inout) :: a(:,:)
+    real(8),intent(inout)    :: w(:)
+    complex(8),allocatable   :: z(:,:), v(:,:)
+    complex(8) :: ph
+    real(8)    :: g, th, t, c, s, off, tmp
+    integer    :: n, p, q, sweep, i, k
+    n = size(a,1)
+    allocate(z(n,n), v(n,n))
+    z = (0d0,0d0)
+    do i=1,n
+       z(i,i) = (1d0,0d0)
+    enddo
+    do sweep=1,100
+       off = 0d0
+       do p=1,n-1
+          do q=p+1,n
+             off = off + abs(a(p,q))**2
+          enddo
+       enddo
+       if(off < 1d-30)exit
+       do p=1,n-1
+          do q=p+1,n
+             g = abs(a(p,q))
+             if(g < 1d-300)cycle
+             ph = a(p,q)/g
+             th = (real(a(q,q)) - real(a(p,p)))/(2d0*g)
+             t = sign(1d0,th)/(abs(th) + sqrt(th*th + 1d0))
+             c = 1d0/sqrt(t*t + 1d0)
+             s = t*c
+             v = (0d0,0d0)
+             do i=1,n
+                v(i,i) = (1d0,0d0)
+             enddo
+             v(p,p) = c
+             v(q,p) = -s*conjg(ph)
+             v(p,q) = s
+             v(q,q) = c*conjg(ph)
+             a = matmul(transpose(conjg(v)),matmul(a,v))
+             z = matmul(z,v)
+             a(p,q) = (0d0,0d0)
+             a(q,p) = (0d0,0d0)
+          enddo
+       enddo
+    enddo
+    do i=1,n
+       w(i) = real(a(i,i))
+    enddo
+    do i=1,n-1
+       k = minloc(w(i:n),1) + i - 1
+       if(k/=i)then
+          tmp = w(i); w(i) = w(k); w(k) = tmp
+          v(:,1) = z(:,i); z(:,i) = z(:,k); z(:,k) = v(:,1)
+       endif
+    enddo
+    a = z
+  end subroutine herm_eig
 
 end module DMFT_ED
