@@ -126,6 +126,34 @@ module DMFT_ED
   interface ed_chi2_fitgf
      module procedure ed_chi2_fitgf_single, ed_fit_bath_sites_normal
   end interface ed_chi2_fitgf
+  interface ed_get_self_matsubara
+     module procedure anomalous_zero_1, anomalous_zero_2, anomalous_zero_3
+     module procedure anomalous_zero_lattice_1, anomalous_zero_lattice_2, anomalous_zero_lattice_3
+  end interface ed_get_self_matsubara
+  interface ed_get_self_real
+     module procedure anomalous_zero_1, anomalous_zero_2, anomalous_zero_3
+     module procedure anomalous_zero_lattice_1, anomalous_zero_lattice_2, anomalous_zero_lattice_3
+  end interface ed_get_self_real
+  interface ed_get_fimp_matsubara
+     module procedure anomalous_zero_1, anomalous_zero_2, anomalous_zero_3
+     module procedure anomalous_zero_lattice_1, anomalous_zero_lattice_2, anomalous_zero_lattice_3
+  end interface ed_get_fimp_matsubara
+  interface ed_get_fimp_real
+     module procedure anomalous_zero_1, anomalous_zero_2, anomalous_zero_3
+     module procedure anomalous_zero_lattice_1, anomalous_zero_lattice_2, anomalous_zero_lattice_3
+  end interface ed_get_fimp_real
+  interface ed_get_phisc
+     module procedure ed_get_phisc_1, ed_get_phisc_2, ed_get_phisc_lattice_1, ed_get_phisc_lattice_2
+  end interface ed_get_phisc
+  interface get_component_bath
+     module procedure get_full_component_bath, get_spin_component_bath, get_spin_orb_component_bath
+  end interface get_component_bath
+  interface set_component_bath
+     module procedure set_full_component_bath, set_spin_component_bath, set_spin_orb_component_bath
+  end interface set_component_bath
+  interface copy_component_bath
+     module procedure copy_full_component_bath, copy_spin_component_bath, copy_spin_orb_component_bath
+  end interface copy_component_bath
   interface ed_get_sigma_matsubara
      module procedure ed_get_sigma_matsubara_1, ed_get_sigma_matsubara_2, ed_get_sigma_matsubara_3
      module procedure ed_get_sigma_matsubara_lattice_1, ed_get_sigma_matsubara_lattice_2, ed_get_sigma_matsubara_lattice_3
@@ -174,7 +202,12 @@ module DMFT_ED
   public :: set_Hloc, lso2nnn_reshape, nnn2lso_reshape, so2nn_reshape, nn2so_reshape
   public :: spin_symmetrize_bath, break_symmetry_bath, ph_symmetrize_bath, ph_trans_bath
   public :: enforce_normal_bath, search_chemical_potential, search_chempot
-  public :: ed_get_self_matsubara, ed_get_density_matrix
+  public :: ed_get_self_matsubara, ed_get_self_real, ed_get_fimp_matsubara, ed_get_fimp_real
+  public :: ed_get_phisc, ed_get_density_matrix
+  public :: get_component_bath_dimension, get_spin_component_bath_dimension
+  public :: get_orb_component_bath_dimension, get_spin_orb_component_bath_dimension
+  public :: get_component_bath, set_component_bath, copy_component_bath
+  public :: so2os_reshape, os2so_reshape
 
 contains
 
@@ -832,12 +865,354 @@ contains
     iattempt = iattempt+1
   end subroutine search_chempot
 
-  !> ed_get_self_matsubara (ED_IO/get_self_matsubara.f90): the anomalous
-  !> self-energy, identically zero in the normal mode
-  subroutine ed_get_self_matsubara(SAmats)
-    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: SAmats
-    SAmats = (0d0,0d0)
-  end subroutine ed_get_self_matsubara
+  !> The anomalous getters ed_get_self_matsubara/real and
+  !> ed_get_fimp_matsubara/real (ED_IO/get_self_matsubara.f90,
+  !> get_self_realaxis.f90, get_fimp_matsubara.f90, get_fimp_realaxis.f90):
+  !> the anomalous self-energy and Green's function, identically zero in the
+  !> normal mode. One set of forms serves all four names: the full
+  !> (Nspin,Nspin,Norb,Norb,L) array, the (Nspin*Norb,Nspin*Norb,L) matrix,
+  !> one element (L, ispin, jspin, iorb, jorb), and the same three with a
+  !> leading site index (Nlat).
+  subroutine anomalous_zero_1(F)
+    complex(8),intent(inout) :: F(:,:,:,:,:)
+    F = (0d0,0d0)
+  end subroutine anomalous_zero_1
+
+  subroutine anomalous_zero_2(F)
+    complex(8),intent(inout) :: F(:,:,:)
+    F = (0d0,0d0)
+  end subroutine anomalous_zero_2
+
+  subroutine anomalous_zero_3(F,ispin,jspin,iorb,jorb)
+    complex(8),intent(inout) :: F(:)
+    integer :: ispin, jspin, iorb, jorb
+    if(ispin>Nspin.or.jspin>Nspin.or.iorb>Norb.or.jorb>Norb)stop "ed_get error: index > N_spin or N_orbital"
+    F = (0d0,0d0)
+  end subroutine anomalous_zero_3
+
+  subroutine anomalous_zero_lattice_1(F,Nsites)
+    complex(8),intent(inout) :: F(:,:,:,:,:,:)
+    integer :: Nsites
+    F(1:Nsites,:,:,:,:,:) = (0d0,0d0)
+  end subroutine anomalous_zero_lattice_1
+
+  subroutine anomalous_zero_lattice_2(F,Nsites)
+    complex(8),intent(inout) :: F(:,:,:,:)
+    integer :: Nsites
+    F(1:Nsites,:,:,:) = (0d0,0d0)
+  end subroutine anomalous_zero_lattice_2
+
+  subroutine anomalous_zero_lattice_3(F,Nsites,ispin,jspin,iorb,jorb)
+    complex(8),intent(inout) :: F(:,:)
+    integer :: Nsites, ispin, jspin, iorb, jorb
+    if(ispin>Nspin.or.jspin>Nspin.or.iorb>Norb.or.jorb>Norb)stop "ed_get error: index > N_spin or N_orbital"
+    F(1:Nsites,:) = (0d0,0d0)
+  end subroutine anomalous_zero_lattice_3
+
+  !> ed_get_phisc (ED_IO/get_phisc.f90): the superconducting order parameter
+  !> per orbital, zero in the normal mode
+  subroutine ed_get_phisc_1(phisc)
+    real(8),dimension(Norb) :: phisc
+    phisc = 0d0
+  end subroutine ed_get_phisc_1
+
+  subroutine ed_get_phisc_2(phisc,iorb)
+    real(8) :: phisc
+    integer :: iorb
+    if(iorb>Norb)stop "ed_get_phisc error: orbital index > N_orbital"
+    phisc = 0d0
+  end subroutine ed_get_phisc_2
+
+  subroutine ed_get_phisc_lattice_1(yii,Nlat)
+    integer :: Nlat
+    real(8),dimension(Nlat,Norb) :: yii
+    yii = 0d0
+  end subroutine ed_get_phisc_lattice_1
+
+  subroutine ed_get_phisc_lattice_2(yii,Nlat,iorb)
+    integer :: Nlat, iorb
+    real(8),dimension(Nlat) :: yii
+    if(iorb>Norb)stop "ed_get_phisc error: orbital index > N_orbital"
+    yii = 0d0
+  end subroutine ed_get_phisc_lattice_2
+
+  !> so2os_reshape / os2so_reshape (ED_AUX_FUNX.f90:354-404): the
+  !> (Nspin*Norb)^2 matrix between the spin-major index io = a + (s-1)*Norb
+  !> and the orbital-major index io = s + (a-1)*Nspin
+  function so2os_reshape(fg,Nspin_,Norb_) result(g)
+    integer    :: Nspin_, Norb_
+    complex(8) :: fg(Nspin_*Norb_,Nspin_*Norb_), g(Nspin_*Norb_,Nspin_*Norb_)
+    integer    :: a, b, s1, s2
+    do s1=1,Nspin_
+       do s2=1,Nspin_
+          do a=1,Norb_
+             do b=1,Norb_
+                g(a+(s1-1)*Norb_, b+(s2-1)*Norb_) = fg(s1+(a-1)*Nspin_, s2+(b-1)*Nspin_)
+             enddo
+          enddo
+       enddo
+    enddo
+  end function so2os_reshape
+
+  function os2so_reshape(fg,Nspin_,Norb_) result(g)
+    integer    :: Nspin_, Norb_
+    complex(8) :: fg(Nspin_*Norb_,Nspin_*Norb_), g(Nspin_*Norb_,Nspin_*Norb_)
+    integer    :: a, b, s1, s2
+    do s1=1,Nspin_
+       do s2=1,Nspin_
+          do a=1,Norb_
+             do b=1,Norb_
+                g(s1+(a-1)*Nspin_, s2+(b-1)*Nspin_) = fg(a+(s1-1)*Norb_, b+(s2-1)*Norb_)
+             enddo
+          enddo
+       enddo
+    enddo
+  end function os2so_reshape
+
+  !---------------------------------------------------------------- bath components
+  !> get/set/copy_component_bath and the component dimensions
+  !> (ED_BATH/user_aux.f90:96-168, 185-334, 431-583, 678-894; type check
+  !> ED_BATH.f90:165-176). itype 1 is the bath levels e, itype 2 the
+  !> hybridisations v. The hybrid bath has one level set per spin,
+  !> e(Nspin,1,Nbath); its v is (Nspin,Norb,Nbath) as for the normal bath.
+  subroutine check_type_bath(itype)
+    integer :: itype
+    if(itype<1.or.itype>2)stop "check_type_bath error: ed_mode=normal, itype!=1,2"
+  end subroutine check_type_bath
+
+  pure integer function bath_levels_per_spin()
+    bath_levels_per_spin = Norb
+    if(trim(bath_type)=='hybrid')bath_levels_per_spin = 1
+  end function bath_levels_per_spin
+
+  function get_component_bath_dimension(itype) result(Ndim)
+    integer :: itype, Ndim(3)
+    call check_type_bath(itype)
+    Ndim = [Nspin,Norb,Nbath]
+    if(itype==1)Ndim(2) = bath_levels_per_spin()
+  end function get_component_bath_dimension
+
+  function get_spin_component_bath_dimension(itype) result(Ndim)
+    integer :: itype, Ndim(2)
+    call check_type_bath(itype)
+    Ndim = [Norb,Nbath]
+    if(itype==1)Ndim(1) = bath_levels_per_spin()
+  end function get_spin_component_bath_dimension
+
+  function get_orb_component_bath_dimension(itype) result(Ndim)
+    integer :: itype, Ndim(2)
+    Ndim = [Nspin,Nbath]
+  end function get_orb_component_bath_dimension
+
+  function get_spin_orb_component_bath_dimension(itype) result(Ndim)
+    integer :: itype, Ndim
+    Ndim = Nbath
+  end function get_spin_orb_component_bath_dimension
+
+  !> flat bath vector -> e(Nspin,levels,Nbath), v(Nspin,Norb,Nbath) and back,
+  !> in the layout of set_bath/get_bath, on arrays of the caller's
+  subroutine bath_split(bath,e,v,who)
+    real(8),intent(in)  :: bath(:)
+    real(8),intent(out) :: e(:,:,:), v(:,:,:)
+    character(len=*)    :: who
+    integer :: s, a, k, ne, st
+    if(size(bath)/=get_bath_dimension())stop trim(who)//" error: wrong bath dimensions"
+    ne = bath_levels_per_spin()
+    st = Nspin*ne*Nbath
+    do s=1,Nspin
+       do a=1,ne
+          do k=1,Nbath
+             e(s,a,k) = bath(k + (a-1)*Nbath + (s-1)*ne*Nbath)
+          enddo
+       enddo
+       do a=1,Norb
+          do k=1,Nbath
+             v(s,a,k) = bath(st + k + (a-1)*Nbath + (s-1)*Norb*Nbath)
+          enddo
+       enddo
+    enddo
+  end subroutine bath_split
+
+  subroutine bath_join(e,v,bath)
+    real(8),intent(in)    :: e(:,:,:), v(:,:,:)
+    real(8),intent(inout) :: bath(:)
+    integer :: s, a, k, ne, st
+    ne = bath_levels_per_spin()
+    st = Nspin*ne*Nbath
+    do s=1,Nspin
+       do a=1,ne
+          do k=1,Nbath
+             bath(k + (a-1)*Nbath + (s-1)*ne*Nbath) = e(s,a,k)
+          enddo
+       enddo
+       do a=1,Norb
+          do k=1,Nbath
+             bath(st + k + (a-1)*Nbath + (s-1)*Norb*Nbath) = v(s,a,k)
+          enddo
+       enddo
+    enddo
+  end subroutine bath_join
+
+  subroutine get_full_component_bath(array,bath_,itype)
+    real(8),dimension(:,:,:) :: array
+    real(8),dimension(:)     :: bath_
+    integer                  :: itype
+    real(8) :: e(Nspin,bath_levels_per_spin(),Nbath), v(Nspin,Norb,Nbath)
+    call bath_split(bath_,e,v,"get_component_bath")
+    if(any(shape(array)/=get_component_bath_dimension(itype)))&
+         stop "get_component_bath error: Array has the wrong shape"
+    if(itype==1)then
+       array = e
+    else
+       array = v
+    endif
+  end subroutine get_full_component_bath
+
+  subroutine get_spin_component_bath(array,bath_,itype,ispin)
+    real(8),dimension(:,:) :: array
+    real(8),dimension(:)   :: bath_
+    integer                :: itype, ispin
+    real(8) :: e(Nspin,bath_levels_per_spin(),Nbath), v(Nspin,Norb,Nbath)
+    call bath_split(bath_,e,v,"get_spin_component_bath")
+    if(any(shape(array)/=get_spin_component_bath_dimension(itype)))&
+         stop "get_spin_component_bath error: Array has the wrong shape"
+    if(ispin>Nspin)stop "get_spin_component_bath error: ispin > Nspin"
+    if(itype==1)then
+       array = e(ispin,:,:)
+    else
+       array = v(ispin,:,:)
+    endif
+  end subroutine get_spin_component_bath
+
+  subroutine get_spin_orb_component_bath(array,bath_,itype,ispin,iorb)
+    real(8),dimension(:) :: array
+    real(8),dimension(:) :: bath_
+    integer              :: itype, ispin, iorb
+    real(8) :: e(Nspin,bath_levels_per_spin(),Nbath), v(Nspin,Norb,Nbath)
+    call bath_split(bath_,e,v,"get_spin_orb_component_bath")
+    call check_type_bath(itype)
+    if(size(array)/=Nbath)stop "get_spin_orb_component_bath error: Array has the wrong shape"
+    if(ispin>Nspin)stop "get_spin_orb_component_bath error: ispin > Nspin"
+    if(iorb>Norb)stop "get_spin_orb_component_bath error: iorb > Norb"
+    if(itype==1)then
+       array = e(ispin,min(iorb,bath_levels_per_spin()),:)
+    else
+       array = v(ispin,iorb,:)
+    endif
+  end subroutine get_spin_orb_component_bath
+
+  subroutine set_full_component_bath(array,bath_,itype)
+    real(8),dimension(:,:,:) :: array
+    real(8),dimension(:)     :: bath_
+    integer                  :: itype
+    real(8) :: e(Nspin,bath_levels_per_spin(),Nbath), v(Nspin,Norb,Nbath)
+    call bath_split(bath_,e,v,"set_component_bath")
+    if(any(shape(array)/=get_component_bath_dimension(itype)))&
+         stop "set_component_bath error: Array has the wrong shape"
+    if(itype==1)then
+       e = array
+    else
+       v = array
+    endif
+    call bath_join(e,v,bath_)
+  end subroutine set_full_component_bath
+
+  subroutine set_spin_component_bath(array,bath_,itype,ispin)
+    real(8),dimension(:,:) :: array
+    real(8),dimension(:)   :: bath_
+    integer                :: itype, ispin
+    real(8) :: e(Nspin,bath_levels_per_spin(),Nbath), v(Nspin,Norb,Nbath)
+    call bath_split(bath_,e,v,"set_spin_component_bath")
+    if(any(shape(array)/=get_spin_component_bath_dimension(itype)))&
+         stop "set_spin_component_bath error: Array has the wrong shape"
+    if(ispin>Nspin)stop "set_spin_component_bath error: ispin > Nspin"
+    if(itype==1)then
+       e(ispin,:,:) = array
+    else
+       v(ispin,:,:) = array
+    endif
+    call bath_join(e,v,bath_)
+  end subroutine set_spin_component_bath
+
+  subroutine set_spin_orb_component_bath(array,bath_,itype,ispin,iorb)
+    real(8),dimension(:) :: array
+    real(8),dimension(:) :: bath_
+    integer              :: itype, ispin, iorb
+    real(8) :: e(Nspin,bath_levels_per_spin(),Nbath), v(Nspin,Norb,Nbath)
+    call bath_split(bath_,e,v,"set_spin_orb_component_bath")
+    call check_type_bath(itype)
+    if(size(array)/=Nbath)stop "set_spin_orb_component_bath error: Array has the wrong shape"
+    if(ispin>Nspin)stop "set_spin_orb_component_bath error: ispin > Nspin"
+    if(iorb>Norb)stop "set_spin_orb_component_bath error: iorb > Norb"
+    if(itype==1)then
+       e(ispin,min(iorb,bath_levels_per_spin()),:) = array
+    else
+       v(ispin,iorb,:) = array
+    endif
+    call bath_join(e,v,bath_)
+  end subroutine set_spin_orb_component_bath
+
+  subroutine copy_full_component_bath(bathIN,bathOUT,itype)
+    real(8),dimension(:) :: bathIN, bathOUT
+    integer              :: itype
+    real(8) :: ei(Nspin,bath_levels_per_spin(),Nbath), vi(Nspin,Norb,Nbath)
+    real(8) :: eo(Nspin,bath_levels_per_spin(),Nbath), vo(Nspin,Norb,Nbath)
+    call bath_split(bathIN,ei,vi,"copy_component_bath")
+    call bath_split(bathOUT,eo,vo,"copy_component_bath")
+    call check_type_bath(itype)
+    if(itype==1)then
+       eo = ei
+    else
+       vo = vi
+    endif
+    call bath_join(eo,vo,bathOUT)
+  end subroutine copy_full_component_bath
+
+  subroutine copy_spin_component_bath(bathIN,ispin,bathOUT,jspin,itype)
+    real(8),dimension(:) :: bathIN, bathOUT
+    integer              :: ispin, jspin
+    integer,optional     :: itype
+    real(8) :: ei(Nspin,bath_levels_per_spin(),Nbath), vi(Nspin,Norb,Nbath)
+    real(8) :: eo(Nspin,bath_levels_per_spin(),Nbath), vo(Nspin,Norb,Nbath)
+    call bath_split(bathIN,ei,vi,"copy_component_bath")
+    call bath_split(bathOUT,eo,vo,"copy_component_bath")
+    if(present(itype))call check_type_bath(itype)
+    if(ispin>Nspin.or.jspin>Nspin)stop "copy_spin_component_bath error: ispin/jspin > Nspin"
+    if(.not.present(itype))then
+       eo(jspin,:,:) = ei(ispin,:,:)
+       vo(jspin,:,:) = vi(ispin,:,:)
+    elseif(itype==1)then
+       eo(jspin,:,:) = ei(ispin,:,:)
+    else
+       vo(jspin,:,:) = vi(ispin,:,:)
+    endif
+    call bath_join(eo,vo,bathOUT)
+  end subroutine copy_spin_component_bath
+
+  subroutine copy_spin_orb_component_bath(bathIN,ispin,iorb,bathOUT,jspin,jorb,itype)
+    real(8),dimension(:) :: bathIN, bathOUT
+    integer              :: ispin, jspin, iorb, jorb
+    integer,optional     :: itype
+    real(8) :: ei(Nspin,bath_levels_per_spin(),Nbath), vi(Nspin,Norb,Nbath)
+    real(8) :: eo(Nspin,bath_levels_per_spin(),Nbath), vo(Nspin,Norb,Nbath)
+    integer :: ai, ao
+    call bath_split(bathIN,ei,vi,"copy_spin_orb_component_bath")
+    call bath_split(bathOUT,eo,vo,"copy_spin_orb_component_bath")
+    if(present(itype))call check_type_bath(itype)
+    if(ispin>Nspin.or.jspin>Nspin)stop "copy_spin_orb_component_bath error: ispin/jspin > Nspin"
+    if(iorb>Norb.or.jorb>Norb)stop "copy_spin_orb_component_bath error: iorb/jorb > Norb"
+    ai = min(iorb,bath_levels_per_spin())
+    ao = min(jorb,bath_levels_per_spin())
+    if(.not.present(itype))then
+       eo(jspin,ao,:) = ei(ispin,ai,:)
+       vo(jspin,jorb,:) = vi(ispin,iorb,:)
+    elseif(itype==1)then
+       eo(jspin,ao,:) = ei(ispin,ai,:)
+    else
+       vo(jspin,jorb,:) = vi(ispin,iorb,:)
+    endif
+    call bath_join(eo,vo,bathOUT)
+  end subroutine copy_spin_orb_component_bath
 
   !> ed_init_solver_single (ED_MAIN.f90:67-107)
   subroutine ed_init_solver_single(bath,Hloc)

@@ -361,3 +361,155 @@ def test_fortran_density_matrix(tmp_path):
     assert np.allclose(eig, np.sort(up), atol=1e-15)
     dm, eig, _, _ = _run_dm_check(tmp_path, norb, 1, "hybrid", up, dw, one)
     assert np.all(dm == 0) and np.all(eig == 0)
+
+
+COMP_CHECK = """program comp_check
+  use DMFT_ED
+  implicit none
+  real(8),allocatable    :: b(:), b2(:), a3(:,:,:), a2(:,:), a1(:)
+  complex(8)             :: m(4,4), r(4,4), back(4,4)
+  complex(8),allocatable :: f5(:,:,:,:,:), f3(:,:,:), f1(:), fl(:,:)
+  real(8)                :: phi(2), p1
+  integer                :: n, i, j, d3(3), d2(2)
+  character(len=7)       :: bt
+  read(*,*) bt
+  Norb = 2; Nspin = 2; Nbath = 3; bath_type = bt
+  n = get_bath_dimension()
+  allocate(b(n), b2(n), a1(Nbath))
+  b  = [(0.1d0*i, i=1,n)]
+  b2 = [(-1d0*i, i=1,n)]
+  d3 = get_component_bath_dimension(1)
+  write(*,"(A,3I4)") "dim_e", d3
+  allocate(a3(d3(1),d3(2),d3(3)))
+  call get_component_bath(a3,b,1)
+  write(*,"(A,100ES24.15)") "get_e", a3
+  deallocate(a3)
+  d3 = get_component_bath_dimension(2)
+  write(*,"(A,3I4)") "dim_v", d3
+  allocate(a3(d3(1),d3(2),d3(3)))
+  call get_component_bath(a3,b,2)
+  write(*,"(A,100ES24.15)") "get_v", a3
+  d2 = get_spin_component_bath_dimension(2)
+  allocate(a2(d2(1),d2(2)))
+  call get_component_bath(a2,b,2,2)
+  write(*,"(A,100ES24.15)") "get_v_s2", a2
+  call get_component_bath(a1,b,1,2,2)
+  write(*,"(A,100ES24.15)") "get_e_s2a2", a1
+  a1 = [7d0,8d0,9d0]
+  call set_component_bath(a1,b,2,1,2)
+  write(*,"(A,100ES24.15)") "set_v_s1a2", b
+  call copy_component_bath(b2,1,b,2)
+  write(*,"(A,100ES24.15)") "copy_s1_s2", b
+  call copy_component_bath(b2,2,1,b,1,2,1)
+  write(*,"(A,100ES24.15)") "copy_e_s2a1_s1a2", b
+  call copy_component_bath(b2,b,2)
+  write(*,"(A,100ES24.15)") "copy_v_full", b
+  a3 = 100d0 + a3
+  call set_component_bath(a3,b,2)
+  write(*,"(A,100ES24.15)") "set_v_full", b
+  do i=1,4
+     do j=1,4
+        m(i,j) = cmplx(10*i+j, -j, kind=8)
+     enddo
+  enddo
+  r = so2os_reshape(m,2,2)
+  back = os2so_reshape(r,2,2)
+  write(*,"(A,100ES24.15)") "so2os", ((real(r(i,j)),j=1,4),i=1,4)
+  write(*,"(A,ES24.15)") "roundtrip", maxval(abs(back-m))
+  allocate(f5(2,2,2,2,5), f3(4,4,5), f1(5), fl(3,5))
+  f5 = (1d0,1d0); f3 = (1d0,1d0); f1 = (1d0,1d0); fl = (1d0,1d0); phi = 1d0; p1 = 1d0
+  call ed_get_self_matsubara(f5)
+  call ed_get_self_real(f3)
+  call ed_get_fimp_matsubara(f1,1,2,2,1)
+  call ed_get_fimp_real(fl,3,1,1,1,1)
+  call ed_get_phisc(phi)
+  call ed_get_phisc(p1,2)
+  write(*,"(A,5ES24.15)") "anom", maxval(abs(f5)), maxval(abs(f3)), maxval(abs(f1)), &
+       maxval(abs(fl)), maxval(abs(phi))+abs(p1)
+end program comp_check
+"""
+
+
+def test_fortran_bath_components(tmp_path):
+    """get/set/copy_component_bath and the component dimensions
+    (ED_BATH/user_aux.f90:96-894) for the normal and the hybrid bath,
+    so2os/os2so_reshape (ED_AUX_FUNX.f90:354-404) and the anomalous getters
+    (zero in the normal mode), against the index rules of the flat bath
+    vector written out here: normal e(s,a,k) at k + a*Nb + s*Nb*Norb and v
+    after Nspin*Norb*Nb; hybrid e(s,k) at k + s*Nb and v(s,a,k) at
+    Nspin*Nb + k + a*Nb + s*Norb*Nb (0-based)."""
+    b_ = os.path.join(ROOT, "dmft-ed_amd", "fortran", "build")
+    if not os.path.exists(os.path.join(b_, "dmft_ed.o")):
+        pytest.skip("Fortran layer not built (run __graft_entry__.build())")
+    import shutil
+    flang = shutil.which("amdflang") or "/opt/rocm/llvm/bin/amdflang"
+    (tmp_path / "check.f90").write_text(COMP_CHECK)
+    lib = os.path.join(ROOT, "dmft-ed_amd")
+    subprocess.run([flang, "-O0", "-I", b_, "check.f90", os.path.join(b_, "dmed_cabi.o"),
+                    os.path.join(b_, "dmft_ed.o"), "-L" + lib, "-ldmed", "-Wl,-rpath," + lib,
+                    "-o", "check"], cwd=str(tmp_path), check=True, capture_output=True)
+    ns, no, nb = 2, 2, 3
+    for bt in ("normal", "hybrid"):
+        out = subprocess.run(["./check"], cwd=str(tmp_path), input=bt + "\n",
+                             capture_output=True, text=True, timeout=60, check=True).stdout
+        res = {l.split()[0]: np.array([float(x) for x in l.split()[1:]])
+               for l in out.splitlines() if l.strip()}
+        ne = 1 if bt == "hybrid" else no
+
+        def split(x):
+            e = np.zeros((ns, ne, nb))
+            v = np.zeros((ns, no, nb))
+            st = ns * ne * nb
+            for s in range(ns):
+                for a in range(ne):
+                    for k in range(nb):
+                        e[s, a, k] = x[k + a * nb + s * ne * nb]
+                for a in range(no):
+                    for k in range(nb):
+                        v[s, a, k] = x[st + k + a * nb + s * no * nb]
+            return e, v
+
+        def join(e, v):
+            x = np.zeros(ns * ne * nb + ns * no * nb)
+            st = ns * ne * nb
+            for s in range(ns):
+                for a in range(ne):
+                    for k in range(nb):
+                        x[k + a * nb + s * ne * nb] = e[s, a, k]
+                for a in range(no):
+                    for k in range(nb):
+                        x[st + k + a * nb + s * no * nb] = v[s, a, k]
+            return x
+
+        n = ns * ne * nb + ns * no * nb
+        b = 0.1 * np.arange(1, n + 1)
+        b2 = -1.0 * np.arange(1, n + 1)
+        assert np.array_equal(join(*split(b)), b)
+        e, v = split(b)
+        e2, v2 = split(b2)
+        assert list(res["dim_e"]) == [ns, ne, nb] and list(res["dim_v"]) == [ns, no, nb]
+        assert np.allclose(res["get_e"], e.ravel(order="F"), rtol=0, atol=1e-15)
+        assert np.allclose(res["get_v"], v.ravel(order="F"), rtol=0, atol=1e-15)
+        assert np.allclose(res["get_v_s2"], v[1].ravel(order="F"), rtol=0, atol=1e-15)
+        assert np.allclose(res["get_e_s2a2"], e[1, min(1, ne - 1)], rtol=0, atol=1e-15)
+        v[0, 1] = [7, 8, 9]
+        assert np.allclose(res["set_v_s1a2"], join(e, v), rtol=0, atol=1e-15)
+        e[1], v[1] = e2[0], v2[0]
+        assert np.allclose(res["copy_s1_s2"], join(e, v), rtol=0, atol=1e-15)
+        e[0, min(1, ne - 1)] = e2[1, 0]
+        assert np.allclose(res["copy_e_s2a1_s1a2"], join(e, v), rtol=0, atol=1e-15)
+        v = v2.copy()
+        assert np.allclose(res["copy_v_full"], join(e, v), rtol=0, atol=1e-15)
+        v = 100.0 + split(0.1 * np.arange(1, n + 1))[1]
+        assert np.allclose(res["set_v_full"], join(e, v), rtol=0, atol=1e-13)
+    # so2os: g(a + s*Norb, b + s'*Norb) = fg(s + a*Nspin, s' + b*Nspin), 0-based
+    m = np.array([[10 * (i + 1) + (j + 1) for j in range(4)] for i in range(4)], float)
+    g = np.zeros((4, 4))
+    for s1 in range(2):
+        for s2 in range(2):
+            for a in range(2):
+                for bb in range(2):
+                    g[a + s1 * 2, bb + s2 * 2] = m[s1 + a * 2, s2 + bb * 2]
+    assert np.array_equal(res["so2os"], g.ravel())
+    assert res["roundtrip"][0] == 0.0
+    assert np.all(res["anom"] == 0.0)
