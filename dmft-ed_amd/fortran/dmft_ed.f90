@@ -2333,7 +2333,7 @@ contains
     real(8),intent(inout)    :: w(:)
     complex(8),allocatable   :: z(:,:), v(:,:)
     complex(8) :: ph
-    real(8)    :: g, th, t, c, s, off, tmp
+    real(8)    :: g, th, t, c, s, off, tmp, fro
     integer    :: n, p, q, sweep, i, k
     n = size(a,1)
     allocate(z(n,n), v(n,n))
@@ -2341,6 +2341,7 @@ contains
     do i=1,n
        z(i,i) = (1d0,0d0)
     enddo
+    fro = sum(abs(a)**2)
     do sweep=1,100
        off = 0d0
        do p=1,n-1
@@ -2348,7 +2349,8 @@ contains
              off = off + abs(a(p,q))**2
           enddo
        enddo
-       if(off < 1d-30)exit
+       ! off-diagonal weight relative to the (rotation-invariant) Frobenius norm
+       if(off <= 1d-32*fro)exit
        do p=1,n-1
           do q=p+1,n
              g = abs(a(p,q))
