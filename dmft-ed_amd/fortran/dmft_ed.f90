@@ -129,18 +129,22 @@ module DMFT_ED
   interface ed_get_self_matsubara
      module procedure anomalous_zero_1, anomalous_zero_2, anomalous_zero_3
      module procedure anomalous_zero_lattice_1, anomalous_zero_lattice_2, anomalous_zero_lattice_3
+     module procedure anomalous_zero_lattice_11, anomalous_zero_lattice_21
   end interface ed_get_self_matsubara
   interface ed_get_self_real
      module procedure anomalous_zero_1, anomalous_zero_2, anomalous_zero_3
      module procedure anomalous_zero_lattice_1, anomalous_zero_lattice_2, anomalous_zero_lattice_3
+     module procedure anomalous_zero_lattice_11, anomalous_zero_lattice_21
   end interface ed_get_self_real
   interface ed_get_fimp_matsubara
      module procedure anomalous_zero_1, anomalous_zero_2, anomalous_zero_3
      module procedure anomalous_zero_lattice_1, anomalous_zero_lattice_2, anomalous_zero_lattice_3
+     module procedure anomalous_zero_lattice_11, anomalous_zero_lattice_21
   end interface ed_get_fimp_matsubara
   interface ed_get_fimp_real
      module procedure anomalous_zero_1, anomalous_zero_2, anomalous_zero_3
      module procedure anomalous_zero_lattice_1, anomalous_zero_lattice_2, anomalous_zero_lattice_3
+     module procedure anomalous_zero_lattice_11, anomalous_zero_lattice_21
   end interface ed_get_fimp_real
   interface ed_get_phisc
      module procedure ed_get_phisc_1, ed_get_phisc_2, ed_get_phisc_lattice_1, ed_get_phisc_lattice_2
@@ -157,18 +161,22 @@ module DMFT_ED
   interface ed_get_sigma_matsubara
      module procedure ed_get_sigma_matsubara_1, ed_get_sigma_matsubara_2, ed_get_sigma_matsubara_3
      module procedure ed_get_sigma_matsubara_lattice_1, ed_get_sigma_matsubara_lattice_2, ed_get_sigma_matsubara_lattice_3
+     module procedure ed_get_sigma_matsubara_lattice_11, ed_get_sigma_matsubara_lattice_21, ed_get_sigma_matsubara_lattice_31
   end interface ed_get_sigma_matsubara
   interface ed_get_sigma_real
      module procedure ed_get_sigma_real_1, ed_get_sigma_real_2, ed_get_sigma_real_3
      module procedure ed_get_sigma_real_lattice_1, ed_get_sigma_real_lattice_2, ed_get_sigma_real_lattice_3
+     module procedure ed_get_sigma_real_lattice_11, ed_get_sigma_real_lattice_21, ed_get_sigma_real_lattice_31
   end interface ed_get_sigma_real
   interface ed_get_gimp_matsubara
      module procedure ed_get_gimp_matsubara_1, ed_get_gimp_matsubara_2, ed_get_gimp_matsubara_3
      module procedure ed_get_gimp_matsubara_lattice_1, ed_get_gimp_matsubara_lattice_2, ed_get_gimp_matsubara_lattice_3
+     module procedure ed_get_gimp_matsubara_lattice_11, ed_get_gimp_matsubara_lattice_21, ed_get_gimp_matsubara_lattice_31
   end interface ed_get_gimp_matsubara
   interface ed_get_gimp_real
      module procedure ed_get_gimp_real_1, ed_get_gimp_real_2, ed_get_gimp_real_3
      module procedure ed_get_gimp_real_lattice_1, ed_get_gimp_real_lattice_2, ed_get_gimp_real_lattice_3
+     module procedure ed_get_gimp_real_lattice_11, ed_get_gimp_real_lattice_21, ed_get_gimp_real_lattice_31
   end interface ed_get_gimp_real
   interface ed_get_dens
      module procedure ed_get_dens_1, ed_get_dens_2, ed_get_dens_lattice_1, ed_get_dens_lattice_2
@@ -908,6 +916,154 @@ contains
     if(ispin>Nspin.or.jspin>Nspin.or.iorb>Norb.or.jorb>Norb)stop "ed_get error: index > N_spin or N_orbital"
     F(1:Nsites,:) = (0d0,0d0)
   end subroutine anomalous_zero_lattice_3
+
+  !> One site of the last lattice solve (ED_IO/get_sigma_matsubara.f90:72-100
+  !> and the same forms of get_sigma_realaxis, get_gimp_matsubara and
+  !> get_gimp_realaxis): _11 the full array, _21 the (Nspin*Norb)^2 matrix
+  !> with io = a + (s-1)*Norb, _31 one element.
+  subroutine ed_get_sigma_matsubara_lattice_11(F,ilat)
+    integer :: ilat
+    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: F
+    if(.not.allocated(Smatsii))stop "ed_get_sigma_matsubara error: no lattice solve"
+    if(ilat<1.or.ilat>size(Smatsii,1))stop "ed_get_sigma_matsubara error: site index out of range"
+    F = Smatsii(ilat,:,:,:,:,:)
+  end subroutine ed_get_sigma_matsubara_lattice_11
+
+  subroutine ed_get_sigma_matsubara_lattice_21(F,ilat)
+    integer :: ilat
+    complex(8),dimension(Nspin*Norb,Nspin*Norb,Lmats),intent(inout) :: F
+    integer :: s1, s2, a, b
+    if(.not.allocated(Smatsii))stop "ed_get_sigma_matsubara error: no lattice solve"
+    if(ilat<1.or.ilat>size(Smatsii,1))stop "ed_get_sigma_matsubara error: site index out of range"
+    do s1=1,Nspin
+       do s2=1,Nspin
+          do a=1,Norb
+             do b=1,Norb
+                F(a+(s1-1)*Norb, b+(s2-1)*Norb, :) = Smatsii(ilat,s1,s2,a,b,:)
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_sigma_matsubara_lattice_21
+
+  subroutine ed_get_sigma_matsubara_lattice_31(F,ilat,ispin,jspin,iorb,jorb)
+    integer :: ilat, ispin, jspin, iorb, jorb
+    complex(8),dimension(Lmats),intent(inout) :: F
+    if(.not.allocated(Smatsii))stop "ed_get_sigma_matsubara error: no lattice solve"
+    if(ilat<1.or.ilat>size(Smatsii,1))stop "ed_get_sigma_matsubara error: site index out of range"
+    F = Smatsii(ilat,ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_sigma_matsubara_lattice_31
+
+  subroutine ed_get_sigma_real_lattice_11(F,ilat)
+    integer :: ilat
+    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lreal),intent(inout) :: F
+    if(.not.allocated(Srealii))stop "ed_get_sigma_real error: no lattice solve"
+    if(ilat<1.or.ilat>size(Srealii,1))stop "ed_get_sigma_real error: site index out of range"
+    F = Srealii(ilat,:,:,:,:,:)
+  end subroutine ed_get_sigma_real_lattice_11
+
+  subroutine ed_get_sigma_real_lattice_21(F,ilat)
+    integer :: ilat
+    complex(8),dimension(Nspin*Norb,Nspin*Norb,Lreal),intent(inout) :: F
+    integer :: s1, s2, a, b
+    if(.not.allocated(Srealii))stop "ed_get_sigma_real error: no lattice solve"
+    if(ilat<1.or.ilat>size(Srealii,1))stop "ed_get_sigma_real error: site index out of range"
+    do s1=1,Nspin
+       do s2=1,Nspin
+          do a=1,Norb
+             do b=1,Norb
+                F(a+(s1-1)*Norb, b+(s2-1)*Norb, :) = Srealii(ilat,s1,s2,a,b,:)
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_sigma_real_lattice_21
+
+  subroutine ed_get_sigma_real_lattice_31(F,ilat,ispin,jspin,iorb,jorb)
+    integer :: ilat, ispin, jspin, iorb, jorb
+    complex(8),dimension(Lreal),intent(inout) :: F
+    if(.not.allocated(Srealii))stop "ed_get_sigma_real error: no lattice solve"
+    if(ilat<1.or.ilat>size(Srealii,1))stop "ed_get_sigma_real error: site index out of range"
+    F = Srealii(ilat,ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_sigma_real_lattice_31
+
+  subroutine ed_get_gimp_matsubara_lattice_11(F,ilat)
+    integer :: ilat
+    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lmats),intent(inout) :: F
+    if(.not.allocated(Gmatsii))stop "ed_get_gimp_matsubara error: no lattice solve"
+    if(ilat<1.or.ilat>size(Gmatsii,1))stop "ed_get_gimp_matsubara error: site index out of range"
+    F = Gmatsii(ilat,:,:,:,:,:)
+  end subroutine ed_get_gimp_matsubara_lattice_11
+
+  subroutine ed_get_gimp_matsubara_lattice_21(F,ilat)
+    integer :: ilat
+    complex(8),dimension(Nspin*Norb,Nspin*Norb,Lmats),intent(inout) :: F
+    integer :: s1, s2, a, b
+    if(.not.allocated(Gmatsii))stop "ed_get_gimp_matsubara error: no lattice solve"
+    if(ilat<1.or.ilat>size(Gmatsii,1))stop "ed_get_gimp_matsubara error: site index out of range"
+    do s1=1,Nspin
+       do s2=1,Nspin
+          do a=1,Norb
+             do b=1,Norb
+                F(a+(s1-1)*Norb, b+(s2-1)*Norb, :) = Gmatsii(ilat,s1,s2,a,b,:)
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_gimp_matsubara_lattice_21
+
+  subroutine ed_get_gimp_matsubara_lattice_31(F,ilat,ispin,jspin,iorb,jorb)
+    integer :: ilat, ispin, jspin, iorb, jorb
+    complex(8),dimension(Lmats),intent(inout) :: F
+    if(.not.allocated(Gmatsii))stop "ed_get_gimp_matsubara error: no lattice solve"
+    if(ilat<1.or.ilat>size(Gmatsii,1))stop "ed_get_gimp_matsubara error: site index out of range"
+    F = Gmatsii(ilat,ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_gimp_matsubara_lattice_31
+
+  subroutine ed_get_gimp_real_lattice_11(F,ilat)
+    integer :: ilat
+    complex(8),dimension(Nspin,Nspin,Norb,Norb,Lreal),intent(inout) :: F
+    if(.not.allocated(Grealii))stop "ed_get_gimp_real error: no lattice solve"
+    if(ilat<1.or.ilat>size(Grealii,1))stop "ed_get_gimp_real error: site index out of range"
+    F = Grealii(ilat,:,:,:,:,:)
+  end subroutine ed_get_gimp_real_lattice_11
+
+  subroutine ed_get_gimp_real_lattice_21(F,ilat)
+    integer :: ilat
+    complex(8),dimension(Nspin*Norb,Nspin*Norb,Lreal),intent(inout) :: F
+    integer :: s1, s2, a, b
+    if(.not.allocated(Grealii))stop "ed_get_gimp_real error: no lattice solve"
+    if(ilat<1.or.ilat>size(Grealii,1))stop "ed_get_gimp_real error: site index out of range"
+    do s1=1,Nspin
+       do s2=1,Nspin
+          do a=1,Norb
+             do b=1,Norb
+                F(a+(s1-1)*Norb, b+(s2-1)*Norb, :) = Grealii(ilat,s1,s2,a,b,:)
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine ed_get_gimp_real_lattice_21
+
+  subroutine ed_get_gimp_real_lattice_31(F,ilat,ispin,jspin,iorb,jorb)
+    integer :: ilat, ispin, jspin, iorb, jorb
+    complex(8),dimension(Lreal),intent(inout) :: F
+    if(.not.allocated(Grealii))stop "ed_get_gimp_real error: no lattice solve"
+    if(ilat<1.or.ilat>size(Grealii,1))stop "ed_get_gimp_real error: site index out of range"
+    F = Grealii(ilat,ispin,jspin,iorb,jorb,:)
+  end subroutine ed_get_gimp_real_lattice_31
+
+  subroutine anomalous_zero_lattice_11(F,ilat)
+    complex(8),intent(inout) :: F(:,:,:,:,:)
+    integer :: ilat
+    F = (0d0,0d0)
+  end subroutine anomalous_zero_lattice_11
+
+  subroutine anomalous_zero_lattice_21(F,ilat)
+    complex(8),intent(inout) :: F(:,:,:)
+    integer :: ilat
+    F = (0d0,0d0)
+  end subroutine anomalous_zero_lattice_21
 
   !> ed_get_phisc (ED_IO/get_phisc.f90): the superconducting order parameter
   !> per orbital, zero in the normal mode

@@ -18,6 +18,7 @@ program ed_hm_bethe_ineq
   real(8),allocatable      :: dens(:,:), docc(:,:), mag(:,:), eimp(:,:), dbl(:,:)
   complex(8),allocatable   :: Hloc(:,:,:,:,:), Weiss(:,:,:,:,:,:), Smats(:,:,:,:,:,:)
   complex(8),allocatable   :: Gloc(:,:,:,:,:,:)
+  complex(8),allocatable   :: S11(:,:,:,:,:), S21(:,:,:), S31(:), Gr(:,:,:,:,:,:), G11(:,:,:,:,:)
   character(len=256)       :: finput
 
   pi = acos(-1d0)
@@ -75,6 +76,20 @@ program ed_hm_bethe_ineq
      write(unit,"(A,I4,100ES24.15)")"site ",ilat,dens(ilat,:),docc(ilat,:),mag(ilat,:),&
           eimp(ilat,:),dbl(ilat,:)
   enddo
+  close(unit)
+  ! the one-site getter forms (_11, _21, _31) against the all-sites form
+  allocate(S11(Nspin,Nspin,Norb,Norb,Lmats), S21(Nspin*Norb,Nspin*Norb,Lmats), S31(Lmats))
+  allocate(Gr(Nlat,Nspin,Nspin,Norb,Norb,Lreal), G11(Nspin,Nspin,Norb,Norb,Lreal))
+  call ed_get_sigma_matsubara(Smats,Nlat)
+  call ed_get_sigma_matsubara(S11,Nlat)
+  call ed_get_sigma_matsubara(S21,Nlat)
+  call ed_get_sigma_matsubara(S31,Nlat,1,1,1,1)
+  call ed_get_gimp_real(Gr,Nlat)
+  call ed_get_gimp_real(G11,2)
+  open(newunit=unit,file="getters_ineq.ed")
+  write(unit,"(A,5ES24.15)")"getters ",maxval(abs(S11-Smats(Nlat,:,:,:,:,:))),&
+       maxval(abs(S21(1,1,:)-Smats(Nlat,1,1,1,1,:))),maxval(abs(S31-Smats(Nlat,1,1,1,1,:))),&
+       maxval(abs(G11-Gr(2,:,:,:,:,:))),maxval(abs(Smats(Nlat,1,1,1,1,:)-Smats(1,1,1,1,1,:)))
   close(unit)
 
 contains

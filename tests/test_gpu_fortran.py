@@ -147,6 +147,10 @@ def test_bethe_ineq_driver_one_iteration(tmp_path):
     summ = [list(map(float, l.split()[2:])) for l in open(tmp_path / "dmft_summary_ineq.ed")
             if l.strip()]
     assert len(summ) == nlat
+    # one-site getter forms (_11, _21, _31) equal the all-sites form; the last
+    # value differs from zero because the sites are inequivalent
+    gl = [float(x) for x in open(tmp_path / "getters_ineq.ed").read().split()[1:]]
+    assert gl[:4] == [0.0, 0.0, 0.0, 0.0] and gl[4] > 0.0
     p = edref.default_params(beta=c["beta"], lmats=lmats, lreal=64, lanc_ngfiter=200,
                              chiflag=True, ltau=100)
     for i in range(nlat):
