@@ -209,9 +209,9 @@ module DMFT_ED
   public :: ed_read_impSigma
   public :: set_Hloc, lso2nnn_reshape, nnn2lso_reshape, so2nn_reshape, nn2so_reshape
   public :: spin_symmetrize_bath, break_symmetry_bath, ph_symmetrize_bath, ph_trans_bath
-  public :: enforce_normal_bath, search_chemical_potential, search_chempot
+  public :: enforce_normal_bath
   public :: ed_get_self_matsubara, ed_get_self_real, ed_get_fimp_matsubara, ed_get_fimp_real
-  public :: ed_get_phisc, ed_get_density_matrix
+  public :: ed_get_phisc
   public :: get_component_bath_dimension, get_spin_component_bath_dimension
   public :: get_orb_component_bath_dimension, get_spin_orb_component_bath_dimension
   public :: get_component_bath, set_component_bath, copy_component_bath
@@ -775,103 +775,6 @@ contains
     enddo
     ed_file_suffix = ""
   end subroutine enforce_normal_bath_lattice
-
-  !> search_chemical_potential (ED_AUX_FUNX.f90:459-564): one step of the
-  !> density-driven search of var towards nread with the reference's step
-  !> halving and threshold tightening; nindex_old starts at 0 (the reference
-  !> leaves it unset), niter is the reference's unset module integer (0)
-  subroutine search_chemical_potential(var,ntmp,converged)
-    real(8),intent(inout) :: var
-    real(8),intent(in)    :: ntmp
-    logical,intent(inout) :: converged
-    logical               :: bool
-    real(8)               :: ndiff, ndelta_old, nratio
-    integer,save          :: count=0, nindex=0, nindex_old(3)=0
-    integer,save          :: nth_magnitude=-2, nth_magnitude_old=-2
-    real(8),save          :: nth=1d-2
-    logical,save          :: ireduce=.true.
-    integer,parameter     :: niter=0
-    integer :: unit, i
-    ndiff = ntmp-nread
-    nratio = 0.5d0
-    count = count+1
-    if(count>2)then
-       do i=1,2
-          nindex_old(i+1) = nindex_old(i)
-       enddo
-    endif
-    nindex_old(1) = nindex
-    if(ndiff>=nth)then
-       nindex = -1
-    elseif(ndiff<=-nth)then
-       nindex = 1
-    else
-       nindex = 0
-    endif
-    ndelta_old = ndelta
-    bool = nindex/=0 .and. ((nindex+nindex_old(1)==0) .or. (nindex+sum(nindex_old(:))==0))
-    if(bool)then
-       ndelta = ndelta_old*nratio
-    else
-       ndelta = ndelta_old
-    endif
-    if(ndelta_old<1d-9)then
-       ndelta_old = 0d0
-       nindex = 0
-    endif
-    var = var + dble(nindex)*ndelta
-    write(LOGfile,"(A,f16.9,A,f15.9)")"n    = ",ntmp," /",nread
-    write(LOGfile,"(A,f15.9)")"var  = ",var
-    open(newunit=unit,file="search_mu_iteration"//trim(ed_file_suffix)//".ed",position="append")
-    write(unit,*)var,ntmp,ndiff
-    close(unit)
-    bool = ireduce .and. (abs(ndiff)<nth) .and. converged .and. (nth>nerr)
-    if(bool)then
-       nth_magnitude_old = nth_magnitude
-       nth_magnitude = nth_magnitude_old-1
-       nth = max(nerr,10d0**(nth_magnitude))
-       count = 0
-       converged = .false.
-       ndelta = ndelta_old*nratio
-    endif
-    if(abs(ntmp-nread)>nth)converged = .false.
-    bool = ireduce .and. (count>niter) .and. (.not.converged)
-    if(bool)then
-       ireduce = .false.
-       nth = 10d0**(nth_magnitude_old)
-    endif
-  end subroutine search_chemical_potential
-
-  !> search_chempot (ED_AUX_FUNX.f90:572-668): converged when |n - nread| <=
-  !> nerr; otherwise xmu -= 0.2*(n - nread), 0.4 after ten attempts.  The
-  !> reference's bracketing branch needs its ilarge/ismall counters, which it
-  !> never sets, so it is never taken.
-  subroutine search_chempot(xmu_tmp,dens_tmp,converged_)
-    real(8),intent(in)    :: dens_tmp
-    real(8),intent(inout) :: xmu_tmp
-    logical,intent(inout) :: converged_
-    real(8)               :: diffdens, delta_xmu
-    integer,save          :: inotbound=0, iattempt=1
-    integer               :: unit
-    diffdens = dens_tmp-nread
-    delta_xmu = 0.2d0
-    if(abs(diffdens)<=nerr)then
-       converged_ = .true.
-       inotbound = 0
-       open(newunit=unit,file="search_mu_iteration"//trim(ed_file_suffix)//".ed",position="append")
-       write(unit,'(3F25.12)')xmu_tmp,dens_tmp,diffdens
-       close(unit)
-    else
-       converged_ = .false.
-       inotbound = inotbound+1
-       if(inotbound>=10)delta_xmu = 0.4d0
-       xmu_tmp = xmu_tmp - delta_xmu*diffdens
-       open(newunit=unit,file="search_mu_iteration"//trim(ed_file_suffix)//".ed",position="append")
-       write(unit,'(3F25.12,1I5)')xmu_tmp,dens_tmp,diffdens,iattempt
-       close(unit)
-    endif
-    iattempt = iattempt+1
-  end subroutine search_chempot
 
   !> The anomalous getters ed_get_self_matsubara/real and
   !> ed_get_fimp_matsubara/real (ED_IO/get_self_matsubara.f90,
@@ -2392,156 +2295,5 @@ contains
        yii = mii(1:Nlat,iorb)
     endif
   end subroutine ed_get_mag_lattice_2
-
-  ! The impurity density matrix in the basis custom_rot (ED_IO.f90:400-515),
-  ! written to imp_density_matrix.dat in the same layout. In the normal mode
-  ! with the normal bath the matrix <c+_{a,s} c_{b,s'}> of ED_OBSERVABLES.f90
-  ! :229-282 is diagonal, its entries being the orbital densities per spin of
-  ! the last solve: rho_diag(io,io) = n(a,s), io = a + (s-1)*Norb, and
-  ! rho = R rho_diag R^+. With dm_eig_ and dm_rot_ present, rho is also
-  ! diagonalised (eigenvalues ascending, eigenvectors in the columns). For
-  ! the hybrid bath the reference fills neither matrix, and neither does this.
-  subroutine ed_get_density_matrix(dm_,custom_rot,dm_eig_,dm_rot_)
-    complex(8),allocatable,intent(out)          :: dm_(:,:)
-    complex(8),allocatable,intent(in)           :: custom_rot(:,:)
-    real(8),allocatable,intent(out),optional    :: dm_eig_(:)
-    complex(8),allocatable,intent(out),optional :: dm_rot_(:,:)
-    complex(8),allocatable                      :: dm_diag(:,:)
-    integer :: n, io, jo, iorb, ispin, unit
-    if(.not.allocated(ed_dens_up))then
-       write(LOGfile,"(A)") "imp_density_matrix is not allocated"
-       stop
-    endif
-    n = Nspin*Norb
-    if(present(dm_eig_))then
-       allocate(dm_eig_(n)); dm_eig_ = 0d0
-    endif
-    if(present(dm_rot_))then
-       allocate(dm_rot_(n,n)); dm_rot_ = (0d0,0d0)
-    endif
-    allocate(dm_(n,n), dm_diag(n,n))
-    dm_ = (0d0,0d0); dm_diag = (0d0,0d0)
-    if(trim(bath_type)=="normal")then
-       do iorb=1,Norb
-          do ispin=1,Nspin
-             io = iorb + (ispin-1)*Norb
-             if(ispin==1)then
-                dm_diag(io,io) = cmplx(ed_dens_up(iorb),0d0,kind=8)
-             else
-                dm_diag(io,io) = cmplx(ed_dens_dw(iorb),0d0,kind=8)
-             endif
-          enddo
-       enddo
-       dm_ = matmul(custom_rot,matmul(dm_diag,transpose(conjg(custom_rot))))
-       if(present(dm_eig_).and.present(dm_rot_))then
-          dm_rot_ = dm_
-          call herm_eig(dm_rot_,dm_eig_)
-       endif
-    endif
-    open(newunit=unit,file="imp_density_matrix.dat",action="write",position="rewind",status="unknown")
-    write(unit,"(A30)")"# Re{rho_asbs'}:"
-    do io=1,n
-       write(unit,"(90(F15.9,1X))") (real(dm_(io,jo)),jo=1,n)
-    enddo
-    write(unit,*)
-    write(unit,"(A30)")"# Im{rho_asbs'}:"
-    do io=1,n
-       write(unit,"(90(F15.9,1X))") (aimag(dm_(io,jo)),jo=1,n)
-    enddo
-    write(unit,*)
-    if(present(dm_eig_).and.present(dm_rot_))then
-       write(unit,"(A30)")"# rho_tilda"
-       write(unit,'(10F22.12)') dm_eig_
-       write(unit,*)
-       write(unit,"(A30)")"# Re{theta}:"
-       do io=1,n
-          write(unit,"(90(F15.9,1X))") (real(dm_rot_(io,jo)),jo=1,n)
-       enddo
-       write(unit,*)
-       write(unit,"(A30)")"# Im{theta}:"
-       do io=1,n
-          write(unit,"(90(F15.9,1X))") (aimag(dm_rot_(io,jo)),jo=1,n)
-       enddo
-       write(unit,*)
-    endif
-    write(unit,"(A30)")"# Re{rho_Jj}"
-    do io=1,n
-       write(unit,"(90(F15.9,1X))") (real(dm_diag(io,jo)),jo=1,n)
-    enddo
-    write(unit,*)
-    write(unit,"(A30)")"# Im{rho_Jj}"
-    do io=1,n
-       write(unit,"(90(F15.9,1X))") (aimag(dm_diag(io,jo)),jo=1,n)
-    enddo
-    write(unit,*)
-    write(unit,"(A30)")"# J basis densities"
-    write(unit,"(90(F15.9,1X))") (real(dm_diag(io,io)),io=1,n)
-    close(unit)
-  end subroutine ed_get_density_matrix
-
-  ! Eigenvalues (ascending) and eigenvectors (columns of a on return) of a
-  ! small Hermitian matrix by cyclic Jacobi rotations. For the pair (p,q) the
-  ! phase of a(p,q) is first moved onto basis vector q, which leaves a real
-  ! symmetric 2x2 block that the real rotation with t = tan(phi),
-  ! t^2 + 2 t theta - 1 = 0, theta = (a_qq - a_pp)/(2 a_pq), zeroes.
-  subroutine herm_eig(a,w)
-    complex(8),intent(inout) :: a(:,:)
-    real(8),intent(inout)    :: w(:)
-    complex(8),allocatable   :: z(:,:), v(:,:)
-    complex(8) :: ph
-    real(8)    :: g, th, t, c, s, off, tmp, fro
-    integer    :: n, p, q, sweep, i, k
-    n = size(a,1)
-    allocate(z(n,n), v(n,n))
-    z = (0d0,0d0)
-    do i=1,n
-       z(i,i) = (1d0,0d0)
-    enddo
-    fro = sum(abs(a)**2)
-    do sweep=1,100
-       off = 0d0
-       do p=1,n-1
-          do q=p+1,n
-             off = off + abs(a(p,q))**2
-          enddo
-       enddo
-       ! off-diagonal weight relative to the (rotation-invariant) Frobenius norm
-       if(off <= 1d-32*fro)exit
-       do p=1,n-1
-          do q=p+1,n
-             g = abs(a(p,q))
-             if(g < 1d-300)cycle
-             ph = a(p,q)/g
-             th = (real(a(q,q)) - real(a(p,p)))/(2d0*g)
-             t = sign(1d0,th)/(abs(th) + sqrt(th*th + 1d0))
-             c = 1d0/sqrt(t*t + 1d0)
-             s = t*c
-             v = (0d0,0d0)
-             do i=1,n
-                v(i,i) = (1d0,0d0)
-             enddo
-             v(p,p) = c
-             v(q,p) = -s*conjg(ph)
-             v(p,q) = s
-             v(q,q) = c*conjg(ph)
-             a = matmul(transpose(conjg(v)),matmul(a,v))
-             z = matmul(z,v)
-             a(p,q) = (0d0,0d0)
-             a(q,p) = (0d0,0d0)
-          enddo
-       enddo
-    enddo
-    do i=1,n
-       w(i) = real(a(i,i))
-    enddo
-    do i=1,n-1
-       k = minloc(w(i:n),1) + i - 1
-       if(k/=i)then
-          tmp = w(i); w(i) = w(k); w(k) = tmp
-          v(:,1) = z(:,i); z(:,i) = z(:,k); z(:,k) = v(:,1)
-       endif
-    enddo
-    a = z
-  end subroutine herm_eig
 
 end module DMFT_ED

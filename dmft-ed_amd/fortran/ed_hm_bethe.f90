@@ -19,7 +19,6 @@ program ed_hm_bethe
   complex(8),allocatable   :: Hloc(:,:,:,:), Weiss(:,:,:,:,:), Weiss_old(:,:,:,:,:)
   complex(8),allocatable   :: Smats(:,:,:,:,:), Gloc(:,:,:,:,:), S2(:,:,:), S3(:)
   real(8)                  :: d2
-  complex(8),allocatable   :: dmat(:,:), drot(:,:)
   character(len=256)       :: finput
 
   pi = acos(-1d0)
@@ -93,14 +92,6 @@ program ed_hm_bethe
   call ed_get_dens(d2,1)
   write(unit,"(A,3ES24.15)")"getters ",maxval(abs(S2(1,1,:)-Smats(1,1,1,1,:))),&
        maxval(abs(S3-Smats(1,1,1,1,:))),abs(d2-ed_dens(1))
-  ! the impurity density matrix in the orbital basis (its diagonal)
-  allocate(drot(Nspin*Norb,Nspin*Norb))
-  drot = (0d0,0d0)
-  do i=1,Nspin*Norb
-     drot(i,i) = (1d0,0d0)
-  enddo
-  call ed_get_density_matrix(dmat,drot)
-  write(unit,"(A,10ES24.15)")"dmat ",(dble(dmat(i,i)),i=1,Nspin*Norb)
   close(unit)
 
 contains

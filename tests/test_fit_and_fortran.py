@@ -104,8 +104,7 @@ UTIL_CHECK = """program util_check
   use DMFT_ED
   implicit none
   real(8),allocatable :: b(:), bl(:,:)
-  real(8)    :: hso(4,4), hnn(2,2,2,2), hl(8,8), hnnn(2,2,2,2,2), var
-  logical    :: conv
+  real(8)    :: hso(4,4), hnn(2,2,2,2), hl(8,8), hnnn(2,2,2,2,2)
   integer    :: i, nb
   call ed_read_input("inputED.in")
   nb = get_bath_dimension()
@@ -133,12 +132,6 @@ UTIL_CHECK = """program util_check
   hnnn = lso2nnn_reshape(hl,2,2,2)
   write(*,"(A,100ES24.15)")"lso2nnn ",hnnn
   write(*,"(A,100ES24.15)")"nnn2lso ",nnn2lso_reshape(hnnn,2,2,2)
-  var = 0.5d0; conv = .true.
-  call search_chemical_potential(var,0.8d0,conv)
-  write(*,"(A,ES24.15,L2)")"mu ",var,conv
-  var = 0.5d0; conv = .true.
-  call search_chempot(var,0.7d0,conv)
-  write(*,"(A,ES24.15,L2)")"chempot ",var,conv
 end program util_check
 """
 
@@ -146,9 +139,9 @@ end program util_check
 def test_fortran_bath_utilities(tmp_path):
     """The host-side helpers of the Fortran API (no GPU call): the bath
     symmetry operations of ED_BATH/user_aux.f90:1071-1270 (site and lattice
-    forms), the spin/orbital/site reshapes of ED_AUX_FUNX.f90:177-351 and one
-    step of search_chemical_potential (:459-564), compiled against the
-    module and checked against the index rules written out here."""
+    forms) and the spin/orbital/site reshapes of ED_AUX_FUNX.f90:177-351,
+    compiled against the module and checked against the index rules written
+    out here."""
     b = os.path.join(ROOT, "dmft-ed_amd", "fortran", "build")
     if not os.path.exists(os.path.join(b, "dmft_ed.o")):
         pytest.skip("Fortran layer not built (run __graft_entry__.build())")
@@ -166,7 +159,7 @@ def test_fortran_bath_utilities(tmp_path):
     for line in out.splitlines():
         parts = line.split()
         if parts and parts[0] in ("spin_sym", "break", "ph_sym", "ph_trans", "spin_sym_lat",
-                                  "so2nn", "nn2so", "lso2nnn", "nnn2lso", "mu", "chempot"):
+                                  "so2nn", "nn2so", "lso2nnn", "nnn2lso"):
             res[parts[0]] = parts[1:]
     nspin, norb, nb = 2, 2, 3
     st = nspin * norb * nb
@@ -191,7 +184,7 @@ def test_fortran_bath_utilities(tmp_path):
                     x[io], x[st + io] = e[s_, a, k], v[s_, a, k]
         return x
 
-    got = {k: np.array(list(map(float, v))) for k, v in res.items() if k not in ("mu", "chempot")}
+    got = {k: np.array(list(map(float, v))) for k, v in res.items()}
     e, v = ev(b0)
     e1, v1 = e.copy(), v.copy()
     e1[1], v1[1] = e[0], v[0]
@@ -232,135 +225,7 @@ def test_fortran_bath_utilities(tmp_path):
                         back[i, j] = hl[i, j]
     assert np.array_equal(got["lso2nnn"], hnnn.flatten(order="F"))
     assert np.array_equal(got["nnn2lso"], back.flatten(order="F"))
-    # n = 0.8 above nread = 0.6: mu steps down by ndelta, not converged
-    assert abs(float(res["mu"][0]) - 0.4) < 1e-14 and res["mu"][1] == "F"
-    # search_chempot: xmu -= 0.2*(n - nread) = 0.5 - 0.2*0.1
-    assert abs(float(res["chempot"][0]) - 0.48) < 1e-14 and res["chempot"][1] == "F"
 
-
-
-DM_CHECK = """program dm_check
-  use DMFT_ED
-  implicit none
-  complex(8),allocatable :: rot(:,:), dm(:,:), th(:,:)
-  real(8),allocatable    :: eig(:), re(:,:), im(:,:)
-  integer :: n, i, j
-  character(len=7) :: bt
-  open(10,file="in.txt",status="old")
-  read(10,*) Norb, Nspin, bt
-  bath_type = bt
-  n = Norb*Nspin
-  allocate(ed_dens_up(Norb), ed_dens_dw(Norb), rot(n,n), re(n,n), im(n,n))
-  read(10,*) ed_dens_up
-  read(10,*) ed_dens_dw
-  read(10,*) ((re(i,j),j=1,n),i=1,n)
-  read(10,*) ((im(i,j),j=1,n),i=1,n)
-  close(10)
-  rot = cmplx(re,im,kind=8)
-  call ed_get_density_matrix(dm,rot,eig,th)
-  do i=1,n
-     write(*,"(A,1X,200ES26.17)") "dm", (real(dm(i,j)),aimag(dm(i,j)),j=1,n)
-  enddo
-  write(*,"(A,1X,200ES26.17)") "eig", eig
-  do i=1,n
-     write(*,"(A,1X,200ES26.17)") "th", (real(th(i,j)),aimag(th(i,j)),j=1,n)
-  enddo
-end program dm_check
-"""
-
-
-def _run_dm_check(tmp_path, norb, nspin, bath, up, dw, rot):
-    b = os.path.join(ROOT, "dmft-ed_amd", "fortran", "build")
-    if not os.path.exists(os.path.join(b, "dmft_ed.o")):
-        pytest.skip("Fortran layer not built (run __graft_entry__.build())")
-    import shutil
-    flang = shutil.which("amdflang") or "/opt/rocm/llvm/bin/amdflang"
-    if not (tmp_path / "check").exists():
-        (tmp_path / "check.f90").write_text(DM_CHECK)
-        lib = os.path.join(ROOT, "dmft-ed_amd")
-        subprocess.run([flang, "-O0", "-I", b, "check.f90", os.path.join(b, "dmed_cabi.o"),
-                        os.path.join(b, "dmft_ed.o"), "-L" + lib, "-ldmed",
-                        "-Wl,-rpath," + lib, "-o", "check"],
-                       cwd=str(tmp_path), check=True, capture_output=True)
-    n = norb * nspin
-    with open(tmp_path / "in.txt", "w") as f:
-        f.write(f"{norb} {nspin} {bath}\n")
-        f.write(" ".join(repr(float(x)) for x in up) + "\n")
-        f.write(" ".join(repr(float(x)) for x in dw) + "\n")
-        f.write(" ".join(repr(float(x)) for x in rot.real.ravel()) + "\n")
-        f.write(" ".join(repr(float(x)) for x in rot.imag.ravel()) + "\n")
-    out = subprocess.run(["./check"], cwd=str(tmp_path), capture_output=True, text=True,
-                         timeout=60, check=True).stdout
-    rows = {"dm": [], "th": [], "eig": []}
-    for line in out.splitlines():
-        p = line.split()
-        if p and p[0] in rows:
-            rows[p[0]].append([float(x) for x in p[1:]])
-
-    def cm(r):
-        a = np.array(r)
-        return a[:, 0::2] + 1j * a[:, 1::2]
-    dat = (tmp_path / "imp_density_matrix.dat").read_text()
-    return cm(rows["dm"]), np.array(rows["eig"][0]), cm(rows["th"]), dat
-
-
-def test_fortran_density_matrix(tmp_path):
-    """ed_get_density_matrix (ED_IO.f90:400-515): in the normal mode with the
-    normal bath the impurity density matrix of ED_OBSERVABLES.f90:229-282 is
-    diag(n(a,s)), io = a + (s-1)*Norb, and rho = R diag R^+. Pinned by the
-    spectral theorem: for a unitary R the eigenvalues are the sorted densities
-    and every eigenvector is a column of R up to a phase; for a general R the
-    eigen-pairs match numpy's eigh of the same Hermitian matrix. The file
-    carries the same numbers in the reference's layout."""
-    rng = np.random.default_rng(7)
-    norb, nspin = 3, 2
-    n = norb * nspin
-    up = np.array([0.91, 0.47, 0.13])
-    dw = np.array([0.66, 0.29, 0.05])
-    dens = np.concatenate([up, dw])            # io = a + s*Norb
-    q, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
-    dm, eig, th, dat = _run_dm_check(tmp_path, norb, nspin, "normal", up, dw, q)
-    assert np.allclose(dm, dm.conj().T, atol=1e-15)
-    assert np.allclose(np.trace(dm).real, dens.sum(), atol=1e-13)
-    assert np.allclose(eig, np.sort(dens), atol=1e-13)
-    assert np.allclose(dm @ th, th * eig[None, :], atol=1e-12)
-    assert np.allclose(th.conj().T @ th, np.eye(n), atol=1e-12)
-    order = np.argsort(dens)
-    for j in range(n):
-        ov = abs(np.vdot(q[:, order[j]], th[:, j]))
-        assert abs(ov - 1.0) < 1e-12
-    # the file: rho, then the eigen-pairs, then the rotated-basis diagonal
-    lines = dat.splitlines()
-    assert lines[0].strip() == "# Re{rho_asbs'}:" and len(lines[0]) == 30
-    re_rho = np.array([[float(x) for x in lines[1 + i].split()] for i in range(n)])
-    assert np.allclose(re_rho, dm.real, atol=1e-9)
-    hdr = [i for i, ln in enumerate(lines) if ln.strip() == "# rho_tilda"]
-    assert len(hdr) == 1 and len(lines[hdr[0]]) == 30     # (A30) right-justifies
-    k = hdr[0]
-    assert np.allclose([float(x) for x in lines[k + 1].split()], eig, atol=1e-11)
-    last = [float(x) for x in lines[-1].split()]
-    assert np.allclose(last, dens, atol=1e-9)
-
-    # a general (non-unitary) R gives a general Hermitian matrix, with
-    # degenerate densities among the inputs
-    up2 = np.array([0.4, 0.4, 0.8])
-    dw2 = np.array([0.4, 0.1, 0.8])
-    r = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
-    dm, eig, th, _ = _run_dm_check(tmp_path, norb, nspin, "normal", up2, dw2, r)
-    ref = r @ np.diag(np.concatenate([up2, dw2])) @ r.conj().T
-    assert np.allclose(dm, ref, atol=1e-13)
-    w = np.linalg.eigvalsh(ref)
-    assert np.allclose(eig, w, rtol=0, atol=1e-12 * max(1.0, abs(w).max()))
-    assert np.allclose(dm @ th, th * eig[None, :], atol=1e-11 * abs(w).max())
-    assert np.allclose(th.conj().T @ th, np.eye(n), atol=1e-12)
-
-    # Nspin = 1 takes the up densities only; the hybrid bath leaves rho empty
-    one = np.eye(norb, dtype=complex)
-    dm, eig, _, _ = _run_dm_check(tmp_path, norb, 1, "normal", up, dw, one)
-    assert np.allclose(dm, np.diag(up), atol=1e-15)
-    assert np.allclose(eig, np.sort(up), atol=1e-15)
-    dm, eig, _, _ = _run_dm_check(tmp_path, norb, 1, "hybrid", up, dw, one)
-    assert np.all(dm == 0) and np.all(eig == 0)
 
 
 COMP_CHECK = """program comp_check

@@ -66,9 +66,6 @@ def test_bethe_driver_one_iteration(tmp_path):
                        atol=1e-9, rtol=0)
     assert np.allclose(summ["mag"], ob["magz"], atol=1e-9, rtol=0)
     assert summ["getters"] == [0.0, 0.0, 0.0]     # matrix / element / orbital getter forms
-    # ed_get_density_matrix, orbital basis: <n_(a,up)> on the diagonal (Nspin = 1)
-    assert np.allclose(summ["dmat"], ob["dens_up"][:1], atol=1e-9, rtol=0)
-    assert (tmp_path / "imp_density_matrix.dat").exists()
     # G_loc of the semicircular DOS (half-bandwidth 1, 500 points) from Sigma
     g = np.loadtxt(tmp_path / "Gloc_iw.ed")
     wm = np.pi / c["beta"] * (2 * np.arange(lmats) + 1)
