@@ -213,7 +213,11 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
 // so all loads of a hop step are in flight before the first is used.
 // Summation order per row: down hops in table order, up hops in table order,
 // then the diagonal.
-template <typename T, int MODE, int RPT>
+//
+// SPLIT: the down hops were already summed into Y by k_hxv_dn (the same FMA
+// chain in the same order); step 2 loads that partial column instead, and the
+// kernel finishes Y in place.  The result is bit-identical to the one-pass form.
+template <typename T, int MODE, int RPT, bool SPLIT>
 __global__ void __launch_bounds__(1024) k_hxv_col(
     const T *__restrict__ X, T *__restrict__ Y, int32_t dup, int32_t ddw, int32_t nhu, int32_t nhd,
     int32_t nimp, int32_t ncoef, const uint32_t *__restrict__ HU, const double *__restrict__ COEF,
@@ -276,10 +280,18 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
         if ((int)threadIdx.x < nimp)
             wt[threadIdx.x] = DD[c] + WTAB[threadIdx.x * nimp + IMPD[c]];
         T acc[RPT];
+        if constexpr (SPLIT) {
+            // 2'. the down-hop sums of this column, written by k_hxv_dn
+            const T *yc = Y + (int64_t)c * dup;
+#pragma unroll
+            for (int r = 0; r < RPT; ++r)
+                acc[r] = __builtin_nontemporal_load(&yc[min((int)threadIdx.x + r * 1024, last)]);
+        } else {
 #pragma unroll
         for (int r = 0; r < RPT; ++r) acc[r] = (T)0;
+        }
         // 2. down hops
-        for (int h0 = 0; h0 < nhd; h0 += HDU) {
+        for (int h0 = 0; !SPLIT && h0 < nhd; h0 += HDU) {
             T v[HDU];
             const T *xj[HDU];
 #pragma unroll
@@ -338,6 +350,58 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
     }
 }
 
+// ------------------------------------------------------------------------
+// Down-spin hops of a large sector over row panels: the first pass of the
+// split H*v.  For every column c and the rows r of one panel,
+//     Y(r, c) = sum_h td_h(c) X(r, jd_h(c))        (table order, from 0)
+// Every column of X is read by the ~nhd columns that hop to it.  With whole
+// columns (D_up rows) those re-reads miss the 4 MB L2 of an XCD; a panel of
+// R rows of all D_dw columns fits, so each element of X is fetched once per
+// H*v and re-read from L2.  Panel p is worked on by the workgroups of XCD
+// p % 8 (workgroup b runs on XCD b % 8), all of them on the same panel at a
+// time, panels in turn.  A 256-lane workgroup covers 256/R column segments:
+// lane t owns row t % R of column segment t / R.  NH hops are loaded and
+// gathered before any is added; a padded hop adds 0 * x, which leaves the sum
+// unchanged.  Rows past D_up load row D_up - 1 (in bounds) and are not stored.
+template <typename T, int MODE, int R>
+__global__ void __launch_bounds__(256) k_hxv_dn(
+    const T *__restrict__ X, T *__restrict__ Y, int32_t dup, int32_t ddw, int32_t nhd,
+    const int32_t *__restrict__ HDI, const T *__restrict__ HDV, const ChainScalars *__restrict__ CS)
+{
+    constexpr int SEG = 256 / R;
+    constexpr int NH = 8;
+    if (MODE == kLanczos && CS->done) return;
+    const int nper = gridDim.x / 8;
+    const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+    const int seg = threadIdx.x / R, rr = threadIdx.x % R;
+    const int npanel = (dup + R - 1) / R;
+    const int ncg = (ddw + SEG - 1) / SEG;
+    for (int p = xcd; p < npanel; p += 8) {
+        const int r0 = p * R + rr;
+        const int row = min(r0, dup - 1);
+        for (int g = loc; g < ncg; g += nper) {
+            const int c = min(g * SEG + seg, ddw - 1);
+            const bool st = r0 < dup && g * SEG + seg < ddw;
+            T acc = (T)0;
+            for (int h0 = 0; h0 < nhd; h0 += NH) {
+                int32_t j[NH];
+                T v[NH], x[NH];
+#pragma unroll
+                for (int q = 0; q < NH; ++q) {
+                    const int h = min(h0 + q, nhd - 1);
+                    j[q] = HDI[(int64_t)h * ddw + c];
+                    v[q] = h0 + q < nhd ? HDV[(int64_t)h * ddw + c] : (T)0;
+                }
+#pragma unroll
+                for (int q = 0; q < NH; ++q) x[q] = X[(int64_t)j[q] * dup + row];
+#pragma unroll
+                for (int q = 0; q < NH; ++q) acc += v[q] * x[q];
+            }
+            if (st) __builtin_nontemporal_store(acc, &Y[(int64_t)c * dup + row]);
+        }
+    }
+}
+
 // rows per lane the column kernel supports per element type (register budget)
 constexpr int kColMaxRptF64 = 13;
 constexpr int kColMaxRptF32 = 16;
@@ -360,12 +424,32 @@ static bool use_col_kernel(const dmed_sector *s, int prec)
            rpt <= maxr && col_lds_bytes(dup, esz) + 2048 <= c.lds_max && s->dw.dim >= 64;
 }
 
+// A kernel's dynamic-LDS limit is process-wide state: it is raised once per
+// kernel, to all the LDS a workgroup may have, so concurrent launches from
+// several host threads never change it under one another.
+template <auto KERN>
+static int lds_attr_once()
+{
+    static const hipError_t e = [] {
+        hipFuncAttributes fa{};
+        hipError_t r = hipFuncGetAttributes(&fa, (const void *)KERN);
+        if (r != hipSuccess) return r;
+        const size_t lim = ctx().lds_max > fa.sharedSizeBytes ? ctx().lds_max - fa.sharedSizeBytes : 0;
+        return hipFuncSetAttribute((const void *)KERN, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lim);
+    }();
+    if (e != hipSuccess)
+        return fail(DMED_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    return 0;
+}
+
 template <typename T, int RPT>
 static int col_fit_per_cu(size_t shmem)
 {
     int nb = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_hxv_col<T, kLanczos, RPT>,
-                                                     1024, shmem) != hipSuccess)
+    if (lds_attr_once<k_hxv_col<T, kLanczos, RPT, false>>() != 0 ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, (const void *)k_hxv_col<T, kLanczos, RPT, false>, 1024, shmem) != hipSuccess)
         nb = 1;
     return nb;
 }
@@ -449,7 +533,7 @@ static unsigned col_grid(const dmed_sector *s, int prec)
     return (unsigned)g;
 }
 
-template <typename T, int MODE, int RPT>
+template <typename T, int MODE, int RPT, bool SPLIT>
 static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScalars *cs,
                         double *partials, unsigned grid, hipStream_t st)
 {
@@ -457,10 +541,8 @@ static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScala
     const size_t shmem = col_lds_bytes(dup, sizeof(T));
     // 16-byte DMA granules need every column start 16-byte aligned
     const int dma16 = ((uintptr_t)x % 16 == 0) && ((size_t)dup * sizeof(T)) % 16 == 0;
-    auto kern = k_hxv_col<T, MODE, RPT>;
-    if (shmem > 64 * 1024)
-        DMED_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)shmem));
+    constexpr auto kern = k_hxv_col<T, MODE, RPT, SPLIT>;
+    if (int rc = lds_attr_once<kern>()) return rc;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(1024),
                        shmem, st, static_cast<const T *>(x), static_cast<T *>(y), (int32_t)dup,
                        (int32_t)s->dw.dim, (int32_t)s->up.nhop, (int32_t)s->dw.nhop, (int32_t)s->nimp,
@@ -471,27 +553,55 @@ static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScala
     return 0;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool SPLIT>
 static int launch_col_rpt(int rpt, const dmed_sector *s, const void *x, void *y,
                           ChainScalars *cs, double *partials, unsigned grid, hipStream_t st)
 {
     switch (rpt) {
 #define DMED_RPT(R) \
-    case R: return launch_col_t<T, MODE, R>(s, x, y, cs, partials, grid, st);
+    case R: return launch_col_t<T, MODE, R, SPLIT>(s, x, y, cs, partials, grid, st);
         DMED_RPT(1) DMED_RPT(2) DMED_RPT(3) DMED_RPT(4) DMED_RPT(5) DMED_RPT(6) DMED_RPT(7) DMED_RPT(8)
         DMED_RPT(9) DMED_RPT(10) DMED_RPT(11) DMED_RPT(12) DMED_RPT(13)
 #undef DMED_RPT
     case 14: case 15: case 16:
         if constexpr (sizeof(T) == 4) {
-            if (rpt == 14) return launch_col_t<T, MODE, 14>(s, x, y, cs, partials, grid, st);
-            if (rpt == 15) return launch_col_t<T, MODE, 15>(s, x, y, cs, partials, grid, st);
-            return launch_col_t<T, MODE, 16>(s, x, y, cs, partials, grid, st);
+            if (rpt == 14) return launch_col_t<T, MODE, 14, SPLIT>(s, x, y, cs, partials, grid, st);
+            if (rpt == 15) return launch_col_t<T, MODE, 15, SPLIT>(s, x, y, cs, partials, grid, st);
+            return launch_col_t<T, MODE, 16, SPLIT>(s, x, y, cs, partials, grid, st);
         }
         break;
     default:
         break;
     }
     return fail(DMED_EINVAL, "column kernel: unsupported rows per lane");
+}
+
+// first pass of the split H*v: down hops over row panels of `panel` rows,
+// `wgs` 256-lane workgroups per CU
+template <typename T, int MODE>
+static int launch_dn(const dmed_sector *s, const void *x, void *y, ChainScalars *cs, int panel,
+                     int wgs, hipStream_t st)
+{
+    const int ncu = std::max(8, ctx().num_cu);
+    const unsigned grid = (unsigned)((ncu / 8) * 8 * std::max(1, wgs));
+    const T *hv;
+    if constexpr (sizeof(T) == 8) hv = reinterpret_cast<const T *>(s->dw.hval);
+    else hv = reinterpret_cast<const T *>(s->dw.hval32);
+    const int32_t dup = (int32_t)s->up.dim, ddw = (int32_t)s->dw.dim, nhd = s->dw.nhop;
+    switch (panel) {
+#define DMED_PANEL(R)                                                                          \
+    case R:                                                                                   \
+        hipLaunchKernelGGL((k_hxv_dn<T, MODE, R>), dim3(grid), dim3(256), 0, st,               \
+                           static_cast<const T *>(x), static_cast<T *>(y), dup, ddw, nhd,      \
+                           s->dw.hidx, hv, (const ChainScalars *)cs);                          \
+        break;
+        DMED_PANEL(8) DMED_PANEL(16) DMED_PANEL(32) DMED_PANEL(64)
+#undef DMED_PANEL
+    default:
+        return fail(DMED_EINVAL, "down-hop pass: unsupported panel rows");
+    }
+    DMED_HIP(hipGetLastError());
+    return 0;
 }
 
 struct LaunchShape {
@@ -570,11 +680,9 @@ static int launch_t(const dmed_sector *s, const void *x, void *y, ChainScalars *
     a.jp = s->jp;
     a.cs = cs;
     a.partials = partials;
-    auto kern = k_hxv<T, MODE, LDS>;
+    constexpr auto kern = k_hxv<T, MODE, LDS>;
     if (sh.shmem > 64 * 1024)
-        DMED_HIP(hipFuncSetAttribute((const void *)kern,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)sh.shmem));
+        if (int rc = lds_attr_once<kern>()) return rc;
     hipLaunchKernelGGL(kern, dim3(sh.grid), dim3(sh.bs), sh.shmem, st, a);
     DMED_HIP(hipGetLastError());
     return 0;
@@ -671,8 +779,8 @@ void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
             unsigned g = (unsigned)((ncu / 8) * 8 * w);
             while (g > 8 && g / 8 > (s->dw.dim + 7) / 8) g -= 8;
             auto fn = [&]() {
-                return f32 ? launch_col_rpt<float, kPlain>(rpt, s, x, y, nullptr, nullptr, g, st)
-                           : launch_col_rpt<double, kPlain>(rpt, s, x, y, nullptr, nullptr, g, st);
+                return f32 ? launch_col_rpt<float, kPlain, false>(rpt, s, x, y, nullptr, nullptr, g, st)
+                           : launch_col_rpt<double, kPlain, false>(rpt, s, x, y, nullptr, nullptr, g, st);
             };
             const float ms = time_launches(fn, rep, st);
             if (ms > 0 && (tbest < 0 || ms < kTuneMargin * tbest)) {
@@ -727,6 +835,21 @@ void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
     }
 }
 
+// Split H*v (down-hop panel pass + column pass) for the column kernel's
+// sectors: panel rows and panel-pass workgroups per CU; panel 0 keeps the
+// one-pass kernel.
+struct SplitShape {
+    int panel = 0, wgs = 4;
+};
+
+static SplitShape split_shape(const dmed_sector *s, int prec)
+{
+    SplitShape sp;
+    if (const char *e = std::getenv("DMED_HXV_PANEL")) sp.panel = std::atoi(e);
+    if (const char *e = std::getenv("DMED_HXV_DNWGS")) sp.wgs = std::max(1, std::atoi(e));
+    return sp;
+}
+
 int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
                ChainScalars *cs, double *partials, int *nparts, hipStream_t st)
 {
@@ -737,11 +860,27 @@ int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
         const unsigned grid = col_grid(s, prec);
         if (nparts) *nparts = (int)grid;
         const int rpt = (int)((s->up.dim + 1023) / 1024);
-        if (prec == DMED_FP32)
-            return mode == kPlain ? launch_col_rpt<float, kPlain>(rpt, s, x, y, cs, partials, grid, st)
-                                  : launch_col_rpt<float, kLanczos>(rpt, s, x, y, cs, partials, grid, st);
-        return mode == kPlain ? launch_col_rpt<double, kPlain>(rpt, s, x, y, cs, partials, grid, st)
-                              : launch_col_rpt<double, kLanczos>(rpt, s, x, y, cs, partials, grid, st);
+        const bool f32 = prec == DMED_FP32;
+        const SplitShape sp = split_shape(s, prec);
+        if (sp.panel > 0) {
+            // down hops into y over L2-resident row panels, then the column
+            // kernel adds the up hops and the diagonal in place
+            int rc = f32 ? (mode == kPlain ? launch_dn<float, kPlain>(s, x, y, cs, sp.panel, sp.wgs, st)
+                                           : launch_dn<float, kLanczos>(s, x, y, cs, sp.panel, sp.wgs, st))
+                         : (mode == kPlain ? launch_dn<double, kPlain>(s, x, y, cs, sp.panel, sp.wgs, st)
+                                           : launch_dn<double, kLanczos>(s, x, y, cs, sp.panel, sp.wgs, st));
+            if (rc) return rc;
+            if (f32)
+                return mode == kPlain ? launch_col_rpt<float, kPlain, true>(rpt, s, x, y, cs, partials, grid, st)
+                                      : launch_col_rpt<float, kLanczos, true>(rpt, s, x, y, cs, partials, grid, st);
+            return mode == kPlain ? launch_col_rpt<double, kPlain, true>(rpt, s, x, y, cs, partials, grid, st)
+                                  : launch_col_rpt<double, kLanczos, true>(rpt, s, x, y, cs, partials, grid, st);
+        }
+        if (f32)
+            return mode == kPlain ? launch_col_rpt<float, kPlain, false>(rpt, s, x, y, cs, partials, grid, st)
+                                  : launch_col_rpt<float, kLanczos, false>(rpt, s, x, y, cs, partials, grid, st);
+        return mode == kPlain ? launch_col_rpt<double, kPlain, false>(rpt, s, x, y, cs, partials, grid, st)
+                              : launch_col_rpt<double, kLanczos, false>(rpt, s, x, y, cs, partials, grid, st);
     }
     const LaunchShape sh = pick_shape(s, prec);
     if (nparts) *nparts = (int)sh.grid;
