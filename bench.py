@@ -1,15 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the impurity-solver hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--reps R]
 
 A step is one FP64 sector H*v over the Ns=16 half-filled sector 8/8
 (dim 12870 x 12870 = 165,636,900; Norb=1, Nbath=15, U=2.5, seeded bath),
 inputs resident in HBM.  Prints ONE JSON line with the metric, the roofline
 of the H*v kernel (algorithmic bytes 16*dim per launch: x read once, y written
 once) measured with HIP events on the library stream, and the CPU baseline:
-the oracle's complex(8) restatement of the reference H*v on a bounded row
-sample of the same sector (rank 0, N=1 only).
+the oracle's complex(8) restatement of the reference H*v on the host cores,
+on all logical CPUs and on one thread (a bounded row sample).
+
+The second half of the metric, the solve time of the batched mixed-precision
+form against the plain FP64 GPU path, is measured by default on two
+workloads, each form repeated R times (speed-up = ratio of the medians):
+  - the all-sector Ns=12 solve at beta=50 (169 sectors, 200-step chains);
+  - the Ns=16 sector pipeline (ground state to 1e-12 and two 500-step chains).
+Their errors against the FP64 path (E0, n, d, G(iw), Sigma(iw)) are reported
+beside the speed-ups.
 """
 import argparse
 import json
@@ -28,9 +36,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg, sample_rows, threads):
+def cpu_baseline(cfg, samples):
     """Oracle H*v (combined map + binary search + complex(8), the reference
-    algorithm) on rows [0, sample_rows) of the sector; returns rows/s."""
+    algorithm) on rows [0, rows) of the sector, for each (rows, threads) in
+    `samples` (rows may be a function of the previous samples' results);
+    returns [(rows, threads, rows/s, seconds)]."""
     import ctypes as C
     import numpy as np
     import edref
@@ -44,25 +54,38 @@ def cpu_baseline(cfg, sample_rows, threads):
     x = np.empty(len(smap), dtype=np.complex128)
     x.real = np.random.default_rng(1).standard_normal(len(smap))
     x.imag = 0.0
-    y = np.zeros(sample_rows, dtype=np.complex128)
     L = edref.lib()
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int64)
-    # y is only written on [0, sample_rows): pass it as the start of hv
-    t0 = time.perf_counter()
-    rc = L.ref_hxv_rows(smap.ctypes.data_as(ip), len(smap), x.view(np.float64).ctypes.data_as(dp),
-                        y.view(np.float64).ctypes.data_as(dp), 0, sample_rows, threads)
-    dt = time.perf_counter() - t0
-    if rc != 0:
-        raise RuntimeError(f"oracle H*v failed ({rc})")
-    return sample_rows / dt, dt
+    out = []
+    for rows, threads in samples:
+        if callable(rows):
+            rows = rows(out)
+        y = np.zeros(rows, dtype=np.complex128)
+        # y is only written on [0, rows): pass it as the start of hv
+        t0 = time.perf_counter()
+        rc = L.ref_hxv_rows(smap.ctypes.data_as(ip), len(smap),
+                            x.view(np.float64).ctypes.data_as(dp),
+                            y.view(np.float64).ctypes.data_as(dp), 0, rows, threads)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"oracle H*v failed ({rc})")
+        out.append((rows, threads, rows / dt, dt))
+    return out
 
 
-def solve_compare():
+def _median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2]
+
+
+def solve_compare(reps=3):
     """Workload 4: Ns=12, all 169 sectors, beta=50, Boltzmann-weighted states.
     Plain FP64 (serial sectors and chains) against the batched mixed form
-    (concurrent per-stream workers; chains of 32 FP64 steps, then FP32 with
-    FP64 reductions)."""
+    (small sectors in one persistent work-queue launch, the rest on
+    concurrent per-stream workers; chains of 32 FP64 steps, then FP32 with
+    FP64 reductions).  The two forms alternate `reps` times; the speed-up is
+    the ratio of the median wall times."""
     import numpy as np
     import dmed
     from dmed import configs
@@ -71,20 +94,27 @@ def solve_compare():
                    c["ust"], c["jh"], 0.0, 0.0, c["xmu"], True, False)
     kw = dict(beta=c["beta"], lmats=c["lmats"], lreal=1000, lanc_ngfiter=200,
               lanc_nstates_sector=4, lanc_nstates_total=16)
-    out = {}
+    forms = (("fp64_plain", dict(batched=0, prec_gf=dmed.FP64)),
+             ("mixed_batched", dict(batched=1, prec_gf=dmed.MIXED)))
+    times = {n: [] for n, _ in forms}
     res = {}
-    for name, extra in (("fp64_plain", dict(batched=0, prec_gf=dmed.FP64)),
-                        ("mixed_batched", dict(batched=1, prec_gf=dmed.MIXED))):
-        t0 = time.perf_counter()
-        r = dmed.solve(1, 1, 12, dmed.default_params(**kw, **extra))
-        dt = time.perf_counter() - t0
-        res[name] = r
-        out[name] = {"seconds": dt, "t_diag": r["t_diag"], "t_gf": r["t_gf"],
-                     "nstates": r["nstates"]}
-        log(f"solve {name}: {dt:.2f}s (diag {r['t_diag']:.2f}s, gf {r['t_gf']:.2f}s), "
-            f"{r['nstates']} states")
+    out = {}
+    for rep in range(reps):
+        for name, extra in forms:
+            t0 = time.perf_counter()
+            r = dmed.solve(1, 1, 12, dmed.default_params(**kw, **extra))
+            dt = time.perf_counter() - t0
+            res[name] = r
+            times[name].append({"seconds": dt, "t_diag": r["t_diag"], "t_gf": r["t_gf"]})
+            log(f"solve {name} #{rep}: {dt:.2f}s (diag {r['t_diag']:.2f}s, gf {r['t_gf']:.2f}s), "
+                f"{r['nstates']} states")
+    for name, _ in forms:
+        out[name] = {"seconds_median": _median([t["seconds"] for t in times[name]]),
+                     "t_diag_median": _median([t["t_diag"] for t in times[name]]),
+                     "t_gf_median": _median([t["t_gf"] for t in times[name]]),
+                     "runs": times[name], "nstates": res[name]["nstates"]}
     a64, amx = res["fp64_plain"], res["mixed_batched"]
-    out["speedup"] = out["fp64_plain"]["seconds"] / out["mixed_batched"]["seconds"]
+    out["speedup"] = out["fp64_plain"]["seconds_median"] / out["mixed_batched"]["seconds_median"]
     out["errors_vs_fp64"] = {
         "E0_rel": abs(amx["egs"] - a64["egs"]) / abs(a64["egs"]),
         "dens": float(np.abs(amx["dens"] - a64["dens"]).max()),
@@ -92,7 +122,8 @@ def solve_compare():
         "G_iw": float(np.abs(amx["gmats"] - a64["gmats"]).max()),
         "Sigma_iw": float(np.abs(amx["smats"] - a64["smats"]).max()),
     }
-    log(f"mixed/batched speedup {out['speedup']:.2f}x, errors {out['errors_vs_fp64']}")
+    log(f"mixed/batched speedup {out['speedup']:.2f}x (median of {reps}), "
+        f"errors {out['errors_vs_fp64']}")
     return out
 
 
@@ -171,12 +202,14 @@ def lanczos_step_rate(sec, D, prec, nsteps=40):
             "frac": 6 * esz * D / dt / 1e9 / HBM_PEAK_GBS, "steps": n}
 
 
-def solve5_compare():
+def solve5_compare(reps=3):
     """Workload 5 sector pipeline (Ns=16, sector 8/8, dim 1.66e8): ground
     state to residual 1e-12, particle and hole chains of 500 steps from it,
     G(iw) and Sigma(iw) on 4096 Matsubara points at beta=100.  Plain FP64
     against the mixed form (ground state started on FP32 vectors and finished
-    in FP64; chains of 32 FP64 steps, then FP32)."""
+    in FP64; chains of 32 FP64 steps, then FP32).  The forms alternate `reps`
+    times after one untimed warm-up of each, which backs every work buffer the
+    pipeline uses; the speed-up is the ratio of the median wall times."""
     import numpy as np
     import dmed
     from dmed import configs
@@ -186,10 +219,10 @@ def solve5_compare():
     s, sp, sh = dmed.Sector(8, 8), dmed.Sector(9, 8), dmed.Sector(7, 8)
     wm = np.pi / c["beta"] * (2 * np.arange(c["lmats"]) + 1)
     z = 1j * wm
-    out, res = {}, {}
-    for name, pgs, pgf in (("fp64_plain", dmed.FP64, dmed.FP64),
-                           ("mixed", dmed.MIXED, dmed.MIXED),
-                           ("fp64_gs_mixed_chains", dmed.FP64, dmed.MIXED)):
+    forms = (("fp64_plain", dmed.FP64, dmed.FP64), ("mixed", dmed.MIXED, dmed.MIXED),
+             ("fp64_gs_mixed_chains", dmed.FP64, dmed.MIXED))
+
+    def run(pgs, pgf, nlanc):
         t0 = time.perf_counter()
         e0, psi, r0, rc = s.ground_state(prec=pgs, tol=1e-12, maxit=6000)
         t_gs = time.perf_counter() - t0
@@ -199,19 +232,37 @@ def solve5_compare():
         t1 = time.perf_counter()
         for dst, dag, sgn in ((sp, 1, 1), (sh, 0, -1)):
             v, n2 = dmed.apply_op(psi, 1, dag, dst)
-            a, b, n = dmed.chain(dst, v, c["nlanc"], prec=pgf)
+            a, b, n = dmed.chain(dst, v, nlanc, prec=pgf)
             dmed.gf_add_chain(n2, e0, a, b, sgn, wm, np.zeros(1), 0.01, g, gr)
             v.free()
         t_gf = time.perf_counter() - t1
         psi.free()
         sig = dmed.sigma(0, 0, z, g)
-        res[name] = dict(e0=e0, dens=dens, docc=docc, g=g, s=sig)
-        out[name] = {"seconds": time.perf_counter() - t0, "t_gs": t_gs, "t_gf": t_gf,
-                     "E0": e0, "gs_residual": r0, "gs_rc": rc}
-        log(f"workload 5 {name}: {out[name]['seconds']:.2f}s (GS {t_gs:.2f}s, chains "
-            f"{t_gf:.2f}s), E0 {e0:.12f}, residual {r0:.2e}")
+        return dict(e0=e0, dens=dens, docc=docc, g=g, s=sig, r0=r0, rc=rc,
+                    seconds=time.perf_counter() - t0, t_gs=t_gs, t_gf=t_gf)
+
+    for _, pgs, pgf in forms[:2]:            # warm-up: back the buffers once
+        run(pgs, pgf, 40)
+    times = {n: [] for n, _, _ in forms}
+    res = {}
+    for rep in range(reps):
+        for name, pgs, pgf in forms:
+            if name == "fp64_gs_mixed_chains" and rep > 0:
+                continue
+            r = run(pgs, pgf, c["nlanc"])
+            res[name] = r
+            times[name].append({k: r[k] for k in ("seconds", "t_gs", "t_gf")})
+            log(f"workload 5 {name} #{rep}: {r['seconds']:.2f}s (GS {r['t_gs']:.2f}s, chains "
+                f"{r['t_gf']:.2f}s), E0 {r['e0']:.12f}, residual {r['r0']:.2e}")
+    out = {}
+    for name, _, _ in forms:
+        out[name] = {"seconds_median": _median([t["seconds"] for t in times[name]]),
+                     "t_gs_median": _median([t["t_gs"] for t in times[name]]),
+                     "t_gf_median": _median([t["t_gf"] for t in times[name]]),
+                     "runs": times[name], "E0": res[name]["e0"],
+                     "gs_residual": res[name]["r0"], "gs_rc": res[name]["rc"]}
     a64, amx = res["fp64_plain"], res["mixed"]
-    out["speedup"] = out["fp64_plain"]["seconds"] / out["mixed"]["seconds"]
+    out["speedup"] = out["fp64_plain"]["seconds_median"] / out["mixed"]["seconds_median"]
     out["errors_vs_fp64"] = {
         "E0_rel": abs(amx["e0"] - a64["e0"]) / abs(a64["e0"]),
         "dens": float(np.abs(amx["dens"] - a64["dens"]).max()),
@@ -224,11 +275,28 @@ def solve5_compare():
         "G_iw": float(np.abs(b["g"] - a64["g"]).max()),
         "Sigma_iw": float(np.abs(b["s"] - a64["s"]).max()),
     }
-    log(f"workload 5 mixed speedup {out['speedup']:.2f}x, errors {out['errors_vs_fp64']}; "
-        f"mixed chains alone {out['errors_vs_fp64_mixed_chains_only']}")
+    log(f"workload 5 mixed speedup {out['speedup']:.2f}x (median of {reps}), errors "
+        f"{out['errors_vs_fp64']}; mixed chains alone {out['errors_vs_fp64_mixed_chains_only']}")
     for v in (s, sp, sh):
         v.free()
     return out
+
+
+def cpu_model():
+    """CPU model and logical CPU count of this host (lscpu)."""
+    import subprocess
+    model, n = None, os.cpu_count()
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for ln in txt.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() == "Model name":
+                model = v.strip()
+            elif k.strip() == "CPU(s)":
+                n = int(v.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return model, n
 
 
 def main():
@@ -241,14 +309,16 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="rows of the oracle H*v sample (default: the whole sector)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--solve", action="store_true",
-                    help="also time the all-sector Ns=12 finite-T solve: plain FP64 vs "
-                         "batched mixed precision")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the all-sector Ns=12 finite-T solve (plain FP64 vs batched "
+                         "mixed precision)")
+    ap.add_argument("--no-solve5", action="store_true",
+                    help="skip the Ns=16 sector pipeline (ground state + 500-step chains, plain "
+                         "FP64 vs mixed precision)")
+    ap.add_argument("--reps", type=int, default=3,
+                    help="repetitions of each solve form (the speed-ups use the medians)")
     ap.add_argument("--configs", action="store_true",
                     help="also time workloads 1-3 beside the oracle (H*v; workload-1 solve)")
-    ap.add_argument("--solve5", action="store_true",
-                    help="also time the Ns=16 sector pipeline (ground state + 500-step chains): "
-                         "plain FP64 vs mixed precision")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solver runs the hot path on a single MI355X "
@@ -329,14 +399,13 @@ def main():
             f"algorithmic (48 B/element FP64, 24 FP32)")
 
     if a.configs:
-        extra["workloads_1_3_vs_oracle"] = configs_table(a.cpu_threads or min(16, os.cpu_count() or 1))
-    if a.solve:
-        extra["solve_ns12_all_sectors"] = solve_compare()
-    if a.solve5:
-        sec.free()
-        x.free()
-        y.free()
-        extra["solve_ns16_sector_pipeline"] = solve5_compare()
+        extra["workloads_1_3_vs_oracle"] = configs_table(a.cpu_threads or (os.cpu_count() or 1))
+    dim_up, dim_dw = sec.dim_up, sec.dim_dw
+    sec.free()
+    x.free()
+    y.free()
+    solve4 = None if a.no_solve else solve_compare(a.reps)
+    solve5 = None if a.no_solve5 else solve5_compare(a.reps)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hxv_ns16_traffic.json")
@@ -348,19 +417,30 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        model, ncpu = cpu_model()
+        threads = a.cpu_threads or ncpu or 1
         rows = min(a.cpu_rows, D) if a.cpu_rows > 0 else D
-        rate, dt = cpu_baseline(cfg, rows, threads)
+        # then one thread on a bounded row sample (about 10 s of work): the
+        # reference itself is serial per MPI rank
+        res = cpu_baseline(cfg, [(rows, threads),
+                                 (lambda r: min(D, max(1 << 20, int(10.0 * r[0][2] / threads))), 1)])
+        (_, _, rate, dt), (rows1, _, rate1, dt1) = res
         what = ("one full H*v of the dim-%d sector" % D if rows == D else
                 f"rows [0,{rows}) of the dim-{D} sector H*v, extrapolated to the full sector")
         cpu = {"value": rate / D, "unit": "Hv/s", "cores": threads, "kind": "port",
-               "sample": f"{what} ({dt:.1f}s): the oracle's complex(8) restatement of "
-                         f"directMatVec_cc (state map + binary search per hop), "
-                         f"{threads} OpenMP threads over row blocks"}
-        log(f"cpu baseline: {rate:.3e} rows/s on {threads} threads -> {rate / D:.4f} Hv/s")
+               "cpu_model": model, "logical_cpus": ncpu,
+               "sample": f"{what} ({dt:.1f}s) on {threads} OpenMP threads over row blocks: "
+                         f"the oracle's complex(8) restatement of directMatVec_cc (state map "
+                         f"+ binary search per hop)",
+               "one_thread": {"value": rate1 / D, "unit": "Hv/s", "cores": 1,
+                              "sample": f"rows [0,{rows1}) ({dt1:.1f}s), extrapolated to the "
+                                        f"full sector"}}
+        log(f"cpu baseline ({model}, {ncpu} logical CPUs): {rate / D:.4f} Hv/s on {threads} "
+            f"threads, {rate1 / D:.5f} Hv/s on 1 thread")
 
     out = {
-        "metric": "H*v/s at dim 1.66e8 (Ns=16 half-filled sector, FP64)",
+        "metric": "H*v/s at dim 1.66e8 (Ns=16 half-filled sector, FP64); batched mixed vs "
+                  "FP64 solve time (solve_speedup)",
         "value": value,
         "unit": "Hv/s",
         "n_gpus": 1,
@@ -372,12 +452,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == dmed.FP64 else "f32",
         "data": "synthetic (seeded SplitMix64 bath, random FP64 vector)",
-        "config": {"workload": cfg["name"], "dim": D, "dim_up": sec.dim_up,
-                   "dim_dw": sec.dim_dw, "norb": 1, "nbath": 15, "U": 2.5,
+        "config": {"workload": cfg["name"], "dim": D, "dim_up": dim_up,
+                   "dim_dw": dim_dw, "norb": 1, "nbath": 15, "U": 2.5,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
+        "solve_speedup": {
+            "workload4_all_sectors_mixed_batched_vs_fp64_plain":
+                solve4["speedup"] if solve4 else None,
+            "workload4_errors_vs_fp64": solve4["errors_vs_fp64"] if solve4 else None,
+            "workload5_sector_pipeline_mixed_vs_fp64_plain":
+                solve5["speedup"] if solve5 else None,
+            "workload5_errors_vs_fp64": solve5["errors_vs_fp64"] if solve5 else None,
+            "repetitions": a.reps, "statistic": "ratio of median wall times"},
+        "solve_ns12_all_sectors": solve4,
+        "solve_ns16_sector_pipeline": solve5,
         "kernel_ms": kernel_ms,
         "extra": extra,
         "algorithmic_bytes_per_launch": alg_bytes,

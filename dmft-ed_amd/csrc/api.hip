@@ -81,6 +81,18 @@ int exec_buf(Exec &e, int slot, size_t bytes, void **p)
     return 0;
 }
 
+// Host-to-device copy on the calling thread's stream, finished on return.
+// Never a null-stream copy: that would break a graph capture running on
+// another worker's stream.
+int copy_h2d(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return 0;
+    hipStream_t st = ex().stream;
+    DMED_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    DMED_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
 int ensure_partials(size_t n)
 {
     Exec &e = ex();

@@ -162,6 +162,7 @@ int model_fill(const dmed_model *m, Model &md);
 int upload_binom();                       // binomial table for rank_of, once per process
 
 int ensure_partials(size_t n);          // on ex()
+int copy_h2d(void *dst, const void *src, size_t bytes);   // on ex().stream, synchronous
 
 // ------------------------------------------------------------- sectors
 int sector_build(dmed_sector *s, int nup, int ndw);

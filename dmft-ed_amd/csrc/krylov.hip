@@ -523,6 +523,17 @@ int chain_init(const dmed_sector *s, void *bufs[3], int prec, int nlanc_total, d
 // captured three-step graph is replayed.  DMED_GRAPHS=0 turns this off.
 constexpr int kGraphMinSteps = 30;
 
+// DMED_TEST_CAPTURE_FAIL=q makes step q (0..2) of the captured three-step
+// window fail before it is recorded: a test of the fallback to eager steps.
+static int capture_fail_step()
+{
+    static const int q = [] {
+        const char *e = std::getenv("DMED_TEST_CAPTURE_FAIL");
+        return e ? std::atoi(e) : -1;
+    }();
+    return q;
+}
+
 static bool graphs_enabled()
 {
     static const bool on = [] {
@@ -578,16 +589,26 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
         ++k;
         hipGraph_t gr = nullptr;
         hipGraphExec_t ge = nullptr;
+        // nothing recorded during a capture has run: if the capture fails
+        // part-way, the eager loop below resumes from these buffers
+        void *const keep[3] = {vp, v, w};
         bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
             int rc = 0;
+            const int fail_at = capture_fail_step();
             for (int q = 0; q < 3 && !rc; ++q) {
-                rc = one(vp, v, w);
+                rc = q == fail_at ? fail(DMED_EHIP, "capture failure forced for a test")
+                                  : one(vp, v, w);
                 rotate();
             }
             // after three rotations the buffers are back where they started
             ok = hipStreamEndCapture(st, &gr) == hipSuccess && rc == 0 && gr != nullptr;
             ok = ok && hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) == hipSuccess;
+        }
+        if (!ok) {
+            vp = keep[0];
+            v = keep[1];
+            w = keep[2];
         }
         bool launched_fail = false;
         if (ok) {
@@ -757,7 +778,11 @@ int dense_eigh(const dmed_sector *s, int neig, double *evals, dmed_vec **evecs, 
             v->prec = DMED_FP64;
             v->dim = n;
             DMED_HIP(hipMalloc(&v->data, std::max<int64_t>(n, 1) * 8));
-            DMED_HIP(hipMemcpy(v->data, H.data() + (size_t)q * n, n * 8, hipMemcpyHostToDevice));
+            if (int rc = copy_h2d(v->data, H.data() + (size_t)q * n, n * 8)) {
+                hipFree(v->data);
+                delete v;
+                return rc;
+            }
             evecs[q] = v;
         }
     }

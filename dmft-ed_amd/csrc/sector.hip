@@ -18,11 +18,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace dmed {
 
 __constant__ uint32_t c_binom[32][32];   // C(p,k), p,k < 32
-static bool g_binom_ready = false;
+static std::once_flag g_binom_once;
+static hipError_t g_binom_status = hipSuccess;
 
 int64_t binom(int n, int k)
 {
@@ -32,17 +34,25 @@ int64_t binom(int n, int k)
     return r;
 }
 
+// The table is written once per process, by whichever thread gets here
+// first; the others wait for it (workers build sectors concurrently).
 int upload_binom()
 {
-    if (g_binom_ready) return 0;
-    uint32_t h[32][32];
-    for (int p = 0; p < 32; ++p)
-        for (int k = 0; k < 32; ++k) {
-            int64_t b = binom(p, k);
-            h[p][k] = (uint32_t)std::min<int64_t>(b, 0xffffffffll);
-        }
-    DMED_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), h, sizeof h));
-    g_binom_ready = true;
+    std::call_once(g_binom_once, [] {
+        uint32_t h[32][32];
+        for (int p = 0; p < 32; ++p)
+            for (int k = 0; k < 32; ++k) {
+                int64_t b = binom(p, k);
+                h[p][k] = (uint32_t)std::min<int64_t>(b, 0xffffffffll);
+            }
+        hipStream_t st = ex().stream;
+        g_binom_status = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_binom), h, sizeof h, 0,
+                                                hipMemcpyHostToDevice, st);
+        if (g_binom_status == hipSuccess) g_binom_status = hipStreamSynchronize(st);
+    });
+    if (g_binom_status != hipSuccess)
+        return fail(DMED_EHIP, std::string("binomial table upload: ") +
+                                   hipGetErrorString(g_binom_status));
     return 0;
 }
 
@@ -198,8 +208,8 @@ static int pack_hops(SpinTables &t)
         DMED_HIP(hipMalloc(&t.hpk, std::max<size_t>(pk.size(), 1) * 4));
         DMED_HIP(hipMalloc(&t.coef, coef.size() * 8));
         if (!pk.empty())
-            DMED_HIP(hipMemcpy(t.hpk, pk.data(), pk.size() * 4, hipMemcpyHostToDevice));
-        DMED_HIP(hipMemcpy(t.coef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice));
+            if (int rc = copy_h2d(t.hpk, pk.data(), pk.size() * 4)) return rc;
+        if (int rc = copy_h2d(t.coef, coef.data(), coef.size() * 8)) return rc;
         t.packed = true;
     }
     return 0;
@@ -342,7 +352,7 @@ static int build_pairs(const Model &m, SpinTables &t)
             }
         }
     DMED_HIP(hipMalloc(&t.ipair, tab.size() * 4));
-    DMED_HIP(hipMemcpy(t.ipair, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+    if (int rc = copy_h2d(t.ipair, tab.data(), tab.size() * 4)) return rc;
     t.h_ipair = std::move(tab);
     return 0;
 }
@@ -375,8 +385,7 @@ int sector_build(dmed_sector *s, int nup, int ndw)
         for (int id = 0; id < s->nimp; ++id)
             s->h_W[iu * s->nimp + id] = interaction(m, iu, id);
     DMED_HIP(hipMalloc(&s->W, s->h_W.size() * sizeof(double)));
-    DMED_HIP(hipMemcpy(s->W, s->h_W.data(), s->h_W.size() * sizeof(double),
-                       hipMemcpyHostToDevice));
+    if (int rc = copy_h2d(s->W, s->h_W.data(), s->h_W.size() * sizeof(double))) return rc;
     return 0;
 }
 
@@ -437,18 +446,18 @@ static int stack_spin(SpinTables &t)
     DMED_HIP(hipMalloc(&t.hval, hn * sizeof(double)));
     DMED_HIP(hipMalloc(&t.hval32, hn * sizeof(float)));
     if (d2) {
-        DMED_HIP(hipMemcpy(t.str, t.h_str.data(), d2 * 4, hipMemcpyHostToDevice));
-        DMED_HIP(hipMemcpy(t.diag, t.h_diag.data(), d2 * 8, hipMemcpyHostToDevice));
-        DMED_HIP(hipMemcpy(t.imp, t.h_imp.data(), d2, hipMemcpyHostToDevice));
+        if (int rc = copy_h2d(t.str, t.h_str.data(), d2 * 4)) return rc;
+        if (int rc = copy_h2d(t.diag, t.h_diag.data(), d2 * 8)) return rc;
+        if (int rc = copy_h2d(t.imp, t.h_imp.data(), d2)) return rc;
     }
     if (!t.h_hidx.empty()) {
-        DMED_HIP(hipMemcpy(t.hidx, t.h_hidx.data(), t.h_hidx.size() * 4, hipMemcpyHostToDevice));
-        DMED_HIP(hipMemcpy(t.hval, t.h_hval.data(), t.h_hval.size() * 8, hipMemcpyHostToDevice));
-        DMED_HIP(hipMemcpy(t.hval32, h32.data(), h32.size() * 4, hipMemcpyHostToDevice));
+        if (int rc = copy_h2d(t.hidx, t.h_hidx.data(), t.h_hidx.size() * 4)) return rc;
+        if (int rc = copy_h2d(t.hval, t.h_hval.data(), t.h_hval.size() * 8)) return rc;
+        if (int rc = copy_h2d(t.hval32, h32.data(), h32.size() * 4)) return rc;
     }
     if (!t.h_ipair.empty()) {
         DMED_HIP(hipMalloc(&t.ipair, t.h_ipair.size() * 4));
-        DMED_HIP(hipMemcpy(t.ipair, t.h_ipair.data(), t.h_ipair.size() * 4, hipMemcpyHostToDevice));
+        if (int rc = copy_h2d(t.ipair, t.h_ipair.data(), t.h_ipair.size() * 4)) return rc;
     }
     return pack_hops(t);
 }
