@@ -62,11 +62,10 @@ def cpu_baseline(cfg, samples):
         if callable(rows):
             rows = rows(out)
         y = np.zeros(rows, dtype=np.complex128)
-        # y is only written on [0, rows): pass it as the start of hv
         t0 = time.perf_counter()
-        rc = L.ref_hxv_rows(smap.ctypes.data_as(ip), len(smap),
-                            x.view(np.float64).ctypes.data_as(dp),
-                            y.view(np.float64).ctypes.data_as(dp), 0, rows, threads)
+        rc = L.ref_hxv_block(smap.ctypes.data_as(ip), len(smap),
+                             x.view(np.float64).ctypes.data_as(dp),
+                             y.view(np.float64).ctypes.data_as(dp), 0, rows, threads)
         dt = time.perf_counter() - t0
         if rc != 0:
             raise RuntimeError(f"oracle H*v failed ({rc})")

@@ -209,8 +209,9 @@ static inline int bath_stride(int a0, int k0)
 /* ----------------------------------------------------------- H * v */
 static int g_abort_flag = 0;
 
+/* Rows [first, last) of the product; row i is stored at hv[i - hoff]. */
 static void hxv_rows(const int64_t *map, long long dim, const zdouble *vin,
-                     zdouble *hv, long long first, long long last)
+                     zdouble *hv, long long first, long long last, long long hoff)
 {
     const int Ns = g_ns, Norb = g_norb, Nbath = g_nbath, Nsp = g_nspin - 1;
     int ib[64];
@@ -332,7 +333,7 @@ static void hxv_rows(const int64_t *map, long long dim, const zdouble *vin,
                     acc += vd * sg1 * sg2 * vin[j - 1];
                 }
             }
-        hv[i] = acc;
+        hv[i - hoff] = acc;
     }
 }
 
@@ -342,7 +343,7 @@ static void hxv_rows(const int64_t *map, long long dim, const zdouble *vin,
 int ref_hxv(const int64_t *map, long long dim, const double *vin, double *hv)
 {
     g_abort_flag = 0;
-    hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)hv, 0, dim);
+    hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)hv, 0, dim, 0);
     return g_abort_flag ? -3 : 0;
 }
 
@@ -360,20 +361,20 @@ int ref_hxv_omp(const int64_t *map, long long dim, const double *vin,
         int r = omp_get_thread_num(), p = omp_get_num_threads();
         long long q = dim / p, rem = (r == p - 1) ? dim % p : 0;
         hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)hv, r * q,
-                 (r + 1) * q + rem);
+                 (r + 1) * q + rem, 0);
     }
 #else
     (void)nthreads;
-    hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)hv, 0, dim);
+    hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)hv, 0, dim, 0);
 #endif
     return g_abort_flag ? -3 : 0;
 }
 
-/* Rows [first, last) of the same product with nthreads contiguous blocks
- * (a bounded sample of a large sector for the CPU timing baseline); hv is
- * written only on those rows. */
-int ref_hxv_rows(const int64_t *map, long long dim, const double *vin, double *hv,
-                 long long first, long long last, int nthreads)
+/* Rows [first, last) of the same product with nthreads contiguous blocks,
+ * stored at out[0 .. last-first): a bounded block of a large sector (the CPU
+ * timing sample, and parity checks of single columns or row ranges). */
+int ref_hxv_block(const int64_t *map, long long dim, const double *vin, double *out,
+                  long long first, long long last, int nthreads)
 {
     g_abort_flag = 0;
     if (first < 0 || last > dim || first > last) return -1;
@@ -383,12 +384,12 @@ int ref_hxv_rows(const int64_t *map, long long dim, const double *vin, double *h
     {
         int r = omp_get_thread_num(), p = omp_get_num_threads();
         long long n = last - first, q = n / p, rem = (r == p - 1) ? n % p : 0;
-        hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)hv, first + r * q,
-                 first + (r + 1) * q + rem);
+        hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)out, first + r * q,
+                 first + (r + 1) * q + rem, first);
     }
 #else
     (void)nthreads;
-    hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)hv, first, last);
+    hxv_rows(map, dim, (const zdouble *)vin, (zdouble *)out, first, last, first);
 #endif
     return g_abort_flag ? -3 : 0;
 }

@@ -70,8 +70,8 @@ def lib():
         L.ref_binary_search.argtypes = [_i64p, C.c_longlong, C.c_int64]
         L.ref_hxv.argtypes = [_i64p, C.c_longlong, _dp, _dp]
         L.ref_hxv_omp.argtypes = [_i64p, C.c_longlong, _dp, _dp, C.c_int]
-        L.ref_hxv_rows.argtypes = [_i64p, C.c_longlong, _dp, _dp, C.c_longlong, C.c_longlong,
-                                   C.c_int]
+        L.ref_hxv_block.argtypes = [_i64p, C.c_longlong, _dp, _dp, C.c_longlong, C.c_longlong,
+                                    C.c_int]
         L.ref_pythag.restype = C.c_double
         L.ref_pythag.argtypes = [C.c_double, C.c_double]
         L.ref_tql2.argtypes = [C.c_int, _dp, _dp, _dp]
@@ -157,6 +157,17 @@ def hxv(smap, x, nthreads=1):
         rc = lib().ref_hxv(_i(smap), len(smap), _d(x.view(np.float64)), _d(y.view(np.float64)))
     if rc != 0:
         raise RuntimeError(f"ref_hxv failed ({rc})")
+    return y
+
+
+def hxv_block(smap, x, first, last, nthreads=1):
+    """Rows [first, last) of the complex(8) on-the-fly H*v (directMatVec_cc)."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.empty(last - first, dtype=np.complex128)
+    rc = lib().ref_hxv_block(_i(smap), len(smap), _d(x.view(np.float64)),
+                             _d(y.view(np.float64)), first, last, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"ref_hxv_block failed ({rc})")
     return y
 
 

@@ -75,6 +75,21 @@ def test_openmp_row_blocks_match_serial():
     assert np.array_equal(edref.hxv(m, x, nthreads=3), edref.hxv(m, x))
 
 
+def test_hxv_block_matches_full_product():
+    """ref_hxv_block (rows [first, last) into their own array) against the
+    rows of the full product, on blocks at the start, inside and at the end
+    of a sector, for 1 and 3 threads."""
+    g = load("m3_norb2_nb2_jh")
+    set_model_from(g)
+    m = edref.build_sector(3, 3)
+    x = np.random.default_rng(2).standard_normal(len(m)) + 0j
+    y = edref.hxv(m, x)
+    n = len(m)
+    for first, last in ((0, 7), (5, n - 3), (n - 11, n), (17, 17)):
+        for t in (1, 3):
+            assert np.array_equal(edref.hxv_block(m, x, first, last, nthreads=t), y[first:last])
+
+
 def test_c_cdg_pauli_and_sign():
     # sign = (-1)^(occupied positions below pos); Pauli violations raise
     out, s = edref.op_c(3, 0b0111)
