@@ -241,6 +241,39 @@ int eigh_sector_ex(const dmed_sector *s, int neig, int ncv, int maxit, double to
 int ground_state(const dmed_sector *s, int prec, int maxit, double tol, double *eval,
                  dmed_vec **evec, double *resid);
 
+// --------------------------------------------- persistent sector queue
+// Small Krylov sectors of a batched solve, diagonalised by one launch of a
+// persistent work-queue kernel (sectorq.hip): each workgroup takes a sector
+// from a device counter and runs its whole thick-restart Lanczos with the
+// multiplet-completing deflation rounds, with no host round trip.
+constexpr int64_t kQueueMaxDim = 16384;
+struct QueueItem {
+    const dmed_sector *s = nullptr;
+    int neig = 1, ncv = 0, maxit = 0;
+    double tol = 1e-12, degen = 0.0;
+    bool lowest = false;                    // lowest level and its copies only
+    std::vector<dmed_vec *> vecs;           // out: nout eigenvectors, ascending
+    std::vector<double> evals, resid;       // out
+    int nout = 0, rc = 0, used = 0;         // out: rc 0 or DMED_ENOCONV
+};
+struct SectorQueue {
+    std::vector<QueueItem> items;
+    float kernel_ms = 0.f;                  // out: duration of the launch
+    double *work = nullptr, *res = nullptr;
+    double **ptrs = nullptr;
+    int *counter = nullptr;
+    void *tasks = nullptr;
+    hipStream_t stream = nullptr;           // set once launched
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+// dimension limit of the queue: kQueueMaxDim, or DMED_SECTOR_QUEUE_MAXDIM
+int64_t queue_max_dim();
+bool queue_takes(const dmed_sector *s, int neig, int ncv);
+// launch on st (asynchronous); queue_finish waits, fills the items and frees
+// the queue's buffers (also after a failed launch)
+int queue_launch(SectorQueue &q, hipStream_t st);
+int queue_finish(SectorQueue &q);
+
 // ------------------------------------------------------------ host LA
 // symmetric eigen-decomposition of a dense n x n matrix (column-major, in
 // place -> eigenvectors), eigenvalues ascending

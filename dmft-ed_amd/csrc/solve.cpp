@@ -341,8 +341,66 @@ extern "C" int dmed_solve(const dmed_solve_params *p, dmed_solve_out *out)
             if (!derived[i]) order.push_back(i);
         std::stable_sort(order.begin(), order.end(),
                          [&](int x, int y) { return secs[x]->dim > secs[y]->dim; });
-        if ((rc = run_tasks(p->batched, (int)order.size(), [&](int k) { solve_sector(order[k]); })))
-            return rc;
+        // Batched form: the small Krylov sectors go to one launch of the
+        // persistent work-queue solver on a stream of their own, while the
+        // host workers take the rest (largest first) on theirs.
+        SectorQueue q;
+        std::vector<int> qsec, rest;
+        const char *q_env = std::getenv("DMED_SECTOR_QUEUE");
+        const bool use_queue = p->batched && !(q_env && q_env[0] == '0');
+        for (int is : order) {
+            const int64_t dim = secs[is]->dim;
+            const int ne = (int)std::min<int64_t>(dim, neigen[is]);
+            const int nblock = (int)std::min<int64_t>(dim, p->lanc_ncv_factor * ne + p->lanc_ncv_add);
+            const int ncv = std::max(nblock, 2 * ne + 10);
+            if (use_queue && dim > p->lanc_dim_threshold && ne < dim &&
+                queue_takes(secs[is], ne, ncv)) {
+                QueueItem it;
+                it.s = secs[is];
+                it.neig = ne;
+                it.ncv = ncv;
+                it.maxit = 50 * std::max(p->lanc_niter, 100);
+                it.tol = p->lanc_tolerance;
+                it.degen = 10.0 * p->gs_threshold;
+                it.lowest = !finite_t;
+                q.items.push_back(std::move(it));
+                qsec.push_back(is);
+            } else {
+                rest.push_back(is);
+            }
+        }
+        hipStream_t qs = nullptr;
+        const double tq0 = now();
+        int qrc = 0;
+        if (!q.items.empty()) {
+            if (hipStreamCreateWithFlags(&qs, hipStreamNonBlocking) != hipSuccess)
+                qrc = fail(DMED_EHIP, "sector queue: stream");
+            if (!qrc) qrc = queue_launch(q, qs);
+        }
+        rc = run_tasks(p->batched, (int)rest.size(), [&](int k) { solve_sector(rest[k]); });
+        const int frc = queue_finish(q);
+        if (!qrc) qrc = frc;
+        if (qs) hipStreamDestroy(qs);
+        if (rc) return rc;
+        if (p->verbose > 1 && !q.items.empty())
+            std::fprintf(stderr, "sector queue: %d sectors, kernel %.3f ms; %.3f ms with the "
+                                 "workers\n", (int)q.items.size(), q.kernel_ms, 1e3 * (now() - tq0));
+        for (size_t k = 0; k < qsec.size(); ++k) {
+            QueueItem &it = q.items[k];
+            SectorRes &r = sres[qsec[k]];
+            if (qrc) {
+                // the queue could not run: these sectors take the host-driven path
+                solve_sector(qsec[k]);
+                continue;
+            }
+            r.nout = it.nout;
+            r.ev = it.evals;
+            r.vecs = it.vecs;
+            r.rc = 0;                      // DMED_ENOCONV: best pairs kept, as ARPACK's
+            if (p->verbose > 1)
+                std::fprintf(stderr, "sector %d dim=%lld neig=%d nout=%d queue, %d H*v\n",
+                             qsec[k], (long long)it.s->dim, it.neig, it.nout, it.used);
+        }
     }
     for (int is = 0; is < nsectors && !rc; ++is) {
         if (!derived[is]) continue;

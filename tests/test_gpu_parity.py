@@ -466,6 +466,36 @@ def test_solve_vs_oracle(name, finite):
             assert rel(r["sreal"][s, s, a, a], o["sreal"][s, a]) < 1e-3
 
 
+@pytest.mark.parametrize("name,finite", [("m1_norb1_nb3", False), ("m4_norb3_nb1", False),
+                                          ("m8_norb2_nb2_nohf", False), ("m5_norb1_nb5_T", True)])
+def test_sector_queue_solve_vs_oracle(name, finite):
+    """The batched solve with its persistent work-queue eigensolver taking
+    every Krylov sector of these models (dimensions above a dense threshold
+    of 16, up to 400) against the oracle's solve: the bars of
+    test_solve_vs_oracle."""
+    g = load(name)
+    set_both(g)
+    norb, nspin = int(g["norb"]), int(g["nspin"])
+    kw = dict(beta=float(g["beta"]), lmats=len(g["wm"]), lreal=len(g["wr"]),
+              wini=float(g["wr"][0]), wfin=float(g["wr"][-1]), eps=float(g["eps"]),
+              lanc_ngfiter=200)
+    if finite:
+        kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
+    say(f"[{name}] batched solve through the sector queue, and the oracle")
+    r, o = _solve_both(norb, int(g["nbath"]), nspin, g["bath"], g["hloc_f"], list(g["uloc"][:norb]),
+                       float(g["ust"]), float(g["jh"]), float(g["xmu"]), bool(g["hfmode"]),
+                       jx=float(g["jx"]), jp=float(g["jp"]), jhflag=bool(g["jhflag"]),
+                       batched=1, lanc_dim_threshold=16, **kw)
+    assert abs(r["egs"] - o["egs"]) <= 1e-10 * abs(o["egs"])
+    assert r["nstates"] == len(o["states"])
+    assert np.allclose(r["dens"], o["dens"], atol=1e-9)
+    assert np.allclose(r["docc"], o["docc"], atol=1e-9)
+    for s_ in range(nspin):
+        for a in range(norb):
+            assert np.abs(r["gmats"][s_, s_, a, a] - o["gmats"][s_, a]).max() < 1e-8
+            assert np.abs(r["smats"][s_, s_, a, a] - o["smats"][s_, a]).max() < 1e-8
+
+
 @pytest.mark.parametrize("name,finite", [("m1_norb1_nb3", False), ("m2_norb2_nb1", False),
                                           ("m7_atomic", False), ("m5_norb1_nb5_T", True)])
 def test_spin_chi_vs_oracle(name, finite):
@@ -734,7 +764,7 @@ def test_solve_config1_vs_oracle():
 def test_batched_solve_identical_to_serial(name, finite):
     """The batched engine (sectors and chains on concurrent per-thread
     streams) keeps the same states and accumulates G in the same order, so
-    it reproduces the serial solve."""
+    with host-driven sector solves it reproduces the serial solve."""
     g = load(name)
     set_both(g)
     norb, nspin = int(g["norb"]), int(g["nspin"])
@@ -744,10 +774,16 @@ def test_batched_solve_identical_to_serial(name, finite):
               lanc_ngfiter=200, lanc_dim_threshold=16)
     if finite:
         kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
-    say(f"[{name}] serial vs batched")
+    say(f"[{name}] serial vs batched (host-driven sector solves)")
     kw.update(chi_spin=1, chi_dens=1, ltau=100)
     r0 = dmed.solve(norb, nspin, ns, dmed.default_params(batched=0, **kw))
-    r1 = dmed.solve(norb, nspin, ns, dmed.default_params(batched=1, **kw))
+    # the persistent sector queue uses its own dense eigensolver for the
+    # projected matrix; without it the batched solve is bit for bit the serial one
+    os.environ["DMED_SECTOR_QUEUE"] = "0"
+    try:
+        r1 = dmed.solve(norb, nspin, ns, dmed.default_params(batched=1, **kw))
+    finally:
+        del os.environ["DMED_SECTOR_QUEUE"]
     assert r0["nstates"] == r1["nstates"]
     assert r0["egs"] == r1["egs"]
     assert np.abs(r0["gmats"] - r1["gmats"]).max() <= 1e-13
