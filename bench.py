@@ -358,6 +358,8 @@ def main():
     alg_bytes = 2 * esz * D
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     value = a.steps / wall
+    shape = sec.hxv_shape(prec)
+    log(f"H*v launch shape: {shape}")
     log(f"H*v: {ms_step:.4f} ms/step wall, kernel {kernel_ms:.4f} ms, "
         f"{achieved:.1f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of HBM peak)")
 
@@ -468,6 +470,7 @@ def main():
         "solve_ns12_all_sectors": solve4,
         "solve_ns16_sector_pipeline": solve5,
         "kernel_ms": kernel_ms,
+        "hxv_launch_shape": shape,
         "extra": extra,
         "algorithmic_bytes_per_launch": alg_bytes,
     }

@@ -407,6 +407,15 @@ int dmed_hxv_timed(const dmed_sector *s, const void *x, void *y, int32_t prec, i
     return rc;
 }
 
+int dmed_hxv_shape_of(const dmed_sector *s, int32_t prec, dmed_hxv_shape *out)
+{
+    REQUIRE_MODEL();
+    if (int rc = check_sector(s)) return rc;
+    if (!out) return fail(DMED_EINVAL, "null argument");
+    if (prec != DMED_FP64 && prec != DMED_FP32) return fail(DMED_EINVAL, "bad precision");
+    return hxv_shape(s, prec, out);
+}
+
 int dmed_sync(void)
 {
     REQUIRE_INIT();

@@ -204,6 +204,9 @@ struct ChainScalars {      // device-resident (inside Context::dscal)
 // thread.
 void set_thread_graphs(bool on);
 
+// the plain-mode launch shape of s (after tuning)
+int hxv_shape(const dmed_sector *s, int prec, dmed_hxv_shape *out);
+
 // mode kLanczos: y = sx*H*x with the block partials of <x, y> (cs->sx)
 int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
                ChainScalars *cs, double *partials, int *nparts, hipStream_t st);

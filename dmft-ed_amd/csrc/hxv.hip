@@ -193,8 +193,8 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
 
 
 // ------------------------------------------------------------------------
-// Persistent column kernel for large columns (D_up up to 1024*RPT).
-// One 1024-lane workgroup per CU (two where registers and LDS allow) walks
+// Persistent column kernel for large columns (D_up up to BS*RPT).
+// BS-lane workgroups (one or more per CU, as registers and LDS allow) walk
 // columns c = xcd*nper + loc + k*grid: chunks of nper consecutive columns are
 // dealt round-robin to the 8 XCDs (workgroup b runs on XCD b%8), so each
 // XCD's concurrent columns are neighbours and the whole chip works on one
@@ -208,17 +208,15 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
 //      stays L2-resident;
 //   4. the diagonal d_up + d_dw + W is added last from the LDS copy of x, and
 //      y is written with non-temporal stores.
-// Each lane owns rows tid + 1024*r, r < RPT.  Rows past D_up load row D_up-1
+// Each lane owns rows tid + BS*r, r < RPT.  Rows past D_up load row D_up-1
 // (in bounds) and are never stored: the inner loops carry no per-row branch,
 // so all loads of a hop step are in flight before the first is used.
 // Summation order per row: down hops in table order, up hops in table order,
-// then the diagonal.
-//
-// SPLIT: the down hops were already summed into Y by k_hxv_dn (the same FMA
-// chain in the same order); step 2 loads that partial column instead, and the
-// kernel finishes Y in place.  The result is bit-identical to the one-pass form.
-template <typename T, int MODE, int RPT, bool SPLIT>
-__global__ void __launch_bounds__(1024) k_hxv_col(
+// then the diagonal -- the same for every (BS, RPT) and grid, so the launch
+// shape never changes y.  In Lanczos mode each column's <x, y> is reduced on
+// its own and stored at PARTIALS[c]: alpha does not depend on the grid either.
+template <typename T, int MODE, int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_hxv_col(
     const T *__restrict__ X, T *__restrict__ Y, int32_t dup, int32_t ddw, int32_t nhu, int32_t nhd,
     int32_t nimp, int32_t ncoef, const uint32_t *__restrict__ HU, const double *__restrict__ COEF,
     const int32_t *__restrict__ HDI, const double *__restrict__ HDV, const double *__restrict__ DU,
@@ -252,16 +250,25 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
     for (int q = 0; q < NP; ++q) imp4[q] = 0u;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        const int ic = min((int)threadIdx.x + r * 1024, last);
+        const int ic = min((int)threadIdx.x + r * BS, last);
         imp4[r / 4] |= (uint32_t)IMPU[ic] << (8 * (r % 4));
     }
-    double dot = 0.0;
+    // Lanczos-mode <x, y> of a column in the order of the 1024-lane form:
+    // row i belongs to virtual lane i % 1024, which sums its rows in order;
+    // the 64-lane groups of virtual lanes are reduced by the same shuffle
+    // tree and their 16 sums by one more.  Thread t of a BS-lane workgroup
+    // holds the virtual lanes t + BS*j, j < M, in dot[j].
+    constexpr int M = 1024 / BS;
+    static_assert(M * BS == 1024, "lanes per workgroup must divide 1024");
     for (int c = xcd * nper + loc; c < ddw; c += gridDim.x) {
+        double dot[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) dot[j] = 0.0;
         const char *xc = reinterpret_cast<const char *>(X + (int64_t)c * dup);
         // 1. x(:,c) -> LDS: 16-byte granules (1 KB per wave instruction)
         //    when every column start is 16-byte aligned, else 4-byte ones
         if (dma16) {
-            for (int k = wave; k * 1024 < colbytes; k += 16) {
+            for (int k = wave; k * 1024 < colbytes; k += BS / 64) {
                 const int off = k * 1024 + lane * 16;
                 if (off < colbytes)
                     __builtin_amdgcn_global_load_lds(
@@ -269,7 +276,7 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
                         (__attribute__((address_space(3))) void *)(smem + k * 1024), 16, 0, 0);
             }
         } else {
-            for (int k = wave; k * 256 < colbytes; k += 16) {
+            for (int k = wave; k * 256 < colbytes; k += BS / 64) {
                 const int off = k * 256 + lane * 4;
                 if (off < colbytes)
                     __builtin_amdgcn_global_load_lds(
@@ -280,18 +287,10 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
         if ((int)threadIdx.x < nimp)
             wt[threadIdx.x] = DD[c] + WTAB[threadIdx.x * nimp + IMPD[c]];
         T acc[RPT];
-        if constexpr (SPLIT) {
-            // 2'. the down-hop sums of this column, written by k_hxv_dn
-            const T *yc = Y + (int64_t)c * dup;
-#pragma unroll
-            for (int r = 0; r < RPT; ++r)
-                acc[r] = __builtin_nontemporal_load(&yc[min((int)threadIdx.x + r * 1024, last)]);
-        } else {
 #pragma unroll
         for (int r = 0; r < RPT; ++r) acc[r] = (T)0;
-        }
         // 2. down hops
-        for (int h0 = 0; !SPLIT && h0 < nhd; h0 += HDU) {
+        for (int h0 = 0; h0 < nhd; h0 += HDU) {
             T v[HDU];
             const T *xj[HDU];
 #pragma unroll
@@ -304,7 +303,7 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
 #pragma unroll
             for (int q = 0; q < HDU; ++q)
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) g[q][r] = xj[q][min((int)threadIdx.x + r * 1024, last)];
+                for (int r = 0; r < RPT; ++r) g[q][r] = xj[q][min((int)threadIdx.x + r * BS, last)];
 #pragma unroll
             for (int r = 0; r < RPT; ++r)
 #pragma unroll
@@ -316,7 +315,7 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
         for (int h = 0; h < nhu; ++h) {
             uint32_t w[RPT];
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) w[r] = HU[(int64_t)h * dup + min((int)threadIdx.x + r * 1024, last)];
+            for (int r = 0; r < RPT; ++r) w[r] = HU[(int64_t)h * dup + min((int)threadIdx.x + r * BS, last)];
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
                 T vv = sc[(w[r] >> 1) & 127u];
@@ -327,7 +326,7 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
         // 4. diagonal last, with the lane's own x from LDS
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
-            const int i = threadIdx.x + r * 1024;
+            const int i = threadIdx.x + r * BS;
             if (i < dup) {
                 const int64_t o = (int64_t)c * dup + i;
                 const T xi = col[i];
@@ -338,73 +337,31 @@ __global__ void __launch_bounds__(1024) k_hxv_col(
                 } else {
                     const T yy = (T)(sx * (double)hv);
                     __builtin_nontemporal_store(yy, &Y[o]);
-                    dot += (double)xi * (double)yy;
+                    dot[r % M] += (double)xi * (double)yy;
                 }
+            }
+        }
+        if (MODE == kLanczos) {
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                const double ws = wave_sum(dot[j]);
+                if (lane == 0) red[wave + (BS / 64) * j] = ws;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                const double v = wave_sum(lane < 16 ? red[lane] : 0.0);
+                if (lane == 0) PARTIALS[c] = v;
             }
         }
         __syncthreads();
     }
-    if (MODE == kLanczos) {
-        const double s = block_sum(dot, red);
-        if (threadIdx.x == 0) PARTIALS[blockIdx.x] = s;
-    }
 }
 
-// ------------------------------------------------------------------------
-// Down-spin hops of a large sector over row panels: the first pass of the
-// split H*v.  For every column c and the rows r of one panel,
-//     Y(r, c) = sum_h td_h(c) X(r, jd_h(c))        (table order, from 0)
-// Every column of X is read by the ~nhd columns that hop to it.  With whole
-// columns (D_up rows) those re-reads miss the 4 MB L2 of an XCD; a panel of
-// R rows of all D_dw columns fits, so each element of X is fetched once per
-// H*v and re-read from L2.  Panel p is worked on by the workgroups of XCD
-// p % 8 (workgroup b runs on XCD b % 8), all of them on the same panel at a
-// time, panels in turn.  A 256-lane workgroup covers 256/R column segments:
-// lane t owns row t % R of column segment t / R.  NH hops are loaded and
-// gathered before any is added; a padded hop adds 0 * x, which leaves the sum
-// unchanged.  Rows past D_up load row D_up - 1 (in bounds) and are not stored.
-template <typename T, int MODE, int R>
-__global__ void __launch_bounds__(256) k_hxv_dn(
-    const T *__restrict__ X, T *__restrict__ Y, int32_t dup, int32_t ddw, int32_t nhd,
-    const int32_t *__restrict__ HDI, const T *__restrict__ HDV, const ChainScalars *__restrict__ CS)
-{
-    constexpr int SEG = 256 / R;
-    constexpr int NH = 8;
-    if (MODE == kLanczos && CS->done) return;
-    const int nper = gridDim.x / 8;
-    const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
-    const int seg = threadIdx.x / R, rr = threadIdx.x % R;
-    const int npanel = (dup + R - 1) / R;
-    const int ncg = (ddw + SEG - 1) / SEG;
-    for (int p = xcd; p < npanel; p += 8) {
-        const int r0 = p * R + rr;
-        const int row = min(r0, dup - 1);
-        for (int g = loc; g < ncg; g += nper) {
-            const int c = min(g * SEG + seg, ddw - 1);
-            const bool st = r0 < dup && g * SEG + seg < ddw;
-            T acc = (T)0;
-            for (int h0 = 0; h0 < nhd; h0 += NH) {
-                int32_t j[NH];
-                T v[NH], x[NH];
-#pragma unroll
-                for (int q = 0; q < NH; ++q) {
-                    const int h = min(h0 + q, nhd - 1);
-                    j[q] = HDI[(int64_t)h * ddw + c];
-                    v[q] = h0 + q < nhd ? HDV[(int64_t)h * ddw + c] : (T)0;
-                }
-#pragma unroll
-                for (int q = 0; q < NH; ++q) x[q] = X[(int64_t)j[q] * dup + row];
-#pragma unroll
-                for (int q = 0; q < NH; ++q) acc += v[q] * x[q];
-            }
-            if (st) __builtin_nontemporal_store(acc, &Y[(int64_t)c * dup + row]);
-        }
-    }
-}
-
-// rows per lane the column kernel supports per element type (register budget)
+// rows per lane the column kernel supports (register budget): per element
+// type for 1024-lane workgroups, and for the 512- and 256-lane forms
 constexpr int kColMaxRptF64 = 13;
 constexpr int kColMaxRptF32 = 16;
+constexpr int kColMaxRptSmall = 8;
 
 // dynamic LDS of the column kernel: the column rounded up to whole 1 KB DMA
 // chunks (the last 16-byte-granule chunk may be partial)
@@ -413,60 +370,143 @@ static size_t col_lds_bytes(int64_t dup, size_t esz)
     return ((size_t)dup * esz + 1023) & ~(size_t)1023;
 }
 
+static int col_max_rpt(int bs, int prec)
+{
+    if (bs != 1024) return kColMaxRptSmall;
+    return prec == DMED_FP32 ? kColMaxRptF32 : kColMaxRptF64;
+}
+
+// rows per lane of a bs-lane column workgroup, 0 when the column does not fit
+static int col_rpt(const dmed_sector *s, int prec, int bs)
+{
+    const int64_t rpt = (s->up.dim + bs - 1) / bs;
+    return rpt >= 1 && rpt <= col_max_rpt(bs, prec) ? (int)rpt : 0;
+}
+
+// The fixed rule: the 1024-lane column kernel for sectors with D_up >= 512
+// whose column fits in LDS and whose hop words pack; the tile kernel otherwise.
 static bool use_col_kernel(const dmed_sector *s, int prec)
 {
     const Context &c = ctx();
     const size_t esz = prec == DMED_FP32 ? 4 : 8;
     const int64_t dup = s->up.dim;
-    const int rpt = (int)((dup + 1023) / 1024);
-    const int maxr = prec == DMED_FP32 ? kColMaxRptF32 : kColMaxRptF64;
     return !s->jterms && s->up.packed && s->up.ncoef <= 128 && s->nimp <= 8 && dup >= 512 &&
-           rpt <= maxr && col_lds_bytes(dup, esz) + 2048 <= c.lds_max && s->dw.dim >= 64;
+           col_rpt(s, prec, 1024) > 0 && col_lds_bytes(dup, esz) + 2048 <= c.lds_max &&
+           s->dw.dim >= 64;
 }
 
 // A kernel's dynamic-LDS limit is process-wide state: it is raised once per
-// kernel, to all the LDS a workgroup may have, so concurrent launches from
-// several host threads never change it under one another.
-template <auto KERN>
-static int lds_attr_once()
+// kernel, to all the LDS a workgroup may have, before the kernel's first
+// launch or occupancy query, under a lock.  Every launch site calls this
+// first, so concurrent launches from several host threads never see the
+// attribute change under them.
+static int lds_attr_once(const void *kern)
 {
-    static const hipError_t e = [] {
+    static std::mutex mu;
+    static std::map<const void *, hipError_t> done;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = done.find(kern);
+    hipError_t e;
+    if (it != done.end()) {
+        e = it->second;
+    } else {
         hipFuncAttributes fa{};
-        hipError_t r = hipFuncGetAttributes(&fa, (const void *)KERN);
-        if (r != hipSuccess) return r;
-        const size_t lim = ctx().lds_max > fa.sharedSizeBytes ? ctx().lds_max - fa.sharedSizeBytes : 0;
-        return hipFuncSetAttribute((const void *)KERN, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lim);
-    }();
+        e = hipFuncGetAttributes(&fa, kern);
+        if (e == hipSuccess) {
+            const size_t lim =
+                ctx().lds_max > fa.sharedSizeBytes ? ctx().lds_max - fa.sharedSizeBytes : 0;
+            e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+        }
+        done[kern] = e;
+    }
     if (e != hipSuccess)
         return fail(DMED_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
     return 0;
 }
 
-template <typename T, int RPT>
-static int col_fit_per_cu(size_t shmem)
+// Column-kernel instantiations: 1024 lanes with 1..13 (FP64) or 1..16 (FP32)
+// rows per lane, 512 and 256 lanes with 1..8.  f(kernel pointer) is called
+// with the instantiation for (bs, rpt); false if there is none.
+template <typename T, int MODE, int BS, typename F>
+static bool col_dispatch_bs(int rpt, F &&f)
+{
+    switch (rpt) {
+#define DMED_R(R)                                                \
+    case R:                                                      \
+        if constexpr (R <= (BS == 1024 ? (sizeof(T) == 4 ? 16 : 13) : 8)) { \
+            f(k_hxv_col<T, MODE, BS, R>);                        \
+            return true;                                         \
+        }                                                        \
+        return false;
+        DMED_R(1) DMED_R(2) DMED_R(3) DMED_R(4) DMED_R(5) DMED_R(6) DMED_R(7) DMED_R(8)
+        DMED_R(9) DMED_R(10) DMED_R(11) DMED_R(12) DMED_R(13) DMED_R(14) DMED_R(15) DMED_R(16)
+#undef DMED_R
+    default:
+        return false;
+    }
+}
+
+template <typename T, int MODE, typename F>
+static bool col_dispatch(int bs, int rpt, F &&f)
+{
+    if (bs == 1024) return col_dispatch_bs<T, MODE, 1024>(rpt, f);
+    if (bs == 512) return col_dispatch_bs<T, MODE, 512>(rpt, f);
+    if (bs == 256) return col_dispatch_bs<T, MODE, 256>(rpt, f);
+    return false;
+}
+
+// workgroups of (bs, rpt) per CU that registers and LDS allow
+template <typename T>
+static int col_fit_per_cu(int bs, int rpt, size_t shmem)
 {
     int nb = 1;
-    if (lds_attr_once<k_hxv_col<T, kLanczos, RPT, false>>() != 0 ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, (const void *)k_hxv_col<T, kLanczos, RPT, false>, 1024, shmem) != hipSuccess)
-        nb = 1;
+    const bool ok = col_dispatch<T, kLanczos>(bs, rpt, [&](auto kern) {
+        if (lds_attr_once((const void *)kern) != 0 ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)kern, bs, shmem) !=
+                hipSuccess)
+            nb = 1;
+    });
+    (void)hipGetLastError();
+    return ok ? std::max(nb, 1) : 1;
+}
+
+static int col_fit(const dmed_sector *s, int prec, int bs)
+{
+    // the occupancy query is the same for every sector of this shape: keep it
+    static std::mutex mu;
+    static std::map<std::tuple<int64_t, int, int>, int> memo;
+    const auto key = std::make_tuple(s->up.dim, prec == DMED_FP32 ? 1 : 0, bs);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = memo.find(key);
+        if (it != memo.end()) return it->second;
+    }
+    const int rpt = col_rpt(s, prec, bs);
+    const size_t shmem = col_lds_bytes(s->up.dim, prec == DMED_FP32 ? 4 : 8);
+    const int nb = rpt == 0 ? 0 : (prec == DMED_FP32 ? col_fit_per_cu<float>(bs, rpt, shmem)
+                                                     : col_fit_per_cu<double>(bs, rpt, shmem));
+    std::lock_guard<std::mutex> lk(mu);
+    memo[key] = nb;
     return nb;
 }
 
 // ------------------------------------------------------------ autotuner
-// Opt-in (DMED_TUNE=1).  The first H*v of a sector shape times the launch
-// shapes that give the same y bit for bit: the tile kernel's columns per
-// workgroup and workgroup size, and the column kernel's workgroups per CU.
-// It keeps the fastest shape for that sector shape and precision.
-// DMED_TUNE_FILE=path loads earlier choices on first use and appends new
-// ones, so later runs reuse them.  Lanczos-mode partial sums follow the
-// grid, so a tuned run's alpha/beta can differ from the fixed rules' in the
-// last bits.  By default the fixed rules below apply, and runs are
-// reproducible bit for bit.
+// On by default (DMED_TUNE=0 turns it off) for sectors of at least
+// DMED_TUNE_MIN_DIM states (2^20 unless set).  The first H*v of a sector
+// shape and precision times the launch shapes that leave every element's
+// sum unchanged -- it never switches between the column and the tile
+// kernel, whose summation orders differ -- and keeps the fastest:
+//   column kernel: lanes per workgroup (1024, 512, 256, i.e. rows per lane
+//                  ceil(D_up/lanes)) x workgroups per CU (as many as fit, <= 4);
+//   tile kernel:   columns per workgroup (the LDS tile, 1..64) x lanes.
+// A candidate replaces the fixed rule only when it is more than 3% faster.
+// The column kernel's Lanczos-mode partial sums are one per column, so a
+// tuned shape changes neither y nor alpha/beta; the tile kernel keeps the
+// fixed rule in Lanczos mode (its partial sums follow the grid).
+// DMED_TUNE_FILE=path loads earlier choices on first use and appends new ones.
 struct TuneChoice {
     int cb = 0, bs = 0;      // tile kernel (0: fixed rule)
-    int wgs = 0;             // column kernel workgroups per CU (0: fixed rule)
+    int cbs = 0, wgs = 0;    // column kernel lanes and workgroups per CU (0: fixed rule)
 };
 using TuneKey = std::tuple<int64_t, int64_t, int, int, int, int>;
 
@@ -492,9 +532,18 @@ static bool tune_enabled()
 {
     static const bool on = [] {
         const char *e = std::getenv("DMED_TUNE");
-        return e && e[0] == '1';
+        return !(e && e[0] == '0');
     }();
     return on;
+}
+
+static int64_t tune_min_dim()
+{
+    static const int64_t d = [] {
+        const char *e = std::getenv("DMED_TUNE_MIN_DIM");
+        return e ? (int64_t)std::atoll(e) : (int64_t)1 << 20;
+    }();
+    return d;
 }
 
 // choice recorded for this shape (no tuning here); zeros when none
@@ -507,99 +556,62 @@ static TuneChoice tune_get(const dmed_sector *s, int prec)
     return it == t.tab.end() ? TuneChoice{} : it->second;
 }
 
-// Workgroups of the persistent column kernel: as many per CU as its
-// registers and LDS allow, at most two (a third lowered the rate at
-// D_up = 3432), in multiples of the 8 XCDs, never more than the columns.
-static unsigned col_grid(const dmed_sector *s, int prec)
+// Column-kernel launch shape: lanes, rows per lane, workgroups per CU, grid.
+// Fixed rule: 1024 lanes, as many workgroups per CU as registers and LDS
+// allow, at most two (a third lowered the rate at D_up = 3432), in multiples
+// of the 8 XCDs, never more than the columns.
+struct ColShape {
+    int bs = 1024, rpt = 0, per_cu = 1, fit = 1;
+    unsigned grid = 8;
+    bool tuned = false;
+};
+
+static unsigned col_grid_for(const dmed_sector *s, int per_cu)
 {
     const int ncu = std::max(8, ctx().num_cu);
-    const int64_t dup = s->up.dim;
-    const int rpt = (int)((dup + 1023) / 1024);
-    const bool f32 = prec == DMED_FP32;
-    const size_t shmem = col_lds_bytes(dup, f32 ? 4 : 8);
-    int fit = 1;
-    switch (rpt) {
-#define DMED_FIT(R) \
-    case R: fit = f32 ? col_fit_per_cu<float, R>(shmem) : col_fit_per_cu<double, R>(shmem); break;
-        DMED_FIT(1) DMED_FIT(2) DMED_FIT(3) DMED_FIT(4) DMED_FIT(5) DMED_FIT(6) DMED_FIT(7)
-#undef DMED_FIT
-    default: fit = 1; break;
-    }
-    int per_cu = std::max(1, std::min(fit, 2));
-    const TuneChoice tc = tune_get(s, prec);
-    if (tc.wgs > 0) per_cu = std::min(tc.wgs, std::max(1, fit));
     int g = (ncu / 8) * 8 * per_cu;
     while (g > 8 && g / 8 > (s->dw.dim + 7) / 8) g -= 8;
     return (unsigned)g;
 }
 
-template <typename T, int MODE, int RPT, bool SPLIT>
-static int launch_col_t(const dmed_sector *s, const void *x, void *y, ChainScalars *cs,
-                        double *partials, unsigned grid, hipStream_t st)
+static ColShape col_shape(const dmed_sector *s, int prec, const TuneChoice &tc)
+{
+    ColShape c;
+    if (tc.cbs > 0 && col_rpt(s, prec, tc.cbs) > 0) {
+        c.bs = tc.cbs;
+        c.fit = col_fit(s, prec, c.bs);
+        c.per_cu = std::max(1, std::min(tc.wgs, c.fit));
+        c.tuned = true;
+    } else {
+        c.fit = col_fit(s, prec, 1024);
+        c.per_cu = std::max(1, std::min(c.fit, 2));
+    }
+    c.rpt = col_rpt(s, prec, c.bs);
+    c.grid = col_grid_for(s, c.per_cu);
+    return c;
+}
+
+template <typename T, int MODE>
+static int launch_col(const dmed_sector *s, const void *x, void *y, ChainScalars *cs,
+                      double *partials, const ColShape &c, hipStream_t st)
 {
     const int64_t dup = s->up.dim;
     const size_t shmem = col_lds_bytes(dup, sizeof(T));
     // 16-byte DMA granules need every column start 16-byte aligned
     const int dma16 = ((uintptr_t)x % 16 == 0) && ((size_t)dup * sizeof(T)) % 16 == 0;
-    constexpr auto kern = k_hxv_col<T, MODE, RPT, SPLIT>;
-    if (int rc = lds_attr_once<kern>()) return rc;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(1024),
-                       shmem, st, static_cast<const T *>(x), static_cast<T *>(y), (int32_t)dup,
-                       (int32_t)s->dw.dim, (int32_t)s->up.nhop, (int32_t)s->dw.nhop, (int32_t)s->nimp,
-                       (int32_t)s->up.ncoef, s->up.hpk, s->up.coef, s->dw.hidx, s->dw.hval,
-                       s->up.diag, s->dw.diag, s->up.imp, s->dw.imp, s->W, (int32_t)dma16,
-                       (const ChainScalars *)cs, partials);
-    DMED_HIP(hipGetLastError());
-    return 0;
-}
-
-template <typename T, int MODE, bool SPLIT>
-static int launch_col_rpt(int rpt, const dmed_sector *s, const void *x, void *y,
-                          ChainScalars *cs, double *partials, unsigned grid, hipStream_t st)
-{
-    switch (rpt) {
-#define DMED_RPT(R) \
-    case R: return launch_col_t<T, MODE, R, SPLIT>(s, x, y, cs, partials, grid, st);
-        DMED_RPT(1) DMED_RPT(2) DMED_RPT(3) DMED_RPT(4) DMED_RPT(5) DMED_RPT(6) DMED_RPT(7) DMED_RPT(8)
-        DMED_RPT(9) DMED_RPT(10) DMED_RPT(11) DMED_RPT(12) DMED_RPT(13)
-#undef DMED_RPT
-    case 14: case 15: case 16:
-        if constexpr (sizeof(T) == 4) {
-            if (rpt == 14) return launch_col_t<T, MODE, 14, SPLIT>(s, x, y, cs, partials, grid, st);
-            if (rpt == 15) return launch_col_t<T, MODE, 15, SPLIT>(s, x, y, cs, partials, grid, st);
-            return launch_col_t<T, MODE, 16, SPLIT>(s, x, y, cs, partials, grid, st);
-        }
-        break;
-    default:
-        break;
-    }
-    return fail(DMED_EINVAL, "column kernel: unsupported rows per lane");
-}
-
-// first pass of the split H*v: down hops over row panels of `panel` rows,
-// `wgs` 256-lane workgroups per CU
-template <typename T, int MODE>
-static int launch_dn(const dmed_sector *s, const void *x, void *y, ChainScalars *cs, int panel,
-                     int wgs, hipStream_t st)
-{
-    const int ncu = std::max(8, ctx().num_cu);
-    const unsigned grid = (unsigned)((ncu / 8) * 8 * std::max(1, wgs));
-    const T *hv;
-    if constexpr (sizeof(T) == 8) hv = reinterpret_cast<const T *>(s->dw.hval);
-    else hv = reinterpret_cast<const T *>(s->dw.hval32);
-    const int32_t dup = (int32_t)s->up.dim, ddw = (int32_t)s->dw.dim, nhd = s->dw.nhop;
-    switch (panel) {
-#define DMED_PANEL(R)                                                                          \
-    case R:                                                                                   \
-        hipLaunchKernelGGL((k_hxv_dn<T, MODE, R>), dim3(grid), dim3(256), 0, st,               \
-                           static_cast<const T *>(x), static_cast<T *>(y), dup, ddw, nhd,      \
-                           s->dw.hidx, hv, (const ChainScalars *)cs);                          \
-        break;
-        DMED_PANEL(8) DMED_PANEL(16) DMED_PANEL(32) DMED_PANEL(64)
-#undef DMED_PANEL
-    default:
-        return fail(DMED_EINVAL, "down-hop pass: unsupported panel rows");
-    }
+    int rc = 0;
+    const bool ok = col_dispatch<T, MODE>(c.bs, c.rpt, [&](auto kern) {
+        rc = lds_attr_once((const void *)kern);
+        if (rc) return;
+        hipLaunchKernelGGL(kern, dim3(c.grid), dim3(c.bs), shmem, st, static_cast<const T *>(x),
+                           static_cast<T *>(y), (int32_t)dup, (int32_t)s->dw.dim,
+                           (int32_t)s->up.nhop, (int32_t)s->dw.nhop, (int32_t)s->nimp,
+                           (int32_t)s->up.ncoef, s->up.hpk, s->up.coef, s->dw.hidx, s->dw.hval,
+                           s->up.diag, s->dw.diag, s->up.imp, s->dw.imp, s->W, (int32_t)dma16,
+                           (const ChainScalars *)cs, partials);
+    });
+    if (!ok) return fail(DMED_EINVAL, "column kernel: unsupported rows per lane");
+    if (rc) return rc;
     DMED_HIP(hipGetLastError());
     return 0;
 }
@@ -633,11 +645,6 @@ static LaunchShape pick_shape(const dmed_sector *s, int prec)
     sh.cb = (int)std::max<int64_t>(cb, 1);
     const int64_t nel = (int64_t)sh.cb * dup;
     sh.bs = nel >= 4096 ? 1024 : (nel >= 1024 ? 512 : 256);
-    const TuneChoice tc = tune_get(s, prec);
-    if (tc.cb > 0) {
-        sh.cb = (int)std::min<int64_t>(tc.cb, std::max<int64_t>(ddw, 1));
-        sh.bs = tc.bs;
-    }
     sh.shmem = (sh.lds ? (((size_t)sh.cb * colb + 15) & ~(size_t)15) : 0) +
                8 * (size_t)sh.cb * s->nimp;
     sh.grid = (unsigned)((ddw + sh.cb - 1) / sh.cb);
@@ -680,9 +687,9 @@ static int launch_t(const dmed_sector *s, const void *x, void *y, ChainScalars *
     a.jp = s->jp;
     a.cs = cs;
     a.partials = partials;
-    constexpr auto kern = k_hxv<T, MODE, LDS>;
+    const auto kern = k_hxv<T, MODE, LDS>;
     if (sh.shmem > 64 * 1024)
-        if (int rc = lds_attr_once<kern>()) return rc;
+        if (int rc = lds_attr_once((const void *)kern)) return rc;
     hipLaunchKernelGGL(kern, dim3(sh.grid), dim3(sh.bs), sh.shmem, st, a);
     DMED_HIP(hipGetLastError());
     return 0;
@@ -722,16 +729,16 @@ static void tune_load_file(TuneTable &t)
     FILE *f = std::fopen(path, "r");
     if (!f) return;
     long long dup, ddw;
-    int nhu, nhd, nimp, prec, cb, bs, wgs;
-    while (std::fscanf(f, "%lld %lld %d %d %d %d %d %d %d", &dup, &ddw, &nhu, &nhd, &nimp, &prec,
-                       &cb, &bs, &wgs) == 9)
-        t.tab[TuneKey{dup, ddw, nhu, nhd, nimp, prec}] = TuneChoice{cb, bs, wgs};
+    int nhu, nhd, nimp, prec, cb, bs, cbs, wgs;
+    while (std::fscanf(f, "%lld %lld %d %d %d %d %d %d %d %d", &dup, &ddw, &nhu, &nhd, &nimp, &prec,
+                       &cb, &bs, &cbs, &wgs) == 10)
+        t.tab[TuneKey{dup, ddw, nhu, nhd, nimp, prec}] = TuneChoice{cb, bs, cbs, wgs};
     std::fclose(f);
 }
 
-// With DMED_TUNE=1, make sure this sector shape has a recorded launch shape,
-// timing the candidates on st the first time (not while st is capturing a
-// graph: then the fixed rules apply until a later eager call).
+// Make sure this sector shape has a recorded launch shape, timing the
+// candidates on st the first time (not while st is capturing a graph: then
+// the fixed rules apply until a later eager call).
 void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
 {
     if (!tune_enabled() || s->dim == 0) return;
@@ -742,6 +749,10 @@ void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
         std::lock_guard<std::mutex> lk(t.mu);
         if (!t.loaded) tune_load_file(t);
         if (t.tab.count(key)) return;
+        if (s->dim < tune_min_dim()) {
+            t.tab[key] = TuneChoice{};
+            return;
+        }
     }
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
@@ -759,34 +770,27 @@ void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
     float tbest = -1.f;
     const bool f32 = prec == DMED_FP32;
     const int rep = s->dim >= (int64_t)1 << 24 ? 3 : 20;
-    if (use_col_kernel(s, prec)) {
-        // workgroups per CU the registers and LDS allow: 1, and 2 if it fits
-        const int rpt = (int)((s->up.dim + 1023) / 1024);
-        const int ncu = std::max(8, ctx().num_cu);
-        int fit = 1;
-        const size_t shmem = col_lds_bytes(s->up.dim, esz);
-        switch (rpt) {
-#define DMED_FIT(R) \
-    case R: fit = f32 ? col_fit_per_cu<float, R>(shmem) : col_fit_per_cu<double, R>(shmem); break;
-            DMED_FIT(1) DMED_FIT(2) DMED_FIT(3) DMED_FIT(4) DMED_FIT(5) DMED_FIT(6) DMED_FIT(7)
-#undef DMED_FIT
-        default: fit = 1; break;
+    auto consider = [&](const TuneChoice &tc, auto fn) {
+        const float ms = time_launches(fn, rep, st);
+        if (ms > 0 && (tbest < 0 || ms < kTuneMargin * tbest)) {
+            tbest = ms;
+            best = tc;
         }
-        // the fixed rule first, then the other count
-        const int w0 = std::max(1, std::min(fit, 2));
-        for (int w : {w0, 3 - w0}) {
-            if (w > std::min(fit, 2)) continue;
-            unsigned g = (unsigned)((ncu / 8) * 8 * w);
-            while (g > 8 && g / 8 > (s->dw.dim + 7) / 8) g -= 8;
-            auto fn = [&]() {
-                return f32 ? launch_col_rpt<float, kPlain, false>(rpt, s, x, y, nullptr, nullptr, g, st)
-                           : launch_col_rpt<double, kPlain, false>(rpt, s, x, y, nullptr, nullptr, g, st);
-            };
-            const float ms = time_launches(fn, rep, st);
-            if (ms > 0 && (tbest < 0 || ms < kTuneMargin * tbest)) {
-                tbest = ms;
-                best = TuneChoice{0, 0, w};
-            }
+    };
+    if (use_col_kernel(s, prec)) {
+        // the fixed rule first, then rows per lane x workgroups per CU
+        std::vector<TuneChoice> cand{TuneChoice{}};
+        for (int bs : {1024, 512, 256}) {
+            if (col_rpt(s, prec, bs) == 0) continue;
+            const int fit = std::min(col_fit(s, prec, bs), 4);
+            for (int w = 1; w <= fit; ++w) cand.push_back(TuneChoice{0, 0, bs, w});
+        }
+        for (const TuneChoice &tc : cand) {
+            const ColShape c = col_shape(s, prec, tc);
+            consider(tc, [&]() {
+                return f32 ? launch_col<float, kPlain>(s, x, y, nullptr, nullptr, c, st)
+                           : launch_col<double, kPlain>(s, x, y, nullptr, nullptr, c, st);
+            });
         }
     } else {
         LaunchShape base = pick_shape(s, prec);     // no recorded choice yet: the fixed rule
@@ -797,26 +801,20 @@ void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
             for (int bs = 256; bs <= 1024; bs *= 2)
                 if (cb != base.cb || bs != base.bs) cand.push_back({cb, bs});
         for (auto [cb, bs] : cand) {
-            {
-                LaunchShape sh = base;
-                sh.cb = cb;
-                sh.bs = bs;
-                if (sh.lds && (size_t)cb * colb + 16 + 8 * (size_t)cb * s->nimp > 64 * 1024) continue;
-                sh.shmem = (sh.lds ? (((size_t)cb * colb + 15) & ~(size_t)15) : 0) + 8 * (size_t)cb * s->nimp;
-                sh.grid = (unsigned)((ddw + cb - 1) / cb);
-                auto fn = [&]() {
-                    if (f32) return sh.lds ? launch_t<float, kPlain, true>(s, x, y, nullptr, nullptr, sh, st)
-                                           : launch_t<float, kPlain, false>(s, x, y, nullptr, nullptr, sh, st);
-                    return sh.lds ? launch_t<double, kPlain, true>(s, x, y, nullptr, nullptr, sh, st)
-                                  : launch_t<double, kPlain, false>(s, x, y, nullptr, nullptr, sh, st);
-                };
-                const float ms = time_launches(fn, rep, st);
-                if (ms > 0 && (tbest < 0 || ms < kTuneMargin * tbest)) {
-                    tbest = ms;
-                    best = TuneChoice{cb, bs, 0};
-                }
-            }
+            LaunchShape sh = base;
+            sh.cb = cb;
+            sh.bs = bs;
+            if (sh.lds && (size_t)cb * colb + 16 + 8 * (size_t)cb * s->nimp > 64 * 1024) continue;
+            sh.shmem = (sh.lds ? (((size_t)cb * colb + 15) & ~(size_t)15) : 0) + 8 * (size_t)cb * s->nimp;
+            sh.grid = (unsigned)((ddw + cb - 1) / cb);
+            consider(TuneChoice{cb, bs, 0, 0}, [&]() {
+                if (f32) return sh.lds ? launch_t<float, kPlain, true>(s, x, y, nullptr, nullptr, sh, st)
+                                       : launch_t<float, kPlain, false>(s, x, y, nullptr, nullptr, sh, st);
+                return sh.lds ? launch_t<double, kPlain, true>(s, x, y, nullptr, nullptr, sh, st)
+                              : launch_t<double, kPlain, false>(s, x, y, nullptr, nullptr, sh, st);
+            });
         }
+        if (best.cb == base.cb && best.bs == base.bs) best = TuneChoice{};
     }
     hipStreamSynchronize(st);
     hipFree(x);
@@ -827,27 +825,31 @@ void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
     t.tab[key] = best;
     if (const char *path = std::getenv("DMED_TUNE_FILE")) {
         if (FILE *f = std::fopen(path, "a")) {
-            std::fprintf(f, "%lld %lld %d %d %d %d %d %d %d\n", (long long)std::get<0>(key),
+            std::fprintf(f, "%lld %lld %d %d %d %d %d %d %d %d\n", (long long)std::get<0>(key),
                          (long long)std::get<1>(key), std::get<2>(key), std::get<3>(key),
-                         std::get<4>(key), std::get<5>(key), best.cb, best.bs, best.wgs);
+                         std::get<4>(key), std::get<5>(key), best.cb, best.bs, best.cbs, best.wgs);
             std::fclose(f);
         }
     }
 }
 
-// Split H*v (down-hop panel pass + column pass) for the column kernel's
-// sectors: panel rows and panel-pass workgroups per CU; panel 0 keeps the
-// one-pass kernel.
-struct SplitShape {
-    int panel = 0, wgs = 4;
-};
-
-static SplitShape split_shape(const dmed_sector *s, int prec)
+// Tile-kernel shape: the tuned one in plain mode, the fixed rule in Lanczos
+// mode (whose block partial sums follow the grid).
+static LaunchShape tile_shape(const dmed_sector *s, int prec, int mode)
 {
-    SplitShape sp;
-    if (const char *e = std::getenv("DMED_HXV_PANEL")) sp.panel = std::atoi(e);
-    if (const char *e = std::getenv("DMED_HXV_DNWGS")) sp.wgs = std::max(1, std::atoi(e));
-    return sp;
+    LaunchShape sh = pick_shape(s, prec);
+    if (mode != kPlain) return sh;
+    const TuneChoice tc = tune_get(s, prec);
+    if (tc.cb > 0) {
+        const size_t esz = prec == DMED_FP32 ? 4 : 8;
+        const int64_t ddw = s->dw.dim;
+        sh.cb = (int)std::min<int64_t>(tc.cb, std::max<int64_t>(ddw, 1));
+        sh.bs = tc.bs;
+        sh.shmem = (sh.lds ? (((size_t)sh.cb * s->up.dim * esz + 15) & ~(size_t)15) : 0) +
+                   8 * (size_t)sh.cb * s->nimp;
+        sh.grid = (unsigned)((ddw + sh.cb - 1) / sh.cb);
+    }
+    return sh;
 }
 
 int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
@@ -856,36 +858,18 @@ int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
     if (s->dim == 0) return 0;
     if (mode == kLanczos && partials == nullptr) return fail(DMED_EINVAL, "no partials");
     tune_ensure(s, prec, st);
-    if (use_col_kernel(s, prec)) {
-        const unsigned grid = col_grid(s, prec);
-        if (nparts) *nparts = (int)grid;
-        const int rpt = (int)((s->up.dim + 1023) / 1024);
-        const bool f32 = prec == DMED_FP32;
-        const SplitShape sp = split_shape(s, prec);
-        if (sp.panel > 0) {
-            // down hops into y over L2-resident row panels, then the column
-            // kernel adds the up hops and the diagonal in place
-            int rc = f32 ? (mode == kPlain ? launch_dn<float, kPlain>(s, x, y, cs, sp.panel, sp.wgs, st)
-                                           : launch_dn<float, kLanczos>(s, x, y, cs, sp.panel, sp.wgs, st))
-                         : (mode == kPlain ? launch_dn<double, kPlain>(s, x, y, cs, sp.panel, sp.wgs, st)
-                                           : launch_dn<double, kLanczos>(s, x, y, cs, sp.panel, sp.wgs, st));
-            if (rc) return rc;
-            if (f32)
-                return mode == kPlain ? launch_col_rpt<float, kPlain, true>(rpt, s, x, y, cs, partials, grid, st)
-                                      : launch_col_rpt<float, kLanczos, true>(rpt, s, x, y, cs, partials, grid, st);
-            return mode == kPlain ? launch_col_rpt<double, kPlain, true>(rpt, s, x, y, cs, partials, grid, st)
-                                  : launch_col_rpt<double, kLanczos, true>(rpt, s, x, y, cs, partials, grid, st);
-        }
-        if (f32)
-            return mode == kPlain ? launch_col_rpt<float, kPlain, false>(rpt, s, x, y, cs, partials, grid, st)
-                                  : launch_col_rpt<float, kLanczos, false>(rpt, s, x, y, cs, partials, grid, st);
-        return mode == kPlain ? launch_col_rpt<double, kPlain, false>(rpt, s, x, y, cs, partials, grid, st)
-                              : launch_col_rpt<double, kLanczos, false>(rpt, s, x, y, cs, partials, grid, st);
-    }
-    const LaunchShape sh = pick_shape(s, prec);
-    if (nparts) *nparts = (int)sh.grid;
-    if (mode == kLanczos && partials == nullptr) return fail(DMED_EINVAL, "no partials");
     const bool f32 = prec == DMED_FP32;
+    if (use_col_kernel(s, prec)) {
+        const ColShape c = col_shape(s, prec, tune_get(s, prec));
+        if (nparts) *nparts = (int)s->dw.dim;         // one partial per column
+        if (f32)
+            return mode == kPlain ? launch_col<float, kPlain>(s, x, y, cs, partials, c, st)
+                                  : launch_col<float, kLanczos>(s, x, y, cs, partials, c, st);
+        return mode == kPlain ? launch_col<double, kPlain>(s, x, y, cs, partials, c, st)
+                              : launch_col<double, kLanczos>(s, x, y, cs, partials, c, st);
+    }
+    const LaunchShape sh = tile_shape(s, prec, mode);
+    if (nparts) *nparts = (int)sh.grid;
     if (mode == kPlain) {
         if (f32) return sh.lds ? launch_t<float, kPlain, true>(s, x, y, cs, partials, sh, st)
                                : launch_t<float, kPlain, false>(s, x, y, cs, partials, sh, st);
@@ -898,11 +882,39 @@ int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
                   : launch_t<double, kLanczos, false>(s, x, y, cs, partials, sh, st);
 }
 
+// partial sums a Lanczos-mode H*v of s leaves (column kernel: one per column)
 int hxv_nparts(const dmed_sector *s, int prec)
 {
-    tune_ensure(s, prec, ex().stream);
-    if (use_col_kernel(s, prec)) return (int)col_grid(s, prec);
+    if (use_col_kernel(s, prec)) return (int)s->dw.dim;
     return (int)pick_shape(s, prec).grid;
+}
+
+// the plain-mode launch shape of s in precision prec (after tuning)
+int hxv_shape(const dmed_sector *s, int prec, dmed_hxv_shape *out)
+{
+    prec = prec == DMED_FP32 ? DMED_FP32 : DMED_FP64;
+    tune_ensure(s, prec, ex().stream);
+    *out = dmed_hxv_shape{};
+    if (use_col_kernel(s, prec)) {
+        const ColShape c = col_shape(s, prec, tune_get(s, prec));
+        out->kernel = 2;
+        out->lanes = c.bs;
+        out->rows_per_lane = c.rpt;
+        out->columns_per_group = 1;
+        out->groups_per_cu = c.per_cu;
+        out->grid = (int32_t)c.grid;
+        out->tuned = c.tuned ? 1 : 0;
+    } else {
+        const LaunchShape sh = tile_shape(s, prec, kPlain);
+        out->kernel = 1;
+        out->lanes = sh.bs;
+        out->rows_per_lane = (int32_t)((sh.cb * s->up.dim + sh.bs - 1) / sh.bs);
+        out->columns_per_group = sh.cb;
+        out->groups_per_cu = 0;
+        out->grid = (int32_t)sh.grid;
+        out->tuned = tune_get(s, prec).cb > 0 ? 1 : 0;
+    }
+    return 0;
 }
 
 }  // namespace dmed

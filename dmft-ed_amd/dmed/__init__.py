@@ -110,6 +110,7 @@ SIGNATURES = [
     ("dmed_hxv_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
     ("dmed_sync", C.c_int, []),
     ("dmed_hxv_timed", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, _dp]),
+    ("dmed_hxv_shape_of", C.c_int, [C.c_void_p, C.c_int32, _i32p]),
     ("dmed_eigh", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                             C.c_int32, _dp, C.POINTER(C.c_void_p), _dp]),
     ("dmed_ground_state", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, _dp,
@@ -246,6 +247,19 @@ class Sector:
         y = np.empty_like(x)
         _check(lib().dmed_hxv_host(self.h, _d(x), _d(y), prec))
         return y
+
+    def hxv_shape(self, prec=FP64):
+        """Launch shape of this sector's plain H*v (after the autotuner ran
+        for it): kernel ("tile" or "column"), lanes, rows_per_lane,
+        columns_per_group, groups_per_cu, grid, tuned."""
+        v = np.zeros(7, dtype=np.int32)
+        _check(lib().dmed_hxv_shape_of(self.h, prec, v.ctypes.data_as(_i32p)))
+        keys = ("kernel", "lanes", "rows_per_lane", "columns_per_group", "groups_per_cu", "grid",
+                "tuned")
+        out = dict(zip(keys, (int(a) for a in v)))
+        out["kernel"] = {1: "tile", 2: "column"}[out["kernel"]]
+        out["tuned"] = bool(out["tuned"])
+        return out
 
     def eigh(self, neig=1, ncv=0, maxit=0, tol=1e-12, dense_threshold=256, vectors=True):
         ev = np.zeros(neig)

@@ -10,8 +10,12 @@
  *     failure; dmed_last_error() then describes the failure.  The reference
  *     aborts with `stop "..."` instead (e.g. ED_HAMILTONIAN_MATVEC.f90:313,317;
  *     ED_SETUP.f90:1061,1075); the Fortran shim turns a nonzero code into stop.
- *   - One process, one GPU, one HIP stream.  Calls are host-synchronous
- *     unless named *_async.  Not thread-safe.
+ *   - One process, one GPU.  Calls are host-synchronous unless named
+ *     *_async.  The caller serialises its calls: the public entry points are
+ *     not meant to be called from several host threads at once.  Inside one
+ *     call the library may run its own host worker threads, each with its own
+ *     HIP stream and scratch (the batched solve, dmed_solve_sites); those are
+ *     joined before the call returns.
  *   - State vectors are real (FP64 or FP32).  The reference's vectors are
  *     complex(8); H is real for a normal bath with real Hloc, so the real
  *     vector is the reference vector up to a global phase.
@@ -135,6 +139,16 @@ int dmed_sync(void);
  * ms_avg receives the average duration of one launch in milliseconds. */
 int dmed_hxv_timed(const dmed_sector *s, const void *x_dev, void *y_dev, int32_t prec,
                    int32_t nrep, double *ms_avg);
+/* Launch shape of the sector's plain H*v in precision prec, after the
+ * autotuner has run for it (no counterpart in the reference: it reports the
+ * library's own choice).  kernel 1: the tile kernel (columns_per_group
+ * columns staged in LDS per workgroup); kernel 2: the persistent column
+ * kernel (one column in LDS, rows_per_lane rows per lane, groups_per_cu
+ * workgroups per CU).  tuned = 1 when the shape came from the autotuner. */
+typedef struct dmed_hxv_shape {
+    int32_t kernel, lanes, rows_per_lane, columns_per_group, groups_per_cu, grid, tuned;
+} dmed_hxv_shape;
+int dmed_hxv_shape_of(const dmed_sector *s, int32_t prec, dmed_hxv_shape *out);
 
 /* ------------------------------------------------------ eigensolver */
 /* Lowest neig eigenpairs of the sector.  Replaces sp_eigh(MatVec, Dim,

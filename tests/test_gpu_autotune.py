@@ -1,11 +1,18 @@
-"""GPU: the opt-in launch-shape autotuner (DMED_TUNE=1, DMED_TUNE_FILE).
+"""GPU: the launch-shape autotuner (on by default for sectors of at least
+DMED_TUNE_MIN_DIM states; DMED_TUNE_FILE keeps the choices).
 
 The tuner only chooses among launch shapes that leave each element's sum
-unchanged, so H*v with tuned shapes must equal the fixed-rule H*v bit for
-bit, in FP64 and FP32.  It is checked in a child process, because the
-setting is read once per process.  Shapes: workload 1 (D_up = 70, tile
-kernel) and workload 2 (D_up = 924, column kernel, one or two workgroups
-per CU).  A second child loads the recorded choices instead of timing again.
+unchanged -- rows per lane (workgroup lanes) and workgroups per CU of the
+column kernel, LDS tile and lanes of the tile kernel -- so H*v with tuned
+shapes must equal the fixed-rule H*v bit for bit, in FP64 and FP32.  The
+column kernel's Lanczos-mode partial sums are one per column and the tile
+kernel keeps its fixed rule in Lanczos mode, so chain alpha/beta are
+bit-identical as well.  Checked in a child process that tunes every sector
+(DMED_TUNE_MIN_DIM=0), since the settings are read once per process; this
+process keeps the fixed rules (both sectors are below the default minimum).
+Shapes: workload 1 (D_up = 70, tile kernel) and workload 2 (D_up = 924,
+column kernel).  A second child loads the recorded choices instead of timing
+again.
 """
 import os
 import subprocess
@@ -38,6 +45,8 @@ for cfg in (1, 2):
     x = np.load({xdir!r} + f"/x{{cfg}}.npy")
     out[f"y{{cfg}}"] = s.hxv(x)
     out[f"z{{cfg}}"] = s.hxv(x, prec=dmed.FP32)
+    sh = s.hxv_shape()
+    out[f"shape{{cfg}}"] = np.array([sh["lanes"], sh["groups_per_cu"], sh["columns_per_group"]])
     a, b, n = dmed.chain(s, s.vec(x), 20)
     out[f"a{{cfg}}"], out[f"b{{cfg}}"] = a, b
 np.savez({xdir!r} + "/child.npz", **out)
@@ -49,7 +58,7 @@ def _child(tmp, tune_file):
     script = os.path.join(tmp, "child.py")
     with open(script, "w") as f:
         f.write(CHILD.format(pkg=PKG, xdir=tmp))
-    env = dict(os.environ, DMED_TUNE="1", DMED_TUNE_FILE=tune_file)
+    env = dict(os.environ, DMED_TUNE="1", DMED_TUNE_MIN_DIM="0", DMED_TUNE_FILE=tune_file)
     r = subprocess.run([sys.executable, "-u", script], env=env, capture_output=True, text=True,
                        timeout=240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
@@ -77,10 +86,10 @@ def test_tuned_launch_shapes_give_identical_hxv(tmp_path):
         for cfg in (1, 2):
             assert np.array_equal(got[f"y{cfg}"], ref[f"y{cfg}"])
             assert np.array_equal(got[f"z{cfg}"], ref[f"z{cfg}"])
-            # Lanczos-mode partial sums follow the grid: the first steps
-            # agree to round-off
-            assert np.allclose(got[f"a{cfg}"][:10], ref[f"a{cfg}"][:10], rtol=0, atol=1e-11)
-            assert np.allclose(got[f"b{cfg}"][1:10], ref[f"b{cfg}"][1:10], rtol=0, atol=1e-11)
+            assert np.array_equal(got[f"a{cfg}"], ref[f"a{cfg}"])
+            assert np.array_equal(got[f"b{cfg}"], ref[f"b{cfg}"])
+        print(f"[autotune] tuned shapes (lanes, per CU, columns): {got['shape1']}, "
+              f"{got['shape2']}", flush=True)
         lines = open(tune_file).read().split("\n")
         lines = [l for l in lines if l.strip()]
         # one line per (shape, precision); the second run only reads them

@@ -805,7 +805,10 @@ def test_mixed_precision_chains_error_bounds():
     r64 = dmed.solve(1, 1, 8, dmed.default_params(prec_gf=dmed.FP64, **kw))
     for prec, gbar, sbar in ((dmed.FP32, 1e-5, 1e-4), (dmed.MIXED, 1e-7, 1e-6)):
         r = dmed.solve(1, 1, 8, dmed.default_params(prec_gf=prec, batched=1, **kw))
-        assert r64["egs"] == r["egs"]
+        # the batched form diagonalises small sectors in the persistent queue,
+        # whose projected eigenproblem is solved by Jacobi: E0 to the solver's
+        # tolerance, not bit for bit
+        assert abs(r64["egs"] - r["egs"]) <= 1e-12 * abs(r64["egs"])
         dg = np.abs(r64["gmats"] - r["gmats"]).max()
         ds = np.abs(r64["smats"] - r["smats"]).max()
         say(f"chains prec={prec}: max|dG|={dg:.3e} max|dSigma|={ds:.3e}")
