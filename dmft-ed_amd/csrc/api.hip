@@ -143,6 +143,7 @@ int dmed_init(int device)
     if (c.lds_max == 0) c.lds_max = 65536;
     if (c.lds_max > 160 * 1024) c.lds_max = 160 * 1024;
     if (int rc = exec_create(c.exec)) return rc;
+    if (int rc = noalloc_setup(device)) return rc;
     DMED_HIP(hipStreamSynchronize(c.exec.stream));
     c.initialized = true;
     return 0;
@@ -626,6 +627,20 @@ extern "C" int dmed_chain(const dmed_sector *s, const dmed_vec *v0, int32_t nlan
     if (!alpha || !beta) return fail(DMED_EINVAL, "null alpha/beta");
     int ns = 0;
     int rc = chain_run(s, v0->data, v0->prec, nlanc, prec, threshold, alpha, beta, &ns);
+    if (nsteps) *nsteps = ns;
+    return rc;
+}
+
+extern "C" int dmed_chain_noalloc(const dmed_sector *s, const dmed_vec *v0, int32_t nlanc,
+                                  int32_t prec, double threshold, double *alpha, double *beta,
+                                  int32_t *nsteps)
+{
+    REQUIRE_MODEL();
+    if (int rc = check_sector(s)) return rc;
+    if (!v0 || v0->sec != s) return fail(DMED_EINVAL, "start vector must belong to the sector");
+    if (!alpha || !beta) return fail(DMED_EINVAL, "null alpha/beta");
+    int ns = 0;
+    int rc = chain_run_noalloc(s, v0->data, v0->prec, nlanc, prec, threshold, alpha, beta, &ns);
     if (nsteps) *nsteps = ns;
     return rc;
 }

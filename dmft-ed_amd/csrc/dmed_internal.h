@@ -211,7 +211,18 @@ int hxv_shape(const dmed_sector *s, int prec, dmed_hxv_shape *out);
 int launch_hxv(const dmed_sector *s, const void *x, void *y, int prec, int mode,
                ChainScalars *cs, double *partials, int *nparts, hipStream_t st);
 
+// While a NoAllocScope lives on a thread, library code on that thread
+// allocates no device memory: the autotuner keeps recorded or fixed launch
+// shapes instead of timing candidates on scratch vectors.
+bool thread_no_alloc();
+struct NoAllocScope {
+    NoAllocScope();
+    ~NoAllocScope();
+    bool prev;
+};
+
 // ---------------------------------------------------------- vector ops
+int launch_zero(void *p, size_t bytes, hipStream_t st);   // kernel fill, no memset node
 int launch_reduce_to(const double *partials, int n, double *out, int op, hipStream_t st);
 int launch_dot(const void *x, const void *y, int64_t n, int prec, double *partials,
                int *nparts, hipStream_t st);
@@ -230,6 +241,12 @@ int dot_host(const double *x, const double *y, int64_t n, double *out);
 // head < 0 selects kMixedHead)
 int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
               double threshold, double *alpha, double *beta, int *nsteps, int head = -1);
+// chain_run with its vectors and partial sums in the library's static device
+// workspace (dmed_chain_noalloc); noalloc_setup takes that workspace's
+// address on a device, once, with the device current (dmed_init)
+int noalloc_setup(int device);
+int chain_run_noalloc(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
+                      double threshold, double *alpha, double *beta, int *nsteps);
 int eigh_sector(const dmed_sector *s, int neig, int ncv, int maxit, double tol,
                 int dense_threshold, double *evals, dmed_vec **evecs, double *resid);
 // lowest_multiplet: the lowest level and its copies within `degen` (at most

@@ -736,6 +736,11 @@ static void tune_load_file(TuneTable &t)
     std::fclose(f);
 }
 
+static thread_local bool t_no_alloc = false;
+bool thread_no_alloc() { return t_no_alloc; }
+NoAllocScope::NoAllocScope() : prev(t_no_alloc) { t_no_alloc = true; }
+NoAllocScope::~NoAllocScope() { t_no_alloc = prev; }
+
 // Make sure this sector shape has a recorded launch shape, timing the
 // candidates on st the first time (not while st is capturing a graph: then
 // the fixed rules apply until a later eager call).
@@ -754,6 +759,7 @@ void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
             return;
         }
     }
+    if (t_no_alloc) return;                   // timing needs scratch vectors
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
     const size_t esz = prec == DMED_FP32 ? 4 : 8;

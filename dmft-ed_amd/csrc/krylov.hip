@@ -15,6 +15,9 @@
 #include <cstdlib>
 #include <cstdio>
 #include <atomic>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include <algorithm>
 #include <cmath>
@@ -216,6 +219,24 @@ __global__ void __launch_bounds__(kVecBS) k_fill_random(T *__restrict__ x, int64
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         x[i] = (T)hash_uniform(seed, (uint64_t)i);
+}
+
+// zero n 32-bit words (a kernel rather than a memset node, so that streams a
+// caller captures into a graph replay it correctly)
+__global__ void __launch_bounds__(kVecBS) k_zero_words(uint32_t *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+int launch_zero(void *p, size_t bytes, hipStream_t st)
+{
+    const int64_t n = (int64_t)(bytes / 4);
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_zero_words, dim3(vec_grid(n)), dim3(kVecBS), 0, st, (uint32_t *)p, n);
+    DMED_HIP(hipGetLastError());
+    return 0;
 }
 
 // ----------------------------------------- block-Krylov helpers (FP64)
@@ -504,15 +525,16 @@ constexpr int kChainMax = (kScalCap - 16) / 3;
 // device scale and zeroes v_prev; chain_steps launches `count` steps.  If
 // psi != nullptr, psi += z[k] * v_k is accumulated before each step (second
 // pass of the ground-state Lanczos), z read from chain_aux().
-int chain_init(const dmed_sector *s, void *bufs[3], int prec, int nlanc_total, double thr)
+int chain_init(const dmed_sector *s, void *bufs[3], int prec, int nlanc_total, double thr,
+               double *partials)
 {
-        hipStream_t st = ex().stream;
+    hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     const size_t esz = prec == DMED_FP32 ? 4 : 8;
     int np = 0;
-    DMED_HIP(hipMemsetAsync(bufs[0], 0, n * esz, st));
-    if (int rc = launch_dot(bufs[1], bufs[1], n, prec, ex().partials, &np, st)) return rc;
-    hipLaunchKernelGGL(k_chain_init, dim3(1), dim3(1024), 0, st, ex().partials, np, chain_scalars(),
+    if (int rc = launch_zero(bufs[0], n * esz, st)) return rc;
+    if (int rc = launch_dot(bufs[1], bufs[1], n, prec, partials, &np, st)) return rc;
+    hipLaunchKernelGGL(k_chain_init, dim3(1), dim3(1024), 0, st, partials, np, chain_scalars(),
                        thr, nlanc_total);
     DMED_HIP(hipGetLastError());
     return 0;
@@ -543,7 +565,8 @@ static bool graphs_enabled()
     return on && !t_graphs_off;
 }
 
-int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double *psi)
+int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double *psi,
+                double *partials, bool allow_graphs = true)
 {
     hipStream_t st = ex().stream;
     const int64_t n = s->dim;
@@ -560,17 +583,17 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
                 hipLaunchKernelGGL(k_accum<double>, dim3(g), dim3(kVecBS), 0, st, psi,
                                    (const double *)v, n, cs, chain_aux());
         }
-        if (int rc = launch_hxv(s, v, w, prec, kLanczos, cs, ex().partials, nullptr, st))
+        if (int rc = launch_hxv(s, v, w, prec, kLanczos, cs, partials, nullptr, st))
             return rc;
-        hipLaunchKernelGGL(k_chain_alpha, dim3(1), dim3(1024), 0, st, ex().partials, nph, cs,
+        hipLaunchKernelGGL(k_chain_alpha, dim3(1), dim3(1024), 0, st, partials, nph, cs,
                            chain_alpha());
         if (prec == DMED_FP32)
             hipLaunchKernelGGL(k_chain_update<float>, dim3(g), dim3(kVecBS), 0, st, (float *)w,
-                               (const float *)v, (const float *)vp, n, cs, ex().partials);
+                               (const float *)v, (const float *)vp, n, cs, partials);
         else
             hipLaunchKernelGGL(k_chain_update<double>, dim3(g), dim3(kVecBS), 0, st, (double *)w,
-                               (const double *)v, (const double *)vp, n, cs, ex().partials);
-        hipLaunchKernelGGL(k_chain_beta, dim3(1), dim3(1024), 0, st, ex().partials, (int)g, cs,
+                               (const double *)v, (const double *)vp, n, cs, partials);
+        hipLaunchKernelGGL(k_chain_beta, dim3(1), dim3(1024), 0, st, partials, (int)g, cs,
                            chain_beta());
         return hipGetLastError() == hipSuccess ? 0 : fail(DMED_EHIP, "chain step launch failed");
     };
@@ -582,7 +605,7 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
         w = t;
     };
     int k = 0;
-    if (count >= kGraphMinSteps && graphs_enabled()) {
+    if (count >= kGraphMinSteps && allow_graphs && graphs_enabled()) {
         // one eager step first (kernel attributes are set outside the capture)
         if (int rc = one(vp, v, w)) return rc;
         rotate();
@@ -646,8 +669,17 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
 
 }  // namespace
 
-int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
-              double threshold, double *alpha, double *beta, int *nsteps, int head)
+// Work vectors and reduction partials of one chain: FP64 (b64) and, for the
+// FP32 tail of a mixed chain or an all-FP32 chain, b32.
+struct ChainWork {
+    void *b64[3] = {};
+    void *b32[3] = {};
+    double *partials = nullptr;
+};
+
+static int chain_run_ws(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
+                        double threshold, double *alpha, double *beta, int *nsteps, int head,
+                        const ChainWork &ws, bool allow_graphs)
 {
     hipStream_t st = ex().stream;
     if (nlanc <= 0) return fail(DMED_EINVAL, "nlanc must be positive");
@@ -662,28 +694,23 @@ int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int p
     const int nhead = prec == DMED_MIXED ? std::min(head, nlanc) : (prec == DMED_FP64 ? nlanc : 0);
     const int p0 = nhead > 0 ? DMED_FP64 : DMED_FP32;
     const int64_t n = s->dim;
-    const size_t n1 = (size_t)std::max<int64_t>(n, 1);
-    const size_t esz = p0 == DMED_FP32 ? 4 : 8;
-    int nparts = std::max(hxv_nparts(s, DMED_FP64), hxv_nparts(s, DMED_FP32));
-    if (int rc = ensure_partials(std::max(nparts, 2 * kVecGrid) + 16)) return rc;
     void *bufs[3];
-    for (int k = 0; k < 3; ++k)
-        if (int rc = exec_buf(ex(), kSlotChain + k, n1 * esz, &bufs[k])) return rc;
+    for (int k = 0; k < 3; ++k) bufs[k] = p0 == DMED_FP32 ? ws.b32[k] : ws.b64[k];
     if (int rc = launch_scale_copy(bufs[1], p0, v0, v0prec, n, 1.0, st)) return rc;
-    DMED_HIP(hipMemsetAsync(chain_alpha(), 0, kChainMax * sizeof(double), st));
-    DMED_HIP(hipMemsetAsync(chain_beta(), 0, kChainMax * sizeof(double), st));
-    if (int rc = chain_init(s, bufs, p0, nlanc, threshold)) return rc;
+    if (int rc = launch_zero(chain_alpha(), kChainMax * sizeof(double), st)) return rc;
+    if (int rc = launch_zero(chain_beta(), kChainMax * sizeof(double), st)) return rc;
+    if (int rc = chain_init(s, bufs, p0, nlanc, threshold, ws.partials)) return rc;
     if (nhead > 0 && nhead < nlanc) {
-        if (int rc = chain_steps(s, bufs, DMED_FP64, nhead, nullptr)) return rc;
+        if (int rc = chain_steps(s, bufs, DMED_FP64, nhead, nullptr, ws.partials, allow_graphs))
+            return rc;
         // the recurrence needs v and v_prev; w is overwritten by the next step
-        void *lo[3];
-        for (int k = 0; k < 3; ++k)
-            if (int rc = exec_buf(ex(), kSlotChainLo + k, n1 * 4, &lo[k])) return rc;
+        void *lo[3] = {ws.b32[0], ws.b32[1], ws.b32[2]};
         if (int rc = launch_scale_copy(lo[0], DMED_FP32, bufs[0], DMED_FP64, n, 1.0, st)) return rc;
         if (int rc = launch_scale_copy(lo[1], DMED_FP32, bufs[1], DMED_FP64, n, 1.0, st)) return rc;
-        if (int rc = chain_steps(s, lo, DMED_FP32, nlanc - nhead, nullptr)) return rc;
+        if (int rc = chain_steps(s, lo, DMED_FP32, nlanc - nhead, nullptr, ws.partials, allow_graphs))
+            return rc;
     } else {
-        if (int rc = chain_steps(s, bufs, p0, nlanc, nullptr)) return rc;
+        if (int rc = chain_steps(s, bufs, p0, nlanc, nullptr, ws.partials, allow_graphs)) return rc;
     }
     ChainScalars h;
     DMED_HIP(hipMemcpyAsync(alpha, chain_alpha(), nlanc * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -692,6 +719,95 @@ int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int p
     DMED_HIP(hipStreamSynchronize(st));
     if (nsteps) *nsteps = h.step;
     return 0;
+}
+
+// partial sums a chain step of s may leave
+static size_t chain_nparts(const dmed_sector *s)
+{
+    const int nparts = std::max(hxv_nparts(s, DMED_FP64), hxv_nparts(s, DMED_FP32));
+    return (size_t)std::max(nparts, 2 * kVecGrid) + 16;
+}
+
+int chain_run(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
+              double threshold, double *alpha, double *beta, int *nsteps, int head)
+{
+    const size_t n1 = (size_t)std::max<int64_t>(s->dim, 1);
+    ChainWork ws;
+    if (int rc = ensure_partials(chain_nparts(s))) return rc;
+    ws.partials = ex().partials;
+    const bool any64 = prec != DMED_FP32, any32 = prec != DMED_FP64;
+    for (int k = 0; k < 3; ++k) {
+        if (any64)
+            if (int rc = exec_buf(ex(), kSlotChain + k, n1 * 8, &ws.b64[k])) return rc;
+        if (any32)
+            if (int rc = exec_buf(ex(), prec == DMED_FP32 ? kSlotChain + k : kSlotChainLo + k,
+                                  n1 * 4, &ws.b32[k]))
+                return rc;
+    }
+    return chain_run_ws(s, v0, v0prec, nlanc, prec, threshold, alpha, beta, nsteps, head, ws, true);
+}
+
+// ---------------------------------------------- allocation-free chain
+// The workspace of dmed_chain_noalloc: a file-scope device array (each GPU
+// the library runs on holds its own copy until the process ends).  Host code
+// never names it in a kernel argument or copy: its per-device address comes
+// from hipGetSymbolAddress at set-up.  Every call on a device shares it, so
+// calls that overlap on one GPU race.
+constexpr size_t kNoAllocBytes = DMED_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static std::mutex g_scratch_mu;
+static std::map<int, unsigned char *> g_scratch;   // device -> address of scratch
+
+int noalloc_setup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch.count(device)) return 0;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        return fail(DMED_EHIP, std::string("no-alloc workspace: hipGetSymbolAddress: ") +
+                                   hipGetErrorString(e));
+    g_scratch[device] = static_cast<unsigned char *>(ptr);
+    return 0;
+}
+
+int chain_run_noalloc(const dmed_sector *s, const void *v0, int v0prec, int nlanc, int prec,
+                      double threshold, double *alpha, double *beta, int *nsteps)
+{
+    unsigned char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        auto it = g_scratch.find(ctx().device);
+        if (it != g_scratch.end()) base = it->second;
+    }
+    if (!base) return fail(DMED_ESTATE, "no-alloc workspace not set up (dmed_init)");
+    const int64_t n = s->dim;
+    if (n > DMED_NOALLOC_MAX_DIM)
+        return fail(DMED_EINVAL, "dmed_chain_noalloc: sector dimension " + std::to_string(n) +
+                                     " exceeds DMED_NOALLOC_MAX_DIM (" +
+                                     std::to_string((long long)DMED_NOALLOC_MAX_DIM) + ")");
+    const size_t n1 = (size_t)std::max<int64_t>(n, 1);
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    // partials, then three FP64 and three FP32 vectors
+    const size_t pbytes = up(chain_nparts(s) * 8);
+    const size_t need = pbytes + 3 * up(n1 * 8) + 3 * up(n1 * 4);
+    if (need > kNoAllocBytes)
+        return fail(DMED_EINVAL, "dmed_chain_noalloc: the sector's partial sums exceed the workspace");
+    ChainWork ws;
+    size_t off = 0;
+    ws.partials = reinterpret_cast<double *>(base);
+    off += pbytes;
+    for (int k = 0; k < 3; ++k) {
+        ws.b64[k] = base + off;
+        off += up(n1 * 8);
+    }
+    for (int k = 0; k < 3; ++k) {
+        ws.b32[k] = base + off;
+        off += up(n1 * 4);
+    }
+    // no timing runs of the autotuner (they allocate) and no graph instantiation
+    NoAllocScope guard;
+    return chain_run_ws(s, v0, v0prec, nlanc, prec, threshold, alpha, beta, nsteps, -1, ws, false);
 }
 
 // ------------------------------------------------------ ground state
@@ -857,12 +973,12 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
         DMED_HIP(hipMemsetAsync(chain_beta(), 0, kChainMax * sizeof(double), st));
         const int budget = std::min(kChainMax - 1, maxit - used);
         void *bufs[3] = {bb[0], bb[1], bb[2]};
-        if (int rc = chain_init(s, bufs, stage, budget, 0.0)) return cleanup(rc);
+        if (int rc = chain_init(s, bufs, stage, budget, 0.0, ex().partials)) return cleanup(rc);
         int steps = 0;
         ChainScalars cs{};
         for (;;) {
             const int chunk = std::min(ncheck, budget - steps);
-            if (int rc = chain_steps(s, bufs, stage, chunk, nullptr)) return cleanup(rc);
+            if (int rc = chain_steps(s, bufs, stage, chunk, nullptr, ex().partials)) return cleanup(rc);
             steps += chunk;
             DMED_HIP(hipMemcpyAsync(a.data(), chain_alpha(), steps * 8, hipMemcpyDeviceToHost, st));
             DMED_HIP(hipMemcpyAsync(b.data(), chain_beta(), (steps + 1) * 8, hipMemcpyDeviceToHost, st));
@@ -884,8 +1000,8 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
             return cleanup(rc);
         DMED_HIP(hipMemsetAsync(psi, 0, n * 8, st));
         void *bufs2[3] = {bb[0], bb[1], bb[2]};
-        if (int rc = chain_init(s, bufs2, stage, steps, 0.0)) return cleanup(rc);
-        if (int rc = chain_steps(s, bufs2, stage, steps, psi)) return cleanup(rc);
+        if (int rc = chain_init(s, bufs2, stage, steps, 0.0, ex().partials)) return cleanup(rc);
+        if (int rc = chain_steps(s, bufs2, stage, steps, psi, ex().partials)) return cleanup(rc);
         used += steps;
         // normalise, FP64 Rayleigh quotient and explicit FP64 residual
         double nn = 0.0;

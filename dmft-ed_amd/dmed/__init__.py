@@ -118,6 +118,8 @@ SIGNATURES = [
     ("dmed_apply_op", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, _dp]),
     ("dmed_chain", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, _dp,
                              _dp, _i32p]),
+    ("dmed_chain_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
+                                     _dp, _dp, _i32p]),
     ("dmed_gf_add_chain", C.c_int, [C.c_double, C.c_double, _dp, _dp, C.c_int32, C.c_int32,
                                     _dp, C.c_int32, _dp, C.c_int32, C.c_double, _dp, _dp]),
     ("dmed_apply_sz", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, _dp]),

@@ -198,6 +198,21 @@ int dmed_chain(const dmed_sector *s, const dmed_vec *v0, int32_t nlanc,
                int32_t prec, double threshold, double *alpha, double *beta,
                int32_t *nsteps);
 
+/* Lanczos chain that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments and results as dmed_chain (the caller still owns v0 and the host
+ * alpha/beta/nsteps); the chain vectors and reduction partials live in a
+ * library-owned static device workspace of DMED_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends, so the sector may have at most
+ * DMED_NOALLOC_MAX_DIM states (DMED_EINVAL beyond).  The autotuner does not
+ * time candidates inside this call and Lanczos steps are launched one by
+ * one.  Calls that overlap on one GPU race on the shared workspace. */
+#define DMED_NOALLOC_MAX_DIM 2097152LL
+#define DMED_NOALLOC_SCRATCH_BYTES (96ull << 20)
+int dmed_chain_noalloc(const dmed_sector *s, const dmed_vec *v0, int32_t nlanc, int32_t prec,
+                       double threshold, double *alpha, double *beta, int32_t *nsteps);
+
 /* add_to_lanczos_gf_normal (build_gf_normal.f90:442-488): accumulate
  *   G(z) += peso_bz * sum_j Z(1,j)^2 / (z - isign*(lambda_j - ei))
  * with (lambda, Z) the tql2 spectrum of the chain, on z = i*wm[n] and
