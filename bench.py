@@ -281,6 +281,85 @@ def solve5_compare(reps=3):
     return out
 
 
+def chi_solves(reps=3, lmats=1024, lreal=1000):
+    """Wall time of one ed_solve with and without the susceptibilities
+    (workload 2: Ns = 12, one orbital), and of a hybrid-bath solve beside a
+    normal-bath solve of the same size (two orbitals, Ns = 12, Hloc
+    off-diagonal so that the hybrid solve runs its mixed chains).  Best of
+    `reps` solves after a warm-up, with the diagonalisation and G/chi times."""
+    import numpy as np
+    import dmed
+    from dmed import configs
+    kw = dict(lmats=lmats, lreal=lreal, lanc_ngfiter=200, ltau=1000)
+    out = {}
+
+    def timed(name, norb, nspin, ns, p):
+        dmed.solve(norb, nspin, ns, p)                        # warm-up: sector builds, tuning
+        best, r = None, None
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = dmed.solve(norb, nspin, ns, p)
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        out[name] = {"wall_s": best, "t_diag": r["t_diag"], "t_gf_chi": r["t_gf"], "E0": r["egs"]}
+        log(f"{name:<40s} wall {best:7.3f} s  diag {r['t_diag']:7.3f} s  "
+            f"gf+chi {r['t_gf']:7.3f} s  E0 {r['egs']:.12f}")
+
+    c = configs.config(2)
+    ns = dmed.set_model(c["norb"], c["nbath"], c["nspin"], c["bath"], c["hloc"], c["uloc"],
+                        c["ust"], c["jh"], 0.0, 0.0, c["xmu"])
+    for name, extra in (("ns12_norb1_G", {}), ("ns12_norb1_G_spin_chi", dict(chi_spin=1)),
+                        ("ns12_norb1_G_spin_charge_chi", dict(chi_spin=1, chi_dens=1))):
+        timed(name, c["norb"], c["nspin"], ns, dmed.default_params(beta=c["beta"], **kw, **extra))
+    rng = np.random.default_rng(5)
+    norb, nspin = 2, 1
+    hloc = np.zeros((nspin, nspin, norb, norb))
+    hloc[0, 0, 0, 1] = hloc[0, 0, 1, 0] = 0.1
+    hloc[0, 0, 0, 0], hloc[0, 0, 1, 1] = -0.1, 0.1
+    uloc, ust, jh = [2.5, 2.0], 1.5, 0.25
+    p = dmed.default_params(beta=100.0, **kw)
+    nb = 5          # normal bath: e(a,k), v(a,k) per orbital
+    bath = np.concatenate([rng.uniform(-1.5, 1.5, size=(norb, nb)).ravel(),
+                           rng.uniform(0.2, 0.7, size=(norb, nb)).ravel()])
+    ns = dmed.set_model(norb, nb, nspin, bath, hloc, uloc, ust, jh)
+    timed("ns12_norb2_normal_bath_nbath5", norb, nspin, ns, p)
+    nb = 10         # hybrid bath: e(k) shared, v(a,k)
+    bath = np.concatenate([rng.uniform(-1.5, 1.5, size=nb),
+                           rng.uniform(0.2, 0.7, size=(norb, nb)).ravel()])
+    ns = dmed.set_model(norb, nb, nspin, bath, hloc, uloc, ust, jh, bath_type="hybrid")
+    timed("ns12_norb2_hybrid_bath_nbath10", norb, nspin, ns, p)
+    return out
+
+
+def sites_solve(workload=1, nsites=16, lmats=1024):
+    """Inequivalent-site solve (dmed_solve_sites): the sites one after another
+    against all sites at once on the GPU.  Site i takes the workload's bath
+    with its levels scaled by 1 + 0.02 i and U + 0.1 i."""
+    import numpy as np
+    import dmed
+    from dmed import configs
+    c = configs.config(workload)
+    sites = []
+    for i in range(nsites):
+        b = np.array(c["bath"], dtype=float)
+        b[:c["nbath"]] *= 1.0 + 0.02 * i
+        sites.append(dict(norb=c["norb"], nbath=c["nbath"], nspin=c["nspin"], bath=b,
+                          hloc_f=c["hloc"], uloc=[c["uloc"][0] + 0.1 * i], xmu=c["xmu"]))
+    kw = dict(beta=c["beta"], lmats=lmats, lreal=100, lanc_ngfiter=200)
+    dmed.solve_sites(sites[:2], dmed.default_params(batched=1, **kw))      # warm-up
+    t = time.perf_counter()
+    seq = dmed.solve_sites(sites, dmed.default_params(batched=0, **kw))
+    t_seq = time.perf_counter() - t
+    t = time.perf_counter()
+    conc = dmed.solve_sites(sites, dmed.default_params(batched=1, **kw))
+    t_conc = time.perf_counter() - t
+    same = all(np.array_equal(x["smats"], y["smats"]) for x, y in zip(seq, conc))
+    log(f"workload {workload} ({c['name']}): {nsites} sites  one after another {t_seq:.2f} s  "
+        f"concurrent {t_conc:.2f} s  ratio {t_seq / t_conc:.2f}  identical={same}")
+    return {"workload": c["name"], "sites": nsites, "sequential_s": t_seq,
+            "concurrent_s": t_conc, "ratio": t_seq / t_conc, "sigma_bit_identical": same}
+
+
 def cpu_model():
     """CPU model and logical CPU count of this host (lscpu)."""
     import subprocess
@@ -318,6 +397,11 @@ def main():
                     help="repetitions of each solve form (the speed-ups use the medians)")
     ap.add_argument("--configs", action="store_true",
                     help="also time workloads 1-3 beside the oracle (H*v; workload-1 solve)")
+    ap.add_argument("--chi", action="store_true",
+                    help="also time solves with the susceptibilities and with a hybrid bath")
+    ap.add_argument("--sites", default="",
+                    help="W:S -- also time S inequivalent sites of workload W, one after "
+                         "another and concurrently")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solver runs the hot path on a single MI355X "
@@ -401,6 +485,11 @@ def main():
 
     if a.configs:
         extra["workloads_1_3_vs_oracle"] = configs_table(a.cpu_threads or (os.cpu_count() or 1))
+    if a.chi:
+        extra["chi_and_hybrid_solves"] = chi_solves()
+    if a.sites:
+        w, n = (int(v) for v in a.sites.split(":"))
+        extra["inequivalent_sites"] = sites_solve(w, n)
     dim_up, dim_dw = sec.dim_up, sec.dim_dw
     sec.free()
     x.free()
