@@ -69,7 +69,7 @@ def build_fortran():
     """DMFT_ED module (ISO_C_BINDING over libdmed.so) and the Bethe drivers
     (single site and inequivalent sites)."""
     os.makedirs(FBUILD, exist_ok=True)
-    srcs = ["dmed_cabi.f90", "dmft_ed.f90"]
+    srcs = ["dmed_cabi.f90", "dmft_ed.f90", "ed_hxv_gpu.f90"]
     objs = []
     for f in srcs:
         src = os.path.join(FDIR, f)

@@ -50,6 +50,8 @@ module DMED_CABI
 
   public :: dmed_init, dmed_finalize, dmed_last_error, dmed_set_model, dmed_solve
   public :: dmed_solve_sites
+  public :: dmed_sector_create, dmed_sector_free, dmed_sector_dims, dmed_vec_create, dmed_vec_free
+  public :: dmed_vec_upload, dmed_vec_download, dmed_vec_data, dmed_hxv
   public :: dmed_solve_params_default, dmed_sigma, dmed_delta, dmed_chi2_fit
   public :: dmed_chi2_fit_hybrid
   public :: c_string
@@ -69,6 +71,49 @@ module DMED_CABI
        import :: c_int, dmed_model_t
        type(dmed_model_t),intent(in) :: m
      end function dmed_set_model
+     integer(c_int) function dmed_sector_create(nup,ndw,sec) bind(C,name="dmed_sector_create")
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int32_t),value :: nup, ndw
+       type(c_ptr),intent(out)  :: sec
+     end function dmed_sector_create
+     integer(c_int) function dmed_sector_free(sec) bind(C,name="dmed_sector_free")
+       import :: c_int, c_ptr
+       type(c_ptr),value :: sec
+     end function dmed_sector_free
+     integer(c_int) function dmed_sector_dims(sec,dim,dimup,dimdw) bind(C,name="dmed_sector_dims")
+       import :: c_int, c_int64_t, c_ptr
+       type(c_ptr),value         :: sec
+       integer(c_int64_t),intent(out) :: dim, dimup, dimdw
+     end function dmed_sector_dims
+     integer(c_int) function dmed_vec_create(sec,prec,v) bind(C,name="dmed_vec_create")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr),value        :: sec
+       integer(c_int32_t),value :: prec
+       type(c_ptr),intent(out)  :: v
+     end function dmed_vec_create
+     integer(c_int) function dmed_vec_free(v) bind(C,name="dmed_vec_free")
+       import :: c_int, c_ptr
+       type(c_ptr),value :: v
+     end function dmed_vec_free
+     integer(c_int) function dmed_vec_upload(v,x) bind(C,name="dmed_vec_upload")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr),value         :: v
+       real(c_double),intent(in) :: x(*)
+     end function dmed_vec_upload
+     integer(c_int) function dmed_vec_download(v,x) bind(C,name="dmed_vec_download")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr),value :: v
+       real(c_double)    :: x(*)
+     end function dmed_vec_download
+     type(c_ptr) function dmed_vec_data(v) bind(C,name="dmed_vec_data")
+       import :: c_ptr
+       type(c_ptr),value :: v
+     end function dmed_vec_data
+     integer(c_int) function dmed_hxv(sec,x,y,prec) bind(C,name="dmed_hxv")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr),value        :: sec, x, y
+       integer(c_int32_t),value :: prec
+     end function dmed_hxv
      subroutine dmed_solve_params_default(p) bind(C,name="dmed_solve_params_default")
        import :: dmed_solve_params_t
        type(dmed_solve_params_t),intent(out) :: p

@@ -197,6 +197,7 @@ module DMFT_ED
   public :: ed_read_input
   public :: get_bath_dimension
   public :: ed_init_solver
+  public :: ed_push_model, set_bath, ed_check
   public :: ed_solve
   public :: ed_get_sigma_matsubara, ed_get_sigma_real
   public :: ed_get_gimp_matsubara, ed_get_gimp_real
@@ -1332,14 +1333,32 @@ contains
     p%ltau = Ltau
   end subroutine fill_params
 
+  !> The model this module holds -- the bath last set by set_bath (or
+  !> ed_solve), impHloc and the interaction parameters -- handed to the
+  !> library, which builds every sector's tables from it.  This is the state
+  !> the reference's H*v reads from module variables (dmft_bath, impHloc,
+  !> Uloc, Ust, Jh, xmu, ...; ED_HAMILTONIAN_MATVEC.f90:60-100).
+  subroutine ed_push_model()
+    type(dmed_model_t)         :: m
+    real(8),allocatable,target :: bcopy(:), hcopy(:,:,:,:)
+    allocate(bcopy(get_bath_dimension()))
+    call get_bath(bcopy)
+    allocate(hcopy(Nspin,Nspin,Norb,Norb)); hcopy = impHloc
+    m%norb = Norb; m%nbath = Nbath; m%nspin = Nspin
+    m%bath = c_loc(bcopy); m%hloc = c_loc(hcopy)
+    m%uloc = Uloc; m%ust = Ust; m%jh = Jh; m%jx = Jx; m%jp = Jp; m%xmu = xmu
+    m%hfmode = merge(1,0,HFmode); m%jhflag = merge(1,0,Jhflag)
+    m%bath_type = merge(DMED_BATH_HYBRID,DMED_BATH_NORMAL,trim(bath_type)=='hybrid')
+    call ed_check(dmed_set_model(m),"dmed_set_model")
+  end subroutine ed_push_model
+
   !> ed_solve_single (ED_MAIN.f90:251-289)
   subroutine ed_solve_single(bath,Hloc)
     real(8),dimension(:),intent(in),target :: bath
     complex(8),optional,intent(in)         :: Hloc(Nspin,Nspin,Norb,Norb)
-    type(dmed_model_t)        :: m
     type(dmed_solve_params_t) :: p
     type(dmed_solve_out_t)    :: o
-    real(8),allocatable,target :: bcopy(:), hcopy(:,:,:,:), dens(:), docc(:)
+    real(8),allocatable,target :: dens(:), docc(:)
     complex(8),allocatable,target :: gm(:,:,:,:,:), gr(:,:,:,:,:), sm(:,:,:,:,:), sr(:,:,:,:,:)
     integer(c_int32_t),target :: ntot
     integer                   :: lt
@@ -1348,14 +1367,7 @@ contains
     if(size(bath)/=get_bath_dimension())stop "ED_SOLVE_SINGLE Error: wrong bath dimensions"
     call set_bath(bath)
     call save_bath(used=.true.)
-    allocate(bcopy(size(bath))); bcopy = bath
-    allocate(hcopy(Nspin,Nspin,Norb,Norb)); hcopy = impHloc
-    m%norb = Norb; m%nbath = Nbath; m%nspin = Nspin
-    m%bath = c_loc(bcopy); m%hloc = c_loc(hcopy)
-    m%uloc = Uloc; m%ust = Ust; m%jh = Jh; m%jx = Jx; m%jp = Jp; m%xmu = xmu
-    m%hfmode = merge(1,0,HFmode); m%jhflag = merge(1,0,Jhflag)
-    m%bath_type = merge(DMED_BATH_HYBRID,DMED_BATH_NORMAL,trim(bath_type)=='hybrid')
-    call ed_check(dmed_set_model(m),"dmed_set_model")
+    call ed_push_model()
     call fill_params(p)
     allocate(dens(Norb), docc(Norb))
     allocate(gm(Nspin,Nspin,Norb,Norb,Lmats), sm(Nspin,Nspin,Norb,Norb,Lmats))

@@ -98,6 +98,10 @@ def test_fortran_driver_and_modules_built():
     for name in ("ed_read_input", "ed_init_solver", "ed_solve", "ed_get_sigma_matsubara",
                  "ed_chi2_fitgf", "get_bath_dimension"):
         assert name in mod, name
+    # the operator-level interface (cc_sparse_HxV shape) of module ED_HXV_GPU
+    shim = open(os.path.join(b, "ed_hxv_gpu.mod"), "rb").read().decode("latin-1").lower()
+    for name in ("setup_hv_sector", "delete_hv_sector", "directmatvec_cc", "cc_sparse_hxv"):
+        assert name in shim, name
 
 
 UTIL_CHECK = """program util_check
