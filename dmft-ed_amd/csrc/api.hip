@@ -487,13 +487,23 @@ __global__ void k_apply_up(const T *__restrict__ psi, int64_t dupI, T *__restric
     __shared__ double red[16];
     double s2 = 0.0;
     const int64_t n = dupJ * ddw;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = e / dupJ, r = e - c * dupJ;
+    // (column, row) of the flat index advance by the fixed grid stride, so
+    // the only 64-bit divisions are the three below
+    const int64_t e0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t qs = stride / dupJ, rs = stride - qs * dupJ;
+    int64_t c = e0 / dupJ, r = e0 - c * dupJ;
+    for (int64_t e = e0; e < n; e += stride) {
         const int32_t j = src[r];
         const double v = j >= 0 ? (double)sgn[r] * (double)psi[j + c * dupI] : 0.0;
         out[e] = (T)v;
         s2 += v * v;
+        c += qs;
+        r += rs;
+        if (r >= dupJ) {
+            r -= dupJ;
+            ++c;
+        }
     }
     double v = s2;
 #pragma unroll
@@ -516,13 +526,21 @@ __global__ void k_apply_dw(const T *__restrict__ psi, int64_t dup, T *__restrict
     __shared__ double red[16];
     double s2 = 0.0;
     const int64_t n = dup * ddwJ;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = e / dup, r = e - c * dup;
+    const int64_t e0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t qs = stride / dup, rs = stride - qs * dup;
+    int64_t c = e0 / dup, r = e0 - c * dup;
+    for (int64_t e = e0; e < n; e += stride) {
         const int32_t j = src[c];
         const double v = j >= 0 ? gsign * (double)sgn[c] * (double)psi[r + (int64_t)j * dup] : 0.0;
         out[e] = (T)v;
         s2 += v * v;
+        c += qs;
+        r += rs;
+        if (r >= dup) {
+            r -= dup;
+            ++c;
+        }
     }
     double v = s2;
 #pragma unroll

@@ -3,10 +3,13 @@
     python dmft-ed_amd/dmed/rocprof.py stats   --dir DIR --out FILE.md --cmd "..."
     python dmft-ed_amd/dmed/rocprof.py traffic --fetch DIR --write DIR --dim D
                                               --out FILE.json --md FILE.md --cmd "..."
+    python dmft-ed_amd/dmed/rocprof.py l2      --dir DIR --out FILE.md --cmd "..."
 
 `stats` condenses a `--kernel-trace --stats` run (kernel_stats.csv) into a
 markdown table.  `traffic` turns two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE`
-runs into HBM bytes per launch of the H*v kernels.  The counters are scaled
+runs into HBM bytes per launch of the H*v kernels.  `l2` turns a
+`--pmc TCC_HIT_sum TCC_MISS_sum` run into L2 requests and hit rates per
+launch of the H*v kernels and of the vector copy.  The traffic counters are scaled
 by what they read for the FP64 vector copy kernel (k_scale_copy<double,
 double>, which reads and writes exactly 8*dim bytes), so the factors between
 counter units and bytes come from the same run.
@@ -102,6 +105,28 @@ def traffic(a):
     print(json.dumps(res))
 
 
+def l2(a):
+    hit = counters(a.dir, "TCC_HIT_sum")
+    miss = counters(a.dir, "TCC_MISS_sum")
+    keys = [("copy", COPY)] + [(p, k) for p, k in HXV.items()] + [
+        ("fp64 Lanczos", "k_hxv_col<double, 1"), ("fp32 Lanczos", "k_hxv_col<float, 1")]
+    lines = ["# L2 hits and misses of the H*v kernel, Ns=16 half-filled sector", "",
+             "Command:", ""] + ["    " + c for c in a.cmd] + [
+        "", "TCC_HIT_sum and TCC_MISS_sum count L2 requests of all 8 XCDs per dispatch "
+        "(reads and writes).", "",
+        "| kernel | form | dispatches | hits | misses | hit rate |", "|---|---|---|---|---|---|"]
+    for form, key in keys:
+        try:
+            name, h, n = _pick(hit, key)
+            _, m, _ = _pick(miss, key)
+        except SystemExit:
+            continue
+        lines.append(f"| `{_short(name)}` | {form} | {n} | {h:.4g} | {m:.4g} | {h / (h + m):.3f} |")
+    with open(a.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def stats(a):
     path = _find(a.dir, "kernel_stats.csv")
     with open(path) as f:
@@ -135,8 +160,12 @@ def main(argv=None):
     s.add_argument("--title", default="")
     s.add_argument("--top", type=int, default=12)
     s.add_argument("--cmd", action="append", default=[])
+    c = sub.add_parser("l2")
+    c.add_argument("--dir", required=True)
+    c.add_argument("--out", required=True)
+    c.add_argument("--cmd", action="append", default=[])
     a = ap.parse_args(argv)
-    traffic(a) if a.cmd_name == "traffic" else stats(a)
+    {"traffic": traffic, "stats": stats, "l2": l2}[a.cmd_name](a)
 
 
 if __name__ == "__main__":
