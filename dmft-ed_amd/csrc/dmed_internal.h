@@ -198,12 +198,6 @@ struct ChainScalars {      // device-resident (inside Context::dscal)
     int32_t pad;
 };
 
-// Lanczos-step graph capture on the calling thread (default on).  Workers
-// that share the device with threads allocating, freeing or building sectors
-// turn it off: those calls fail or break a capture in flight on another
-// thread.
-void set_thread_graphs(bool on);
-
 // the plain-mode launch shape of s (after tuning)
 int hxv_shape(const dmed_sector *s, int prec, dmed_hxv_shape *out);
 

@@ -503,10 +503,6 @@ int launch_axpy(void *y, int yprec, const void *x, int xprec, int64_t n, double 
 }
 
 // ----------------------------------------------------------------- chain
-static thread_local bool t_graphs_off = false;
-
-void set_thread_graphs(bool on) { t_graphs_off = !on; }
-
 namespace {
 
 struct DevBuf {
@@ -562,7 +558,7 @@ static bool graphs_enabled()
         const char *e = std::getenv("DMED_GRAPHS");
         return !(e && e[0] == '0');
     }();
-    return on && !t_graphs_off;
+    return on;
 }
 
 int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double *psi,

@@ -1161,14 +1161,13 @@ extern "C" int dmed_solve_sites(int32_t nsites, const dmed_model *models,
     if (conc) q.batched = 0;
     std::vector<int> rcs(nsites, 0);
     std::vector<std::string> errs(nsites);
-    // concurrent sites build sectors and allocate while others run chains:
-    // their chains launch step by step (no stream capture)
+    // concurrent sites build sectors and allocate while others capture chain
+    // steps: captures are thread-local and every host-to-device copy of a
+    // solve goes on the worker's own stream, so neither disturbs the other
     int rc = run_tasks(conc ? 1 : 0, nsites, [&](int i) {
         bind_model(&ms[i]);
-        if (conc) set_thread_graphs(false);
         rcs[i] = dmed_solve(&q, &outs[i]);
         if (rcs[i]) errs[i] = dmed_last_error();
-        set_thread_graphs(true);
         bind_model(nullptr);
     });
     if (rc) return rc;
