@@ -117,26 +117,47 @@ __global__ void __launch_bounds__(1024) k_chain_alpha(const double *__restrict__
     }
 }
 
+// 16-byte vectors of T
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { using type = double2; };
+template <> struct Vec16<float> { using type = float4; };
+
 // w = Hv - b*v_prev - alpha*v in place of y = Hv (stored scales sx, svp), with
-// the block partials of ||w||^2 (first gridDim.x) and <w, x> (next gridDim.x)
+// the block partials of ||w||^2 (first gridDim.x) and <w, x> (next gridDim.x).
+// Each thread takes the elements i = tid + k*stride in increasing order, as a
+// plain grid-stride loop would, with four of them loaded before the first is
+// used; the partial sums (hence beta) do not depend on that grouping.
 template <typename T>
 __global__ void __launch_bounds__(kVecBS) k_chain_update(T *__restrict__ w, const T *__restrict__ v,
                                                          const T *__restrict__ vp, int64_t n,
                                                          const ChainScalars *cs,
                                                          double *__restrict__ partials)
 {
+    constexpr int U = 4;
     __shared__ double red[16];
     if (cs->done) return;
     const double ca = cs->alpha * cs->sx, cb = cs->b * cs->svp;
     double s1 = 0.0, s2 = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const double vi = (double)v[i];
-        const T wt = (T)((double)w[i] - cb * (double)vp[i] - ca * vi);
-        w[i] = wt;
-        const double wi = (double)wt;
-        s1 += wi * wi;
-        s2 += wi * vi;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += U * stride) {
+        T a[U], b[U], c[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int64_t e = min(i + k * stride, n - 1);
+            a[k] = w[e];
+            b[k] = v[e];
+            c[k] = vp[e];
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            if (i + k * stride >= n) break;
+            const double vi = (double)b[k];
+            const T wt = (T)((double)a[k] - cb * (double)c[k] - ca * vi);
+            w[i + k * stride] = wt;
+            const double wi = (double)wt;
+            s1 += wi * wi;
+            s2 += wi * vi;
+        }
     }
     s1 = bsum(s1, red);
     __syncthreads();
@@ -173,16 +194,34 @@ __global__ void __launch_bounds__(1024) k_chain_beta(const double *__restrict__ 
     }
 }
 
-// psi += z_k * sx * v  (second pass of the ground-state Lanczos)
+// psi += z_k * sx * v  (second pass of the ground-state Lanczos); 16-byte
+// loads of v and of psi (psi and v 16-byte aligned), the last n % (16/sizeof(T))
+// elements one per thread
 template <typename T>
 __global__ void __launch_bounds__(kVecBS) k_accum(double *__restrict__ psi, const T *__restrict__ v,
                                                   int64_t n, const ChainScalars *cs,
                                                   const double *__restrict__ z)
 {
+    using V = typename Vec16<T>::type;
+    constexpr int E = 16 / sizeof(T);
     const double c = z[cs->step] * cs->sx;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        psi[i] += c * (double)v[i];
+    const int64_t nv = n / E;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const V *vv = reinterpret_cast<const V *>(v);
+    double2 *pp = reinterpret_cast<double2 *>(psi);
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += stride) {
+        const V b = vv[j];
+        const T *y = reinterpret_cast<const T *>(&b);
+#pragma unroll
+        for (int k = 0; k < E / 2; ++k) {
+            double2 p = pp[j * (E / 2) + k];
+            p.x += c * (double)y[2 * k];
+            p.y += c * (double)y[2 * k + 1];
+            pp[j * (E / 2) + k] = p;
+        }
+    }
+    const int64_t t = nv * E + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) psi[t] += c * (double)v[t];
 }
 
 template <typename TD, typename TS>
@@ -297,14 +336,24 @@ __device__ __forceinline__ void rowsums(const double *__restrict__ P, int nr, in
 }
 
 // Partials of the projections of w on the rows Y_0..Y_{ny-1}, V_0..V_{nv-1}
-// and, with one extra row, on w itself: P[r*gridDim.x + block].
+// and, as row ny + nv, on w itself (when nrows says so): P[r*gridDim.x + block].
+// Each block takes rpb <= kProjRows consecutive rows (blockIdx.y), so w is
+// read once per rpb rows; every row keeps its own per-thread order and block
+// reduction, as with one row per block, so rpb does not change the result.
 // Second pass (dgks != 0): runs only when the first pass cancelled more than
 // half of ||w||^2 (Daniel-Gragg-Kaufman-Stewart), judged from the first
 // pass's partials: n0 (||w||^2 before, g0 values) and n1 (after, g0 values).
+constexpr int kProjRows = 8;
+
+// rows per block of k_proj_dot: several for long vectors, whose re-reads of w
+// cost HBM traffic; one for short ones, which need the blocks for parallelism
+static int proj_rpb(int64_t n) { return n >= ((int64_t)1 << 14) ? kProjRows : 1; }
+static unsigned proj_groups(int nrows, int rpb) { return (unsigned)((nrows + rpb - 1) / rpb); }
+
 __global__ void __launch_bounds__(kVecBS) k_proj_dot(
     const double *__restrict__ Y, int ny, const double *__restrict__ V, int nv,
     const double *__restrict__ w, int64_t n, double *__restrict__ P, int dgks,
-    const double *__restrict__ n0, const double *__restrict__ n1)
+    const double *__restrict__ n0, const double *__restrict__ n1, int nrows, int rpb)
 {
     __shared__ double red[16];
     if (dgks) {
@@ -312,14 +361,31 @@ __global__ void __launch_bounds__(kVecBS) k_proj_dot(
         const double a1 = psum_all(n1, gridDim.x, red);
         if (!(a1 < 0.5 * a0)) return;
     }
-    const int r = blockIdx.y;
-    const double *b = r < ny ? Y + (int64_t)r * n : (r < ny + nv ? V + (int64_t)(r - ny) * n : w);
-    double s = 0.0;
+    const int r0 = blockIdx.y * rpb;
+    const int nr = min(rpb, nrows - r0);
+    const double *b[kProjRows];
+#pragma unroll
+    for (int q = 0; q < kProjRows; ++q) {
+        const int r = min(r0 + q, nrows - 1);
+        b[q] = r < ny ? Y + (int64_t)r * n : (r < ny + nv ? V + (int64_t)(r - ny) * n : w);
+    }
+    double s[kProjRows];
+#pragma unroll
+    for (int q = 0; q < kProjRows; ++q) s[q] = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        s += b[i] * w[i];
-    s = bsum(s, red);
-    if (threadIdx.x == 0) P[(int64_t)r * gridDim.x + blockIdx.x] = s;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double wi = w[i];
+#pragma unroll
+        for (int q = 0; q < kProjRows; ++q)
+            if (q < nr) s[q] += b[q][i] * wi;
+    }
+#pragma unroll
+    for (int q = 0; q < kProjRows; ++q) {
+        if (q >= nr) break;
+        const double t = bsum(s[q], red);
+        if (threadIdx.x == 0) P[(int64_t)(r0 + q) * gridDim.x + blockIdx.x] = t;
+        __syncthreads();
+    }
 }
 
 // w -= sum_r Y_r cy_r + sum_r V_r cv_r, the coefficients summed from the
@@ -594,6 +660,8 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
         return hipGetLastError() == hipSuccess ? 0 : fail(DMED_EHIP, "chain step launch failed");
     };
     void *vp = bufs[0], *v = bufs[1], *w = bufs[2];
+    if (((uintptr_t)vp | (uintptr_t)v | (uintptr_t)w | (uintptr_t)psi) % 16 != 0)
+        return fail(DMED_EINVAL, "chain work vectors must be 16-byte aligned");
     auto rotate = [&]() {
         void *t = vp;
         vp = v;
@@ -1070,17 +1138,19 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
         if (noproj) {
             // only ||w||^2: the w row of k_proj_dot, into Pn1
             hipLaunchKernelGGL(k_proj_dot, dim3(g, 1), dim3(kVecBS), 0, st, Y, 0, V, 0, w, n, Pn1,
-                               0, (const double *)nullptr, (const double *)nullptr);
+                               0, (const double *)nullptr, (const double *)nullptr, 1, 1);
             DMED_HIP(hipGetLastError());
             return 0;
         }
         const double *n0 = P1 + (size_t)rows * g;   // the w row
-        hipLaunchKernelGGL(k_proj_dot, dim3(g, rows + 1), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n,
-                           P1, 0, (const double *)nullptr, (const double *)nullptr);
+        const int rpb = proj_rpb(n);
+        hipLaunchKernelGGL(k_proj_dot, dim3(g, proj_groups(rows + 1, rpb)), dim3(kVecBS), 0, st, Y,
+                           nl, V, jmax, w, n, P1, 0, (const double *)nullptr,
+                           (const double *)nullptr, rows + 1, rpb);
         hipLaunchKernelGGL(k_proj_axpy, dim3(g), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n, P1, Pn1,
                            cacc, 1, 0, (const double *)nullptr, (const double *)nullptr);
-        hipLaunchKernelGGL(k_proj_dot, dim3(g, rows), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n, P2,
-                           1, n0, Pn1);
+        hipLaunchKernelGGL(k_proj_dot, dim3(g, proj_groups(rows, rpb)), dim3(kVecBS), 0, st, Y, nl,
+                           V, jmax, w, n, P2, 1, n0, Pn1, rows, rpb);
         hipLaunchKernelGGL(k_proj_axpy, dim3(g), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n, P2, Pn2,
                            cacc, 0, 1, n0, Pn1);
         DMED_HIP(hipGetLastError());
