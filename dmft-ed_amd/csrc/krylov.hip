@@ -1023,26 +1023,55 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
     std::vector<double> a(kChainMax), b(kChainMax), z(kChainMax);
     // DMED_MIXED: restarts run the recurrence on FP32 vectors (FP64 scalars
     // and reductions) until the FP64 residual of the accumulated vector is
-    // below kMixedSwitch; the later restarts, which take the residual down to
-    // tol, run in FP64.  The Rayleigh quotient and the residual are FP64.
+    // below kMixedSwitch.  The later restarts, which take the residual down to
+    // tol, start from that vector psi0 with kRefineHead FP64 steps and go on
+    // in FP32: the first step gives alpha_0 = <psi0|H|psi0> and
+    // beta_1 = ||H psi0 - alpha_0 psi0|| in FP64, and every later Krylov
+    // vector enters the new psi with a weight of the order of the residual
+    // over the gap, so its FP32 rounding lands far below tol.  The Rayleigh
+    // quotient and the residual are FP64.
     constexpr double kMixedSwitch = 1e-5;
-    int stage = prec == DMED_MIXED ? DMED_FP32 : DMED_FP64;
+    constexpr int kRefineHead = 2;
+    enum { kFp32Stage, kRefineStage, kFp64Stage };
+    int stage = prec == DMED_MIXED ? kFp32Stage : kFp64Stage;
+    // (v_prev, v) of an FP64 head into the FP32 buffers; w is rewritten by
+    // the next step
+    auto to_fp32 = [&](void *cb[3]) -> int {
+        if (int rc = launch_scale_copy(b32[0], DMED_FP32, cb[0], DMED_FP64, n, 1.0, st)) return rc;
+        if (int rc = launch_scale_copy(b32[1], DMED_FP32, cb[1], DMED_FP64, n, 1.0, st)) return rc;
+        cb[0] = b32[0];
+        cb[1] = b32[1];
+        cb[2] = b32[2];
+        return 0;
+    };
     for (int restart = 0; restart < 12 && used < maxit; ++restart) {
-        void **bb = stage == DMED_FP32 ? b32 : b64;
-        const double stol = stage == DMED_FP32 ? std::max(tol, 1e-7) : tol;
+        const bool refine = stage == kRefineStage;
+        const int p0 = stage == kFp32Stage ? DMED_FP32 : DMED_FP64;
+        void **bb = p0 == DMED_FP32 ? b32 : b64;
+        const double stol = stage == kFp32Stage ? std::max(tol, 1e-7) : tol;
         // pass 1, in chunks of ncheck steps with a host convergence check
-        if (int rc = launch_scale_copy(bb[1], stage, psi, DMED_FP64, n, 1.0, st))
+        if (int rc = launch_scale_copy(bb[1], p0, psi, DMED_FP64, n, 1.0, st))
             return cleanup(rc);
         DMED_HIP(hipMemsetAsync(chain_alpha(), 0, kChainMax * sizeof(double), st));
         DMED_HIP(hipMemsetAsync(chain_beta(), 0, kChainMax * sizeof(double), st));
         const int budget = std::min(kChainMax - 1, maxit - used);
+        const int nhead = refine ? std::min(kRefineHead, budget) : 0;
         void *bufs[3] = {bb[0], bb[1], bb[2]};
-        if (int rc = chain_init(s, bufs, stage, budget, 0.0, ex().partials)) return cleanup(rc);
+        if (int rc = chain_init(s, bufs, p0, budget, 0.0, ex().partials)) return cleanup(rc);
         int steps = 0;
+        if (nhead > 0) {
+            if (int rc = chain_steps(s, bufs, DMED_FP64, nhead, nullptr, ex().partials))
+                return cleanup(rc);
+            if (int rc = to_fp32(bufs)) return cleanup(rc);
+            steps = nhead;
+        }
+        const int cur = refine ? DMED_FP32 : p0;
         ChainScalars cs{};
         for (;;) {
             const int chunk = std::min(ncheck, budget - steps);
-            if (int rc = chain_steps(s, bufs, stage, chunk, nullptr, ex().partials)) return cleanup(rc);
+            if (chunk > 0)
+                if (int rc = chain_steps(s, bufs, cur, chunk, nullptr, ex().partials))
+                    return cleanup(rc);
             steps += chunk;
             DMED_HIP(hipMemcpyAsync(a.data(), chain_alpha(), steps * 8, hipMemcpyDeviceToHost, st));
             DMED_HIP(hipMemcpyAsync(b.data(), chain_beta(), (steps + 1) * 8, hipMemcpyDeviceToHost, st));
@@ -1058,14 +1087,21 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
             }
         }
         used += steps;
-        // pass 2: rerun the recurrence from the same start accumulating psi (FP64)
+        // pass 2: rerun the recurrence from the same start, in the same
+        // precisions, accumulating psi (FP64)
         DMED_HIP(hipMemcpyAsync(chain_aux(), z.data(), steps * 8, hipMemcpyHostToDevice, st));
-        if (int rc = launch_scale_copy(bb[1], stage, psi, DMED_FP64, n, 1.0, st))
+        if (int rc = launch_scale_copy(bb[1], p0, psi, DMED_FP64, n, 1.0, st))
             return cleanup(rc);
         DMED_HIP(hipMemsetAsync(psi, 0, n * 8, st));
         void *bufs2[3] = {bb[0], bb[1], bb[2]};
-        if (int rc = chain_init(s, bufs2, stage, steps, 0.0, ex().partials)) return cleanup(rc);
-        if (int rc = chain_steps(s, bufs2, stage, steps, psi, ex().partials)) return cleanup(rc);
+        if (int rc = chain_init(s, bufs2, p0, steps, 0.0, ex().partials)) return cleanup(rc);
+        const int h2 = std::min(nhead, steps);
+        if (h2 > 0) {
+            if (int rc = chain_steps(s, bufs2, DMED_FP64, h2, psi, ex().partials)) return cleanup(rc);
+            if (int rc = to_fp32(bufs2)) return cleanup(rc);
+        }
+        if (steps > h2)
+            if (int rc = chain_steps(s, bufs2, cur, steps - h2, psi, ex().partials)) return cleanup(rc);
         used += steps;
         // normalise, FP64 Rayleigh quotient and explicit FP64 residual
         double nn = 0.0;
@@ -1080,8 +1116,8 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
         if (int rc = residual(s, psi, theta, (double *)b64[2], &res)) return cleanup(rc);
         used += 2;
         if (res <= tol * std::max(1.0, fabs(theta))) break;
-        if (stage == DMED_FP32 && (res <= kMixedSwitch * std::max(1.0, fabs(theta)) || restart >= 3))
-            stage = DMED_FP64;
+        if (stage == kFp32Stage && (res <= kMixedSwitch * std::max(1.0, fabs(theta)) || restart >= 3))
+            stage = kRefineStage;
     }
     *eval = theta;
     if (resid) *resid = res;
