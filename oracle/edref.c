@@ -1,8 +1,8 @@
 /*
  * oracle/edref.c -- CPU restatement of the reference DMFT_ED hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the parity checker and the CPU
- * timing baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * TEST INFRASTRUCTURE ONLY.  The GPU results are tested against this file,
+ * and it is the CPU timing baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it; the product library (dmft-ed_amd/) never
  * links, loads or calls it.
  *

@@ -1,6 +1,6 @@
 """oracle/edref.py -- ctypes front end and solve orchestration of the oracle.
 
-TEST INFRASTRUCTURE ONLY (the checker): imported by tests/, by
+TEST INFRASTRUCTURE ONLY (the GPU results are tested against it): imported by tests/, by
 __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
 product library.  The arithmetic lives in oracle/edref.c (a C restatement of
 the reference); this file only sequences it the way the reference's
