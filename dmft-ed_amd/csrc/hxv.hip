@@ -215,8 +215,14 @@ __global__ void __launch_bounds__(1024) k_hxv(const HxvArgs<T> a)
 // then the diagonal -- the same for every (BS, RPT) and grid, so the launch
 // shape never changes y.  In Lanczos mode each column's <x, y> is reduced on
 // its own and stored at PARTIALS[c]: alpha does not depend on the grid either.
-template <typename T, int MODE, int BS, int RPT>
-__global__ void __launch_bounds__(BS) k_hxv_col(
+// HALVES = 2: steps 2-4 run twice, for the first and the second half of the
+// lane's rows, one neighbour column at a time.  Half the accumulators, loads
+// and hop words are live at once, which brings the kernel within the
+// registers of two workgroups per CU (2*BS/256 waves per SIMD); two
+// workgroups let one's LDS gathers overlap the other's far loads.  The sums
+// are the same, term for term, as with HALVES = 1.
+template <typename T, int MODE, int BS, int RPT, int HALVES = 1>
+__global__ void __launch_bounds__(BS, HALVES == 2 ? 2 * BS / 256 : 1) k_hxv_col(
     const T *__restrict__ X, T *__restrict__ Y, int32_t dup, int32_t ddw, int32_t nhu, int32_t nhd,
     int32_t nimp, int32_t ncoef, const uint32_t *__restrict__ HU, const double *__restrict__ COEF,
     const int32_t *__restrict__ HDI, const double *__restrict__ HDV, const double *__restrict__ DU,
@@ -286,58 +292,116 @@ __global__ void __launch_bounds__(BS) k_hxv_col(
         }
         if ((int)threadIdx.x < nimp)
             wt[threadIdx.x] = DD[c] + WTAB[threadIdx.x * nimp + IMPD[c]];
-        T acc[RPT];
+        if constexpr (HALVES == 2) {
+            constexpr int RH = (RPT + 1) / 2;
+#pragma unroll 1
+            for (int part = 0; part < 2; ++part) {
+                const int rb = part * RH;
+                T acc[RH];
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) acc[r] = (T)0;
-        // 2. down hops
-        for (int h0 = 0; h0 < nhd; h0 += HDU) {
-            T v[HDU];
-            const T *xj[HDU];
+                for (int r = 0; r < RH; ++r) acc[r] = (T)0;
+                // 2. down hops for rows rb .. rb+RH-1
+                for (int h = 0; h < nhd; ++h) {
+                    const T v = (T)HDV[(int64_t)h * ddw + c];
+                    const T *xj = X + (int64_t)HDI[(int64_t)h * ddw + c] * dup;
+                    T g[RH];
 #pragma unroll
-            for (int q = 0; q < HDU; ++q) {
-                const int h = min(h0 + q, nhd - 1);
-                v[q] = h0 + q < nhd ? (T)HDV[(int64_t)h * ddw + c] : (T)0;
-                xj[q] = X + (int64_t)HDI[(int64_t)h * ddw + c] * dup;
+                    for (int r = 0; r < RH; ++r)
+                        g[r] = xj[min((int)threadIdx.x + (rb + r) * BS, last)];
+#pragma unroll
+                    for (int r = 0; r < RH; ++r) acc[r] += v * g[r];
+                }
+                // 3. the DMA has landed (first half), up hops from LDS
+                if (part == 0) {
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __syncthreads();
+                }
+                for (int h = 0; h < nhu; ++h) {
+                    uint32_t w[RH];
+#pragma unroll
+                    for (int r = 0; r < RH; ++r)
+                        w[r] = HU[(int64_t)h * dup + min((int)threadIdx.x + (rb + r) * BS, last)];
+#pragma unroll
+                    for (int r = 0; r < RH; ++r) {
+                        T vv = sc[(w[r] >> 1) & 127u];
+                        vv = (w[r] & 1u) ? -vv : vv;
+                        acc[r] += vv * col[w[r] >> 8];
+                    }
+                }
+                // 4. diagonal and store
+#pragma unroll
+                for (int q = 0; q < RH; ++q) {
+                    const int r = rb + q;
+                    const int i = threadIdx.x + r * BS;
+                    if (r < RPT && i < dup) {
+                        const int64_t o = (int64_t)c * dup + i;
+                        const T xi = col[i];
+                        const uint32_t ib = (imp4[r / 4] >> (8 * (r % 4))) & 0xffu;
+                        const T hv = acc[q] + (T)(DU[i] + wt[ib]) * xi;
+                        if (MODE == kPlain) {
+                            __builtin_nontemporal_store(hv, &Y[o]);
+                        } else {
+                            const T yy = (T)(sx * (double)hv);
+                            __builtin_nontemporal_store(yy, &Y[o]);
+                            dot[r % M] += (double)xi * (double)yy;
+                        }
+                    }
+                }
             }
-            T g[HDU][RPT];
+        } else {
+            T acc[RPT];
 #pragma unroll
-            for (int q = 0; q < HDU; ++q)
+            for (int r = 0; r < RPT; ++r) acc[r] = (T)0;
+            // 2. down hops
+            for (int h0 = 0; h0 < nhd; h0 += HDU) {
+                T v[HDU];
+                const T *xj[HDU];
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) g[q][r] = xj[q][min((int)threadIdx.x + r * BS, last)];
+                for (int q = 0; q < HDU; ++q) {
+                    const int h = min(h0 + q, nhd - 1);
+                    v[q] = h0 + q < nhd ? (T)HDV[(int64_t)h * ddw + c] : (T)0;
+                    xj[q] = X + (int64_t)HDI[(int64_t)h * ddw + c] * dup;
+                }
+                T g[HDU][RPT];
 #pragma unroll
-            for (int r = 0; r < RPT; ++r)
+                for (int q = 0; q < HDU; ++q)
 #pragma unroll
-                for (int q = 0; q < HDU; ++q) acc[r] += v[q] * g[q][r];
-        }
-        // 3. the DMA has landed for every wave, then up hops from LDS
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        for (int h = 0; h < nhu; ++h) {
-            uint32_t w[RPT];
+                    for (int r = 0; r < RPT; ++r) g[q][r] = xj[q][min((int)threadIdx.x + r * BS, last)];
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) w[r] = HU[(int64_t)h * dup + min((int)threadIdx.x + r * BS, last)];
+                for (int r = 0; r < RPT; ++r)
+#pragma unroll
+                    for (int q = 0; q < HDU; ++q) acc[r] += v[q] * g[q][r];
+            }
+            // 3. the DMA has landed for every wave, then up hops from LDS
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            for (int h = 0; h < nhu; ++h) {
+                uint32_t w[RPT];
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) w[r] = HU[(int64_t)h * dup + min((int)threadIdx.x + r * BS, last)];
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    T vv = sc[(w[r] >> 1) & 127u];
+                    vv = (w[r] & 1u) ? -vv : vv;
+                    acc[r] += vv * col[w[r] >> 8];
+                }
+            }
+            // 4. diagonal last, with the lane's own x from LDS
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
-                T vv = sc[(w[r] >> 1) & 127u];
-                vv = (w[r] & 1u) ? -vv : vv;
-                acc[r] += vv * col[w[r] >> 8];
-            }
-        }
-        // 4. diagonal last, with the lane's own x from LDS
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int i = threadIdx.x + r * BS;
-            if (i < dup) {
-                const int64_t o = (int64_t)c * dup + i;
-                const T xi = col[i];
-                const uint32_t ib = (imp4[r / 4] >> (8 * (r % 4))) & 0xffu;
-                const T hv = acc[r] + (T)(DU[i] + wt[ib]) * xi;
-                if (MODE == kPlain) {
-                    __builtin_nontemporal_store(hv, &Y[o]);
-                } else {
-                    const T yy = (T)(sx * (double)hv);
-                    __builtin_nontemporal_store(yy, &Y[o]);
-                    dot[r % M] += (double)xi * (double)yy;
+                const int i = threadIdx.x + r * BS;
+                if (i < dup) {
+                    const int64_t o = (int64_t)c * dup + i;
+                    const T xi = col[i];
+                    const uint32_t ib = (imp4[r / 4] >> (8 * (r % 4))) & 0xffu;
+                    const T hv = acc[r] + (T)(DU[i] + wt[ib]) * xi;
+                    if (MODE == kPlain) {
+                        __builtin_nontemporal_store(hv, &Y[o]);
+                    } else {
+                        const T yy = (T)(sx * (double)hv);
+                        __builtin_nontemporal_store(yy, &Y[o]);
+                        dot[r % M] += (double)xi * (double)yy;
+                    }
                 }
             }
         }
@@ -425,18 +489,20 @@ static int lds_attr_once(const void *kern)
 }
 
 // Column-kernel instantiations: 1024 lanes with 1..13 (FP64) or 1..16 (FP32)
-// rows per lane, 512 and 256 lanes with 1..8.  f(kernel pointer) is called
-// with the instantiation for (bs, rpt); false if there is none.
-template <typename T, int MODE, int BS, typename F>
+// rows per lane, 512 and 256 lanes with 1..8; the two-half form (HALVES = 2)
+// for 1024 lanes and at least 4 rows per lane.  f(kernel pointer) is called
+// with the instantiation for (bs, rpt, halves); false if there is none.
+template <typename T, int MODE, int BS, int H, typename F>
 static bool col_dispatch_bs(int rpt, F &&f)
 {
     switch (rpt) {
-#define DMED_R(R)                                                \
-    case R:                                                      \
-        if constexpr (R <= (BS == 1024 ? (sizeof(T) == 4 ? 16 : 13) : 8)) { \
-            f(k_hxv_col<T, MODE, BS, R>);                        \
-            return true;                                         \
-        }                                                        \
+#define DMED_R(R)                                                                  \
+    case R:                                                                        \
+        if constexpr (R <= (BS == 1024 ? (sizeof(T) == 4 ? 16 : 13) : 8) &&        \
+                      (H == 1 || (BS == 1024 && R >= 4))) {                        \
+            f(k_hxv_col<T, MODE, BS, R, H>);                                       \
+            return true;                                                           \
+        }                                                                          \
         return false;
         DMED_R(1) DMED_R(2) DMED_R(3) DMED_R(4) DMED_R(5) DMED_R(6) DMED_R(7) DMED_R(8)
         DMED_R(9) DMED_R(10) DMED_R(11) DMED_R(12) DMED_R(13) DMED_R(14) DMED_R(15) DMED_R(16)
@@ -447,34 +513,54 @@ static bool col_dispatch_bs(int rpt, F &&f)
 }
 
 template <typename T, int MODE, typename F>
-static bool col_dispatch(int bs, int rpt, F &&f)
+static bool col_dispatch(int bs, int rpt, bool halves, F &&f)
 {
-    if (bs == 1024) return col_dispatch_bs<T, MODE, 1024>(rpt, f);
-    if (bs == 512) return col_dispatch_bs<T, MODE, 512>(rpt, f);
-    if (bs == 256) return col_dispatch_bs<T, MODE, 256>(rpt, f);
+    if (bs == 1024)
+        return halves ? col_dispatch_bs<T, MODE, 1024, 2>(rpt, f)
+                      : col_dispatch_bs<T, MODE, 1024, 1>(rpt, f);
+    if (halves) return false;
+    if (bs == 512) return col_dispatch_bs<T, MODE, 512, 1>(rpt, f);
+    if (bs == 256) return col_dispatch_bs<T, MODE, 256, 1>(rpt, f);
     return false;
 }
 
-// workgroups of (bs, rpt) per CU that registers and LDS allow
+// DMED_COL_HALVES=0 keeps the column kernel to its one-pass form
+static bool halves_enabled()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("DMED_COL_HALVES");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// workgroups of (bs, rpt, halves) per CU that registers and LDS allow; 0
+// when there is no such instantiation
 template <typename T>
-static int col_fit_per_cu(int bs, int rpt, size_t shmem)
+static int col_fit_per_cu(int bs, int rpt, bool halves, size_t shmem)
 {
     int nb = 1;
-    const bool ok = col_dispatch<T, kLanczos>(bs, rpt, [&](auto kern) {
+    const bool ok = col_dispatch<T, kLanczos>(bs, rpt, halves, [&](auto kern) {
         if (lds_attr_once((const void *)kern) != 0 ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)kern, bs, shmem) !=
                 hipSuccess)
             nb = 1;
     });
     (void)hipGetLastError();
-    return ok ? std::max(nb, 1) : 1;
+    return ok ? std::max(nb, 1) : 0;
 }
 
-static int col_fit(const dmed_sector *s, int prec, int bs)
+// {workgroups per CU of the one-pass form, of the two-half form (0: none)}
+struct ColFit {
+    int one = 1, two = 0;
+    int best() const { return std::max(one, two); }
+};
+
+static ColFit col_fit(const dmed_sector *s, int prec, int bs)
 {
     // the occupancy query is the same for every sector of this shape: keep it
     static std::mutex mu;
-    static std::map<std::tuple<int64_t, int, int>, int> memo;
+    static std::map<std::tuple<int64_t, int, int>, ColFit> memo;
     const auto key = std::make_tuple(s->up.dim, prec == DMED_FP32 ? 1 : 0, bs);
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -483,11 +569,20 @@ static int col_fit(const dmed_sector *s, int prec, int bs)
     }
     const int rpt = col_rpt(s, prec, bs);
     const size_t shmem = col_lds_bytes(s->up.dim, prec == DMED_FP32 ? 4 : 8);
-    const int nb = rpt == 0 ? 0 : (prec == DMED_FP32 ? col_fit_per_cu<float>(bs, rpt, shmem)
-                                                     : col_fit_per_cu<double>(bs, rpt, shmem));
+    ColFit f;
+    if (rpt == 0) {
+        f.one = 0;
+    } else {
+        const bool f32 = prec == DMED_FP32;
+        f.one = std::max(1, f32 ? col_fit_per_cu<float>(bs, rpt, false, shmem)
+                                : col_fit_per_cu<double>(bs, rpt, false, shmem));
+        if (halves_enabled() && bs == 1024)
+            f.two = f32 ? col_fit_per_cu<float>(bs, rpt, true, shmem)
+                        : col_fit_per_cu<double>(bs, rpt, true, shmem);
+    }
     std::lock_guard<std::mutex> lk(mu);
-    memo[key] = nb;
-    return nb;
+    memo[key] = f;
+    return f;
 }
 
 // ------------------------------------------------------------ autotuner
@@ -564,6 +659,7 @@ struct ColShape {
     int bs = 1024, rpt = 0, per_cu = 1, fit = 1;
     unsigned grid = 8;
     bool tuned = false;
+    bool halves = false;     // the two-half form (more workgroups per CU)
 };
 
 static unsigned col_grid_for(const dmed_sector *s, int per_cu)
@@ -574,18 +670,24 @@ static unsigned col_grid_for(const dmed_sector *s, int per_cu)
     return (unsigned)g;
 }
 
+// Workgroups per CU: the tuned count, or two where they fit (the fixed rule).
+// The one-pass form is kept whenever it fits that many workgroups; the
+// two-half form takes over where only it does.
 static ColShape col_shape(const dmed_sector *s, int prec, const TuneChoice &tc)
 {
     ColShape c;
+    ColFit f;
     if (tc.cbs > 0 && col_rpt(s, prec, tc.cbs) > 0) {
         c.bs = tc.cbs;
-        c.fit = col_fit(s, prec, c.bs);
-        c.per_cu = std::max(1, std::min(tc.wgs, c.fit));
+        f = col_fit(s, prec, c.bs);
+        c.per_cu = std::max(1, std::min(tc.wgs, f.best()));
         c.tuned = true;
     } else {
-        c.fit = col_fit(s, prec, 1024);
-        c.per_cu = std::max(1, std::min(c.fit, 2));
+        f = col_fit(s, prec, 1024);
+        c.per_cu = std::max(1, std::min(f.best(), 2));
     }
+    c.fit = f.best();
+    c.halves = c.per_cu > f.one && f.two >= c.per_cu;
     c.rpt = col_rpt(s, prec, c.bs);
     c.grid = col_grid_for(s, c.per_cu);
     return c;
@@ -600,7 +702,7 @@ static int launch_col(const dmed_sector *s, const void *x, void *y, ChainScalars
     // 16-byte DMA granules need every column start 16-byte aligned
     const int dma16 = ((uintptr_t)x % 16 == 0) && ((size_t)dup * sizeof(T)) % 16 == 0;
     int rc = 0;
-    const bool ok = col_dispatch<T, MODE>(c.bs, c.rpt, [&](auto kern) {
+    const bool ok = col_dispatch<T, MODE>(c.bs, c.rpt, c.halves, [&](auto kern) {
         rc = lds_attr_once((const void *)kern);
         if (rc) return;
         hipLaunchKernelGGL(kern, dim3(c.grid), dim3(c.bs), shmem, st, static_cast<const T *>(x),
@@ -788,7 +890,7 @@ void tune_ensure(const dmed_sector *s, int prec, hipStream_t st)
         std::vector<TuneChoice> cand{TuneChoice{}};
         for (int bs : {1024, 512, 256}) {
             if (col_rpt(s, prec, bs) == 0) continue;
-            const int fit = std::min(col_fit(s, prec, bs), 4);
+            const int fit = std::min(col_fit(s, prec, bs).best(), 4);
             for (int w = 1; w <= fit; ++w) cand.push_back(TuneChoice{0, 0, bs, w});
         }
         for (const TuneChoice &tc : cand) {
@@ -910,6 +1012,7 @@ int hxv_shape(const dmed_sector *s, int prec, dmed_hxv_shape *out)
         out->groups_per_cu = c.per_cu;
         out->grid = (int32_t)c.grid;
         out->tuned = c.tuned ? 1 : 0;
+        out->row_passes = c.halves ? 2 : 1;
     } else {
         const LaunchShape sh = tile_shape(s, prec, kPlain);
         out->kernel = 1;
@@ -919,6 +1022,7 @@ int hxv_shape(const dmed_sector *s, int prec, dmed_hxv_shape *out)
         out->groups_per_cu = 0;
         out->grid = (int32_t)sh.grid;
         out->tuned = tune_get(s, prec).cb > 0 ? 1 : 0;
+        out->row_passes = 1;
     }
     return 0;
 }

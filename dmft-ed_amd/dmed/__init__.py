@@ -253,11 +253,11 @@ class Sector:
     def hxv_shape(self, prec=FP64):
         """Launch shape of this sector's plain H*v (after the autotuner ran
         for it): kernel ("tile" or "column"), lanes, rows_per_lane,
-        columns_per_group, groups_per_cu, grid, tuned."""
-        v = np.zeros(7, dtype=np.int32)
+        columns_per_group, groups_per_cu, grid, tuned, row_passes."""
+        v = np.zeros(8, dtype=np.int32)
         _check(lib().dmed_hxv_shape_of(self.h, prec, v.ctypes.data_as(_i32p)))
         keys = ("kernel", "lanes", "rows_per_lane", "columns_per_group", "groups_per_cu", "grid",
-                "tuned")
+                "tuned", "row_passes")
         out = dict(zip(keys, (int(a) for a in v)))
         out["kernel"] = {1: "tile", 2: "column"}[out["kernel"]]
         out["tuned"] = bool(out["tuned"])

@@ -144,9 +144,12 @@ int dmed_hxv_timed(const dmed_sector *s, const void *x_dev, void *y_dev, int32_t
  * library's own choice).  kernel 1: the tile kernel (columns_per_group
  * columns staged in LDS per workgroup); kernel 2: the persistent column
  * kernel (one column in LDS, rows_per_lane rows per lane, groups_per_cu
- * workgroups per CU).  tuned = 1 when the shape came from the autotuner. */
+ * workgroups per CU; row_passes 2 when each lane takes its rows in two
+ * halves, the form that fits two workgroups per CU in registers).  tuned = 1
+ * when the shape came from the autotuner. */
 typedef struct dmed_hxv_shape {
     int32_t kernel, lanes, rows_per_lane, columns_per_group, groups_per_cu, grid, tuned;
+    int32_t row_passes;
 } dmed_hxv_shape;
 int dmed_hxv_shape_of(const dmed_sector *s, int32_t prec, dmed_hxv_shape *out);
 
