@@ -43,3 +43,29 @@ def test_traffic_scales_by_the_copy_kernel(tmp_path):
     assert abs(r["fp32_bytes_per_launch"] - (1000 * 16 + 4 * 1024)) < 1e-6
     assert r["fp64_algorithmic_bytes"] == 16 * dim
     assert "traffic / algorithmic" in (tmp_path / "t.md").read_text()
+
+
+def test_l2_hit_rates_per_kernel_form(tmp_path):
+    copy = "void dmed::k_scale_copy<double, double>(double*, double const*, long, double)"
+    h64 = "void dmed::k_hxv_col<double, 0, 1024, 13, 1>(double const*, ...)"
+    l32 = "void dmed::k_hxv_col<float, 1, 1024, 13, 2>(float const*, ...)"
+    d = tmp_path / "l2"
+    os.makedirs(d)
+    with open(d / "x_counter_collection.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, FIELDS)
+        w.writeheader()
+        rows = [(copy, "TCC_HIT_sum", 10.0), (copy, "TCC_MISS_sum", 30.0),
+                (h64, "TCC_HIT_sum", 50.0), (h64, "TCC_MISS_sum", 50.0),
+                (h64, "TCC_HIT_sum", 70.0), (h64, "TCC_MISS_sum", 30.0),
+                (l32, "TCC_HIT_sum", 3.0), (l32, "TCC_MISS_sum", 1.0)]
+        for i, (k, c, v) in enumerate(rows):
+            w.writerow({"Dispatch_Id": i, "Kernel_Name": k, "Counter_Name": c,
+                        "Counter_Value": v})
+    out = tmp_path / "l2.md"
+    rocprof.main(["l2", "--dir", str(d), "--out", str(out), "--cmd", "rocprofv3 ..."])
+    lines = {l.split("|")[2].strip(): l for l in out.read_text().splitlines()
+             if l.startswith("| `")}
+    assert lines["copy"].rstrip(" |").endswith("0.250")
+    assert lines["fp64"].rstrip(" |").endswith("0.600")          # (50+70)/(100+100)
+    assert lines["fp32 Lanczos"].rstrip(" |").endswith("0.750")
+    assert "fp32" not in lines                                     # no plain FP32 rows
