@@ -1048,7 +1048,10 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
         const bool refine = stage == kRefineStage;
         const int p0 = stage == kFp32Stage ? DMED_FP32 : DMED_FP64;
         void **bb = p0 == DMED_FP32 ? b32 : b64;
-        const double stol = stage == kFp32Stage ? std::max(tol, 1e-7) : tol;
+        // the FP32 stage only has to hand the refinement a vector with a
+        // residual near 1e-6: tighter FP32 restarts cost more steps than the
+        // refinement saves
+        const double stol = stage == kFp32Stage ? std::max(tol, 1e-6) : tol;
         // pass 1, in chunks of ncheck steps with a host convergence check
         if (int rc = launch_scale_copy(bb[1], p0, psi, DMED_FP64, n, 1.0, st))
             return cleanup(rc);
