@@ -126,11 +126,58 @@ def solve_compare(reps=3):
     return out
 
 
+def sector_pipeline_gpu(c):
+    """Ground state of the workload's sector to residual 1e-12 (FP64), then
+    the particle and hole chains of every orbital (FP64, c["nlanc"] steps):
+    seconds, E0 and the chain lengths."""
+    import dmed
+    nu, nd = c["sector"]
+    s = dmed.Sector(nu, nd)
+    t0 = time.perf_counter()
+    e0, psi, r0, rc = s.ground_state(prec=dmed.FP64, tol=1e-12, maxit=6000)
+    steps = []
+    for a in range(1, c["norb"] + 1):
+        for dag, dn in ((1, 1), (0, -1)):
+            dst = dmed.Sector(nu + dn, nd)
+            v, n2 = dmed.apply_op(psi, a, dag, dst)
+            al, bl, n = dmed.chain(dst, v, c["nlanc"], prec=dmed.FP64)
+            steps.append(n)
+            v.free()
+            dst.free()
+    dt = time.perf_counter() - t0
+    psi.free()
+    s.free()
+    return dt, e0, steps
+
+
+def sector_pipeline_oracle(c, threads):
+    """The same pipeline on the oracle: the reference's plain Lanczos ground
+    state (lanczos_plain_c stop rule) and sp_lanc_tridiag chains, on `threads`
+    OpenMP threads."""
+    import numpy as np
+    import edref
+    nu, nd = c["sector"]
+    m = edref.build_sector(nu, nd)
+    t0 = time.perf_counter()
+    v0 = np.random.default_rng(7).standard_normal(len(m))
+    egs, psi = edref.lanczos_gs(m, v0 / np.linalg.norm(v0), nitermax=1000, nthreads=threads)
+    for a in range(1, c["norb"] + 1):
+        for dag, dn in ((1, 1), (0, -1)):
+            mj = edref.build_sector(nu + dn, nd)
+            v = edref.apply_op(m, psi, mj, a, dag)
+            edref.lanc_tridiag(mj, v / np.linalg.norm(v), c["nlanc"], nthreads=threads)
+    return time.perf_counter() - t0, float(egs[0])
+
+
 def configs_table(threads):
     """Workloads 1-3 beside the oracle on the host cores: GPU FP64 H*v of the
     ground-state sector (HIP events) against one full oracle H*v (the
-    reference's map + binary-search algorithm, `threads` OpenMP threads), and
-    for workload 1 the whole ed_solve on both (T=0, Lmats=4096, 200-step chains)."""
+    reference's map + binary-search algorithm, `threads` OpenMP threads); for
+    workload 1 the whole ed_solve on both (T=0, Lmats=4096, 200-step chains);
+    for workloads 2 and 3 the sector pipeline (ground state, then the particle
+    and hole chains of every orbital) on the GPU, and for workload 2 on the
+    oracle too (workload 3's would take about half an hour: its H*v count
+    times the oracle H*v time gives the estimate)."""
     import ctypes as C
     import numpy as np
     import dmed
@@ -177,9 +224,21 @@ def configs_table(threads):
                        "(dense/eigsh sector eigensolves, C chains, 1 thread)",
                        solve_E0_rel=abs(r["egs"] - o["egs"]) / abs(o["egs"]),
                        solve_Sigma_iw=float(np.abs(r["smats"][0, 0, 0, 0] - o["smats"][0, 0]).max()))
+        if n in (2, 3):
+            dmed.set_model(*args)
+            tg, e0, steps = sector_pipeline_gpu(c)
+            row.update(gpu_pipeline_s=tg, gpu_pipeline_E0=e0, chain_steps=steps)
+            if n == 2:
+                edref.set_model(*args)
+                to, eo = sector_pipeline_oracle(c, threads)
+                row.update(cpu_pipeline_s=to, cpu_pipeline_threads=threads,
+                           pipeline_E0_rel=abs(e0 - eo) / abs(eo))
         log(f"{c['name']}: GPU H*v {row['gpu_hxv_ms']:.4f} ms, oracle H*v {row['cpu_hxv_ms']:.1f} ms"
             + (f"; solve GPU {row['gpu_solve_s']:.2f}s vs oracle {row['cpu_solve_s']:.2f}s"
-               if n == 1 else ""))
+               if n == 1 else "")
+            + (f"; sector pipeline GPU {row['gpu_pipeline_s']:.2f}s" if n in (2, 3) else "")
+            + (f" vs oracle {row['cpu_pipeline_s']:.1f}s (dE0 {row['pipeline_E0_rel']:.1e})"
+               if n == 2 else ""))
         out[f"workload{n}"] = row
     return out
 
