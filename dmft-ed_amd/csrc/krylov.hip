@@ -117,26 +117,26 @@ __global__ void __launch_bounds__(1024) k_chain_alpha(const double *__restrict__
     }
 }
 
+// 16-byte vectors of T
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { using type = double2; };
+template <> struct Vec16<float> { using type = float4; };
+
 // w = Hv - b*v_prev - alpha*v in place of y = Hv (stored scales sx, svp), with
 // the block partials of ||w||^2 (first gridDim.x) and <w, x> (next gridDim.x).
 // Each thread takes the elements i = tid + k*stride in increasing order, as a
 // plain grid-stride loop would, with four of them loaded before the first is
 // used; the partial sums (hence beta) do not depend on that grouping.
-// psi != nullptr (second pass of the ground-state Lanczos): also
-// psi += z_k * sx * v, with the v already loaded for the update.
 template <typename T>
 __global__ void __launch_bounds__(kVecBS) k_chain_update(T *__restrict__ w, const T *__restrict__ v,
                                                          const T *__restrict__ vp, int64_t n,
                                                          const ChainScalars *cs,
-                                                         double *__restrict__ partials,
-                                                         double *__restrict__ psi,
-                                                         const double *__restrict__ z)
+                                                         double *__restrict__ partials)
 {
     constexpr int U = 4;
     __shared__ double red[16];
     if (cs->done) return;
     const double ca = cs->alpha * cs->sx, cb = cs->b * cs->svp;
-    const double cz = psi ? z[cs->step] * cs->sx : 0.0;
     double s1 = 0.0, s2 = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += U * stride) {
@@ -154,7 +154,6 @@ __global__ void __launch_bounds__(kVecBS) k_chain_update(T *__restrict__ w, cons
             const double vi = (double)b[k];
             const T wt = (T)((double)a[k] - cb * (double)c[k] - ca * vi);
             w[i + k * stride] = wt;
-            if (psi) psi[i + k * stride] += cz * vi;
             const double wi = (double)wt;
             s1 += wi * wi;
             s2 += wi * vi;
@@ -193,6 +192,36 @@ __global__ void __launch_bounds__(1024) k_chain_beta(const double *__restrict__ 
         cs->step = k + 1;
         if (fabs(b) < cs->thr || k + 1 >= cs->nlanc) cs->done = 1;
     }
+}
+
+// psi += z_k * sx * v  (second pass of the ground-state Lanczos); 16-byte
+// loads of v and of psi (psi and v 16-byte aligned), the last n % (16/sizeof(T))
+// elements one per thread
+template <typename T>
+__global__ void __launch_bounds__(kVecBS) k_accum(double *__restrict__ psi, const T *__restrict__ v,
+                                                  int64_t n, const ChainScalars *cs,
+                                                  const double *__restrict__ z)
+{
+    using V = typename Vec16<T>::type;
+    constexpr int E = 16 / sizeof(T);
+    const double c = z[cs->step] * cs->sx;
+    const int64_t nv = n / E;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const V *vv = reinterpret_cast<const V *>(v);
+    double2 *pp = reinterpret_cast<double2 *>(psi);
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += stride) {
+        const V b = vv[j];
+        const T *y = reinterpret_cast<const T *>(&b);
+#pragma unroll
+        for (int k = 0; k < E / 2; ++k) {
+            double2 p = pp[j * (E / 2) + k];
+            p.x += c * (double)y[2 * k];
+            p.y += c * (double)y[2 * k + 1];
+            pp[j * (E / 2) + k] = p;
+        }
+    }
+    const int64_t t = nv * E + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) psi[t] += c * (double)v[t];
 }
 
 template <typename TD, typename TS>
@@ -608,24 +637,31 @@ int chain_steps(const dmed_sector *s, void *bufs[3], int prec, int count, double
     const unsigned g = vec_grid(n);
     // the kernels of one step; no memsets, no host synchronisation
     auto one = [&](void *vp, void *v, void *w) -> int {
+        if (psi) {
+            if (prec == DMED_FP32)
+                hipLaunchKernelGGL(k_accum<float>, dim3(g), dim3(kVecBS), 0, st, psi,
+                                   (const float *)v, n, cs, chain_aux());
+            else
+                hipLaunchKernelGGL(k_accum<double>, dim3(g), dim3(kVecBS), 0, st, psi,
+                                   (const double *)v, n, cs, chain_aux());
+        }
         if (int rc = launch_hxv(s, v, w, prec, kLanczos, cs, partials, nullptr, st))
             return rc;
         hipLaunchKernelGGL(k_chain_alpha, dim3(1), dim3(1024), 0, st, partials, nph, cs,
                            chain_alpha());
-        // psi (pass 2 of the ground state) takes z_k * v_k in the update
         if (prec == DMED_FP32)
             hipLaunchKernelGGL(k_chain_update<float>, dim3(g), dim3(kVecBS), 0, st, (float *)w,
-                               (const float *)v, (const float *)vp, n, cs, partials, psi,
-                               (const double *)chain_aux());
+                               (const float *)v, (const float *)vp, n, cs, partials);
         else
             hipLaunchKernelGGL(k_chain_update<double>, dim3(g), dim3(kVecBS), 0, st, (double *)w,
-                               (const double *)v, (const double *)vp, n, cs, partials, psi,
-                               (const double *)chain_aux());
+                               (const double *)v, (const double *)vp, n, cs, partials);
         hipLaunchKernelGGL(k_chain_beta, dim3(1), dim3(1024), 0, st, partials, (int)g, cs,
                            chain_beta());
         return hipGetLastError() == hipSuccess ? 0 : fail(DMED_EHIP, "chain step launch failed");
     };
     void *vp = bufs[0], *v = bufs[1], *w = bufs[2];
+    if (((uintptr_t)vp | (uintptr_t)v | (uintptr_t)w | (uintptr_t)psi) % 16 != 0)
+        return fail(DMED_EINVAL, "chain work vectors must be 16-byte aligned");
     auto rotate = [&]() {
         void *t = vp;
         vp = v;
