@@ -289,11 +289,23 @@ int launch_zero(void *p, size_t bytes, hipStream_t st)
 constexpr int kGsGrid = 512;
 constexpr int kGsMaxRows = 160;     // Y and V rows (nl + m <= 64 + 64) plus slack
 
-// sum of n partials in index order, valid in every thread
+// at most this many partials per thread and row when n <= kGsGrid and the
+// block has kVecBS threads: their loads are all issued before the first add
+constexpr int kGsPer = (kGsGrid + kVecBS - 1) / kVecBS;
+
+// sum of 1 <= n <= kGsGrid partials in index order, valid in every thread
+// (kVecBS threads per block).  The loads are unconditional, at indices
+// clamped into range (a select between a load and a constant would make the
+// compiler wait for each load); only the in-range ones are added.
 __device__ __forceinline__ double psum_all(const double *__restrict__ p, int n, double *red)
 {
+    double v[kGsPer];
+#pragma unroll
+    for (int k = 0; k < kGsPer; ++k) v[k] = p[min((int)threadIdx.x + k * kVecBS, n - 1)];
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+#pragma unroll
+    for (int k = 0; k < kGsPer; ++k)
+        if ((int)threadIdx.x + k * kVecBS < n) s += v[k];
     s = bsum(s, red);
     __shared__ double out;
     if (threadIdx.x == 0) out = s;
@@ -303,21 +315,30 @@ __device__ __forceinline__ double psum_all(const double *__restrict__ p, int n, 
     return s;
 }
 
-// c[r] = sum of the g partials of row r of P (row stride g), r < nr, into
-// shared c; sixteen rows at a time with all their loads in flight, reduced
-// by wave shuffles and one pass over the waves (a fixed order for a given
-// g and block size).  red: shared, >= 16 * waves per block.
+// c[r] = sum of the 1 <= g <= kGsGrid partials of row r of P (row stride
+// g), r < nr, into shared c; sixteen rows at a time with all their loads
+// issued before the first add (unconditional, at clamped indices, as in
+// psum_all), each thread summing its in-range partials in index order,
+// reduced by wave shuffles and one pass over the waves (a fixed order for a
+// given g).  kVecBS threads per block; red: shared, >= 16 * waves per block.
 __device__ __forceinline__ void rowsums(const double *__restrict__ P, int nr, int g,
                                         double *c, double *red)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = kVecBS >> 6;
     for (int r0 = 0; r0 < nr; r0 += 16) {
+        double v[16][kGsPer];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+#pragma unroll
+            for (int k = 0; k < kGsPer; ++k)
+                v[q][k] = P[(int64_t)min(r0 + q, nr - 1) * g + min((int)threadIdx.x + k * kVecBS, g - 1)];
         double s[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             s[q] = 0.0;
-            if (r0 + q < nr)
-                for (int i = threadIdx.x; i < g; i += blockDim.x) s[q] += P[(int64_t)(r0 + q) * g + i];
+#pragma unroll
+            for (int k = 0; k < kGsPer; ++k)
+                if ((int)threadIdx.x + k * kVecBS < g) s[q] += v[q][k];
         }
 #pragma unroll
         for (int q = 0; q < 16; ++q) s[q] = wsum(s[q]);
