@@ -393,12 +393,19 @@ __global__ void __launch_bounds__(kVecBS) k_proj_dot(
     double s[kProjRows];
 #pragma unroll
     for (int q = 0; q < kProjRows; ++q) s[q] = 0.0;
+    // the loads of w and of the block's rows are all issued before the first
+    // product (a load and its use under one condition make the compiler
+    // wait for each load in turn)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const double wi = w[i];
+        double x[kProjRows];
 #pragma unroll
         for (int q = 0; q < kProjRows; ++q)
-            if (q < nr) s[q] += b[q][i] * wi;
+            if (q < nr) x[q] = b[q][i];
+#pragma unroll
+        for (int q = 0; q < kProjRows; ++q)
+            if (q < nr) s[q] += x[q] * wi;
     }
 #pragma unroll
     for (int q = 0; q < kProjRows; ++q) {
@@ -432,13 +439,25 @@ __global__ void __launch_bounds__(kVecBS) k_proj_axpy(
     if (blockIdx.x == 0 && acc)
         for (int r = ny + (int)threadIdx.x; r < nr; r += blockDim.x)
             acc[r - ny] = first ? c[r] : acc[r - ny] + c[r];
+    // rows in chunks of kProjRows, the loads of a chunk all issued before
+    // its first product; the terms are added in row order
     double s2 = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
+        const double wv = w[i];
         double s = 0.0;
-        for (int j = 0; j < ny; ++j) s += Y[(int64_t)j * n + i] * c[j];
-        for (int j = 0; j < nv; ++j) s += V[(int64_t)j * n + i] * c[ny + j];
-        const double x = w[i] - s;
+        for (int j0 = 0; j0 < nr; j0 += kProjRows) {
+            double x[kProjRows];
+#pragma unroll
+            for (int q = 0; q < kProjRows; ++q) {
+                const int j = j0 + q;
+                if (j < nr) x[q] = (j < ny ? Y + (int64_t)j * n : V + (int64_t)(j - ny) * n)[i];
+            }
+#pragma unroll
+            for (int q = 0; q < kProjRows; ++q)
+                if (j0 + q < nr) s += x[q] * c[j0 + q];
+        }
+        const double x = wv - s;
         w[i] = x;
         s2 += x * x;
     }
