@@ -465,6 +465,49 @@ __global__ void __launch_bounds__(kVecBS) k_proj_axpy(
     if (threadIdx.x == 0) Pn[blockIdx.x] = s2;
 }
 
+// Lanczos three-term step of a thick-restart expansion step j (not the first
+// of a restart cycle): w -= alpha v_j + beta v_{j-1} before the full
+// Gram-Schmidt pass, with alpha = <v_j, H v_j> summed from the np partials
+// that the Lanczos-mode H*v left and beta = *bprev, the norm that normalised
+// v_j.  In exact arithmetic H v_j has no other component in the basis, so
+// the full pass that follows removes only rounding-level remainders and
+// rarely cancels enough to need its second pass.  Block 0 stores alpha and
+// beta at acc[j] and acc[j-1] (0 below); the full pass adds its
+// coefficients to them.  np >= 1.
+__global__ void __launch_bounds__(kVecBS) k_trl_local(double *__restrict__ w,
+                                                      const double *__restrict__ vj,
+                                                      const double *__restrict__ vjm1, int64_t n,
+                                                      const double *__restrict__ pa, int np,
+                                                      const double *__restrict__ bprev,
+                                                      double *__restrict__ acc, int j)
+{
+    __shared__ double red[16];
+    __shared__ double out;
+    // alpha: each thread sums its partials in index order, four loads in
+    // flight (unconditional, at clamped indices)
+    double s = 0.0;
+    for (int i0 = 0; i0 < np; i0 += 4 * kVecBS) {
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = pa[min(i0 + (int)threadIdx.x + k * kVecBS, np - 1)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i0 + (int)threadIdx.x + k * kVecBS < np) s += v[k];
+    }
+    s = bsum(s, red);
+    if (threadIdx.x == 0) out = s;
+    __syncthreads();
+    const double al = out, b = *bprev;
+    if (blockIdx.x == 0)
+        for (int r = threadIdx.x; r <= j; r += blockDim.x)
+            acc[r] = r == j ? al : (r == j - 1 ? b : 0.0);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = w[i], x = vj[i], y = vjm1[i];
+        w[i] = a - al * x - b * y;
+    }
+}
+
 // dst = w / ||w||, with ||w||^2 the sum of the partials p1 (after the first
 // pass) or, when the second pass ran (same test as above on n0, p1), p2;
 // 0 when the norm is 0.  Block 0 stores the norm at *nrm.  Without a
@@ -1189,29 +1232,40 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
     if (int rc = exec_buf(ex(), kSlotTrlBasis, (size_t)(m + 1) * n * 8, (void **)&V)) return rc;
     if (int rc = exec_buf(ex(), kSlotTrlW, n * 8, (void **)&w)) return rc;
     // dS: Ritz rotation (m*m), and per expansion step j the accumulated
-    // CGS2 coefficients (j+1) and the new norm, in rows of m+2 (kept on the
-    // device until the restart)
+    // coefficients (j+1) and the new norm, in rows of m+2 (kept on the
+    // device until the restart); then the scalars of the Lanczos-mode H*v
+    // (scale 1: y = Hx, with the partials of <x, y>)
     const size_t ldc = (size_t)m + 2;
     if (nl + m > kGsMaxRows) return fail(DMED_EINVAL, "too many projection rows");
-    if (int rc = exec_buf(ex(), kSlotTrlSmall, ((size_t)m * m + (size_t)m * ldc + 4) * 8,
-                          (void **)&dS))
+    const size_t csoff = (size_t)m * m + (size_t)m * ldc + 8;
+    if (int rc = exec_buf(ex(), kSlotTrlSmall, csoff * 8 + sizeof(ChainScalars), (void **)&dS))
         return rc;
     double *dcoef = dS + (size_t)m * m;
+    ChainScalars *tcs = reinterpret_cast<ChainScalars *>(dS + csoff);
+    {
+        ChainScalars h{};
+        h.sx = 1.0;
+        if (int rc = copy_h2d(tcs, &h, sizeof h)) return rc;
+    }
     // partial sums of one orthogonalisation: rows of the first pass
     // (nl + m + 1, the last one ||w||^2 before it), ||w||^2 after it, rows
-    // of the second pass, ||w||^2 after that
+    // of the second pass, ||w||^2 after that; then the <v_j, H v_j> partials
+    // of the H*v
     const unsigned g = (unsigned)std::min<unsigned>(vec_grid(n), kGsGrid);
     const int nr = nl + m;
-    if (int rc = ensure_partials((size_t)(2 * nr + 3) * g + 16)) return rc;
+    const int nph = hxv_nparts(s, DMED_FP64);
+    if (int rc = ensure_partials((size_t)(2 * nr + 3) * g + (size_t)nph + 16)) return rc;
     double *P1 = ex().partials, *Pn1 = P1 + (size_t)(nr + 1) * g, *P2 = Pn1 + g, *Pn2 = P2 + (size_t)nr * g;
+    double *Pa = Pn2 + g;
     // classical Gram-Schmidt of w against Y[0..nl) and V[0..jmax), repeated
     // once when the first pass cancelled more than half of ||w||^2 (Daniel-
     // Gragg-Kaufman-Stewart criterion, decided on the device); the V
-    // coefficients of both passes are accumulated at cacc when given.  Four
-    // launches, no host sync; normalize_into then needs no reduction launch.
+    // coefficients of both passes are stored at cacc when given (first:
+    // overwrite, else add to what is there).  Four launches, no host sync;
+    // normalize_into then needs no reduction launch.
     bool noproj = false;
     const double *n0_last = nullptr;
-    auto orth = [&](int jmax, double *cacc) -> int {
+    auto orth = [&](int jmax, double *cacc, int first) -> int {
         const int rows = nl + jmax;
         noproj = rows <= 0;
         if (noproj) {
@@ -1227,7 +1281,7 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
                            nl, V, jmax, w, n, P1, 0, (const double *)nullptr,
                            (const double *)nullptr, rows + 1, rpb);
         hipLaunchKernelGGL(k_proj_axpy, dim3(g), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n, P1, Pn1,
-                           cacc, 1, 0, (const double *)nullptr, (const double *)nullptr);
+                           cacc, first, 0, (const double *)nullptr, (const double *)nullptr);
         hipLaunchKernelGGL(k_proj_dot, dim3(g, proj_groups(rows, rpb)), dim3(kVecBS), 0, st, Y, nl,
                            V, jmax, w, n, P2, 1, n0, Pn1, rows, rpb);
         hipLaunchKernelGGL(k_proj_axpy, dim3(g), dim3(kVecBS), 0, st, Y, nl, V, jmax, w, n, P2, Pn2,
@@ -1246,7 +1300,7 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
     std::vector<double> T((size_t)m * m, 0.0), S, theta(m), hc((size_t)m * ldc);
     // start vector in the complement of Y
     if (int rc = launch_fill_random(w, n, DMED_FP64, seed, st)) return rc;
-    if (int rc = orth(0, nullptr)) return rc;
+    if (int rc = orth(0, nullptr, 1)) return rc;
     if (int rc = normalize_into(V, dcoef + m + 1)) return rc;
     int k = 0, used = 0;
     double beta_last = 0.0;
@@ -1254,13 +1308,26 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
     for (;;) {
         // expansion k..m-1 entirely on the stream; one copy of the
         // coefficients per restart
+        // expansion step j > k: H v_j has, in exact arithmetic, only the
+        // components alpha_j v_j and beta_{j-1} v_{j-1} in the basis; they are
+        // taken off first (k_trl_local), so the full pass that follows sees no
+        // large cancellation and its second pass seldom runs.  Step k (the
+        // arrow of the kept Ritz vectors, or the first step) goes straight to
+        // the full passes.
         for (int j = k; j < m; ++j) {
             double *vj = V + (size_t)j * n;
-            if (int rc = launch_hxv(s, vj, w, DMED_FP64, kPlain, nullptr, nullptr, nullptr, st))
+            const bool local = j > k;
+            if (int rc = launch_hxv(s, vj, w, DMED_FP64, local ? kLanczos : kPlain,
+                                    local ? tcs : nullptr, local ? Pa : nullptr, nullptr, st))
                 return rc;
             ++used;
             double *cj = dcoef + (size_t)j * ldc;
-            if (int rc = orth(j + 1, cj)) return rc;
+            if (local) {
+                hipLaunchKernelGGL(k_trl_local, dim3(g), dim3(kVecBS), 0, st, w, vj, vj - n, n, Pa,
+                                   nph, dcoef + (size_t)(j - 1) * ldc + m + 1, cj, j);
+                DMED_HIP(hipGetLastError());
+            }
+            if (int rc = orth(j + 1, cj, local ? 0 : 1)) return rc;
             if (int rc = normalize_into(V + (size_t)(j + 1) * n, cj + m + 1)) return rc;
         }
         DMED_HIP(hipMemcpyAsync(hc.data() + (size_t)k * ldc, dcoef + (size_t)k * ldc,

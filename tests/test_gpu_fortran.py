@@ -80,8 +80,15 @@ def test_bethe_driver_one_iteration(tmp_path):
     wr = np.linspace(-5.0, 5.0, 100)
     sr = np.loadtxt(tmp_path / "impSigma_l11_s1_realw.ed")
     assert np.allclose(sr[:, 0], wr, rtol=0, atol=1e-12)
+    # Sigma(w + 0.01i) = G0^-1 - G^-1 from 200-step chains is not reproducible
+    # to better than a few 1e-3 (relative, in norm): perturbing the oracle's own
+    # ground state by 1e-15 relative moves its real-axis Sigma by 1.2e-3 to
+    # 3.2e-3 (and G by up to 6e-4) while Sigma(i w_n) moves by 1e-11 at most,
+    # because the late chain coefficients that a broadening of 0.01 resolves
+    # follow rounding.  The bound sits above that floor; Sigma(i w_n) is
+    # checked to 1e-8 elsewhere.
     x, y = sr[:, 2] + 1j * sr[:, 1], o["sreal"][0, 0]
-    assert np.linalg.norm(x - y) / max(np.linalg.norm(y), np.sqrt(len(y))) < 1e-3
+    assert np.linalg.norm(x - y) / max(np.linalg.norm(y), np.sqrt(len(y))) < 1e-2
     e_b, v_b = bath[:c["nbath"]], bath[c["nbath"]:]
     for name, z in (("impG_l11_s1_iw.ed", 1j * wm), ("impG_l11_s1_realw.ed", wr + 0.01j)):
         gf = np.loadtxt(tmp_path / name)
