@@ -185,6 +185,35 @@ def test_eigh_vs_golden_spectra(name):
             k += 1
 
 
+@pytest.mark.parametrize("name", ["m3_norb2_nb2_jh", "m4_norb3_nb1", "m5_norb1_nb5_T"])
+def test_eigh_small_basis_many_restarts(name):
+    """Thick restart with the smallest basis the solver takes for 4 states
+    (ncv = 8: each restart keeps 6 Ritz vectors and adds 2), so a solve runs
+    through about two hundred restarts. Every restart cycle starts with the
+    arrow step, which takes the full Gram-Schmidt passes, and continues with
+    three-term steps. That exercises the bookkeeping of the coefficients
+    across restarts. Sector (3,3) of three Ns = 6 fixtures (400 states; m4 has
+    a degenerate pair) against the fixtures' exact spectra."""
+    g = load(name)
+    ns = set_both(g)
+    eoffs = g["eig_offsets"]
+    nu = nd = 3
+    k = nu * (ns + 1) + nd
+    ref = g["eigs"][eoffs[k]:eoffs[k + 1]]
+    s = dmed.Sector(nu, nd)
+    assert s.dim == 400
+    say(f"[{name}] eigh (3,3) ncv=8")
+    ev, vecs, res, rc = s.eigh(4, ncv=8, tol=1e-12, dense_threshold=0)
+    assert rc == 0
+    assert np.allclose(ev, ref[:4], rtol=0, atol=1e-10 * max(1, abs(ref[0])))
+    assert res.max() <= 1e-12 * max(1, np.abs(ev).max())
+    X = np.array([v.download() for v in vecs])
+    assert np.abs(X @ X.T - np.eye(4)).max() < 1e-12
+    for v in vecs:
+        v.free()
+    s.free()
+
+
 @pytest.mark.parametrize("cfg", [1, 2])
 def test_ground_state_energy_vs_oracle(cfg):
     c = configs.config(cfg)
