@@ -8,8 +8,8 @@
 // step, and the kernels are far too small to fill the chip.  Here one launch
 // takes a queue of such sectors (largest first): each 1024-lane workgroup
 // pops a sector from an atomic counter and runs the whole solve for it --
-// H*v, classical Gram-Schmidt twice with the Daniel-Gragg-Kaufman-Stewart
-// test, restarts, the Rayleigh-Ritz eigenproblem of the projected matrix
+// H*v, the three-term step and classical Gram-Schmidt with a second pass on
+// the Daniel-Gragg-Kaufman-Stewart test, restarts, the Rayleigh-Ritz eigenproblem of the projected matrix
 // (parallel cyclic Jacobi on the matrix in LDS), the deflation rounds and the
 // explicit residuals -- with the Krylov basis in its own slice of HBM, which
 // stays in the XCD's L2 at these sizes.  Workgroups leave when the queue is
@@ -17,9 +17,9 @@
 // Jacobi sweeps, deflation rounds by neig + 1).
 //
 // The algorithm is the one of trl_run / trl_deflated (krylov.hip) step for
-// step: the same start vectors, Gram-Schmidt passes, restart rule,
-// convergence test and keep/evict rules; only the dense eigensolver of the
-// projected matrix differs (Jacobi here, Householder + QL on the host), so
+// step: the same start vectors, three-term steps, Gram-Schmidt passes,
+// restart rule, convergence test and keep/evict rules; only the dense
+// eigensolver of the projected matrix differs (Jacobi here, Householder + QL on the host), so
 // results agree to the solver tolerance rather than bit for bit.
 #include "dmed_internal.h"
 
@@ -388,11 +388,31 @@ __device__ int wg_trl(const SqTask &t, int want, int m, int maxit, double tol, i
     bool converged = false;
     for (;;) {
         for (int j = k; j < m; ++j) {
-            wg_hxv(t, V + (size_t)j * n, w);
+            const double *vj = V + (size_t)j * n;
+            wg_hxv(t, vj, w);
             ++used;
+            // step j > k: the three-term step first, w -= alpha v_j +
+            // beta_{j-1} v_{j-1} (as trl_run does), so that the full pass
+            // seldom needs its second pass
+            double al = 0.0;
+            const double bprev = beta_last;
+            if (j > k) {
+                const double *vjm = vj - n;
+                double s = 0.0;
+                for (int e = threadIdx.x; e < n; e += kQBlock) s += vj[e] * w[e];
+                al = wg_sum(s, sh);
+                for (int e = threadIdx.x; e < n; e += kQBlock)
+                    w[e] = w[e] - al * vj[e] - bprev * vjm[e];
+                __syncthreads();
+            }
             // coefficients on V[0..j] land in sh.A's first column as scratch
             double *cj = sh.A;
             const double nn = wg_orth(t.Y, nl, V, j + 1, w, n, cj, sh);
+            if (j > k && threadIdx.x == 0) {
+                cj[j] += al;
+                cj[j - 1] += bprev;
+            }
+            __syncthreads();
             wg_normalize(V + (size_t)(j + 1) * n, w, n, nn);
             const double bn = sqrt(nn);
             if (threadIdx.x == 0) {
