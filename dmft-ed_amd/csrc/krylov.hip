@@ -1055,7 +1055,7 @@ int dense_eigh(const dmed_sector *s, int neig, double *evals, dmed_vec **evecs, 
 // explicit residual ||H y - theta y|| for a device FP64 vector y
 int residual(const dmed_sector *s, const double *y, double theta, double *work, double *out)
 {
-        hipStream_t st = ex().stream;
+    hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     if (int rc = launch_hxv(s, y, work, DMED_FP64, kPlain, nullptr, nullptr, nullptr, st))
         return rc;
@@ -1212,15 +1212,18 @@ int plain_gs(const dmed_sector *s, int maxit, double tol, double *eval, dmed_vec
     return res <= tol * std::max(1.0, fabs(theta)) ? 0 : DMED_ENOCONV;
 }
 
-// Thick-restart Lanczos with full (twice classical Gram-Schmidt)
-// reorthogonalisation in a device basis, for the neig lowest eigenpairs of H
-// restricted to the orthogonal complement of nl locked orthonormal vectors Y
-// (Wu & Simon, SIAM J. Matrix Anal. Appl. 22 (2000) 602).  Ritz vectors are
-// written to X (neig contiguous device vectors).
+// Thick-restart Lanczos with full reorthogonalisation in a device basis, for
+// the neig lowest eigenpairs of H restricted to the orthogonal complement of
+// nl locked orthonormal vectors Y (Wu & Simon, SIAM J. Matrix Anal. Appl. 22
+// (2000) 602).  Each expansion step after the first of a restart cycle takes
+// the three-term Lanczos step, then one classical Gram-Schmidt pass against
+// Y and the basis, and a second pass when the first cancelled more than half
+// of ||w||^2; the first step of a cycle takes the two passes directly.  Ritz
+// vectors are written to X (neig contiguous device vectors).
 int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const double *Y,
             int nl, uint64_t seed, double *theta_out, double *X, double *resid_out, int *used_out)
 {
-        hipStream_t st = ex().stream;
+    hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     const int64_t avail = n - nl;                 // dimension of the complement
     if (avail < neig) return fail(DMED_EINVAL, "complement smaller than neig");
@@ -1398,7 +1401,7 @@ int trl_run(const dmed_sector *s, int neig, int m, int maxit, double tol, const 
 int trl_deflated(const dmed_sector *s, int neig, int m, int maxit, double tol, double degen,
                  bool lowest_multiplet, int *nout, double *evals, dmed_vec **evecs, double *resid)
 {
-        hipStream_t st = ex().stream;
+    hipStream_t st = ex().stream;
     const int64_t n = s->dim;
     double *Y = nullptr, *X = nullptr;
     if (int rc = exec_buf(ex(), kSlotDeflY, (size_t)neig * n * 8, (void **)&Y)) return rc;

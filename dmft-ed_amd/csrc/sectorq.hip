@@ -4,8 +4,8 @@
 // The per-sector work of ed_diag_c (ED_DIAG.f90:115-228) for a sector of
 // dimension D is a thick-restart Lanczos with full reorthogonalisation and
 // multiplet-completing deflation rounds.  Driven from the host, a sector of a
-// few thousand states costs six launches and a host round trip per Krylov
-// step, and the kernels are far too small to fill the chip.  Here one launch
+// few thousand states costs seven launches per Krylov step and a host round
+// trip per restart, and the kernels are far too small to fill the chip.  Here one launch
 // takes a queue of such sectors (largest first): each 1024-lane workgroup
 // pops a sector from an atomic counter and runs the whole solve for it --
 // H*v, the three-term step and classical Gram-Schmidt with a second pass on
