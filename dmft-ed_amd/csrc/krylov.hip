@@ -69,13 +69,32 @@ __global__ void __launch_bounds__(kVecBS) k_dot(const TX *__restrict__ x, const 
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// per-thread sum of p[threadIdx.x + k*blockDim.x], k increasing (the order
+// of a plain strided loop), with eight loads in flight: they are issued
+// unconditionally at indices clamped into range, and only the in-range ones
+// are added (a loop that loads and adds one element at a time waits for
+// each load in turn)
+__device__ __forceinline__ double strided_sum(const double *__restrict__ p, int n)
+{
+    double s = 0.0;
+    const int bd = blockDim.x;
+    for (int i0 = threadIdx.x; i0 < n; i0 += 8 * bd) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = p[min(i0 + k * bd, n - 1)];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (i0 + k * bd < n) s += v[k];
+    }
+    return s;
+}
+
 // fixed-order final sum; op 0: plain sum, 1: sqrt of sum
 __global__ void __launch_bounds__(1024) k_reduce(const double *__restrict__ p, int n,
                                                  double *__restrict__ out, int op)
 {
     __shared__ double red[16];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    double s = strided_sum(p, n);
     s = bsum(s, red);
     if (threadIdx.x == 0) *out = op == 1 ? sqrt(s) : s;
 }
@@ -85,8 +104,7 @@ __global__ void __launch_bounds__(1024) k_chain_init(const double *__restrict__ 
                                                      ChainScalars *cs, double thr, int nlanc)
 {
     __shared__ double red[16];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    double s = strided_sum(p, n);
     s = bsum(s, red);
     if (threadIdx.x == 0) {
         cs->sx = 1.0 / sqrt(s);
@@ -106,8 +124,7 @@ __global__ void __launch_bounds__(1024) k_chain_alpha(const double *__restrict__
 {
     __shared__ double red[16];
     if (cs->done) return;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    double s = strided_sum(p, n);
     s = bsum(s, red);
     if (threadIdx.x == 0) {
         // alpha = <v, Hv - b v_prev> = sx*(<x, y> - b*svp*<x, x_prev>)
@@ -173,11 +190,7 @@ __global__ void __launch_bounds__(1024) k_chain_beta(const double *__restrict__ 
 {
     __shared__ double red[16];
     if (cs->done) return;
-    double s = 0.0, q = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        s += p[i];
-        q += p[n + i];
-    }
+    double s = strided_sum(p, n), q = strided_sum(p + n, n);
     s = bsum(s, red);
     __syncthreads();
     q = bsum(q, red);
