@@ -153,6 +153,7 @@ int dmed_finalize(void)
 {
     Context &c = ctx();
     if (!c.initialized) return 0;
+    exec_pool_clear();
     exec_destroy(c.exec);
     c.initialized = false;
     c.has_model = false;

@@ -138,6 +138,7 @@ struct Exec {
 };
 int exec_create(Exec &e);
 void exec_destroy(Exec &e);
+void exec_pool_clear();                 // destroys the idle worker contexts (solve.cpp)
 Exec &ex();                             // context of the calling thread
 void bind_exec(Exec *e);                // nullptr: back to the default context
 int exec_buf(Exec &e, int slot, size_t bytes, void **p);

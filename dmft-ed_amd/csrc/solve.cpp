@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "dmed_internal.h"
@@ -110,10 +111,69 @@ inline size_t gidx(int s1, int s2, int a, int b, int l, int nspin, int norb)
     return (size_t)s1 + nspin * ((size_t)s2 + nspin * ((size_t)a + norb * ((size_t)b + (size_t)norb * l)));
 }
 
-// Run ntask tasks: serially on the calling thread, or (batched) on a pool of
-// host workers, each bound to its own stream/scratch context, taking tasks
-// from a shared counter.  Task results must not depend on which worker ran
-// them.
+// Worker execution contexts, kept between calls.  Their streams and
+// grow-only buffers are reused, so once the buffers have reached the sizes a
+// workload needs, a batched phase allocates and frees no device memory while
+// other workers run (hipMalloc and hipFree synchronise with the device).
+// Contexts are handed out exclusively, so nested or concurrent callers get
+// distinct ones; at most kPoolKeep idle contexts are kept, and the rest are
+// destroyed on release.  exec_pool_clear (dmed_finalize) destroys them all.
+constexpr int kPoolKeep = 16;
+std::mutex g_pool_mu;
+std::vector<Exec *> g_pool;
+
+int pool_acquire(int nw, std::vector<Exec *> &out)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    while ((int)out.size() < nw) {
+        if (!g_pool.empty()) {
+            out.push_back(g_pool.back());
+            g_pool.pop_back();
+            continue;
+        }
+        Exec *e = new Exec;
+        if (int rc = exec_create(*e)) {
+            exec_destroy(*e);
+            delete e;
+            return rc;
+        }
+        out.push_back(e);
+    }
+    return 0;
+}
+
+void pool_release(std::vector<Exec *> &v)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (Exec *e : v) {
+        if ((int)g_pool.size() < kPoolKeep) {
+            g_pool.push_back(e);
+        } else {
+            exec_destroy(*e);
+            delete e;
+        }
+    }
+    v.clear();
+}
+
+}  // namespace
+
+void exec_pool_clear()
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (Exec *e : g_pool) {
+        exec_destroy(*e);
+        delete e;
+    }
+    g_pool.clear();
+}
+
+namespace {
+
+// Run ntask tasks: serially on the calling thread, or (batched) on host
+// workers, each bound to its own stream/scratch context from the pool above,
+// taking tasks from a shared counter.  Task results must not depend on which
+// worker ran them.
 int run_tasks(int batched, int ntask, const std::function<void(int)> &fn)
 {
     if (!batched || ntask <= 1) {
@@ -123,29 +183,28 @@ int run_tasks(int batched, int ntask, const std::function<void(int)> &fn)
     int nw = 8;
     if (const char *e = std::getenv("DMED_WORKERS")) nw = std::max(1, std::atoi(e));
     nw = std::min(nw, ntask);
-    std::vector<Exec> execs(nw);
-    for (auto &e : execs)
-        if (int rc = exec_create(e)) {
-            for (auto &x : execs) exec_destroy(x);
-            return rc;
-        }
+    std::vector<Exec *> execs;
+    if (int rc = pool_acquire(nw, execs)) {
+        pool_release(execs);
+        return rc;
+    }
     std::atomic<int> next(0);
     const int dev = ctx().device;
     std::vector<std::thread> th;
     for (int w = 0; w < nw; ++w)
         th.emplace_back([&, w]() {
             hipSetDevice(dev);
-            bind_exec(&execs[w]);
+            bind_exec(execs[w]);
             for (;;) {
                 const int k = next.fetch_add(1);
                 if (k >= ntask) break;
                 fn(k);
             }
-            hipStreamSynchronize(execs[w].stream);
+            hipStreamSynchronize(execs[w]->stream);
             bind_exec(nullptr);
         });
     for (auto &t : th) t.join();
-    for (auto &e : execs) exec_destroy(e);
+    pool_release(execs);
     return 0;
 }
 
