@@ -55,6 +55,8 @@ extern "C" {
  * module initialisation of the reference (init_ed_structure,
  * ED_SETUP.f90:135-332). */
 int dmed_init(int device);
+/* Release the library stream, its work buffers and the worker contexts that
+ * batched solves keep between calls (streams and grow-only work buffers). */
 int dmed_finalize(void);
 const char *dmed_last_error(void);
 
