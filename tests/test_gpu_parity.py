@@ -823,6 +823,37 @@ def test_batched_solve_identical_to_serial(name, finite):
         assert np.abs(r0[k] - r1[k]).max() <= 1e-13 * max(1.0, np.abs(r0[k]).max()), k
 
 
+def test_batched_solves_repeat_bit_for_bit_on_kept_worker_contexts():
+    """The worker contexts of batched solves are kept between calls, and their
+    grow-only buffers hold whatever the previous solve left.  Interleaved
+    batched solves of two models (queue on, mixed chains, susceptibilities)
+    must repeat bit for bit: no result may depend on stale buffer contents or
+    on which worker ran a task."""
+    def run(name, finite):
+        g = load(name)
+        set_both(g)
+        norb, nspin = int(g["norb"]), int(g["nspin"])
+        ns = (int(g["nbath"]) + 1) * norb
+        kw = dict(beta=float(g["beta"]), lmats=len(g["wm"]), lreal=len(g["wr"]),
+                  wini=float(g["wr"][0]), wfin=float(g["wr"][-1]), eps=float(g["eps"]),
+                  lanc_ngfiter=200, lanc_dim_threshold=16, chi_spin=1, chi_dens=1, ltau=100,
+                  batched=1, prec_gf=dmed.MIXED)
+        if finite:
+            kw.update(lanc_nstates_sector=8, lanc_nstates_total=40)
+        return dmed.solve(norb, nspin, ns, dmed.default_params(**kw))
+
+    keys = ("gmats", "smats", "greal", "dens", "spin_chi_iv", "dens_chi_iv")
+    say("[kept contexts] m4, m5, m4, m5")
+    a1 = run("m4_norb3_nb1", False)
+    b1 = run("m5_norb1_nb5_T", True)
+    a2 = run("m4_norb3_nb1", False)
+    b2 = run("m5_norb1_nb5_T", True)
+    for x, y in ((a1, a2), (b1, b2)):
+        assert x["egs"] == y["egs"] and x["nstates"] == y["nstates"]
+        for k in keys:
+            assert np.array_equal(x[k], y[k]), k
+
+
 def test_mixed_precision_chains_error_bounds():
     """Green's-function chains against the FP64 solve on workload 1: all-FP32
     chains (FP64 reductions) within dG < 1e-5, dSigma < 1e-4 (reported); mixed
